@@ -1,0 +1,459 @@
+"""Python-facing wrappers of the gfx950 particle-filter / CBGL libraries.
+
+`PfKernels` and `GlKernels` call the reference ABI (pfc_* / pfgl_*) one to one, so code
+written against pf_kernels.cu reads the same here.  `ParticleFilter`, `GlobalLoc` and
+`quat_mean` drive the C++ host layer (libpf_host.so) that restates the reference's Rust
+engine (crates/pf_core/src/{particle_filter,cuda,global_loc,types}.rs).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+
+OBS_DTYPE = np.dtype([("id", "<i4"), ("mode", "<i4"), ("dx", "<f4"), ("dy", "<f4"), ("dz", "<f4"),
+                      ("range", "<f4")])
+assert OBS_DTYPE.itemsize == 24  # ObsF / CObservation, pf_kernels.cu:79-84, cuda.rs:18-27
+
+STAGES = ["reinit", "predict", "weight", "normalize", "resample_scan", "resample_gather", "inject", "est_sums",
+          "est_covs"]
+GL_STAGES = ["generate", "score", "topk"]
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise RuntimeError(f"{what} failed (rc={rc})")
+
+
+def device_available() -> bool:
+    return _lib.kernels().pfc_device_available() == 1
+
+
+def make_obs(ids, modes, vecs, ranges) -> np.ndarray:
+    o = np.zeros(len(ids), dtype=OBS_DTYPE)
+    o["id"] = ids
+    o["mode"] = modes
+    v = np.asarray(vecs, dtype=np.float64).reshape(-1, 3)
+    o["dx"], o["dy"], o["dz"] = v[:, 0], v[:, 1], v[:, 2]
+    o["range"] = ranges
+    return o
+
+
+class PfKernels:
+    """One pfc_* handle (pf_kernels.cu:475-735)."""
+
+    def __init__(self, cap_obs: int = 256):
+        self.L = _lib.kernels()
+        self.h = self.L.pfc_create(cap_obs)
+        if not self.h:
+            raise RuntimeError("pfc_create failed")
+        self.n = 0
+
+    def close(self):
+        if self.h:
+            self.L.pfc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def init(self, n: int, seed: int):
+        _check(self.L.pfc_init(self.h, n, seed & 0xFFFFFFFF), "pfc_init")
+        self.n = n
+
+    def upload_landmarks(self, xyz):
+        a = np.ascontiguousarray(np.asarray(xyz, dtype=np.float32).reshape(-1))
+        _check(self.L.pfc_upload_landmarks(self.h, _fp(a), a.size // 3), "pfc_upload_landmarks")
+
+    def reinit(self, pose7, pos_std, rot_std):
+        ps = np.asarray(pos_std, np.float32)
+        rs = np.asarray(rot_std, np.float32)
+        _check(self.L.pfc_reinit(self.h, *[float(v) for v in pose7], _fp(ps), _fp(rs)), "pfc_reinit")
+
+    def predict(self, t, q, tstd, rstd):
+        ts = np.asarray(tstd, np.float32)
+        rs = np.asarray(rstd, np.float32)
+        _check(self.L.pfc_predict(self.h, *[float(v) for v in t], *[float(v) for v in q], _fp(ts), _fp(rs)),
+               "pfc_predict")
+
+    def weight(self, obs: np.ndarray | None, sigma_r: float, sigma_b: float, gate: float):
+        if obs is None or len(obs) == 0:
+            rc = self.L.pfc_weight(self.h, None, 0, sigma_r, sigma_b, gate)
+        else:
+            o = np.ascontiguousarray(obs, dtype=OBS_DTYPE)
+            rc = self.L.pfc_weight(self.h, o.ctypes.data, len(o), sigma_r, sigma_b, gate)
+        _check(rc, "pfc_weight")
+
+    def resample(self, ratio: float = 0.0):
+        _check(self.L.pfc_resample(self.h, ratio), "pfc_resample")
+
+    def est_sums(self) -> np.ndarray:
+        out = np.zeros(14)
+        _check(self.L.pfc_est_sums(self.h, _dp(out)), "pfc_est_sums")
+        return out
+
+    def covs(self, mean_pos, mean_quat) -> np.ndarray:
+        out = np.zeros(12)
+        _check(self.L.pfc_covs(self.h, *[float(v) for v in mean_pos], *[float(v) for v in mean_quat], _dp(out)),
+               "pfc_covs")
+        return out
+
+    def snapshot(self):
+        pos = np.zeros((self.n, 3), np.float32)
+        quat = np.zeros((self.n, 4), np.float32)
+        _check(self.L.pfc_snapshot(self.h, _fp(pos), _fp(quat)), "pfc_snapshot")
+        return pos, quat
+
+    # extensions
+    def set_state(self, pos, quat, w=None):
+        p = np.ascontiguousarray(pos, np.float32)
+        q = np.ascontiguousarray(quat, np.float32)
+        wp = None
+        if w is not None:
+            wa = np.ascontiguousarray(w, np.float32)
+            wp = _fp(wa)
+        _check(self.L.pfx_set_state(self.h, _fp(p), _fp(q), wp), "pfx_set_state")
+
+    def get_state(self):
+        pos = np.zeros((self.n, 3), np.float32)
+        quat = np.zeros((self.n, 4), np.float32)
+        w = np.zeros(self.n, np.float32)
+        _check(self.L.pfx_get_state(self.h, _fp(pos), _fp(quat), _fp(w)), "pfx_get_state")
+        return pos, quat, w
+
+    def get_ll(self):
+        ll = np.zeros(self.n)
+        _check(self.L.pfx_get_ll(self.h, _dp(ll)), "pfx_get_ll")
+        return ll
+
+    def ancestors(self):
+        a = np.zeros(self.n, np.int32)
+        _check(self.L.pfx_get_ancestors(self.h, _ip(a)), "pfx_get_ancestors")
+        return a
+
+    def set_lcg(self, v: int):
+        _check(self.L.pfx_set_lcg(self.h, v), "pfx_set_lcg")
+
+    def get_lcg(self) -> int:
+        v = C.c_uint64()
+        _check(self.L.pfx_get_lcg(self.h, C.byref(v)), "pfx_get_lcg")
+        return v.value
+
+    def rng(self):
+        g, n = C.c_uint64(), C.c_uint64()
+        _check(self.L.pfx_get_rng(self.h, C.byref(g), C.byref(n)), "pfx_get_rng")
+        return g.value, n.value
+
+    def weight_scalars(self):
+        out = np.zeros(4)
+        _check(self.L.pfx_weight_scalars(self.h, _dp(out)), "pfx_weight_scalars")
+        return out
+
+    def profile(self, enable: bool = True):
+        _check(self.L.pfx_profile_enable(self.h, 1 if enable else 0), "pfx_profile_enable")
+
+    def profile_read(self):
+        ms = np.zeros(len(STAGES))
+        cnt = np.zeros(len(STAGES), np.int32)
+        _check(self.L.pfx_profile_read(self.h, _dp(ms), _ip(cnt)), "pfx_profile_read")
+        return {s: (float(ms[i]), int(cnt[i])) for i, s in enumerate(STAGES)}
+
+    def sync(self):
+        _check(self.L.pfx_sync(self.h), "pfx_sync")
+
+
+class GlKernels:
+    """One pfgl_* handle (pf_kernels.cu:895-1025)."""
+
+    def __init__(self, grid: np.ndarray, res: float, ox: float, oy: float, cap_hyp: int):
+        self.L = _lib.kernels()
+        g = np.ascontiguousarray(grid, dtype=np.uint8)
+        H, W = g.shape
+        self.W, self.H = W, H
+        self.h = self.L.pfgl_create(W, H, res, ox, oy, g.ctypes.data_as(C.POINTER(C.c_uint8)), cap_hyp)
+        if not self.h:
+            raise RuntimeError("pfgl_create failed")
+
+    def close(self):
+        if self.h:
+            self.L.pfgl_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def free_cells(self) -> int:
+        return self.L.pfgl_free_cells(self.h)
+
+    def generate(self, n: int, seed: int):
+        _check(self.L.pfgl_generate(self.h, n, seed), "pfgl_generate")
+
+    def score(self, scan, rmax: float, ang_span: float):
+        s = np.ascontiguousarray(scan, np.float32)
+        _check(self.L.pfgl_score(self.h, _fp(s), s.size, rmax, ang_span), "pfgl_score")
+
+    def topk(self, k: int):
+        out = np.zeros((k, 3), np.float32)
+        r = self.L.pfgl_topk(self.h, k, _fp(out))
+        if r < 0:
+            raise RuntimeError("pfgl_topk failed")
+        return out[:r]
+
+    def topk_ex(self, k: int):
+        out = np.zeros((k, 3), np.float32)
+        caer = np.zeros(k, np.float32)
+        idx = np.zeros(k, np.int32)
+        r = self.L.pfglx_topk_ex(self.h, k, _fp(out), _fp(caer), _ip(idx))
+        if r < 0:
+            raise RuntimeError("pfglx_topk_ex failed")
+        return out[:r], caer[:r], idx[:r]
+
+    def set_hypotheses(self, hyp):
+        a = np.ascontiguousarray(hyp, np.float32).reshape(-1, 3)
+        _check(self.L.pfglx_set_hypotheses(self.h, _fp(a), len(a)), "pfglx_set_hypotheses")
+
+    def hypotheses(self):
+        n = self.L.pfglx_hypothesis_count(self.h)
+        a = np.zeros((max(n, 0), 3), np.float32)
+        if n > 0 and self.L.pfglx_get_hypotheses(self.h, _fp(a)) < 0:
+            raise RuntimeError("pfglx_get_hypotheses failed")
+        return a
+
+    def caer(self):
+        n = self.L.pfglx_hypothesis_count(self.h)
+        a = np.zeros(max(n, 0), np.float32)
+        if n > 0 and self.L.pfglx_get_caer(self.h, _fp(a)) < 0:
+            raise RuntimeError("pfglx_get_caer failed")
+        return a
+
+    def render(self, poses, nrays: int, rmax: float, ang_span: float):
+        p = np.ascontiguousarray(poses, np.float32).reshape(-1, 3)
+        out = np.zeros((len(p), nrays), np.float32)
+        _check(self.L.pfglx_render(self.h, _fp(p), len(p), nrays, rmax, ang_span, _fp(out)), "pfglx_render")
+        return out
+
+    def profile(self, enable: bool = True):
+        _check(self.L.pfglx_profile_enable(self.h, 1 if enable else 0), "pfglx_profile_enable")
+
+    def profile_read(self):
+        ms = np.zeros(len(GL_STAGES))
+        cnt = np.zeros(len(GL_STAGES), np.int32)
+        _check(self.L.pfglx_profile_read(self.h, _dp(ms), _ip(cnt)), "pfglx_profile_read")
+        return {s: (float(ms[i]), int(cnt[i])) for i, s in enumerate(GL_STAGES)}
+
+    def sync(self):
+        _check(self.L.pfglx_sync(self.h), "pfglx_sync")
+
+
+# ------------------------------------------------------------------ host layer
+
+
+@dataclass
+class PfConfig:
+    """config.rs:3-64 (defaults config.rs:49-64); obs_cap = cuda.rs:95 OBS_CAP."""
+    particle_count: int = 4096
+    range_noise: float = 0.05
+    bearing_noise: float = 0.02
+    resample_ess_ratio: float = 0.5
+    random_inject_ratio: float = 0.0
+    gate_sigma: float = 0.0
+    seed: int = 42
+    obs_cap: int = 256
+    fast: bool = False
+
+
+@dataclass
+class Observation:
+    """types.rs:58-67.  mode 0 Cartesian (vector = body offset); 1 range-bearing
+    (vector = unit direction, range in m)."""
+    landmark_id: int
+    mode: int
+    vector: tuple
+    range: float = 0.0
+
+
+@dataclass
+class OdomDelta:
+    """types.rs:69-90 with its defaults."""
+    translation: tuple = (0.0, 0.0, 0.0)
+    rotation: tuple = (1.0, 0.0, 0.0, 0.0)
+    trans_noise: tuple = (0.05, 0.05, 0.05)
+    rot_noise: tuple = (0.02, 0.02, 0.03)
+
+
+@dataclass
+class Estimate:
+    position: np.ndarray = field(default_factory=lambda: np.zeros(3))
+    attitude: np.ndarray = field(default_factory=lambda: np.array([1.0, 0, 0, 0]))
+    pos_cov: np.ndarray = field(default_factory=lambda: np.ones((3, 3)))
+    att_cov: np.ndarray = field(default_factory=lambda: np.ones((3, 3)))
+    ess: float = 0.0
+    valid: bool = False
+    resampled: bool = False
+
+
+def _est(e: _lib.PfhEstimate) -> Estimate:
+    return Estimate(np.array(e.pos[:]), np.array(e.quat[:]), np.array(e.pos_cov[:]).reshape(3, 3),
+                    np.array(e.att_cov[:]).reshape(3, 3), float(e.ess), bool(e.valid), bool(e.resampled))
+
+
+def _odom(o: OdomDelta) -> _lib.PfhOdom:
+    c = _lib.PfhOdom()
+    c.t[:] = list(o.translation)
+    c.q[:] = list(o.rotation)
+    c.trans_noise[:] = list(o.trans_noise)
+    c.rot_noise[:] = list(o.rot_noise)
+    return c
+
+
+def _obs_array(obs) -> C.Array:
+    arr = (_lib.PfhObs * max(1, len(obs)))()
+    for i, o in enumerate(obs):
+        arr[i].id = o.landmark_id
+        arr[i].mode = o.mode
+        arr[i].vx, arr[i].vy, arr[i].vz = (float(v) for v in o.vector)
+        arr[i].range = float(o.range)
+    return arr
+
+
+class ParticleFilter:
+    """particle_filter.rs:72-133 driving the HIP backend through the C++ host layer."""
+
+    def __init__(self, cfg: PfConfig):
+        self.H = _lib.host()
+        c = _lib.PfhConfig(cfg.particle_count, cfg.range_noise, cfg.bearing_noise, cfg.resample_ess_ratio,
+                           cfg.random_inject_ratio, cfg.gate_sigma, cfg.seed, cfg.obs_cap, 1 if cfg.fast else 0)
+        self.f = self.H.pfh_filter_new(C.byref(c))
+        if not self.f:
+            raise RuntimeError("no usable GPU (pfc_device_available() == 0) or allocation failed")
+        self.cfg = cfg
+
+    def close(self):
+        if self.f:
+            self.H.pfh_filter_free(self.f)
+            self.f = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_map(self, landmarks: dict):
+        ids = np.array(sorted(landmarks), np.int32)
+        xyz = np.array([landmarks[i] for i in ids], np.float64).reshape(-1, 3)
+        _check(self.H.pfh_filter_set_map(self.f, _ip(ids), _dp(xyz), len(ids)), "pfh_filter_set_map")
+
+    def reinitialize(self, pose7, pos_std, rot_std):
+        p = np.asarray(pose7, np.float64)
+        ps = np.asarray(pos_std, np.float64)
+        rs = np.asarray(rot_std, np.float64)
+        _check(self.H.pfh_filter_reinitialize(self.f, _dp(p), _dp(ps), _dp(rs)), "pfh_filter_reinitialize")
+
+    def update(self, odom: OdomDelta, obs) -> Estimate:
+        e = _lib.PfhEstimate()
+        arr = _obs_array(obs)
+        _check(self.H.pfh_filter_update(self.f, C.byref(_odom(odom)), arr, len(obs), C.byref(e)),
+               "pfh_filter_update")
+        return _est(e)
+
+    def update_raw(self, odom_c: _lib.PfhOdom, obs_arr, nobs: int, est_c: _lib.PfhEstimate) -> None:
+        """update() with pre-built ctypes structures (benchmark loop, no per-call packing)."""
+        _check(self.H.pfh_filter_update(self.f, C.byref(odom_c), obs_arr, nobs, C.byref(est_c)),
+               "pfh_filter_update")
+
+    def estimate(self) -> Estimate:
+        e = _lib.PfhEstimate()
+        _check(self.H.pfh_filter_estimate(self.f, C.byref(e)), "pfh_filter_estimate")
+        return _est(e)
+
+    def snapshot(self):
+        n = self.cfg.particle_count
+        pos = np.zeros((n, 3))
+        quat = np.zeros((n, 4))
+        w = np.zeros(n)
+        _check(self.H.pfh_filter_snapshot(self.f, _dp(pos), _dp(quat), _dp(w)), "pfh_filter_snapshot")
+        return pos, quat, w
+
+    def kernels_handle(self):
+        return self.H.pfh_filter_kernels(self.f)
+
+
+def quat_mean(M) -> np.ndarray:
+    """types.rs:128-144 through the host layer's Jacobi solver."""
+    m = np.ascontiguousarray(M, np.float64).reshape(16)
+    q = np.zeros(4)
+    _check(_lib.host().pfh_quat_mean(_dp(m), _dp(q)), "pfh_quat_mean")
+    return q
+
+
+@dataclass
+class GlParams:
+    """global_loc.rs:236-257 defaults."""
+    hypotheses: int = 50_000
+    rays: int = 360
+    rmax: float = 10.0
+    ang_span: float = 2 * np.pi
+    topk: int = 10
+    seed: int = 42
+
+
+class GlobalLoc:
+    """CudaGlobalLoc (global_loc.rs:274-332) over the pfgl_* ABI; also returns CAER."""
+
+    def __init__(self, grid: np.ndarray, res: float, origin, params: GlParams):
+        self.H = _lib.host()
+        g = np.ascontiguousarray(grid, np.uint8)
+        p = _lib.PfhGlParams(params.hypotheses, params.rays, params.rmax, params.ang_span, params.topk, params.seed)
+        self.g = self.H.pfh_gl_new(g.shape[1], g.shape[0], res, float(origin[0]), float(origin[1]),
+                                   g.ctypes.data_as(C.POINTER(C.c_uint8)), C.byref(p))
+        if not self.g:
+            raise RuntimeError("pfgl_create failed")
+        self.params = params
+
+    def close(self):
+        if self.g:
+            self.H.pfh_gl_free(self.g)
+            self.g = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_seed(self, seed: int):
+        self.params.seed = seed
+        _check(self.H.pfh_gl_set_seed(self.g, seed), "pfh_gl_set_seed")
+
+    def localize(self, scan) -> np.ndarray:
+        s = np.ascontiguousarray(scan, np.float32)
+        out = np.zeros((self.params.topk, 4))
+        k = self.H.pfh_gl_localize(self.g, _fp(s), s.size, _dp(out))
+        if k <= 0:
+            raise RuntimeError("localize failed")
+        return out[:k]
+
+    def kernels_handle(self):
+        return self.H.pfh_gl_kernels(self.g)

@@ -1,0 +1,712 @@
+// gl_kernels.hip -- CBGL global localisation on MI355X (gfx950): pfgl_* / pfglx_* C ABI.
+//
+// Drop-in for src/cuda/pf_kernels.cu:737-1025 of the reference (bound by
+// crates/pf_core/src/global_loc.rs:262-271).  Pipeline per localize():
+//   generate  hypotheses uniform over the free cells (counter-based splitmix64 draws;
+//             free cells compacted in row-major order, like global_loc.rs:365-368)
+//   score     one wave64 per hypothesis, lane l casts rays l, l+64, ...; 2-D DDA over a
+//             bit-packed occupancy grid staged in LDS; CAER = sum |real - virtual| with a
+//             fixed lane order and wave tree (deterministic)
+//   topk      bottom-k by radix *select* over unique 64-bit keys (ordered CAER bits << 32 |
+//             hypothesis index), then one workgroup bitonic-sorts the k winners; ties keep
+//             index order as the CPU twin's stable sort does (global_loc.rs:395-397).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../../include/pf_gl.h"
+#include "pf_device.h"
+
+using namespace pfd;
+
+#define GL_CHECK(x)                                                                              \
+    do {                                                                                         \
+        hipError_t e_ = (x);                                                                     \
+        if (e_ != hipSuccess) {                                                                  \
+            fprintf(stderr, "pf_hip %s:%d %s: %s\n", __FILE__, __LINE__, #x, hipGetErrorString(e_)); \
+            return -1;                                                                           \
+        }                                                                                        \
+    } while (0)
+
+#define GL_LAUNCH_CHECK()                                                                        \
+    do {                                                                                         \
+        hipError_t e_ = hipGetLastError();                                                       \
+        if (e_ != hipSuccess) {                                                                  \
+            fprintf(stderr, "pf_hip %s:%d launch failed: %s\n", __FILE__, __LINE__,              \
+                    hipGetErrorString(e_));                                                      \
+            return -1;                                                                           \
+        }                                                                                        \
+    } while (0)
+
+namespace glk {
+
+constexpr int kTB = 256;
+constexpr int kLdsBitsMaxWords = 12288;  // 48 KB of grid bits in LDS (393,216 cells)
+constexpr int kLdsRaysMax = 1024;        // real scan + ray table staged in LDS up to this many rays
+constexpr int kSortMax = 4096;           // winners sorted in one workgroup up to this k
+
+struct Grid {
+    const uint32_t* bits;  // row-major bit cy*W+cx, 1 = occupied
+    int W, H, nwords;
+    float res, ox, oy;
+};
+
+struct GlState {
+    hipStream_t stream = nullptr;
+    Grid g{};
+    uint32_t* d_bits = nullptr;
+    int32_t* d_free = nullptr;  // (cx, cy) pairs, row-major order
+    int nfree = 0;
+    int cap_hyp = 0;
+    int nh = 0;
+    float* d_hyp = nullptr;
+    float* d_caer = nullptr;
+    // scan + ray table
+    float* d_scan = nullptr;
+    double* d_cphi = nullptr;
+    double* d_sphi = nullptr;
+    int ray_cap = 0;
+    int tab_nrays = -1;
+    float tab_span = 0.f;
+    // top-k
+    uint64_t* d_tk = nullptr;  // [0] prefix, [1] threshold, [2] remaining|done, [3] count
+    unsigned* d_hist = nullptr;
+    uint64_t* d_win = nullptr;
+    uint64_t* d_sortbuf = nullptr;
+    int sort_cap = 0;
+    float* d_out = nullptr;
+    float* d_outcaer = nullptr;
+    int32_t* d_outidx = nullptr;
+    // render scratch
+    float* d_poses = nullptr;
+    float* d_ranges = nullptr;
+    size_t render_poses_cap = 0, render_ranges_cap = 0;
+    // profiling
+    bool prof_on = false;
+    struct Ev {
+        int stage;
+        hipEvent_t a, b;
+    };
+    std::vector<Ev> prof;
+};
+
+__device__ inline bool occ_bit(const uint32_t* bits, int idx) { return (bits[idx >> 5] >> (idx & 31)) & 1u; }
+
+// f32 DDA with the operation sequence of pf_kernels.cu:784-808 (distance to the first
+// occupied cell along a unit direction; rmax on exit, miss or beyond rmax).
+__device__ inline float cast_ray(const uint32_t* bits, int W, int H, float res, float ox, float oy, float px, float py,
+                                 float dirx, float diry, float rmax) {
+    const float fx = (px - ox) / res, fy = (py - oy) / res;
+    int cx = (int)floorf(fx), cy = (int)floorf(fy);
+    if (!(cx >= 0 && cy >= 0 && cx < W && cy < H)) return rmax;
+    const int sx = dirx > 0.f ? 1 : -1, sy = diry > 0.f ? 1 : -1;
+    const float adx = fabsf(dirx), ady = fabsf(diry);
+    const float tdx = adx > 1e-9f ? res / adx : INFINITY;
+    const float tdy = ady > 1e-9f ? res / ady : INFINITY;
+    const float frx = fx - (float)cx, fry = fy - (float)cy;
+    float tmx = adx > 1e-9f ? ((dirx > 0.f ? 1.f - frx : frx) * tdx) : INFINITY;
+    float tmy = ady > 1e-9f ? ((diry > 0.f ? 1.f - fry : fry) * tdy) : INFINITY;
+    int idx = cy * W + cx;
+    const int dxi = sx, dyi = sy * W;
+    for (int step = 0; step < 8192; ++step) {
+        float dist;
+        if (tmx <= tmy) {
+            dist = tmx;
+            tmx += tdx;
+            cx += sx;
+            idx += dxi;
+        } else {
+            dist = tmy;
+            tmy += tdy;
+            cy += sy;
+            idx += dyi;
+        }
+        if (dist > rmax) return rmax;
+        if ((unsigned)cx >= (unsigned)W || (unsigned)cy >= (unsigned)H) return rmax;
+        if (occ_bit(bits, idx)) return dist;
+    }
+    return rmax;
+}
+
+// Hypothesis h (deterministic-draw mode): cell = floor(u53 * nfree); x/y offsets and yaw
+// from 24-bit uniforms; f32 arithmetic in the order of pf_kernels.cu:817-822.
+__global__ __launch_bounds__(kTB) void k_gl_gen(const int32_t* __restrict__ freexy, int nfree, float res, float ox,
+                                                float oy, uint64_t seed, int n, float* __restrict__ hyp) {
+    const int h = blockIdx.x * kTB + threadIdx.x;
+    if (h >= n) return;
+    uint64_t x = sm_final(seed * kPhi + (uint64_t)h * kC1);
+    const uint64_t z0 = sm_next(x), z1 = sm_next(x), z2 = sm_next(x), z3 = sm_next(x);
+    long c = (long)(u01_co(z0) * (double)nfree);
+    if (c > nfree - 1) c = nfree - 1;
+    const float ux = (float)(z1 >> 40) * 0x1.0p-24f;
+    const float uy = (float)(z2 >> 40) * 0x1.0p-24f;
+    const float ut = (float)(z3 >> 40) * 0x1.0p-24f;
+    const float fxc = (float)freexy[2 * c] + ux;
+    const float fyc = (float)freexy[2 * c + 1] + uy;
+    const float fxr = fxc * res;
+    const float fyr = fyc * res;
+    hyp[3 * h] = ox + fxr;
+    hyp[3 * h + 1] = oy + fyr;
+    const float tt = 6.28318530f * ut;
+    hyp[3 * h + 2] = -3.14159265f + tt;
+}
+
+// CAER scoring.  LDS layout: [grid bits][real scan f32][cos phi f64][sin phi f64].
+template <bool kLdsGrid, bool kLdsRays>
+__global__ __launch_bounds__(kTB) void k_gl_score(Grid g, const float* __restrict__ hyp, int nh,
+                                                  const float* __restrict__ scan, const double* __restrict__ cphi,
+                                                  const double* __restrict__ sphi, int nrays, float rmax,
+                                                  float* __restrict__ caer) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    uint32_t* sbits = reinterpret_cast<uint32_t*>(smem);
+    const int gw = kLdsGrid ? ((g.nwords + 3) & ~3) : 0;
+    double* scphi = reinterpret_cast<double*>(smem + 4 * (size_t)gw);
+    double* ssphi = scphi + (kLdsRays ? nrays : 0);
+    float* sscan = reinterpret_cast<float*>(ssphi + (kLdsRays ? nrays : 0));
+    if (kLdsGrid)
+        for (int i = threadIdx.x; i < g.nwords; i += kTB) sbits[i] = g.bits[i];
+    if (kLdsRays)
+        for (int i = threadIdx.x; i < nrays; i += kTB) {
+            scphi[i] = cphi[i];
+            ssphi[i] = sphi[i];
+            sscan[i] = scan[i];
+        }
+    __syncthreads();
+    const uint32_t* bits = kLdsGrid ? sbits : g.bits;
+    const double* tc = kLdsRays ? scphi : cphi;
+    const double* ts = kLdsRays ? ssphi : sphi;
+    const float* sc = kLdsRays ? sscan : scan;
+    const int lane = threadIdx.x & 63;
+    const int waves_total = gridDim.x * (kTB / 64);
+    for (int h = blockIdx.x * (kTB / 64) + (threadIdx.x >> 6); h < nh; h += waves_total) {
+        const float x = hyp[3 * h], y = hyp[3 * h + 1];
+        double st, ct;
+        det_sincos((double)hyp[3 * h + 2], st, ct);
+        float acc = 0.f;
+        for (int r = lane; r < nrays; r += 64) {
+            const double cp = tc[r], sp = ts[r];
+            const float dx = (float)(ct * cp - st * sp);
+            const float dy = (float)(st * cp + ct * sp);
+            const float vm = cast_ray(bits, g.W, g.H, g.res, g.ox, g.oy, x, y, dx, dy, rmax);
+            acc += fabsf(sc[r] - vm);
+        }
+        acc = wave_sum_down(acc);
+        if (lane == 0) caer[h] = acc;
+    }
+}
+
+// Virtual scans for poses[3n] -> out[n][nrays] (training data / tests).
+__global__ __launch_bounds__(kTB) void k_gl_render(Grid g, const float* __restrict__ poses, int n,
+                                                   const double* __restrict__ cphi, const double* __restrict__ sphi,
+                                                   int nrays, float rmax, float* __restrict__ out) {
+    const int lane = threadIdx.x & 63;
+    const int waves_total = gridDim.x * (kTB / 64);
+    for (int h = blockIdx.x * (kTB / 64) + (threadIdx.x >> 6); h < n; h += waves_total) {
+        const float x = poses[3 * h], y = poses[3 * h + 1];
+        double st, ct;
+        det_sincos((double)poses[3 * h + 2], st, ct);
+        for (int r = lane; r < nrays; r += 64) {
+            const double cp = cphi[r], sp = sphi[r];
+            const float dx = (float)(ct * cp - st * sp);
+            const float dy = (float)(st * cp + ct * sp);
+            out[(size_t)h * nrays + r] = cast_ray(g.bits, g.W, g.H, g.res, g.ox, g.oy, x, y, dx, dy, rmax);
+        }
+    }
+}
+
+// ---------------------------------------------------------------- bottom-k selection
+
+__device__ inline uint64_t comp_key(float c, int i) {
+    uint32_t b = __float_as_uint(c);
+    b = (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+    return ((uint64_t)b << 32) | (uint32_t)i;
+}
+
+// tk[0] prefix, tk[1] threshold, tk[2] remaining, tk[3] done, tk[4] collect counter
+__global__ void k_tk_init(uint64_t* tk, unsigned* hist, int k) {
+    const int t = threadIdx.x;
+    if (t < 256) hist[t] = 0;
+    if (t == 0) {
+        tk[0] = 0;
+        tk[1] = ~0ull;
+        tk[2] = (uint64_t)k;
+        tk[3] = 0;
+        tk[4] = 0;
+    }
+}
+
+__global__ __launch_bounds__(kTB) void k_tk_hist(const float* __restrict__ caer, int n, const uint64_t* tk,
+                                                 unsigned* hist, int shift) {
+    __shared__ unsigned sh[256];
+    if (tk[3]) return;
+    sh[threadIdx.x] = 0;
+    __syncthreads();
+    const uint64_t prefix = tk[0];
+    const int hs = shift + 8;
+    const uint64_t hiref = hs >= 64 ? 0 : (prefix >> hs);
+    for (int i = blockIdx.x * kTB + threadIdx.x; i < n; i += gridDim.x * kTB) {
+        const uint64_t c = comp_key(caer[i], i);
+        const uint64_t hi = hs >= 64 ? 0 : (c >> hs);
+        if (hi == hiref) atomicAdd(&sh[(c >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    const unsigned v = sh[threadIdx.x];
+    if (v) atomicAdd(&hist[threadIdx.x], v);
+}
+
+__global__ void k_tk_pick(uint64_t* tk, unsigned* hist, int shift) {
+    __shared__ unsigned sh[256];
+    const int t = threadIdx.x;
+    const bool done = tk[3] != 0;
+    sh[t] = hist[t];
+    hist[t] = 0;
+    __syncthreads();
+    if (done || t != 0) return;
+    uint64_t rem = tk[2];
+    uint64_t before = 0;
+    int b = 0;
+    for (; b < 256; ++b) {
+        if (before + sh[b] >= rem) break;
+        before += sh[b];
+    }
+    if (b > 255) b = 255;  // unreachable when k <= n
+    rem -= before;
+    const uint64_t prefix = tk[0] | ((uint64_t)b << shift);
+    tk[0] = prefix;
+    tk[2] = rem;
+    const uint64_t low = shift == 0 ? 0 : ((1ull << shift) - 1);
+    if (sh[b] == rem || shift == 0) {
+        tk[1] = prefix | low;
+        tk[3] = 1;
+    }
+}
+
+__global__ __launch_bounds__(kTB) void k_tk_collect(const float* __restrict__ caer, int n, uint64_t* tk,
+                                                    uint64_t* __restrict__ win) {
+    const uint64_t thr = tk[1];
+    for (int i = blockIdx.x * kTB + threadIdx.x; i < n; i += gridDim.x * kTB) {
+        const uint64_t c = comp_key(caer[i], i);
+        if (c <= thr) {
+            const unsigned long long slot = atomicAdd(reinterpret_cast<unsigned long long*>(&tk[4]), 1ull);
+            win[slot] = c;
+        }
+    }
+}
+
+// Sorts k <= kSortMax keys in one workgroup (bitonic in LDS) and gathers the poses.
+__global__ __launch_bounds__(1024) void k_tk_sort(const uint64_t* __restrict__ win, int k,
+                                                  const float* __restrict__ hyp, const float* __restrict__ caer,
+                                                  float* __restrict__ out, float* __restrict__ outcaer,
+                                                  int32_t* __restrict__ outidx) {
+    __shared__ uint64_t sk[kSortMax];
+    int m = 1;
+    while (m < k) m <<= 1;
+    for (int i = threadIdx.x; i < m; i += 1024) sk[i] = i < k ? win[i] : ~0ull;
+    __syncthreads();
+    for (int size = 2; size <= m; size <<= 1)
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int i = threadIdx.x; i < m; i += 1024) {
+                const int j = i ^ stride;
+                if (j > i) {
+                    const bool up = (i & size) == 0;
+                    const uint64_t a = sk[i], b = sk[j];
+                    if ((a > b) == up) {
+                        sk[i] = b;
+                        sk[j] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    for (int i = threadIdx.x; i < k; i += 1024) {
+        const int src = (int)(uint32_t)(sk[i] & 0xffffffffu);
+        out[3 * i] = hyp[3 * src];
+        out[3 * i + 1] = hyp[3 * src + 1];
+        out[3 * i + 2] = hyp[3 * src + 2];
+        outcaer[i] = caer[src];
+        outidx[i] = src;
+    }
+}
+
+// Global bitonic sort step over m (power of two) composite keys (large-k fallback).
+__global__ __launch_bounds__(kTB) void k_bitonic_step(uint64_t* __restrict__ a, int m, int size, int stride) {
+    const int i = blockIdx.x * kTB + threadIdx.x;
+    if (i >= m) return;
+    const int j = i ^ stride;
+    if (j > i) {
+        const bool up = (i & size) == 0;
+        const uint64_t x = a[i], y = a[j];
+        if ((x > y) == up) {
+            a[i] = y;
+            a[j] = x;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kTB) void k_fill_keys(const float* __restrict__ caer, int n, int m,
+                                                   uint64_t* __restrict__ a) {
+    const int i = blockIdx.x * kTB + threadIdx.x;
+    if (i < m) a[i] = i < n ? comp_key(caer[i], i) : ~0ull;
+}
+
+__global__ __launch_bounds__(kTB) void k_gather_sorted(const uint64_t* __restrict__ a, int k,
+                                                       const float* __restrict__ hyp, const float* __restrict__ caer,
+                                                       float* __restrict__ out, float* __restrict__ outcaer,
+                                                       int32_t* __restrict__ outidx) {
+    const int i = blockIdx.x * kTB + threadIdx.x;
+    if (i >= k) return;
+    const int src = (int)(uint32_t)(a[i] & 0xffffffffu);
+    out[3 * i] = hyp[3 * src];
+    out[3 * i + 1] = hyp[3 * src + 1];
+    out[3 * i + 2] = hyp[3 * src + 2];
+    outcaer[i] = caer[src];
+    outidx[i] = src;
+}
+
+// ---------------------------------------------------------------- host helpers
+
+void prof_b(GlState* s, int stage) {
+    if (!s->prof_on) return;
+    GlState::Ev e{stage, nullptr, nullptr};
+    (void)hipEventCreate(&e.a);
+    (void)hipEventCreate(&e.b);
+    (void)hipEventRecord(e.a, s->stream);
+    s->prof.push_back(e);
+}
+void prof_e(GlState* s) {
+    if (s->prof_on && !s->prof.empty()) (void)hipEventRecord(s->prof.back().b, s->stream);
+}
+
+int ensure_rays(GlState* s, int nrays, float span) {
+    if (nrays > s->ray_cap) {
+        (void)hipFree(s->d_scan);
+        (void)hipFree(s->d_cphi);
+        (void)hipFree(s->d_sphi);
+        s->d_scan = nullptr;
+        s->d_cphi = s->d_sphi = nullptr;
+        GL_CHECK(hipMalloc((void**)&s->d_scan, sizeof(float) * (size_t)nrays));
+        GL_CHECK(hipMalloc((void**)&s->d_cphi, sizeof(double) * (size_t)nrays));
+        GL_CHECK(hipMalloc((void**)&s->d_sphi, sizeof(double) * (size_t)nrays));
+        s->ray_cap = nrays;
+        s->tab_nrays = -1;
+    }
+    if (s->tab_nrays != nrays || s->tab_span != span) {
+        // ray r at phi = span * r / (nrays - 1) relative to the hypothesis yaw (global_loc.rs:204-209)
+        std::vector<double> c((size_t)nrays), sn((size_t)nrays);
+        for (int r = 0; r < nrays; ++r) {
+            const double phi = nrays > 1 ? (double)span * (double)r / (double)(nrays - 1) : 0.0;
+            c[r] = cos(phi);
+            sn[r] = sin(phi);
+        }
+        GL_CHECK(hipMemcpyAsync(s->d_cphi, c.data(), sizeof(double) * (size_t)nrays, hipMemcpyHostToDevice, s->stream));
+        GL_CHECK(hipMemcpyAsync(s->d_sphi, sn.data(), sizeof(double) * (size_t)nrays, hipMemcpyHostToDevice, s->stream));
+        GL_CHECK(hipStreamSynchronize(s->stream));
+        s->tab_nrays = nrays;
+        s->tab_span = span;
+    }
+    return 0;
+}
+
+int score_launch(GlState* s, int nrays, float rmax) {
+    const bool lds_grid = s->g.nwords <= kLdsBitsMaxWords;
+    const bool lds_rays = nrays <= kLdsRaysMax;
+    const size_t shm = (lds_grid ? 4 * (size_t)((s->g.nwords + 3) & ~3) : 0) +
+                       (lds_rays ? (size_t)nrays * (8 + 8 + 4) : 0);
+    const int waves = (s->nh + 0);
+    int blocks = (waves + 3) / 4;
+    if (blocks > 4096) blocks = 4096;
+    if (blocks < 1) blocks = 1;
+    Grid g = s->g;
+    g.bits = s->d_bits;
+    prof_b(s, PFGLX_STAGE_SCORE);
+    if (lds_grid && lds_rays)
+        hipLaunchKernelGGL((k_gl_score<true, true>), dim3(blocks), dim3(kTB), shm, s->stream, g, s->d_hyp, s->nh,
+                           s->d_scan, s->d_cphi, s->d_sphi, nrays, rmax, s->d_caer);
+    else if (lds_grid)
+        hipLaunchKernelGGL((k_gl_score<true, false>), dim3(blocks), dim3(kTB), shm, s->stream, g, s->d_hyp, s->nh,
+                           s->d_scan, s->d_cphi, s->d_sphi, nrays, rmax, s->d_caer);
+    else if (lds_rays)
+        hipLaunchKernelGGL((k_gl_score<false, true>), dim3(blocks), dim3(kTB), shm, s->stream, g, s->d_hyp, s->nh,
+                           s->d_scan, s->d_cphi, s->d_sphi, nrays, rmax, s->d_caer);
+    else
+        hipLaunchKernelGGL((k_gl_score<false, false>), dim3(blocks), dim3(kTB), shm, s->stream, g, s->d_hyp, s->nh,
+                           s->d_scan, s->d_cphi, s->d_sphi, nrays, rmax, s->d_caer);
+    prof_e(s);
+    GL_LAUNCH_CHECK();
+    return 0;
+}
+
+int topk_launch(GlState* s, int k) {
+    const int n = s->nh;
+    int blocks = (n + kTB - 1) / kTB;
+    if (blocks > 1024) blocks = 1024;
+    prof_b(s, PFGLX_STAGE_TOPK);
+    if (k <= kSortMax) {
+        hipLaunchKernelGGL(k_tk_init, dim3(1), dim3(256), 0, s->stream, s->d_tk, s->d_hist, k);
+        for (int pass = 0; pass < 8; ++pass) {
+            const int shift = 56 - 8 * pass;
+            hipLaunchKernelGGL(k_tk_hist, dim3(blocks), dim3(kTB), 0, s->stream, s->d_caer, n, s->d_tk, s->d_hist,
+                               shift);
+            hipLaunchKernelGGL(k_tk_pick, dim3(1), dim3(256), 0, s->stream, s->d_tk, s->d_hist, shift);
+        }
+        hipLaunchKernelGGL(k_tk_collect, dim3(blocks), dim3(kTB), 0, s->stream, s->d_caer, n, s->d_tk, s->d_win);
+        hipLaunchKernelGGL(k_tk_sort, dim3(1), dim3(1024), 0, s->stream, s->d_win, k, s->d_hyp, s->d_caer, s->d_out,
+                           s->d_outcaer, s->d_outidx);
+    } else {
+        int m = 1;
+        while (m < n) m <<= 1;
+        if (m > s->sort_cap) {
+            (void)hipFree(s->d_sortbuf);
+            s->d_sortbuf = nullptr;
+            GL_CHECK(hipMalloc((void**)&s->d_sortbuf, sizeof(uint64_t) * (size_t)m));
+            s->sort_cap = m;
+        }
+        const int gb = (m + kTB - 1) / kTB;
+        hipLaunchKernelGGL(k_fill_keys, dim3(gb), dim3(kTB), 0, s->stream, s->d_caer, n, m, s->d_sortbuf);
+        for (int size = 2; size <= m; size <<= 1)
+            for (int stride = size >> 1; stride > 0; stride >>= 1)
+                hipLaunchKernelGGL(k_bitonic_step, dim3(gb), dim3(kTB), 0, s->stream, s->d_sortbuf, m, size, stride);
+        hipLaunchKernelGGL(k_gather_sorted, dim3((k + kTB - 1) / kTB), dim3(kTB), 0, s->stream, s->d_sortbuf, k,
+                           s->d_hyp, s->d_caer, s->d_out, s->d_outcaer, s->d_outidx);
+    }
+    prof_e(s);
+    GL_LAUNCH_CHECK();
+    return 0;
+}
+
+}  // namespace glk
+
+using namespace glk;
+
+extern "C" {
+
+void* pfgl_create(int W, int H, float res, float ox, float oy, const unsigned char* occ, int cap_hyp) {
+    if (W <= 0 || H <= 0 || cap_hyp <= 0 || !occ || (long long)W * H > (1ll << 30)) return nullptr;
+    GlState* s = new (std::nothrow) GlState();
+    if (!s) return nullptr;
+    const long long ncell = (long long)W * H;
+    const int nwords = (int)((ncell + 31) / 32);
+    std::vector<uint32_t> bits((size_t)nwords, 0u);
+    std::vector<int32_t> freexy;
+    freexy.reserve((size_t)ncell * 2);
+    for (long long i = 0; i < ncell; ++i) {
+        if (occ[i]) {
+            bits[(size_t)(i >> 5)] |= 1u << (i & 31);
+        } else {
+            freexy.push_back((int32_t)(i % W));
+            freexy.push_back((int32_t)(i / W));
+        }
+    }
+    s->nfree = (int)(freexy.size() / 2);
+    s->cap_hyp = cap_hyp;
+    s->g.W = W;
+    s->g.H = H;
+    s->g.nwords = nwords;
+    s->g.res = res;
+    s->g.ox = ox;
+    s->g.oy = oy;
+    bool ok = hipSetDevice(0) == hipSuccess && hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) == hipSuccess;
+    ok = ok && hipMalloc((void**)&s->d_bits, sizeof(uint32_t) * (size_t)nwords) == hipSuccess;
+    ok = ok && hipMalloc((void**)&s->d_free, sizeof(int32_t) * (freexy.size() + 2)) == hipSuccess;
+    ok = ok && hipMalloc((void**)&s->d_hyp, sizeof(float) * 3 * (size_t)cap_hyp) == hipSuccess;
+    ok = ok && hipMalloc((void**)&s->d_caer, sizeof(float) * (size_t)cap_hyp) == hipSuccess;
+    ok = ok && hipMalloc((void**)&s->d_tk, sizeof(uint64_t) * 8) == hipSuccess;
+    ok = ok && hipMalloc((void**)&s->d_hist, sizeof(unsigned) * 256) == hipSuccess;
+    ok = ok && hipMalloc((void**)&s->d_win, sizeof(uint64_t) * (size_t)std::min(cap_hyp, kSortMax)) == hipSuccess;
+    ok = ok && hipMalloc((void**)&s->d_out, sizeof(float) * 3 * (size_t)cap_hyp) == hipSuccess;
+    ok = ok && hipMalloc((void**)&s->d_outcaer, sizeof(float) * (size_t)cap_hyp) == hipSuccess;
+    ok = ok && hipMalloc((void**)&s->d_outidx, sizeof(int32_t) * (size_t)cap_hyp) == hipSuccess;
+    ok = ok && hipMemcpy(s->d_bits, bits.data(), sizeof(uint32_t) * (size_t)nwords, hipMemcpyHostToDevice) == hipSuccess;
+    if (ok && !freexy.empty())
+        ok = hipMemcpy(s->d_free, freexy.data(), sizeof(int32_t) * freexy.size(), hipMemcpyHostToDevice) == hipSuccess;
+    ok = ok && hipMemset(s->d_hist, 0, sizeof(unsigned) * 256) == hipSuccess;
+    if (!ok) {
+        fprintf(stderr, "pf_hip pfgl_create: device setup failed\n");
+        pfgl_destroy(s);
+        return nullptr;
+    }
+    return s;
+}
+
+int pfgl_destroy(void* h) {
+    GlState* s = static_cast<GlState*>(h);
+    if (!s) return 0;
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    void* ptrs[] = {s->d_bits, s->d_free,    s->d_hyp,      s->d_caer,   s->d_scan, s->d_cphi,
+                    s->d_sphi, s->d_tk,      s->d_hist,     s->d_win,    s->d_sortbuf, s->d_out,
+                    s->d_outcaer, s->d_outidx, s->d_poses, s->d_ranges};
+    for (void* p : ptrs) (void)hipFree(p);
+    for (auto& e : s->prof) {
+        (void)hipEventDestroy(e.a);
+        (void)hipEventDestroy(e.b);
+    }
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
+    return 0;
+}
+
+int pfgl_free_cells(void* h) {
+    GlState* s = static_cast<GlState*>(h);
+    return s ? s->nfree : -1;
+}
+
+int pfgl_generate(void* h, int n, unsigned long long seed) {
+    GlState* s = static_cast<GlState*>(h);
+    if (!s || s->nfree <= 0) return -1;
+    if (n <= 0 || n > s->cap_hyp) return -1;
+    s->nh = n;
+    prof_b(s, PFGLX_STAGE_GENERATE);
+    hipLaunchKernelGGL(k_gl_gen, dim3((n + kTB - 1) / kTB), dim3(kTB), 0, s->stream, s->d_free, s->nfree, s->g.res,
+                       s->g.ox, s->g.oy, (uint64_t)seed, n, s->d_hyp);
+    prof_e(s);
+    GL_LAUNCH_CHECK();
+    return 0;
+}
+
+int pfgl_score(void* h, const float* real_scan, int nrays, float rmax, float ang_span) {
+    GlState* s = static_cast<GlState*>(h);
+    if (!s || s->nh <= 0 || !real_scan || nrays <= 0) return -1;
+    if (ensure_rays(s, nrays, ang_span) != 0) return -1;
+    GL_CHECK(hipMemcpyAsync(s->d_scan, real_scan, sizeof(float) * (size_t)nrays, hipMemcpyHostToDevice, s->stream));
+    if (score_launch(s, nrays, rmax) != 0) return -1;
+    GL_CHECK(hipStreamSynchronize(s->stream));  // real_scan is borrowed for the call only
+    return 0;
+}
+
+int pfglx_topk_ex(void* h, int k, float* out3k, float* caer_k, int32_t* idx_k) {
+    GlState* s = static_cast<GlState*>(h);
+    if (!s || k <= 0 || s->nh <= 0) return -1;
+    if (k > s->nh) k = s->nh;
+    if (topk_launch(s, k) != 0) return -1;
+    if (out3k) GL_CHECK(hipMemcpyAsync(out3k, s->d_out, sizeof(float) * 3 * (size_t)k, hipMemcpyDeviceToHost, s->stream));
+    if (caer_k)
+        GL_CHECK(hipMemcpyAsync(caer_k, s->d_outcaer, sizeof(float) * (size_t)k, hipMemcpyDeviceToHost, s->stream));
+    if (idx_k)
+        GL_CHECK(hipMemcpyAsync(idx_k, s->d_outidx, sizeof(int32_t) * (size_t)k, hipMemcpyDeviceToHost, s->stream));
+    GL_CHECK(hipStreamSynchronize(s->stream));
+    return k;
+}
+
+int pfgl_topk(void* h, int k, float* out) {
+    if (!out) return -1;
+    return pfglx_topk_ex(h, k, out, nullptr, nullptr);
+}
+
+int pfglx_set_hypotheses(void* h, const float* hyp3n, int n) {
+    GlState* s = static_cast<GlState*>(h);
+    if (!s || !hyp3n || n <= 0 || n > s->cap_hyp) return -1;
+    GL_CHECK(hipMemcpyAsync(s->d_hyp, hyp3n, sizeof(float) * 3 * (size_t)n, hipMemcpyHostToDevice, s->stream));
+    GL_CHECK(hipStreamSynchronize(s->stream));
+    s->nh = n;
+    return 0;
+}
+
+int pfglx_get_hypotheses(void* h, float* hyp3n) {
+    GlState* s = static_cast<GlState*>(h);
+    if (!s || !hyp3n || s->nh <= 0) return -1;
+    GL_CHECK(hipMemcpyAsync(hyp3n, s->d_hyp, sizeof(float) * 3 * (size_t)s->nh, hipMemcpyDeviceToHost, s->stream));
+    GL_CHECK(hipStreamSynchronize(s->stream));
+    return s->nh;
+}
+
+int pfglx_get_caer(void* h, float* caer) {
+    GlState* s = static_cast<GlState*>(h);
+    if (!s || !caer || s->nh <= 0) return -1;
+    GL_CHECK(hipMemcpyAsync(caer, s->d_caer, sizeof(float) * (size_t)s->nh, hipMemcpyDeviceToHost, s->stream));
+    GL_CHECK(hipStreamSynchronize(s->stream));
+    return s->nh;
+}
+
+int pfglx_hypothesis_count(void* h) {
+    GlState* s = static_cast<GlState*>(h);
+    return s ? s->nh : -1;
+}
+
+int pfglx_render(void* h, const float* poses3n, int n, int nrays, float rmax, float ang_span, float* out) {
+    GlState* s = static_cast<GlState*>(h);
+    if (!s || !poses3n || !out || n <= 0 || nrays <= 0) return -1;
+    if (ensure_rays(s, nrays, ang_span) != 0) return -1;
+    const size_t need = (size_t)n;
+    if (need > s->render_poses_cap) {
+        (void)hipFree(s->d_poses);
+        s->d_poses = nullptr;
+        GL_CHECK(hipMalloc((void**)&s->d_poses, sizeof(float) * 3 * need));
+        s->render_poses_cap = need;
+    }
+    if (need * (size_t)nrays > s->render_ranges_cap) {
+        (void)hipFree(s->d_ranges);
+        s->d_ranges = nullptr;
+        GL_CHECK(hipMalloc((void**)&s->d_ranges, sizeof(float) * need * (size_t)nrays));
+        s->render_ranges_cap = need * (size_t)nrays;
+    }
+    GL_CHECK(hipMemcpyAsync(s->d_poses, poses3n, sizeof(float) * 3 * need, hipMemcpyHostToDevice, s->stream));
+    Grid g = s->g;
+    g.bits = s->d_bits;
+    int blocks = (n + 3) / 4;
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(k_gl_render, dim3(blocks), dim3(kTB), 0, s->stream, g, s->d_poses, n, s->d_cphi, s->d_sphi,
+                       nrays, rmax, s->d_ranges);
+    GL_LAUNCH_CHECK();
+    GL_CHECK(hipMemcpyAsync(out, s->d_ranges, sizeof(float) * need * (size_t)nrays, hipMemcpyDeviceToHost, s->stream));
+    GL_CHECK(hipStreamSynchronize(s->stream));
+    return 0;
+}
+
+// Device-resident render for callers that keep data on the GPU (training-set generation).
+int pfglx_render_device(void* h, const float* d_poses3n, int n, int nrays, float rmax, float ang_span, float* d_out,
+                        void* stream) {
+    GlState* s = static_cast<GlState*>(h);
+    if (!s || !d_poses3n || !d_out || n <= 0 || nrays <= 0) return -1;
+    if (ensure_rays(s, nrays, ang_span) != 0) return -1;
+    Grid g = s->g;
+    g.bits = s->d_bits;
+    int blocks = (n + 3) / 4;
+    if (blocks > 8192) blocks = 8192;
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : s->stream;
+    hipLaunchKernelGGL(k_gl_render, dim3(blocks), dim3(kTB), 0, st, g, d_poses3n, n, s->d_cphi, s->d_sphi, nrays, rmax,
+                       d_out);
+    GL_LAUNCH_CHECK();
+    return 0;
+}
+
+int pfglx_profile_enable(void* h, int enable) {
+    GlState* s = static_cast<GlState*>(h);
+    if (!s) return -1;
+    s->prof_on = enable != 0;
+    return 0;
+}
+
+int pfglx_profile_read(void* h, double* ms, int* count) {
+    GlState* s = static_cast<GlState*>(h);
+    if (!s) return -1;
+    GL_CHECK(hipStreamSynchronize(s->stream));
+    for (int k = 0; k < PFGLX_STAGE_COUNT; ++k) {
+        if (ms) ms[k] = 0.0;
+        if (count) count[k] = 0;
+    }
+    for (auto& e : s->prof) {
+        float t = 0.f;
+        if (hipEventElapsedTime(&t, e.a, e.b) == hipSuccess) {
+            if (ms) ms[e.stage] += t;
+            if (count) count[e.stage] += 1;
+        }
+        (void)hipEventDestroy(e.a);
+        (void)hipEventDestroy(e.b);
+    }
+    s->prof.clear();
+    return 0;
+}
+
+int pfglx_sync(void* h) {
+    GlState* s = static_cast<GlState*>(h);
+    if (!s) return -1;
+    GL_CHECK(hipStreamSynchronize(s->stream));
+    return 0;
+}
+
+}  // extern "C"

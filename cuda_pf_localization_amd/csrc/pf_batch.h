@@ -1,0 +1,166 @@
+// pf_batch.h -- particle-store layout, per-filter parameter blocks and handle state.
+//
+// One handle holds nf >= 1 filters back to back in the same SoA streams.  Filter f owns
+// particles [off_f, off_f + n_f) of every stream, off_f a multiple of kTile, so a
+// 256-particle block or a 1024-particle tile never straddles two filters.  The reference
+// ABI (pfc_*) drives a handle with nf = 1; the batched ABI (pfb_*) drives nf > 1 with the
+// same kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <cstddef>
+#include <vector>
+
+namespace pfk {
+
+constexpr int kTB = 256;     // threads per block
+constexpr int kTile = 1024;  // filter alignment, scan tile and estimate tile
+constexpr int kEstN = 14;    // pfc_est_sums outputs (pf_kernels.cu:71)
+constexpr int kCovN = 12;    // pfc_covs outputs (pf_kernels.cu:72)
+
+struct PoseSet {
+    float* c[7];  // px py pz qw qx qy qz
+};
+struct PoseBufs {
+    PoseSet s[2];  // ping-pong: F[f].parity selects the current set
+};
+
+// Device-resident per-filter state (scalars of the last weight / scan / estimate passes).
+struct FilterDev {
+    double max_ll, sum_e, sum_e2, ess;
+    int uniform;      // last weight pass degenerate -> weights are 1/n
+    int parity;       // current pose set
+    int do_resample;  // resample predicate of the current cycle
+    uint32_t u32;     // top 32 bits of the advanced LCG (systematic offset)
+    uint64_t total;   // fixed-point weight total of the last scan
+    uint64_t lcg;     // resampling LCG state (pf_kernels.cu:116, 653)
+    double sums[kEstN];
+    double covs[kCovN];
+    unsigned cnt_weight, cnt_gather, cnt_sums, cnt_covs;
+};
+
+struct Layout {
+    const int* blk_filter;  // filter of every 256-particle block
+    const int* f_n;
+    const int* f_off;
+    const int* f_padn;      // n rounded up to kTile
+    const float* f_winv;    // (float)(1/n)
+};
+
+// Observation joined with its landmark, read as wave-uniform scalar loads (64 B).
+struct ObsRec {
+    double lx, ly, lz;
+    double vx, vy, vz;
+    double range;
+    int32_t mode, pad;
+};
+
+struct MotionParams {
+    double t[3];    // predict: body translation; reinit: prior position
+    double q[4];    // predict: odometry rotation; reinit: prior attitude (wxyz)
+    double s_t[3];  // translation / position std
+    double s_r[3];  // rotation std
+    uint64_t gseed; // generation seed of this call
+    float winv;     // reinit: 1/n
+    int pad;
+};
+
+struct WeightParams {
+    double inv2rr, inv2bb, gate, cart_gate2, rng_gate2, brg_gate2, cart_gated, half_g2, t_gate;
+    const ObsRec* obs;
+    int nobs, pad;
+};
+
+struct InjectParams {
+    uint64_t gen;  // generation counter (pf_kernels.cu:337)
+    int k;         // number of picks
+    int pad;
+};
+
+struct CovParams {
+    double mp[3];
+    double mq[4];
+};
+
+// Parameters by value (one filter) or from a device array indexed by filter.
+template <class T>
+struct ParamRef {
+    const T* dev;
+    T val;
+    __device__ T get(int f) const { return dev ? dev[f] : val; }
+    static ParamRef value(const T& v) {
+        ParamRef r;
+        r.dev = nullptr;
+        r.val = v;
+        return r;
+    }
+    static ParamRef array(const T* d) {
+        ParamRef r;
+        r.dev = d;
+        r.val = T{};
+        return r;
+    }
+};
+
+struct ProfEvent {
+    int stage;
+    hipEvent_t a, b;
+};
+
+struct PfState {
+    hipStream_t stream = nullptr;
+    int obs_cap = 0;
+    // layout
+    int nf = 0, NP = 0;
+    std::vector<int> n, off, padn;
+    int *d_blk_filter = nullptr, *d_fn = nullptr, *d_foff = nullptr, *d_fpadn = nullptr;
+    float* d_fwinv = nullptr;
+    Layout L{};
+    // particle streams
+    PoseBufs pose{};
+    float* w = nullptr;
+    double* ll = nullptr;
+    uint64_t* cum = nullptr;
+    int32_t* anc = nullptr;
+    int32_t *inj_idx = nullptr, *inj_next = nullptr, *inj_head = nullptr;
+    double* inj_jit = nullptr;
+    // reduction partials / scan state
+    double *part_w = nullptr, *part_est = nullptr, *part_cov = nullptr;
+    uint64_t* scan_status = nullptr;
+    unsigned* scan_ctr = nullptr;
+    FilterDev* fdev = nullptr;
+    // observations (reference ABI path)
+    ObsRec* d_obs = nullptr;
+    ObsRec* h_obs = nullptr;  // pinned staging
+    hipEvent_t obs_ev = nullptr;
+    std::vector<float> lm;  // host mirror of the dense landmark table
+    int lm_cap = 0;
+    // deterministic-draw state (single-filter ABI)
+    uint64_t gseed = 42, gen = 0;
+    // profiling
+    bool prof_on = false;
+    std::vector<ProfEvent> prof_pending;
+};
+
+int state_alloc(PfState* s, const std::vector<int>& ns);
+void state_free_particles(PfState* s);
+void prof_begin(PfState* s, int stage);
+void prof_end(PfState* s);
+int launch_motion(PfState* s, ParamRef<MotionParams> A, int reinit);
+int launch_weight(PfState* s, ParamRef<WeightParams> A);
+int launch_normalize(PfState* s, int force_uniform);
+int launch_resample(PfState* s, ParamRef<InjectParams> inj, int any_inject);
+int launch_est_sums(PfState* s);
+int launch_est_covs(PfState* s, ParamRef<CovParams> A);
+int launch_predicate(PfState* s, const float* d_ratio, int force);
+WeightParams make_weight_params(double sigma_r, double sigma_b, double gate);
+int join_obs(const struct pfc_obs* obs, int nobs, const float* lm, int lm_cap, ObsRec* out);
+
+// Device address of a FilterDev member of filter f.
+template <class M>
+inline M* fdev_field(PfState* s, int f, size_t offset) {
+    return reinterpret_cast<M*>(reinterpret_cast<char*>(s->fdev + f) + offset);
+}
+
+}  // namespace pfk

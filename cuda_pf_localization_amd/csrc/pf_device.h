@@ -1,0 +1,265 @@
+// pf_device.h -- device helpers shared by the particle-filter and CBGL kernels (gfx950).
+//
+// Deterministic-draw mode.  Process noise is a pure function of (generation seed,
+// particle index): the particle key is the splitmix64 finaliser of
+// gseed*phi + i*C1 (crates/pf_core/src/cpu.rs:151-161), its stream is the splitmix64
+// sequence of src/cuda/pf_kernels.cu:323-328, and normals come from a Box-Muller
+// transform whose log, sqrt and sincos are built from IEEE fma/mul/add only.  Every
+// operation here is explicit (the translation units are compiled with
+// -ffp-contract=off), so a CPU evaluation of the same sequence is bit-identical.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pfd {
+
+constexpr uint64_t kPhi = 0x9E3779B97F4A7C15ull;
+constexpr uint64_t kC1 = 0xBF58476D1CE4E5B9ull;
+constexpr uint64_t kC2 = 0x94D049BB133111EBull;
+constexpr uint64_t kGenBump = 0x9E37797979797979ull;  // cpu.rs:136
+
+__host__ __device__ inline uint64_t sm_final(uint64_t z) {
+    z = (z ^ (z >> 30)) * kC1;
+    z = (z ^ (z >> 27)) * kC2;
+    return z ^ (z >> 31);
+}
+__host__ __device__ inline uint64_t particle_key(uint64_t gseed, uint64_t i) {
+    return sm_final(gseed * kPhi + i * kC1);
+}
+__host__ __device__ inline uint64_t sm_next(uint64_t& x) {
+    x += kPhi;
+    return sm_final(x);
+}
+__device__ inline double u01_co(uint64_t z) { return (double)(z >> 11) * 0x1.0p-53; }
+__device__ inline double u01_oc(uint64_t z) { return (double)((z >> 11) + 1) * 0x1.0p-53; }
+
+// Natural log of a positive normal double: exponent split, m in [1/sqrt2, sqrt2),
+// ln m = 2 atanh(s), s = (m-1)/(m+1) with a Newton reciprocal.
+__device__ inline double det_log(double x) {
+    uint64_t b = __double_as_longlong(x);
+    int e = (int)((b >> 52) & 0x7ff) - 1023;
+    double m = __longlong_as_double((long long)((b & 0x000FFFFFFFFFFFFFull) | 0x3FF0000000000000ull));
+    if (m > 1.4142135623730951) {
+        m = m * 0.5;
+        e += 1;
+    }
+    const double f = m - 1.0;
+    const double d = m + 1.0;
+    double y = __fma_rn(-0.25, d, 1.0);
+#pragma unroll
+    for (int it = 0; it < 5; ++it) {
+        const double r = __fma_rn(-d, y, 1.0);
+        y = __fma_rn(y, r, y);
+    }
+    const double s = f * y;
+    const double s2 = s * s;
+    double p = 1.0 / 23.0;
+    p = __fma_rn(p, s2, 1.0 / 21.0);
+    p = __fma_rn(p, s2, 1.0 / 19.0);
+    p = __fma_rn(p, s2, 1.0 / 17.0);
+    p = __fma_rn(p, s2, 1.0 / 15.0);
+    p = __fma_rn(p, s2, 1.0 / 13.0);
+    p = __fma_rn(p, s2, 1.0 / 11.0);
+    p = __fma_rn(p, s2, 1.0 / 9.0);
+    p = __fma_rn(p, s2, 1.0 / 7.0);
+    p = __fma_rn(p, s2, 1.0 / 5.0);
+    p = __fma_rn(p, s2, 1.0 / 3.0);
+    const double lnm = 2.0 * __fma_rn(s * s2, p, s);
+    const double de = (double)e;
+    const double r = __fma_rn(de, 1.9082149292705877e-10, lnm);
+    return __fma_rn(de, 0.6931471803691238, r);
+}
+
+// Square root by an integer-seeded reciprocal-sqrt Newton iteration.
+__device__ inline double det_sqrt(double x) {
+    if (!(x > 0.0)) return 0.0;
+    uint64_t b = 0x5FE6EB50C7B537A9ull - (__double_as_longlong(x) >> 1);
+    double y = __longlong_as_double((long long)b);
+    const double h = 0.5 * x;
+#pragma unroll
+    for (int it = 0; it < 5; ++it) {
+        const double t = h * y;
+        const double r = __fma_rn(-t, y, 0.5);
+        y = __fma_rn(y, r, y);
+    }
+    const double s = x * y;
+    const double r = __fma_rn(-s, s, x);
+    return __fma_rn(r, 0.5 * y, s);
+}
+
+// sin and cos of x radians: pi/2 Cody-Waite reduction, Taylor on |r| <= pi/4.
+__device__ inline void det_sincos(double x, double& so, double& co) {
+    const double k = floor(x * 0.63661977236758138 + 0.5);
+    double r = __fma_rn(-k, 1.5707963267948966, x);
+    r = __fma_rn(-k, 6.123233995736766e-17, r);
+    const double r2 = r * r;
+    double ps = 1.0 / 355687428096000.0;
+    ps = __fma_rn(ps, r2, -1.0 / 1307674368000.0);
+    ps = __fma_rn(ps, r2, 1.0 / 6227020800.0);
+    ps = __fma_rn(ps, r2, -1.0 / 39916800.0);
+    ps = __fma_rn(ps, r2, 1.0 / 362880.0);
+    ps = __fma_rn(ps, r2, -1.0 / 5040.0);
+    ps = __fma_rn(ps, r2, 1.0 / 120.0);
+    ps = __fma_rn(ps, r2, -1.0 / 6.0);
+    const double sn = __fma_rn(r * r2, ps, r);
+    double pc = 1.0 / 6402373705728000.0;
+    pc = __fma_rn(pc, r2, -1.0 / 20922789888000.0);
+    pc = __fma_rn(pc, r2, 1.0 / 87178291200.0);
+    pc = __fma_rn(pc, r2, -1.0 / 479001600.0);
+    pc = __fma_rn(pc, r2, 1.0 / 3628800.0);
+    pc = __fma_rn(pc, r2, -1.0 / 40320.0);
+    pc = __fma_rn(pc, r2, 1.0 / 720.0);
+    pc = __fma_rn(pc, r2, -1.0 / 24.0);
+    pc = __fma_rn(pc, r2, 0.5);
+    const double cs = __fma_rn(-r2, pc, 1.0);
+    const long q = (long)k & 3;
+    if (q == 0) {
+        so = sn;
+        co = cs;
+    } else if (q == 1) {
+        so = cs;
+        co = -sn;
+    } else if (q == 2) {
+        so = -sn;
+        co = -cs;
+    } else {
+        so = -cs;
+        co = sn;
+    }
+}
+
+// cos(h) and sin(h)/h from h2 = h^2: Taylor in h2 after halving h2 <= 1, then doubling.
+__device__ inline void det_cos_sinc(double h2, double& co, double& sinc) {
+    int m = 0;
+    while (h2 > 1.0 && m < 64) {
+        h2 *= 0.25;
+        ++m;
+    }
+    double c = 1.0 / 2432902008176640000.0;
+    c = __fma_rn(c, -h2, 1.0 / 6402373705728000.0);
+    c = __fma_rn(c, -h2, 1.0 / 20922789888000.0);
+    c = __fma_rn(c, -h2, 1.0 / 87178291200.0);
+    c = __fma_rn(c, -h2, 1.0 / 479001600.0);
+    c = __fma_rn(c, -h2, 1.0 / 3628800.0);
+    c = __fma_rn(c, -h2, 1.0 / 40320.0);
+    c = __fma_rn(c, -h2, 1.0 / 720.0);
+    c = __fma_rn(c, -h2, 1.0 / 24.0);
+    c = __fma_rn(c, -h2, 0.5);
+    c = __fma_rn(c, -h2, 1.0);
+    double s = 1.0 / 51090942171709440000.0;
+    s = __fma_rn(s, -h2, 1.0 / 121645100408832000.0);
+    s = __fma_rn(s, -h2, 1.0 / 355687428096000.0);
+    s = __fma_rn(s, -h2, 1.0 / 1307674368000.0);
+    s = __fma_rn(s, -h2, 1.0 / 6227020800.0);
+    s = __fma_rn(s, -h2, 1.0 / 39916800.0);
+    s = __fma_rn(s, -h2, 1.0 / 362880.0);
+    s = __fma_rn(s, -h2, 1.0 / 5040.0);
+    s = __fma_rn(s, -h2, 1.0 / 120.0);
+    s = __fma_rn(s, -h2, 1.0 / 6.0);
+    s = __fma_rn(s, -h2, 1.0);
+    for (int i = 0; i < m; ++i) {
+        s = s * c;
+        c = __fma_rn(2.0 * c, c, -1.0);
+    }
+    co = c;
+    sinc = s;
+}
+
+// Six standard normals of particle i (three Box-Muller pairs).
+__device__ inline void particle_normals(uint64_t gseed, uint64_t i, double n6[6]) {
+    uint64_t x = particle_key(gseed, i);
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+        const uint64_t za = sm_next(x);
+        const uint64_t zb = sm_next(x);
+        const double u1 = u01_oc(za);
+        const double u2 = u01_co(zb);
+        const double r = det_sqrt(-2.0 * det_log(u1));
+        double s, c;
+        det_sincos(u2 * 6.283185307179586, s, c);
+        n6[2 * p] = r * c;
+        n6[2 * p + 1] = r * s;
+    }
+}
+
+struct dquat {
+    double w, x, y, z;
+};
+
+// Hamilton product a (x) b.
+__device__ inline dquat qmul(const dquat& a, const dquat& b) {
+    dquat r;
+    r.w = a.w * b.w - a.x * b.x - a.y * b.y - a.z * b.z;
+    r.x = a.w * b.x + a.x * b.w + a.y * b.z - a.z * b.y;
+    r.y = a.w * b.y - a.x * b.z + a.y * b.w + a.z * b.x;
+    r.z = a.w * b.z + a.x * b.y - a.y * b.x + a.z * b.w;
+    return r;
+}
+
+// v' = v + w t + qv x t with t = 2 qv x v (the rotation formula of the reference:
+// pf_kernels.cu:119-131, nalgebra UnitQuaternion::transform_vector).
+__device__ inline void qrot(const dquat& q, double vx, double vy, double vz, double& ox, double& oy,
+                            double& oz) {
+    const double tx = 2.0 * (q.y * vz - q.z * vy);
+    const double ty = 2.0 * (q.z * vx - q.x * vz);
+    const double tz = 2.0 * (q.x * vy - q.y * vx);
+    const double cx = q.y * tz - q.z * ty;
+    const double cy = q.z * tx - q.x * tz;
+    const double cz = q.x * ty - q.y * tx;
+    ox = vx + q.w * tx + cx;
+    oy = vy + q.w * ty + cy;
+    oz = vz + q.w * tz + cz;
+}
+
+// Axis-angle noise quaternion exp(a/2) = (cos h, a sinc(h)/2), |a| = 2h (cpu.rs:30-42).
+__device__ inline dquat noise_dq(double sx, double sy, double sz, const double* n3) {
+    const double ax = sx * n3[0], ay = sy * n3[1], az = sz * n3[2];
+    const double ang2 = ax * ax + ay * ay + az * az;
+    dquat d{1.0, 0.0, 0.0, 0.0};
+    if (ang2 > 1e-24) {
+        double c, s;
+        det_cos_sinc(0.25 * ang2, c, s);
+        const double hs = 0.5 * s;
+        d.w = c;
+        d.x = ax * hs;
+        d.y = ay * hs;
+        d.z = az * hs;
+    }
+    return d;
+}
+
+// ---------------------------------------------------------------- wave64 reductions
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+__device__ inline double wave_max(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    return v;
+}
+// Tree sum landing in lane 0 with the order l += l+32, +16, ..., +1.
+__device__ inline float wave_sum_down(float v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = v + __shfl_down(v, off, 64);
+    return v;
+}
+
+// Exact unsigned 64x64 -> 128 product compare: a*b < c*d.
+__device__ inline bool mul128_lt(uint64_t a, uint64_t b, uint64_t c, uint64_t d) {
+    const uint64_t h1 = __umul64hi(a, b), l1 = a * b;
+    const uint64_t h2 = __umul64hi(c, d), l2 = c * d;
+    return (h1 < h2) || (h1 == h2 && l1 < l2);
+}
+
+// Fixed-point systematic-resampling weight floor(w * 2^56), w clamped to [0, 64).
+__device__ inline uint64_t qweight(float wf) {
+    const double w = (double)wf;
+    if (!(w > 0.0)) return 0;
+    if (w >= 64.0) return (uint64_t)1 << 62;
+    return (uint64_t)(w * 0x1.0p56);
+}
+
+}  // namespace pfd

@@ -1,0 +1,438 @@
+// pf_host.cpp -- C++ host layer over the pfc_* / pfgl_* ABI (the role the reference's Rust
+// crate pf_core plays above libpf_kernels.so).
+//
+//   HipBackend      crates/pf_core/src/cuda.rs:97-357 (f64->f32 casts, map.has filtering,
+//                   OBS_CAP truncation, landmark upload only on change, host Markley mean,
+//                   f32 round trip of the mean position, packed-triangle unpacking)
+//   ParticleFilter  crates/pf_core/src/particle_filter.rs:72-133 (update cycle order)
+//   quat_mean       crates/pf_core/src/types.rs:128-144 (largest eigenvector of sym(M);
+//                   cyclic Jacobi here, nalgebra is not available)
+//   GlobalLoc       crates/pf_core/src/global_loc.rs:274-332 (generate -> score -> topk)
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <array>
+#include <optional>
+#include <vector>
+
+#include "pf_gl.h"
+#include "pf_host.h"
+#include "pf_kernels.h"
+
+namespace pfh {
+
+// ------------------------------------------------------------------ Markley mean
+
+// Largest-eigenvalue unit eigenvector of (M + M^T)/2 by cyclic Jacobi rotations,
+// sign chosen so that w >= 0 (types.rs:139-142).
+int quat_mean(const double* M, double* q) {
+    double a[4][4], v[4][4];
+    for (int r = 0; r < 4; ++r)
+        for (int c = 0; c < 4; ++c) {
+            a[r][c] = 0.5 * (M[4 * r + c] + M[4 * c + r]);
+            v[r][c] = r == c ? 1.0 : 0.0;
+        }
+    for (int sweep = 0; sweep < 50; ++sweep) {
+        double off = 0.0, diag = 0.0;
+        for (int r = 0; r < 4; ++r) {
+            diag += a[r][r] * a[r][r];
+            for (int c = r + 1; c < 4; ++c) off += a[r][c] * a[r][c];
+        }
+        if (off <= 1e-32 * diag || off == 0.0) break;
+        for (int p = 0; p < 3; ++p)
+            for (int r = p + 1; r < 4; ++r) {
+                const double apr = a[p][r];
+                if (apr == 0.0) continue;
+                // rotation angle zeroing a[p][r]: tan(2 phi) = 2 a_pr / (a_rr - a_pp)
+                const double tau = (a[r][r] - a[p][p]) / (2.0 * apr);
+                const double t = (tau >= 0.0 ? 1.0 : -1.0) / (std::fabs(tau) + std::sqrt(1.0 + tau * tau));
+                const double c = 1.0 / std::sqrt(1.0 + t * t), s = t * c;
+                for (int k = 0; k < 4; ++k) {
+                    const double kp = a[k][p], kr = a[k][r];
+                    a[k][p] = c * kp - s * kr;
+                    a[k][r] = s * kp + c * kr;
+                }
+                for (int k = 0; k < 4; ++k) {
+                    const double pk = a[p][k], rk = a[r][k];
+                    a[p][k] = c * pk - s * rk;
+                    a[r][k] = s * pk + c * rk;
+                }
+                for (int k = 0; k < 4; ++k) {
+                    const double kp = v[k][p], kr = v[k][r];
+                    v[k][p] = c * kp - s * kr;
+                    v[k][r] = s * kp + c * kr;
+                }
+            }
+    }
+    int best = 0;
+    for (int i = 1; i < 4; ++i)
+        if (a[i][i] > a[best][best]) best = i;
+    double nrm = 0.0;
+    for (int r = 0; r < 4; ++r) nrm += v[r][best] * v[r][best];
+    nrm = std::sqrt(nrm);
+    if (!(nrm > 0.0)) return -1;
+    const double sg = v[0][best] < 0.0 ? -1.0 : 1.0;
+    for (int r = 0; r < 4; ++r) q[r] = sg * v[r][best] / nrm;
+    return 0;
+}
+
+// ------------------------------------------------------------------ landmark map
+
+class LandmarkMap {
+   public:
+    void add(int id, double x, double y, double z) {
+        if (id < 0) return;
+        if ((int)pos_.size() <= id) pos_.resize((size_t)id + 1);
+        pos_[(size_t)id] = std::array<double, 3>{x, y, z};
+    }
+    bool has(int id) const { return id >= 0 && id < (int)pos_.size() && pos_[(size_t)id].has_value(); }
+    int capacity() const { return (int)pos_.size(); }
+    const std::vector<std::optional<std::array<double, 3>>>& entries() const { return pos_; }
+
+   private:
+    std::vector<std::optional<std::array<double, 3>>> pos_;
+};
+
+// ------------------------------------------------------------------ backend
+
+struct Estimate {
+    double pos[3]{0, 0, 0}, quat[4]{1, 0, 0, 0};
+    double pos_cov[3][3]{{1, 1, 1}, {1, 1, 1}, {1, 1, 1}}, att_cov[3][3]{{1, 1, 1}, {1, 1, 1}, {1, 1, 1}};
+    double ess = 0.0;
+    bool valid = false;  // Estimate::default (types.rs:105-115)
+};
+
+class HipBackend {
+   public:
+    explicit HipBackend(int obs_cap) : obs_cap_(obs_cap) {}
+    ~HipBackend() {
+        if (h_) pfc_destroy(h_);
+    }
+    bool create() {
+        if (pfc_device_available() != 1) return false;  // cuda.rs:113-115
+        h_ = pfc_create(obs_cap_);
+        return h_ != nullptr;
+    }
+    bool init(int64_t n, const pfh_config& cfg) {  // cuda.rs:180-185
+        n_ = n;
+        cfg_ = cfg;
+        return pfc_init(h_, (int)n, (unsigned)cfg.seed) == 0;
+    }
+    int reinit(const double* pose7, const double* pos_std, const double* rot_std) {  // cuda.rs:187-205
+        const float ps[3] = {(float)pos_std[0], (float)pos_std[1], (float)pos_std[2]};
+        const float rs[3] = {(float)rot_std[0], (float)rot_std[1], (float)rot_std[2]};
+        return pfc_reinit(h_, (float)pose7[0], (float)pose7[1], (float)pose7[2], (float)pose7[3], (float)pose7[4],
+                          (float)pose7[5], (float)pose7[6], ps, rs);
+    }
+    int predict(const pfh_odom& o) {  // cuda.rs:207-225
+        const float t[3] = {(float)o.trans_noise[0], (float)o.trans_noise[1], (float)o.trans_noise[2]};
+        const float r[3] = {(float)o.rot_noise[0], (float)o.rot_noise[1], (float)o.rot_noise[2]};
+        return pfc_predict(h_, (float)o.t[0], (float)o.t[1], (float)o.t[2], (float)o.q[0], (float)o.q[1],
+                           (float)o.q[2], (float)o.q[3], t, r);
+    }
+    int weight(const pfh_obs* obs, int nobs, const LandmarkMap& map) {  // cuda.rs:227-256
+        const float sr = (float)cfg_.range_noise, sb = (float)cfg_.bearing_noise, g = (float)cfg_.gate_sigma;
+        cobs_.clear();
+        for (int k = 0; k < nobs; ++k) {
+            if (!map.has(obs[k].id)) continue;
+            if ((int)cobs_.size() >= obs_cap_) break;  // take(OBS_CAP), cuda.rs:159
+            pfc_obs c;
+            c.id = obs[k].id;
+            c.mode = obs[k].mode == 0 ? 0 : 1;
+            c.dx = (float)obs[k].vx;
+            c.dy = (float)obs[k].vy;
+            c.dz = (float)obs[k].vz;
+            c.range = (float)obs[k].range;
+            cobs_.push_back(c);
+        }
+        if (cobs_.empty()) return pfc_weight(h_, nullptr, 0, sr, sb, g);
+        if (sync_landmarks(map) != 0) return -1;
+        return pfc_weight(h_, cobs_.data(), (int)cobs_.size(), sr, sb, g);
+    }
+    int resample(double ratio) { return pfc_resample(h_, (float)ratio); }  // cuda.rs:258-262
+
+    // pass 1 only (sums; ESS)
+    int sums(double* s14) { return pfc_est_sums(h_, s14); }
+
+    // cuda.rs:264-329, optionally reusing the sums of a preceding pass-1 call
+    int estimate(Estimate& e, const double* sums_in = nullptr) {
+        if (n_ == 0) {
+            e = Estimate();
+            return 0;
+        }
+        double s[14];
+        if (sums_in)
+            memcpy(s, sums_in, sizeof(s));
+        else if (pfc_est_sums(h_, s) != 0)
+            return -1;
+        double M[16];
+        for (int r = 0; r < 4; ++r)
+            for (int c = r; c < 4; ++c) {
+                const int off = 4 * r - r * (r - 1) / 2 + (c - r);
+                M[4 * r + c] = s[3 + off];
+                M[4 * c + r] = s[3 + off];
+            }
+        double q[4];
+        if (quat_mean(M, q) != 0) return -1;
+        double cv[12];
+        if (pfc_covs(h_, (float)s[0], (float)s[1], (float)s[2], (float)q[0], (float)q[1], (float)q[2], (float)q[3],
+                     cv) != 0)
+            return -1;
+        auto sym3 = [&](int o, double (&m)[3][3]) {
+            for (int r = 0; r < 3; ++r)
+                for (int c = r; c < 3; ++c) {
+                    const double x = cv[o + 3 * r - r * (r - 1) / 2 + (c - r)];
+                    m[r][c] = x;
+                    m[c][r] = x;
+                }
+        };
+        for (int k = 0; k < 3; ++k) e.pos[k] = s[k];
+        for (int k = 0; k < 4; ++k) e.quat[k] = q[k];
+        sym3(0, e.pos_cov);
+        sym3(6, e.att_cov);
+        e.ess = s[13] > 0.0 ? 1.0 / s[13] : 0.0;
+        e.valid = true;
+        return 0;
+    }
+    int snapshot(double* pos, double* quat, double* w) {  // cuda.rs:331-357
+        std::vector<float> p((size_t)n_ * 3), q((size_t)n_ * 4);
+        if (pfc_snapshot(h_, p.data(), q.data()) != 0) return -1;
+        for (int64_t i = 0; i < n_; ++i) {
+            for (int c = 0; c < 3; ++c) pos[3 * i + c] = p[3 * i + c];
+            double qq[4], nrm = 0.0;
+            for (int c = 0; c < 4; ++c) {
+                qq[c] = q[4 * i + c];
+                nrm += qq[c] * qq[c];
+            }
+            nrm = std::sqrt(nrm);  // UnitQuaternion::from_quaternion normalises
+            for (int c = 0; c < 4; ++c) quat[4 * i + c] = nrm > 0.0 ? qq[c] / nrm : qq[c];
+            if (w) w[i] = 1.0 / (double)n_;
+        }
+        return 0;
+    }
+    void* handle() const { return h_; }
+    int64_t n() const { return n_; }
+
+   private:
+    int sync_landmarks(const LandmarkMap& map) {  // cuda.rs:140-154
+        const int cap = map.capacity() > 1 ? map.capacity() : 1;
+        std::vector<float> flat((size_t)cap * 3, 0.f);
+        const auto& ent = map.entries();
+        for (size_t id = 0; id < ent.size(); ++id)
+            if (ent[id])
+                for (int c = 0; c < 3; ++c) flat[3 * id + c] = (float)(*ent[id])[c];
+        if (flat != lm_cache_) {
+            if (pfc_upload_landmarks(h_, flat.data(), cap) != 0) return -1;
+            lm_cache_.swap(flat);
+        }
+        return 0;
+    }
+    void* h_ = nullptr;
+    int obs_cap_;
+    int64_t n_ = 0;
+    pfh_config cfg_{};
+    std::vector<float> lm_cache_;
+    std::vector<pfc_obs> cobs_;
+};
+
+class ParticleFilter {
+   public:
+    explicit ParticleFilter(const pfh_config& cfg) : cfg_(cfg), be_(cfg.obs_cap > 0 ? cfg.obs_cap : 256) {}
+    bool create() { return be_.create() && be_.init(cfg_.particle_count, cfg_); }
+    LandmarkMap& map() { return map_; }
+    int reinitialize(const double* pose7, const double* ps, const double* rs) {  // pf.rs:93-96
+        last_ = Estimate();
+        return be_.reinit(pose7, ps, rs);
+    }
+    // pf.rs:110-124: predict -> weight -> estimate -> [resample -> estimate]
+    int update(const pfh_odom& odom, const pfh_obs* obs, int nobs, Estimate& est, bool& resampled) {
+        resampled = false;
+        if (be_.predict(odom) != 0) return -1;
+        if (be_.weight(obs, nobs, map_) != 0) return -1;
+        const double thr = cfg_.resample_ess_ratio * (double)cfg_.particle_count;
+        if (cfg_.fast) {
+            // The first estimate's mean and covariances are discarded when the filter
+            // resamples; only its ESS is read, so compute pass 1 alone first.
+            double s[14];
+            if (be_.sums(s) != 0) return -1;
+            const double ess = s[13] > 0.0 ? 1.0 / s[13] : 0.0;
+            if (cfg_.resample_ess_ratio > 0.0 && ess < thr) {
+                if (be_.resample(cfg_.random_inject_ratio) != 0) return -1;
+                resampled = true;
+                if (be_.estimate(est) != 0) return -1;
+            } else if (be_.estimate(est, s) != 0) {
+                return -1;
+            }
+        } else {
+            if (be_.estimate(est) != 0) return -1;
+            if (cfg_.resample_ess_ratio > 0.0 && est.ess < thr) {
+                if (be_.resample(cfg_.random_inject_ratio) != 0) return -1;
+                resampled = true;
+                if (be_.estimate(est) != 0) return -1;
+            }
+        }
+        last_ = est;
+        return 0;
+    }
+    int estimate(Estimate& e) { return be_.estimate(e); }
+    HipBackend& backend() { return be_; }
+
+   private:
+    pfh_config cfg_;
+    HipBackend be_;
+    LandmarkMap map_;
+    Estimate last_;
+};
+
+// ------------------------------------------------------------------ global localisation
+
+class GlobalLoc {
+   public:
+    ~GlobalLoc() {
+        if (h_) pfgl_destroy(h_);
+    }
+    bool create(int W, int H, double res, double ox, double oy, const uint8_t* cells, const pfh_gl_params& p) {
+        p_ = p;
+        h_ = pfgl_create(W, H, (float)res, (float)ox, (float)oy, cells, p.hypotheses);  // global_loc.rs:287-303
+        return h_ != nullptr;
+    }
+    // global_loc.rs:306-327 (the CAER of each winner is returned as well)
+    int localize(const float* scan, int nrays, double* out4k) {
+        if (pfgl_generate(h_, p_.hypotheses, (unsigned long long)p_.seed) != 0) return -1;
+        if (pfgl_score(h_, scan, nrays, (float)p_.rmax, (float)p_.ang_span) != 0) return -1;
+        std::vector<float> xyz((size_t)p_.topk * 3), caer((size_t)p_.topk);
+        const int k = pfglx_topk_ex(h_, p_.topk, xyz.data(), caer.data(), nullptr);
+        if (k <= 0) return -1;
+        for (int i = 0; i < k; ++i) {
+            out4k[4 * i] = xyz[3 * i];
+            out4k[4 * i + 1] = xyz[3 * i + 1];
+            out4k[4 * i + 2] = xyz[3 * i + 2];
+            out4k[4 * i + 3] = caer[i];
+        }
+        return k;
+    }
+    void set_seed(uint64_t s) { p_.seed = s; }
+    void* handle() const { return h_; }
+
+   private:
+    void* h_ = nullptr;
+    pfh_gl_params p_{};
+};
+
+void to_c(const Estimate& e, bool resampled, pfh_estimate* out) {
+    for (int k = 0; k < 3; ++k) out->pos[k] = e.pos[k];
+    for (int k = 0; k < 4; ++k) out->quat[k] = e.quat[k];
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) {
+            out->pos_cov[3 * r + c] = e.pos_cov[r][c];
+            out->att_cov[3 * r + c] = e.att_cov[r][c];
+        }
+    out->ess = e.ess;
+    out->valid = e.valid ? 1 : 0;
+    out->resampled = resampled ? 1 : 0;
+}
+
+}  // namespace pfh
+
+using namespace pfh;
+
+extern "C" {
+
+void* pfh_filter_new(const pfh_config* cfg) {
+    if (!cfg || cfg->particle_count <= 0) return nullptr;
+    ParticleFilter* f = new (std::nothrow) ParticleFilter(*cfg);
+    if (!f) return nullptr;
+    if (!f->create()) {
+        delete f;
+        return nullptr;
+    }
+    return f;
+}
+
+void pfh_filter_free(void* f) { delete static_cast<ParticleFilter*>(f); }
+
+int pfh_filter_set_map(void* fp, const int32_t* ids, const double* xyz, int n) {
+    ParticleFilter* f = static_cast<ParticleFilter*>(fp);
+    if (!f || (n > 0 && (!ids || !xyz))) return -1;
+    f->map() = LandmarkMap();
+    for (int i = 0; i < n; ++i) {
+        if (ids[i] < 0) return -1;  // landmark_map.rs:19
+        f->map().add(ids[i], xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
+    }
+    return 0;
+}
+
+int pfh_filter_reinitialize(void* fp, const double pose7[7], const double pos_std[3], const double rot_std[3]) {
+    ParticleFilter* f = static_cast<ParticleFilter*>(fp);
+    if (!f || !pose7 || !pos_std || !rot_std) return -1;
+    return f->reinitialize(pose7, pos_std, rot_std);
+}
+
+int pfh_filter_update(void* fp, const pfh_odom* odom, const pfh_obs* obs, int nobs, pfh_estimate* out) {
+    ParticleFilter* f = static_cast<ParticleFilter*>(fp);
+    if (!f || !odom || (nobs > 0 && !obs)) return -1;
+    Estimate e;
+    bool rs = false;
+    if (f->update(*odom, obs, nobs, e, rs) != 0) return -1;
+    if (out) to_c(e, rs, out);
+    return 0;
+}
+
+int pfh_filter_estimate(void* fp, pfh_estimate* out) {
+    ParticleFilter* f = static_cast<ParticleFilter*>(fp);
+    if (!f || !out) return -1;
+    Estimate e;
+    if (f->estimate(e) != 0) return -1;
+    to_c(e, false, out);
+    return 0;
+}
+
+int pfh_filter_snapshot(void* fp, double* pos3n, double* quat4n, double* w) {
+    ParticleFilter* f = static_cast<ParticleFilter*>(fp);
+    if (!f || !pos3n || !quat4n) return -1;
+    return f->backend().snapshot(pos3n, quat4n, w);
+}
+
+void* pfh_filter_kernels(void* fp) {
+    ParticleFilter* f = static_cast<ParticleFilter*>(fp);
+    return f ? f->backend().handle() : nullptr;
+}
+
+int pfh_quat_mean(const double M[16], double q[4]) {
+    if (!M || !q) return -1;
+    return quat_mean(M, q);
+}
+
+void* pfh_gl_new(int W, int H, double res, double ox, double oy, const uint8_t* cells, const pfh_gl_params* p) {
+    if (!p || !cells) return nullptr;
+    GlobalLoc* g = new (std::nothrow) GlobalLoc();
+    if (!g) return nullptr;
+    if (!g->create(W, H, res, ox, oy, cells, *p)) {
+        delete g;
+        return nullptr;
+    }
+    return g;
+}
+
+void pfh_gl_free(void* g) { delete static_cast<GlobalLoc*>(g); }
+
+int pfh_gl_localize(void* gp, const float* scan, int nrays, double* out4k) {
+    GlobalLoc* g = static_cast<GlobalLoc*>(gp);
+    if (!g || !scan || !out4k || nrays <= 0) return -1;
+    return g->localize(scan, nrays, out4k);
+}
+
+int pfh_gl_set_seed(void* gp, uint64_t seed) {
+    GlobalLoc* g = static_cast<GlobalLoc*>(gp);
+    if (!g) return -1;
+    g->set_seed(seed);
+    return 0;
+}
+
+void* pfh_gl_kernels(void* gp) {
+    GlobalLoc* g = static_cast<GlobalLoc*>(gp);
+    return g ? g->handle() : nullptr;
+}
+
+}  // extern "C"

@@ -1,0 +1,1249 @@
+// pf_kernels.hip -- MI355X (gfx950) particle-filter kernels and the pfc_* / pfx_* C ABI.
+//
+// Drop-in for the reference library src/cuda/pf_kernels.cu (rus1ru/cuda_pf_localization):
+// the 12 pfc_* entry points keep its signatures and semantics (bound by
+// crates/pf_core/src/cuda.rs:33-88); the kernels are a new design for CDNA4:
+//
+//   * SoA particle store: seven f32 pose streams (px py pz qw qx qy qz) in two ping-pong
+//     sets, f32 weights, f64 log-likelihoods.  Filters are laid out back to back at
+//     1024-particle alignment, so one launch serves one filter or a batch of filters.
+//   * k_motion: reinit/predict with counter-based splitmix64 draws and an fma-only
+//     Box-Muller (no RNG state in memory), pose algebra in f64, f32 storage.
+//   * k_weight: one particle per lane, observation+landmark records read as wave-uniform
+//     scalar loads, cancellation-prone residuals in f64 (full-rate FP64 FMA on CDNA4),
+//     fused online-softmax block partials and a deterministic last-block finalise.
+//   * resampling: fixed-point (2^-56) weights, integer decoupled-lookback scan with
+//     dynamic tile tickets, exact 128-bit pointer comparisons, ping-pong gather (no copies),
+//     deterministic injection (per-target pick lists summed in pick order).
+//   * estimate: wave64 butterfly reductions into f64 block partials with a deterministic
+//     last-block finalise per filter.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../../include/pf_kernels.h"
+#include "pf_batch.h"
+#include "pf_device.h"
+
+using namespace pfd;
+
+#define PF_CHECK(x)                                                                              \
+    do {                                                                                         \
+        hipError_t e_ = (x);                                                                     \
+        if (e_ != hipSuccess) {                                                                  \
+            fprintf(stderr, "pf_hip %s:%d %s: %s\n", __FILE__, __LINE__, #x, hipGetErrorString(e_)); \
+            return -1;                                                                           \
+        }                                                                                        \
+    } while (0)
+
+#define PF_LAUNCH_CHECK()                                                                        \
+    do {                                                                                         \
+        hipError_t e_ = hipGetLastError();                                                       \
+        if (e_ != hipSuccess) {                                                                  \
+            fprintf(stderr, "pf_hip %s:%d launch failed: %s\n", __FILE__, __LINE__,              \
+                    hipGetErrorString(e_));                                                      \
+            return -1;                                                                           \
+        }                                                                                        \
+    } while (0)
+
+namespace pfk {
+
+// ============================================================================ kernels
+
+// Reinit (cpu.rs:180-195) and predict (cpu.rs:197-218), one particle per lane.
+__global__ __launch_bounds__(kTB) void k_motion(Layout L, PoseBufs B, FilterDev* F, float* __restrict__ w,
+                                                ParamRef<MotionParams> A, int reinit) {
+    const int blk = blockIdx.x;
+    const int f = L.blk_filter[blk];
+    const int off = L.f_off[f];
+    const int li = blk * kTB + (int)threadIdx.x - off;
+    if (li >= L.f_n[f]) return;
+    const MotionParams& p = A.get(f);
+    const int gi = off + li;
+    const PoseSet S = B.s[F[f].parity];
+    double nn[6];
+    particle_normals(p.gseed, (uint64_t)li, nn);
+    if (reinit) {
+        const double x = p.t[0] + p.s_t[0] * nn[0];
+        const double y = p.t[1] + p.s_t[1] * nn[1];
+        const double z = p.t[2] + p.s_t[2] * nn[2];
+        const dquat q0{p.q[0], p.q[1], p.q[2], p.q[3]};
+        const dquat dq = noise_dq(p.s_r[0], p.s_r[1], p.s_r[2], nn + 3);
+        const dquat q = qmul(q0, dq);
+        S.c[0][gi] = (float)x;
+        S.c[1][gi] = (float)y;
+        S.c[2][gi] = (float)z;
+        S.c[3][gi] = (float)q.w;
+        S.c[4][gi] = (float)q.x;
+        S.c[5][gi] = (float)q.y;
+        S.c[6][gi] = (float)q.z;
+        w[gi] = p.winv;
+    } else {
+        const dquat q{(double)S.c[3][gi], (double)S.c[4][gi], (double)S.c[5][gi], (double)S.c[6][gi]};
+        const double bx = p.t[0] + p.s_t[0] * nn[0];
+        const double by = p.t[1] + p.s_t[1] * nn[1];
+        const double bz = p.t[2] + p.s_t[2] * nn[2];
+        double wx, wy, wz;
+        qrot(q, bx, by, bz, wx, wy, wz);
+        const double x = (double)S.c[0][gi] + wx;
+        const double y = (double)S.c[1][gi] + wy;
+        const double z = (double)S.c[2][gi] + wz;
+        const dquat qo{p.q[0], p.q[1], p.q[2], p.q[3]};
+        const dquat dq = noise_dq(p.s_r[0], p.s_r[1], p.s_r[2], nn + 3);
+        const dquat qn = qmul(qmul(q, qo), dq);
+        S.c[0][gi] = (float)x;
+        S.c[1][gi] = (float)y;
+        S.c[2][gi] = (float)z;
+        S.c[3][gi] = (float)qn.w;
+        S.c[4][gi] = (float)qn.x;
+        S.c[5][gi] = (float)qn.y;
+        S.c[6][gi] = (float)qn.z;
+    }
+}
+
+// Block-wide f64 reductions (4 waves): deterministic tree order.
+__device__ inline double block_sum(double v, double* sh) {
+    v = wave_sum(v);
+    const int wv = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) sh[wv] = v;
+    __syncthreads();
+    return (sh[0] + sh[1]) + (sh[2] + sh[3]);
+}
+__device__ inline double block_max(double v, double* sh) {
+    v = wave_max(v);
+    const int wv = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) sh[wv] = v;
+    __syncthreads();
+    return fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
+}
+
+// eb^2 = acos(1 - omd)^2 as a power series in t = 2*omd: eb^2 = 2 sum t^n / (n^2 C(2n,n)).
+__device__ inline double acos_sq_series(double t) {
+    double p;
+    if (t <= 0.01) {
+        p = 1.0 / 33264.0;
+        p = __fma_rn(p, t, 1.0 / 6300.0);
+        p = __fma_rn(p, t, 1.0 / 1120.0);
+        p = __fma_rn(p, t, 1.0 / 180.0);
+        p = __fma_rn(p, t, 1.0 / 24.0);
+        p = __fma_rn(p, t, 0.5);
+    } else {
+        p = 1.0 / 389398464.0;
+        p = __fma_rn(p, t, 1.0 / 85357272.0);
+        p = __fma_rn(p, t, 1.0 / 18475600.0);
+        p = __fma_rn(p, t, 1.0 / 3938220.0);
+        p = __fma_rn(p, t, 1.0 / 823680.0);
+        p = __fma_rn(p, t, 1.0 / 168168.0);
+        p = __fma_rn(p, t, 1.0 / 33264.0);
+        p = __fma_rn(p, t, 1.0 / 6300.0);
+        p = __fma_rn(p, t, 1.0 / 1120.0);
+        p = __fma_rn(p, t, 1.0 / 180.0);
+        p = __fma_rn(p, t, 1.0 / 24.0);
+        p = __fma_rn(p, t, 0.5);
+    }
+    return 2.0 * t * p;
+}
+
+// Gated log-likelihood (cpu.rs:44-97, 220-248) + fused online-softmax partials.
+__global__ __launch_bounds__(kTB) void k_weight(Layout L, PoseBufs B, FilterDev* F, double* __restrict__ ll,
+                                                double* __restrict__ part, ParamRef<WeightParams> A) {
+    __shared__ double sh[4];
+    __shared__ int s_last;
+    const int blk = blockIdx.x;
+    const int f = L.blk_filter[blk];
+    const int off = L.f_off[f];
+    const int nf = L.f_n[f];
+    const int li = blk * kTB + (int)threadIdx.x - off;
+    const bool valid = li < nf;
+    const WeightParams& p = A.get(f);
+    const int gi = off + (valid ? li : 0);
+    const PoseSet S = B.s[F[f].parity];
+
+    double acc = 0.0;
+    if (valid) {
+        const double px = (double)S.c[0][gi], py = (double)S.c[1][gi], pz = (double)S.c[2][gi];
+        const double qw = (double)S.c[3][gi], qx = (double)S.c[4][gi], qy = (double)S.c[5][gi],
+                     qz = (double)S.c[6][gi];
+        // Linear map of qrot(conj(q), .): world -> body (exact for non-unit q as well).
+        const double a = 1.0 - 2.0 * (qx * qx + qy * qy + qz * qz);
+        const double m00 = a + 2.0 * qx * qx, m11 = a + 2.0 * qy * qy, m22 = a + 2.0 * qz * qz;
+        const double m01 = 2.0 * (qx * qy + qw * qz), m10 = 2.0 * (qx * qy - qw * qz);
+        const double m02 = 2.0 * (qx * qz - qw * qy), m20 = 2.0 * (qx * qz + qw * qy);
+        const double m12 = 2.0 * (qy * qz + qw * qx), m21 = 2.0 * (qy * qz - qw * qx);
+        const ObsRec* __restrict__ obs = p.obs;
+        const int nobs = p.nobs;
+        for (int k = 0; k < nobs; ++k) {
+            const ObsRec o = obs[k];
+            const double dx = o.lx - px, dy = o.ly - py, dz = o.lz - pz;
+            const double bx = __fma_rn(m00, dx, __fma_rn(m01, dy, m02 * dz));
+            const double by = __fma_rn(m10, dx, __fma_rn(m11, dy, m12 * dz));
+            const double bz = __fma_rn(m20, dx, __fma_rn(m21, dy, m22 * dz));
+            if (o.mode == 0) {
+                const double ex = bx - o.vx, ey = by - o.vy, ez = bz - o.vz;
+                const double r2 = __fma_rn(ex, ex, __fma_rn(ey, ey, ez * ez));
+                acc += (p.gate > 0.0 && r2 > p.cart_gate2) ? p.cart_gated : -r2 * p.inv2rr;
+            } else {
+                const double r2 = __fma_rn(bx, bx, __fma_rn(by, by, bz * bz));
+                const double pv = __fma_rn(bx, o.vx, __fma_rn(by, o.vy, bz * o.vz));
+                double rn, rinv;
+                if (r2 > 1e-18) {
+                    double y = (double)__frsqrt_rn((float)r2);
+                    y = y * __fma_rn(-0.5 * r2 * y, y, 1.5);
+                    rn = r2 * y;
+                    rinv = y;
+                } else {
+                    rn = 1e-9;
+                    rinv = 1e9;
+                }
+                const double er = rn - o.range;
+                const double er2 = er * er;
+                acc += (p.gate > 0.0 && er2 > p.rng_gate2) ? p.half_g2 : -er2 * p.inv2rr;
+                double omd = __fma_rn(pv, rinv, 1.0);
+                omd = fmin(fmax(omd, 0.0), 2.0);
+                const double t = 2.0 * omd;
+                double bterm;
+                if (p.gate > 0.0 && t > p.t_gate) {
+                    bterm = p.half_g2;
+                } else {
+                    double eb2;
+                    if (t <= 0.25) {
+                        eb2 = acos_sq_series(t);
+                    } else {
+                        const double eb = acos(1.0 - omd);
+                        eb2 = eb * eb;
+                    }
+                    bterm = (p.gate > 0.0 && eb2 > p.brg_gate2) ? p.half_g2 : -eb2 * p.inv2bb;
+                }
+                acc += bterm;
+            }
+        }
+        ll[gi] = acc;
+    }
+    // Online softmax partials of this block: m_b, sum exp(ll-m_b), sum exp(2(ll-m_b)).
+    const double mb = block_max(valid ? acc : -INFINITY, sh);
+    double e = 0.0;
+    if (valid && mb > -INFINITY) e = exp(acc - mb);
+    const double sb = block_sum(e, sh);
+    const double s2b = block_sum(e * e, sh);
+    if (threadIdx.x == 0) {
+        part[3 * blk] = mb;
+        part[3 * blk + 1] = sb;
+        part[3 * blk + 2] = s2b;
+        __threadfence();
+        const unsigned nblk = (unsigned)(L.f_padn[f] / kTB);
+        const unsigned old = atomicAdd(&F[f].cnt_weight, 1u);
+        s_last = (old == nblk - 1);
+    }
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();
+    const int b0 = off / kTB, nb = L.f_padn[f] / kTB;
+    double m = -INFINITY;
+    for (int b = (int)threadIdx.x; b < nb; b += kTB) m = fmax(m, part[3 * (b0 + b)]);
+    const double M = block_max(m, sh);
+    double s = 0.0, s2 = 0.0;
+    for (int b = (int)threadIdx.x; b < nb; b += kTB) {
+        const double mbb = part[3 * (b0 + b)];
+        if (mbb > -INFINITY) {
+            const double sc = exp(mbb - M);
+            s += part[3 * (b0 + b) + 1] * sc;
+            s2 += part[3 * (b0 + b) + 2] * (sc * sc);
+        }
+    }
+    const double Ssum = block_sum(s, sh);
+    const double S2sum = block_sum(s2, sh);
+    if (threadIdx.x == 0) {
+        FilterDev& fd = F[f];
+        const bool ok = (Ssum > 0.0) && isfinite(Ssum) && isfinite(M);
+        fd.max_ll = M;
+        fd.sum_e = Ssum;
+        fd.sum_e2 = S2sum;
+        fd.uniform = ok ? 0 : 1;
+        fd.ess = ok ? (Ssum * Ssum) / S2sum : (double)nf;
+        fd.cnt_weight = 0;
+    }
+}
+
+// w = exp(ll - max) / sum (or 1/n when the weight pass was degenerate / had no observations).
+__global__ __launch_bounds__(kTB) void k_normalize(Layout L, FilterDev* F, const double* __restrict__ ll,
+                                                   float* __restrict__ w, int force_uniform) {
+    const int blk = blockIdx.x;
+    const int f = L.blk_filter[blk];
+    const int off = L.f_off[f];
+    const int li = blk * kTB + (int)threadIdx.x - off;
+    if (li >= L.f_n[f]) return;
+    const FilterDev& fd = F[f];
+    const float winv = L.f_winv[f];
+    if (force_uniform || fd.uniform) {
+        w[off + li] = winv;
+    } else {
+        w[off + li] = (float)(exp(ll[off + li] - fd.max_ll) / fd.sum_e);
+    }
+}
+
+// --------------------------------------------------------------------------- resample
+
+constexpr uint64_t kFlagAgg = 1ull << 62, kFlagIncl = 2ull << 62, kValMask = (1ull << 62) - 1;
+
+// Inclusive scan of fixed-point weights, one 1024-particle tile per block (256 x 4),
+// decoupled look-back within each filter, tiles taken in ticket order.
+__global__ __launch_bounds__(kTB) void k_scan(Layout L, FilterDev* F, const float* __restrict__ w,
+                                              uint64_t* __restrict__ cum, uint64_t* __restrict__ status,
+                                              unsigned* __restrict__ ctr, int ntiles) {
+    __shared__ int s_tile;
+    __shared__ uint64_t s_wsum[4];
+    __shared__ uint64_t s_prefix;
+    if (threadIdx.x == 0) s_tile = (int)atomicAdd(&ctr[0], 1u);
+    __syncthreads();
+    const int tile = s_tile;
+    const int f = L.blk_filter[tile * (kTile / kTB)];
+    const int off = L.f_off[f];
+    const int nf = L.f_n[f];
+    const int ftile0 = off / kTile;
+    const bool active = F[f].do_resample != 0;
+    if (active) {
+        const int base = tile * kTile + (int)threadIdx.x * 4;
+        const int li0 = base - off;
+        const float4 wv = *reinterpret_cast<const float4*>(w + base);
+        uint64_t q[4];
+        q[0] = (li0 + 0 < nf) ? qweight(wv.x) : 0;
+        q[1] = (li0 + 1 < nf) ? qweight(wv.y) : 0;
+        q[2] = (li0 + 2 < nf) ? qweight(wv.z) : 0;
+        q[3] = (li0 + 3 < nf) ? qweight(wv.w) : 0;
+        q[1] += q[0];
+        q[2] += q[1];
+        q[3] += q[2];
+        const uint64_t tsum = q[3];
+        // wave inclusive scan of thread sums
+        const int lane = threadIdx.x & 63, wv_id = threadIdx.x >> 6;
+        uint64_t x = tsum;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint64_t y = __shfl_up(x, d, 64);
+            if (lane >= d) x += y;
+        }
+        if (lane == 63) s_wsum[wv_id] = x;
+        __syncthreads();
+        uint64_t wpre = 0;
+        for (int k = 0; k < wv_id; ++k) wpre += s_wsum[k];
+        const uint64_t texcl = wpre + x - tsum;
+        const uint64_t agg = s_wsum[0] + s_wsum[1] + s_wsum[2] + s_wsum[3];
+        if (threadIdx.x == 0) {
+            uint64_t prefix = 0;
+            if (tile == ftile0) {
+                __hip_atomic_store(&status[tile], kFlagIncl | (agg & kValMask), __ATOMIC_RELEASE,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            } else {
+                __hip_atomic_store(&status[tile], kFlagAgg | (agg & kValMask), __ATOMIC_RELEASE,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+                int j = tile - 1;
+                while (true) {
+                    const uint64_t st = __hip_atomic_load(&status[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+                    const uint64_t fl = st & ~kValMask;
+                    if (fl == 0) {
+                        __builtin_amdgcn_s_sleep(1);
+                        continue;
+                    }
+                    prefix += st & kValMask;
+                    if (fl == kFlagIncl) break;
+                    --j;
+                }
+                __hip_atomic_store(&status[tile], kFlagIncl | ((prefix + agg) & kValMask), __ATOMIC_RELEASE,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            }
+            s_prefix = prefix;
+        }
+        __syncthreads();
+        const uint64_t pre = s_prefix + texcl;
+        uint64_t* cp = cum + base;
+        cp[0] = pre + q[0];
+        cp[1] = pre + q[1];
+        cp[2] = pre + q[2];
+        cp[3] = pre + q[3];
+        // the tile holding the filter's last particle publishes the total and advances the LCG
+        const int last = nf - 1;
+        if (li0 <= last && last < li0 + 4) {
+            const uint64_t T = pre + q[last - li0];
+            FilterDev& fd = F[f];
+            fd.total = T;
+            if (T > 0) {
+                const uint64_t lcg = fd.lcg * 6364136223846793005ull + 1442695040888963407ull;  // cu:653
+                fd.lcg = lcg;
+                fd.u32 = (uint32_t)(lcg >> 32);
+            }
+        }
+    }
+    // self-cleaning: the last tile to finish resets statuses and tickets for the next launch
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const unsigned done = atomicAdd(&ctr[1], 1u);
+        s_tile = (done == (unsigned)ntiles - 1) ? -1 : 0;
+    }
+    __syncthreads();
+    if (s_tile == -1) {
+        __threadfence();
+        for (int t = (int)threadIdx.x; t < ntiles; t += kTB) status[t] = 0;
+        if (threadIdx.x == 0) {
+            ctr[0] = 0;
+            ctr[1] = 0;
+        }
+    }
+}
+
+// Systematic pick: ancestor(i) = first j with cum_j * n 2^32 >= T (U32 + i 2^32); gather poses
+// from the current set into the other set, weights -> 1/n, last block flips the parity.
+__global__ __launch_bounds__(kTB) void k_gather(Layout L, FilterDev* F, PoseBufs B, const uint64_t* __restrict__ cum,
+                                                float* __restrict__ w, int32_t* __restrict__ anc) {
+    __shared__ int s_last;
+    const int blk = blockIdx.x;
+    const int f = L.blk_filter[blk];
+    FilterDev& fd = F[f];
+    if (!fd.do_resample) return;
+    const int off = L.f_off[f];
+    const int nf = L.f_n[f];
+    const int li = blk * kTB + (int)threadIdx.x - off;
+    const uint64_t T = fd.total;
+    const int par = fd.parity;
+    if (li < nf) {
+        if (T == 0) {
+            w[off + li] = L.f_winv[f];  // cu:647-650: degenerate total -> uniform weights only
+        } else {
+            const uint64_t A = (uint64_t)nf << 32;
+            const uint64_t Bi = (uint64_t)fd.u32 + ((uint64_t)li << 32);
+            const uint64_t* c = cum + off;
+            int lo = 0, hi = nf - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (mul128_lt(c[mid], A, T, Bi))
+                    lo = mid + 1;
+                else
+                    hi = mid;
+            }
+            const PoseSet S = B.s[par], D = B.s[par ^ 1];
+#pragma unroll
+            for (int k = 0; k < 7; ++k) D.c[k][off + li] = S.c[k][off + lo];
+            anc[off + li] = lo;
+            w[off + li] = L.f_winv[f];
+        }
+    }
+    if (T == 0) return;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const unsigned nblk = (unsigned)(L.f_padn[f] / kTB);
+        const unsigned old = atomicAdd(&fd.cnt_gather, 1u);
+        s_last = (old == nblk - 1);
+    }
+    __syncthreads();
+    if (s_last && threadIdx.x == 0) {
+        fd.parity = par ^ 1;
+        fd.cnt_gather = 0;
+        __threadfence();
+    }
+}
+
+// Injection picks (cu:333-352 formula): target index and f64 jitter of pick j.
+__global__ __launch_bounds__(kTB) void k_inj_pick(Layout L, FilterDev* F, ParamRef<InjectParams> A,
+                                                  int32_t* __restrict__ idx, double* __restrict__ jit,
+                                                  int32_t* __restrict__ head) {
+    const int blk = blockIdx.x;
+    const int f = L.blk_filter[blk];
+    const FilterDev& fd = F[f];
+    const InjectParams& p = A.get(f);
+    const int off = L.f_off[f];
+    const int j = blk * kTB + (int)threadIdx.x - off;
+    if (!fd.do_resample || fd.total == 0 || j >= p.k) return;
+    const int nf = L.f_n[f];
+    uint64_t z = fd.lcg ^ (p.gen * kPhi) ^ (((uint64_t)j + 1) * kC1);
+    if (z == 0) z = 0x243F6A8885A308D3ull;
+    int t = (int)(u01_co(sm_next(z)) * (double)nf);
+    if (t >= nf) t = nf - 1;
+    const double jx = u01_co(sm_next(z)) - 0.5;
+    const double jy = u01_co(sm_next(z)) - 0.5;
+    const double jz = u01_co(sm_next(z)) - 0.5;
+    idx[off + j] = t;
+    jit[3 * (off + j)] = jx;
+    jit[3 * (off + j) + 1] = jy;
+    jit[3 * (off + j) + 2] = jz;
+    head[off + t] = -1;
+}
+
+__global__ __launch_bounds__(kTB) void k_inj_link(Layout L, FilterDev* F, ParamRef<InjectParams> A,
+                                                  const int32_t* __restrict__ idx, int32_t* __restrict__ head,
+                                                  int32_t* __restrict__ next) {
+    const int blk = blockIdx.x;
+    const int f = L.blk_filter[blk];
+    const FilterDev& fd = F[f];
+    const InjectParams& p = A.get(f);
+    const int off = L.f_off[f];
+    const int j = blk * kTB + (int)threadIdx.x - off;
+    if (!fd.do_resample || fd.total == 0 || j >= p.k) return;
+    const int t = idx[off + j];
+    next[off + j] = atomicExch(&head[off + t], j);
+}
+
+// The pick that ended up as list head applies every pick of its target in ascending pick order.
+__global__ __launch_bounds__(kTB) void k_inj_apply(Layout L, FilterDev* F, PoseBufs B, ParamRef<InjectParams> A,
+                                                   const int32_t* __restrict__ idx, const double* __restrict__ jit,
+                                                   const int32_t* __restrict__ head,
+                                                   const int32_t* __restrict__ next) {
+    const int blk = blockIdx.x;
+    const int f = L.blk_filter[blk];
+    const FilterDev& fd = F[f];
+    const InjectParams& p = A.get(f);
+    const int off = L.f_off[f];
+    const int j = blk * kTB + (int)threadIdx.x - off;
+    if (!fd.do_resample || fd.total == 0 || j >= p.k) return;
+    const int t = idx[off + j];
+    if (head[off + t] != j) return;
+    const PoseSet S = B.s[fd.parity];
+    double ax = (double)S.c[0][off + t], ay = (double)S.c[1][off + t], az = (double)S.c[2][off + t];
+    // ascending traversal of the (unordered) list: repeatedly take the smallest pick > prev
+    int prev = -1;
+    while (true) {
+        int best = 0x7fffffff;
+        for (int q = j; q >= 0; q = next[off + q])
+            if (q > prev && q < best) best = q;
+        if (best == 0x7fffffff) break;
+        ax += jit[3 * (off + best)];
+        ay += jit[3 * (off + best) + 1];
+        az += jit[3 * (off + best) + 2];
+        prev = best;
+    }
+    S.c[0][off + t] = (float)ax;
+    S.c[1][off + t] = (float)ay;
+    S.c[2][off + t] = (float)az;
+}
+
+// ---------------------------------------------------------------------------- estimate
+
+// Writes the block partial of NV values and, in the filter's last block, their
+// deterministic sum to out (fixed thread->block assignment + fixed tree order).
+template <int NV>
+__device__ inline void reduce_partials(double (&acc)[NV], double* __restrict__ part, int blk, int b0, int nb,
+                                       unsigned* cnt, double* out, double* sh) {
+    __shared__ int s_last;
+    double tot[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) tot[k] = block_sum(acc[k], sh);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) part[(size_t)NV * blk + k] = tot[k];
+        __threadfence();
+        const unsigned old = atomicAdd(cnt, 1u);
+        s_last = (old == (unsigned)nb - 1);
+    }
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();
+    double v[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) v[k] = 0.0;
+    for (int b = (int)threadIdx.x; b < nb; b += kTB)
+#pragma unroll
+        for (int k = 0; k < NV; ++k) v[k] += part[(size_t)NV * (b0 + b) + k];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) tot[k] = block_sum(v[k], sh);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) out[k] = tot[k];
+        *cnt = 0;
+    }
+}
+
+// est layout (cu:354-404): [0..2] sum w p, [3..12] sum w q_r q_c (r<=c), [13] sum w^2.
+__global__ __launch_bounds__(kTB) void k_est_sums(Layout L, FilterDev* F, PoseBufs B, const float* __restrict__ w,
+                                                  double* __restrict__ part) {
+    __shared__ double sh[4];
+    const int blk = blockIdx.x;  // 1024-particle tile
+    const int f = L.blk_filter[blk * (kTile / kTB)];
+    const int off = L.f_off[f];
+    const int nf = L.f_n[f];
+    const PoseSet S = B.s[F[f].parity];
+    double acc[kEstN];
+#pragma unroll
+    for (int k = 0; k < kEstN; ++k) acc[k] = 0.0;
+    const int base = blk * kTile + (int)threadIdx.x * 4;
+    const int li0 = base - off;
+    if (li0 < nf) {
+        float4 v[8];
+        v[7] = *reinterpret_cast<const float4*>(w + base);
+#pragma unroll
+        for (int c = 0; c < 7; ++c) v[c] = *reinterpret_cast<const float4*>(S.c[c] + base);
+        const float* vv = reinterpret_cast<const float*>(v);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            if (li0 + e < nf) {
+                const double wi = (double)vv[28 + e];
+                const double px = vv[0 + e], py = vv[4 + e], pz = vv[8 + e];
+                const double q[4] = {(double)vv[12 + e], (double)vv[16 + e], (double)vv[20 + e], (double)vv[24 + e]};
+                acc[0] = __fma_rn(wi, px, acc[0]);
+                acc[1] = __fma_rn(wi, py, acc[1]);
+                acc[2] = __fma_rn(wi, pz, acc[2]);
+                int k = 3;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const double wq = wi * q[r];
+#pragma unroll
+                    for (int c = r; c < 4; ++c) {
+                        acc[k] = __fma_rn(wq, q[c], acc[k]);
+                        ++k;
+                    }
+                }
+                acc[13] = __fma_rn(wi, wi, acc[13]);
+            }
+        }
+    }
+    reduce_partials<kEstN>(acc, part, blk, off / kTile, L.f_padn[f] / kTile, &F[f].cnt_sums, F[f].sums, sh);
+}
+
+// cov layout (cu:406-471): [0..5] position, [6..11] attitude tangent (axis-angle of qm^-1 q,
+// canonical sign, angle = 2 atan2(|v|, w)).
+__global__ __launch_bounds__(kTB) void k_est_covs(Layout L, FilterDev* F, PoseBufs B, const float* __restrict__ w,
+                                                  double* __restrict__ part, ParamRef<CovParams> A) {
+    __shared__ double sh[4];
+    const int blk = blockIdx.x;
+    const int f = L.blk_filter[blk * (kTile / kTB)];
+    const int off = L.f_off[f];
+    const int nf = L.f_n[f];
+    const CovParams& p = A.get(f);
+    const PoseSet S = B.s[F[f].parity];
+    double acc[kCovN];
+#pragma unroll
+    for (int k = 0; k < kCovN; ++k) acc[k] = 0.0;
+    const int base = blk * kTile + (int)threadIdx.x * 4;
+    const int li0 = base - off;
+    if (li0 < nf) {
+        float4 v[8];
+        v[7] = *reinterpret_cast<const float4*>(w + base);
+#pragma unroll
+        for (int c = 0; c < 7; ++c) v[c] = *reinterpret_cast<const float4*>(S.c[c] + base);
+        const float* vv = reinterpret_cast<const float*>(v);
+        const dquat mc{p.mq[0], -p.mq[1], -p.mq[2], -p.mq[3]};
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            if (li0 + e < nf) {
+                const double wi = (double)vv[28 + e];
+                const double dp[3] = {(double)vv[0 + e] - p.mp[0], (double)vv[4 + e] - p.mp[1],
+                                      (double)vv[8 + e] - p.mp[2]};
+                int k = 0;
+#pragma unroll
+                for (int r = 0; r < 3; ++r) {
+                    const double wd = wi * dp[r];
+#pragma unroll
+                    for (int c = r; c < 3; ++c) {
+                        acc[k] = __fma_rn(wd, dp[c], acc[k]);
+                        ++k;
+                    }
+                }
+                const dquat qi{(double)vv[12 + e], (double)vv[16 + e], (double)vv[20 + e], (double)vv[24 + e]};
+                dquat d = qmul(mc, qi);
+                if (d.w < 0.0) {
+                    d.w = -d.w;
+                    d.x = -d.x;
+                    d.y = -d.y;
+                    d.z = -d.z;
+                }
+                const double hv2 = d.x * d.x + d.y * d.y + d.z * d.z;
+                double aav[3] = {0.0, 0.0, 0.0};
+                if (hv2 > 1e-24) {
+                    const double hv = sqrt(hv2);
+                    const double ratio = 2.0 * atan2(hv, d.w) / hv;
+                    aav[0] = d.x * ratio;
+                    aav[1] = d.y * ratio;
+                    aav[2] = d.z * ratio;
+                }
+#pragma unroll
+                for (int r = 0; r < 3; ++r) {
+                    const double wa = wi * aav[r];
+#pragma unroll
+                    for (int c = r; c < 3; ++c) {
+                        acc[k] = __fma_rn(wa, aav[c], acc[k]);
+                        ++k;
+                    }
+                }
+            }
+        }
+    }
+    reduce_partials<kCovN>(acc, part, blk, off / kTile, L.f_padn[f] / kTile, &F[f].cnt_covs, F[f].covs, sh);
+}
+
+// Sets the per-filter resample predicate: force (ABI call) or ESS < ratio * n (fused cycle).
+__global__ void k_resample_predicate(FilterDev* F, int nfilt, const float* ratio_ess, const int* fn, int force) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nfilt) return;
+    FilterDev& fd = F[f];
+    if (force) {
+        fd.do_resample = 1;
+    } else {
+        const double r = (double)ratio_ess[f];
+        fd.do_resample = (r > 0.0 && fd.ess < r * (double)fn[f]) ? 1 : 0;
+    }
+}
+
+}  // namespace pfk
+
+// ============================================================================ host side
+
+namespace pfk {
+
+int state_alloc(PfState* s, const std::vector<int>& ns) {
+    state_free_particles(s);
+    const int nf = (int)ns.size();
+    s->nf = nf;
+    s->n.assign(ns.begin(), ns.end());
+    s->off.resize(nf);
+    s->padn.resize(nf);
+    int NP = 0;
+    for (int f = 0; f < nf; ++f) {
+        s->off[f] = NP;
+        s->padn[f] = (ns[f] + kTile - 1) / kTile * kTile;
+        NP += s->padn[f];
+    }
+    s->NP = NP;
+    const size_t fb = sizeof(float) * (size_t)NP;
+    for (int b = 0; b < 2; ++b)
+        for (int c = 0; c < 7; ++c) {
+            PF_CHECK(hipMalloc((void**)&s->pose.s[b].c[c], fb));
+            PF_CHECK(hipMemsetAsync(s->pose.s[b].c[c], 0, fb, s->stream));
+        }
+    PF_CHECK(hipMalloc((void**)&s->w, fb));
+    PF_CHECK(hipMemsetAsync(s->w, 0, fb, s->stream));
+    PF_CHECK(hipMalloc((void**)&s->ll, sizeof(double) * (size_t)NP));
+    PF_CHECK(hipMemsetAsync(s->ll, 0, sizeof(double) * (size_t)NP, s->stream));
+    PF_CHECK(hipMalloc((void**)&s->cum, sizeof(uint64_t) * (size_t)NP));
+    PF_CHECK(hipMalloc((void**)&s->anc, sizeof(int32_t) * (size_t)NP));
+    PF_CHECK(hipMemsetAsync(s->anc, 0, sizeof(int32_t) * (size_t)NP, s->stream));
+    PF_CHECK(hipMalloc((void**)&s->inj_idx, sizeof(int32_t) * (size_t)NP));
+    PF_CHECK(hipMalloc((void**)&s->inj_next, sizeof(int32_t) * (size_t)NP));
+    PF_CHECK(hipMalloc((void**)&s->inj_head, sizeof(int32_t) * (size_t)NP));
+    PF_CHECK(hipMalloc((void**)&s->inj_jit, sizeof(double) * 3 * (size_t)NP));
+    const int nb256 = NP / kTB, ntile = NP / kTile;
+    PF_CHECK(hipMalloc((void**)&s->part_w, sizeof(double) * 3 * (size_t)nb256));
+    PF_CHECK(hipMalloc((void**)&s->part_est, sizeof(double) * kEstN * (size_t)ntile));
+    PF_CHECK(hipMalloc((void**)&s->part_cov, sizeof(double) * kCovN * (size_t)ntile));
+    PF_CHECK(hipMalloc((void**)&s->scan_status, sizeof(uint64_t) * (size_t)ntile));
+    PF_CHECK(hipMemsetAsync(s->scan_status, 0, sizeof(uint64_t) * (size_t)ntile, s->stream));
+    PF_CHECK(hipMalloc((void**)&s->scan_ctr, sizeof(unsigned) * 4));
+    PF_CHECK(hipMemsetAsync(s->scan_ctr, 0, sizeof(unsigned) * 4, s->stream));
+    // layout tables
+    std::vector<int> blk_filter(nb256);
+    for (int f = 0; f < nf; ++f)
+        for (int b = s->off[f] / kTB; b < (s->off[f] + s->padn[f]) / kTB; ++b) blk_filter[b] = f;
+    std::vector<float> winv(nf);
+    for (int f = 0; f < nf; ++f) winv[f] = (float)(1.0 / (double)ns[f]);
+    PF_CHECK(hipMalloc((void**)&s->d_blk_filter, sizeof(int) * (size_t)nb256));
+    PF_CHECK(hipMalloc((void**)&s->d_fn, sizeof(int) * (size_t)nf));
+    PF_CHECK(hipMalloc((void**)&s->d_foff, sizeof(int) * (size_t)nf));
+    PF_CHECK(hipMalloc((void**)&s->d_fpadn, sizeof(int) * (size_t)nf));
+    PF_CHECK(hipMalloc((void**)&s->d_fwinv, sizeof(float) * (size_t)nf));
+    PF_CHECK(hipMalloc((void**)&s->fdev, sizeof(FilterDev) * (size_t)nf));
+    PF_CHECK(hipMemcpyAsync(s->d_blk_filter, blk_filter.data(), sizeof(int) * (size_t)nb256, hipMemcpyHostToDevice,
+                            s->stream));
+    PF_CHECK(hipMemcpyAsync(s->d_fn, s->n.data(), sizeof(int) * (size_t)nf, hipMemcpyHostToDevice, s->stream));
+    PF_CHECK(hipMemcpyAsync(s->d_foff, s->off.data(), sizeof(int) * (size_t)nf, hipMemcpyHostToDevice, s->stream));
+    PF_CHECK(hipMemcpyAsync(s->d_fpadn, s->padn.data(), sizeof(int) * (size_t)nf, hipMemcpyHostToDevice, s->stream));
+    PF_CHECK(hipMemcpyAsync(s->d_fwinv, winv.data(), sizeof(float) * (size_t)nf, hipMemcpyHostToDevice, s->stream));
+    std::vector<FilterDev> fd(nf);
+    for (int f = 0; f < nf; ++f) {
+        memset(&fd[f], 0, sizeof(FilterDev));
+        fd[f].lcg = 0x243F6A8885A308D3ull;  // cu:116
+        fd[f].ess = (double)ns[f];
+    }
+    PF_CHECK(hipMemcpyAsync(s->fdev, fd.data(), sizeof(FilterDev) * (size_t)nf, hipMemcpyHostToDevice, s->stream));
+    PF_CHECK(hipStreamSynchronize(s->stream));
+    s->L.blk_filter = s->d_blk_filter;
+    s->L.f_n = s->d_fn;
+    s->L.f_off = s->d_foff;
+    s->L.f_padn = s->d_fpadn;
+    s->L.f_winv = s->d_fwinv;
+    return 0;
+}
+
+void state_free_particles(PfState* s) {
+    for (int b = 0; b < 2; ++b)
+        for (int c = 0; c < 7; ++c) {
+            (void)hipFree(s->pose.s[b].c[c]);
+            s->pose.s[b].c[c] = nullptr;
+        }
+    void** ptrs[] = {(void**)&s->w,        (void**)&s->ll,          (void**)&s->cum,       (void**)&s->anc,
+                     (void**)&s->inj_idx,  (void**)&s->inj_next,    (void**)&s->inj_head,  (void**)&s->inj_jit,
+                     (void**)&s->part_w,   (void**)&s->part_est,    (void**)&s->part_cov,  (void**)&s->scan_status,
+                     (void**)&s->scan_ctr, (void**)&s->d_blk_filter, (void**)&s->d_fn,     (void**)&s->d_foff,
+                     (void**)&s->d_fpadn,  (void**)&s->d_fwinv,     (void**)&s->fdev};
+    for (void** p : ptrs) {
+        (void)hipFree(*p);
+        *p = nullptr;
+    }
+    s->nf = 0;
+    s->NP = 0;
+}
+
+// ---- profiling helpers
+void prof_begin(PfState* s, int stage) {
+    if (!s->prof_on) return;
+    hipEvent_t a, b;
+    (void)hipEventCreate(&a);
+    (void)hipEventCreate(&b);
+    (void)hipEventRecord(a, s->stream);
+    s->prof_pending.push_back({stage, a, b});
+}
+void prof_end(PfState* s) {
+    if (!s->prof_on || s->prof_pending.empty()) return;
+    (void)hipEventRecord(s->prof_pending.back().b, s->stream);
+}
+
+int launch_motion(PfState* s, ParamRef<MotionParams> A, int reinit) {
+    prof_begin(s, reinit ? PFX_STAGE_REINIT : PFX_STAGE_PREDICT);
+    hipLaunchKernelGGL(k_motion, dim3(s->NP / kTB), dim3(kTB), 0, s->stream, s->L, s->pose, s->fdev, s->w, A, reinit);
+    prof_end(s);
+    PF_LAUNCH_CHECK();
+    return 0;
+}
+
+int launch_weight(PfState* s, ParamRef<WeightParams> A) {
+    prof_begin(s, PFX_STAGE_WEIGHT);
+    hipLaunchKernelGGL(k_weight, dim3(s->NP / kTB), dim3(kTB), 0, s->stream, s->L, s->pose, s->fdev, s->ll, s->part_w,
+                       A);
+    prof_end(s);
+    PF_LAUNCH_CHECK();
+    return 0;
+}
+
+int launch_normalize(PfState* s, int force_uniform) {
+    prof_begin(s, PFX_STAGE_NORMALIZE);
+    hipLaunchKernelGGL(k_normalize, dim3(s->NP / kTB), dim3(kTB), 0, s->stream, s->L, s->fdev, s->ll, s->w,
+                       force_uniform);
+    prof_end(s);
+    PF_LAUNCH_CHECK();
+    return 0;
+}
+
+int launch_resample(PfState* s, ParamRef<InjectParams> inj, int any_inject) {
+    const int ntile = s->NP / kTile;
+    prof_begin(s, PFX_STAGE_RESAMPLE_SCAN);
+    hipLaunchKernelGGL(k_scan, dim3(ntile), dim3(kTB), 0, s->stream, s->L, s->fdev, s->w, s->cum, s->scan_status,
+                       s->scan_ctr, ntile);
+    prof_end(s);
+    PF_LAUNCH_CHECK();
+    prof_begin(s, PFX_STAGE_RESAMPLE_GATHER);
+    hipLaunchKernelGGL(k_gather, dim3(s->NP / kTB), dim3(kTB), 0, s->stream, s->L, s->fdev, s->pose, s->cum, s->w,
+                       s->anc);
+    prof_end(s);
+    PF_LAUNCH_CHECK();
+    if (any_inject) {
+        prof_begin(s, PFX_STAGE_INJECT);
+        const dim3 g(s->NP / kTB), b(kTB);
+        hipLaunchKernelGGL(k_inj_pick, g, b, 0, s->stream, s->L, s->fdev, inj, s->inj_idx, s->inj_jit, s->inj_head);
+        hipLaunchKernelGGL(k_inj_link, g, b, 0, s->stream, s->L, s->fdev, inj, s->inj_idx, s->inj_head, s->inj_next);
+        hipLaunchKernelGGL(k_inj_apply, g, b, 0, s->stream, s->L, s->fdev, s->pose, inj, s->inj_idx, s->inj_jit,
+                           s->inj_head, s->inj_next);
+        prof_end(s);
+        PF_LAUNCH_CHECK();
+    }
+    return 0;
+}
+
+int launch_est_sums(PfState* s) {
+    prof_begin(s, PFX_STAGE_EST_SUMS);
+    hipLaunchKernelGGL(k_est_sums, dim3(s->NP / kTile), dim3(kTB), 0, s->stream, s->L, s->fdev, s->pose, s->w,
+                       s->part_est);
+    prof_end(s);
+    PF_LAUNCH_CHECK();
+    return 0;
+}
+
+int launch_est_covs(PfState* s, ParamRef<CovParams> A) {
+    prof_begin(s, PFX_STAGE_EST_COVS);
+    hipLaunchKernelGGL(k_est_covs, dim3(s->NP / kTile), dim3(kTB), 0, s->stream, s->L, s->fdev, s->pose, s->w,
+                       s->part_cov, A);
+    prof_end(s);
+    PF_LAUNCH_CHECK();
+    return 0;
+}
+
+int launch_predicate(PfState* s, const float* d_ratio, int force) {
+    hipLaunchKernelGGL(k_resample_predicate, dim3((s->nf + 63) / 64), dim3(64), 0, s->stream, s->fdev, s->nf, d_ratio,
+                       s->d_fn, force);
+    PF_LAUNCH_CHECK();
+    return 0;
+}
+
+WeightParams make_weight_params(double sigma_r, double sigma_b, double gate) {
+    WeightParams p{};
+    const double sr = sigma_r > 1e-6 ? sigma_r : 1e-6;  // cpu.rs:56-57
+    const double sb = sigma_b > 1e-6 ? sigma_b : 1e-6;
+    const double g2 = gate * gate;
+    p.inv2rr = 1.0 / (2.0 * sr * sr);
+    p.inv2bb = 1.0 / (2.0 * sb * sb);
+    p.gate = gate;
+    p.cart_gate2 = g2 * 3.0 * sr * sr;
+    p.rng_gate2 = g2 * sr * sr;
+    p.brg_gate2 = g2 * sb * sb;
+    p.cart_gated = -0.5 * gate * gate * 3.0;
+    p.half_g2 = -0.5 * gate * gate;
+    // t = 2(1 - cos eb) beyond which the bearing term is certainly gated
+    const double gb = gate * sb;
+    p.t_gate = (gb >= M_PI) ? 1e300 : 2.0 * (1.0 - cos(gb)) * (1.0 + 1e-6) + 1e-300;
+    return p;
+}
+
+// Joins observations with the landmark table (skips ids outside [0, lm_cap), cu:242).
+int join_obs(const pfc_obs* obs, int nobs, const float* lm, int lm_cap, ObsRec* out) {
+    int m = 0;
+    for (int k = 0; k < nobs; ++k) {
+        const pfc_obs& o = obs[k];
+        if (o.id < 0 || o.id >= lm_cap) continue;
+        ObsRec& r = out[m++];
+        r.lx = lm[3 * o.id];
+        r.ly = lm[3 * o.id + 1];
+        r.lz = lm[3 * o.id + 2];
+        r.vx = o.dx;
+        r.vy = o.dy;
+        r.vz = o.dz;
+        r.range = o.range;
+        r.mode = o.mode == 0 ? 0 : 1;
+        r.pad = 0;
+    }
+    return m;
+}
+
+}  // namespace pfk
+
+using namespace pfk;
+
+// ============================================================================ C ABI
+
+extern "C" {
+
+int pfc_device_available(void) {
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return 0;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, 0) != hipSuccess) return 0;
+    return prop.major >= 9 ? 1 : 0;
+}
+
+void* pfc_create(int cap_obs) {
+    PfState* s = new (std::nothrow) PfState();
+    if (!s) return nullptr;
+    if (cap_obs < 1 || cap_obs > 4096) cap_obs = 64;  // cu:488
+    s->obs_cap = cap_obs;
+    if (hipSetDevice(0) != hipSuccess || hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipMalloc((void**)&s->d_obs, sizeof(ObsRec) * (size_t)cap_obs) != hipSuccess ||
+        hipHostMalloc((void**)&s->h_obs, sizeof(ObsRec) * (size_t)cap_obs, hipHostMallocDefault) != hipSuccess ||
+        hipEventCreateWithFlags(&s->obs_ev, hipEventDisableTiming) != hipSuccess) {
+        fprintf(stderr, "pf_hip pfc_create: device setup failed\n");
+        pfc_destroy(s);
+        return nullptr;
+    }
+    return s;
+}
+
+int pfc_destroy(void* h) {
+    PfState* s = static_cast<PfState*>(h);
+    if (!s) return 0;
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    state_free_particles(s);
+    (void)hipFree(s->d_obs);
+    (void)hipHostFree(s->h_obs);
+    for (auto& p : s->prof_pending) {
+        (void)hipEventDestroy(p.a);
+        (void)hipEventDestroy(p.b);
+    }
+    if (s->obs_ev) (void)hipEventDestroy(s->obs_ev);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
+    return 0;
+}
+
+int pfc_init(void* h, int n, unsigned int seed) {
+    PfState* s = static_cast<PfState*>(h);
+    if (!s || n <= 0) return -1;
+    if (state_alloc(s, std::vector<int>{n}) != 0) return -1;
+    s->gseed = seed ? (uint64_t)seed : 42ull;  // cu:523
+    s->gen = 0;
+    return 0;
+}
+
+int pfc_upload_landmarks(void* h, const float* data, int count) {
+    PfState* s = static_cast<PfState*>(h);
+    if (!s || count < 0 || (count > 0 && !data)) return -1;
+    s->lm.assign(data, data + 3 * (size_t)count);  // host mirror; joined with obs per weight call
+    s->lm_cap = count;
+    return 0;
+}
+
+int pfc_reinit(void* h, float x, float y, float z, float qw, float qx, float qy, float qz, const float* pos_std,
+               const float* rot_std) {
+    PfState* s = static_cast<PfState*>(h);
+    if (!s || s->nf != 1 || !pos_std || !rot_std) return -1;
+    s->gseed += kGenBump;  // cpu.rs:181
+    s->gen += 1;           // cu:572
+    MotionParams p{};
+    p.t[0] = x;
+    p.t[1] = y;
+    p.t[2] = z;
+    p.q[0] = qw;
+    p.q[1] = qx;
+    p.q[2] = qy;
+    p.q[3] = qz;
+    for (int k = 0; k < 3; ++k) {
+        p.s_t[k] = pos_std[k];
+        p.s_r[k] = rot_std[k];
+    }
+    p.gseed = s->gseed;
+    p.winv = (float)(1.0 / (double)s->n[0]);
+    return launch_motion(s, ParamRef<MotionParams>::value(p), 1);
+}
+
+int pfc_predict(void* h, float tx, float ty, float tz, float qw, float qx, float qy, float qz, const float* tstd,
+                const float* rstd) {
+    PfState* s = static_cast<PfState*>(h);
+    if (!s || s->nf != 1 || !tstd || !rstd) return -1;
+    s->gseed += kGenBump;  // cpu.rs:198
+    s->gen += 1;           // cu:588
+    MotionParams p{};
+    p.t[0] = tx;
+    p.t[1] = ty;
+    p.t[2] = tz;
+    p.q[0] = qw;
+    p.q[1] = qx;
+    p.q[2] = qy;
+    p.q[3] = qz;
+    for (int k = 0; k < 3; ++k) {
+        p.s_t[k] = tstd[k];
+        p.s_r[k] = rstd[k];
+    }
+    p.gseed = s->gseed;
+    return launch_motion(s, ParamRef<MotionParams>::value(p), 0);
+}
+
+int pfc_weight(void* h, const void* obs, int nobs, float sigma_r, float sigma_b, float gate) {
+    PfState* s = static_cast<PfState*>(h);
+    if (!s || s->nf != 1) return -1;
+    nobs = nobs > s->obs_cap ? s->obs_cap : nobs;  // cu:604
+    if (nobs <= 0 || !obs) return launch_normalize(s, 1);
+    // the staging buffer may still feed the previous call's copy
+    PF_CHECK(hipEventSynchronize(s->obs_ev));
+    const int m = join_obs(static_cast<const pfc_obs*>(obs), nobs, s->lm.data(), s->lm_cap, s->h_obs);
+    if (m == 0) return launch_normalize(s, 1);  // every id unknown: ll = 0 for all -> uniform
+    PF_CHECK(hipMemcpyAsync(s->d_obs, s->h_obs, sizeof(ObsRec) * (size_t)m, hipMemcpyHostToDevice, s->stream));
+    PF_CHECK(hipEventRecord(s->obs_ev, s->stream));
+    WeightParams p = make_weight_params(sigma_r, sigma_b, gate);
+    p.obs = s->d_obs;
+    p.nobs = m;
+    if (launch_weight(s, ParamRef<WeightParams>::value(p)) != 0) return -1;
+    return launch_normalize(s, 0);
+}
+
+int pfc_resample(void* h, float random_inject_ratio) {
+    PfState* s = static_cast<PfState*>(h);
+    if (!s || s->nf != 1) return -1;
+    if (launch_predicate(s, nullptr, 1) != 0) return -1;
+    InjectParams ip{};
+    ip.gen = s->gen;
+    int k = 0;
+    if (random_inject_ratio > 0.f) {
+        k = (int)(random_inject_ratio * (float)s->n[0]);  // cu:667-671
+        if (k > s->n[0]) k = s->n[0];
+    }
+    ip.k = k;
+    return launch_resample(s, ParamRef<InjectParams>::value(ip), k > 0);
+}
+
+int pfc_est_sums(void* h, double* out) {
+    PfState* s = static_cast<PfState*>(h);
+    if (!s || s->nf != 1 || !out) return -1;
+    if (launch_est_sums(s) != 0) return -1;
+    PF_CHECK(hipMemcpyAsync(out, fdev_field<double>(s, 0, offsetof(FilterDev, sums)), sizeof(double) * kEstN,
+                            hipMemcpyDeviceToHost, s->stream));
+    PF_CHECK(hipStreamSynchronize(s->stream));
+    return 0;
+}
+
+int pfc_covs(void* h, float mx, float my, float mz, float qw, float qx, float qy, float qz, double* out) {
+    PfState* s = static_cast<PfState*>(h);
+    if (!s || s->nf != 1 || !out) return -1;
+    CovParams p{};
+    p.mp[0] = mx;
+    p.mp[1] = my;
+    p.mp[2] = mz;
+    p.mq[0] = qw;
+    p.mq[1] = qx;
+    p.mq[2] = qy;
+    p.mq[3] = qz;
+    if (launch_est_covs(s, ParamRef<CovParams>::value(p)) != 0) return -1;
+    PF_CHECK(hipMemcpyAsync(out, fdev_field<double>(s, 0, offsetof(FilterDev, covs)), sizeof(double) * kCovN,
+                            hipMemcpyDeviceToHost, s->stream));
+    PF_CHECK(hipStreamSynchronize(s->stream));
+    return 0;
+}
+
+static int read_parity(PfState* s, int f, int* par) {
+    PF_CHECK(hipMemcpyAsync(par, fdev_field<int>(s, f, offsetof(FilterDev, parity)), sizeof(int), hipMemcpyDeviceToHost,
+                            s->stream));
+    PF_CHECK(hipStreamSynchronize(s->stream));
+    return 0;
+}
+
+int pfx_get_state(void* h, float* pos, float* quat, float* w) {
+    PfState* s = static_cast<PfState*>(h);
+    if (!s || s->nf != 1) return -1;
+    int par = 0;
+    if (read_parity(s, 0, &par) != 0) return -1;
+    const int n = s->n[0];
+    std::vector<float> tmp((size_t)n);
+    for (int c = 0; c < 7; ++c) {
+        if ((c < 3 && !pos) || (c >= 3 && !quat)) continue;
+        PF_CHECK(hipMemcpy(tmp.data(), s->pose.s[par].c[c], sizeof(float) * (size_t)n, hipMemcpyDeviceToHost));
+        if (c < 3)
+            for (int i = 0; i < n; ++i) pos[3 * i + c] = tmp[i];
+        else
+            for (int i = 0; i < n; ++i) quat[4 * i + (c - 3)] = tmp[i];
+    }
+    if (w) PF_CHECK(hipMemcpy(w, s->w, sizeof(float) * (size_t)n, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int pfc_snapshot(void* h, float* pos, float* quat) {
+    if (!pos || !quat) return -1;
+    return pfx_get_state(h, pos, quat, nullptr);
+}
+
+int pfx_set_state(void* h, const float* pos, const float* quat, const float* w) {
+    PfState* s = static_cast<PfState*>(h);
+    if (!s || s->nf != 1 || !pos || !quat) return -1;
+    const int n = s->n[0];
+    if (w) {
+        double tot = 0.0;
+        for (int i = 0; i < n; ++i) {
+            if (!(w[i] >= 0.f) || !std::isfinite(w[i])) return -1;
+            tot += w[i];
+        }
+        if (tot >= 16.0) return -1;  // fixed-point resampling headroom
+    }
+    PF_CHECK(hipStreamSynchronize(s->stream));
+    int par = 0;
+    if (read_parity(s, 0, &par) != 0) return -1;
+    std::vector<float> tmp((size_t)n);
+    for (int c = 0; c < 7; ++c) {
+        if (c < 3)
+            for (int i = 0; i < n; ++i) tmp[i] = pos[3 * i + c];
+        else
+            for (int i = 0; i < n; ++i) tmp[i] = quat[4 * i + (c - 3)];
+        PF_CHECK(hipMemcpy(s->pose.s[par].c[c], tmp.data(), sizeof(float) * (size_t)n, hipMemcpyHostToDevice));
+    }
+    if (w) {
+        PF_CHECK(hipMemcpy(s->w, w, sizeof(float) * (size_t)n, hipMemcpyHostToDevice));
+    } else {
+        const float u = (float)(1.0 / (double)n);
+        for (int i = 0; i < n; ++i) tmp[i] = u;
+        PF_CHECK(hipMemcpy(s->w, tmp.data(), sizeof(float) * (size_t)n, hipMemcpyHostToDevice));
+    }
+    return 0;
+}
+
+int pfx_get_ll(void* h, double* ll) {
+    PfState* s = static_cast<PfState*>(h);
+    if (!s || s->nf != 1 || !ll) return -1;
+    PF_CHECK(hipStreamSynchronize(s->stream));
+    PF_CHECK(hipMemcpy(ll, s->ll, sizeof(double) * (size_t)s->n[0], hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int pfx_get_ancestors(void* h, int32_t* anc) {
+    PfState* s = static_cast<PfState*>(h);
+    if (!s || s->nf != 1 || !anc) return -1;
+    PF_CHECK(hipStreamSynchronize(s->stream));
+    PF_CHECK(hipMemcpy(anc, s->anc, sizeof(int32_t) * (size_t)s->n[0], hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int pfx_set_lcg(void* h, uint64_t lcg) {
+    PfState* s = static_cast<PfState*>(h);
+    if (!s || s->nf < 1) return -1;
+    PF_CHECK(hipStreamSynchronize(s->stream));
+    PF_CHECK(hipMemcpy(fdev_field<uint64_t>(s, 0, offsetof(FilterDev, lcg)), &lcg, sizeof(uint64_t), hipMemcpyHostToDevice));
+    return 0;
+}
+
+int pfx_get_lcg(void* h, uint64_t* lcg) {
+    PfState* s = static_cast<PfState*>(h);
+    if (!s || s->nf < 1 || !lcg) return -1;
+    PF_CHECK(hipStreamSynchronize(s->stream));
+    PF_CHECK(hipMemcpy(lcg, fdev_field<uint64_t>(s, 0, offsetof(FilterDev, lcg)), sizeof(uint64_t), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int pfx_get_rng(void* h, uint64_t* gseed, uint64_t* gen) {
+    PfState* s = static_cast<PfState*>(h);
+    if (!s) return -1;
+    if (gseed) *gseed = s->gseed;
+    if (gen) *gen = s->gen;
+    return 0;
+}
+
+int pfx_weight_scalars(void* h, double* out4) {
+    PfState* s = static_cast<PfState*>(h);
+    if (!s || s->nf < 1 || !out4) return -1;
+    FilterDev fd;
+    PF_CHECK(hipStreamSynchronize(s->stream));
+    PF_CHECK(hipMemcpy(&fd, s->fdev, sizeof(FilterDev), hipMemcpyDeviceToHost));
+    out4[0] = fd.max_ll;
+    out4[1] = fd.sum_e;
+    out4[2] = fd.sum_e2;
+    out4[3] = fd.ess;
+    return 0;
+}
+
+int pfx_profile_enable(void* h, int enable) {
+    PfState* s = static_cast<PfState*>(h);
+    if (!s) return -1;
+    s->prof_on = enable != 0;
+    return 0;
+}
+
+int pfx_profile_read(void* h, double* ms, int* count) {
+    PfState* s = static_cast<PfState*>(h);
+    if (!s) return -1;
+    PF_CHECK(hipStreamSynchronize(s->stream));
+    for (int k = 0; k < PFX_STAGE_COUNT; ++k) {
+        if (ms) ms[k] = 0.0;
+        if (count) count[k] = 0;
+    }
+    for (auto& p : s->prof_pending) {
+        float t = 0.f;
+        if (hipEventElapsedTime(&t, p.a, p.b) == hipSuccess) {
+            if (ms) ms[p.stage] += t;
+            if (count) count[p.stage] += 1;
+        }
+        (void)hipEventDestroy(p.a);
+        (void)hipEventDestroy(p.b);
+    }
+    s->prof_pending.clear();
+    return 0;
+}
+
+int pfx_particle_count(void* h) {
+    PfState* s = static_cast<PfState*>(h);
+    return (s && s->nf == 1) ? s->n[0] : 0;
+}
+
+int pfx_sync(void* h) {
+    PfState* s = static_cast<PfState*>(h);
+    if (!s) return -1;
+    PF_CHECK(hipStreamSynchronize(s->stream));
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,128 @@
+"""GPU parity of the CBGL global-localisation path (pfgl_* ABI) against the CPU oracle.
+
+  * hypotheses (deterministic-draw mode)  bit-exact vs the oracle's f32 generator
+  * rendered ranges (f32 DDA)             bit-exact vs the oracle's f32 DDA (same directions)
+  * CAER                                  bit-exact (same lane order and tree); contract 1e-4 m
+  * bottom-k                              identical (CAER, index) order
+  * reference tests global_loc.rs:11-121 ported (known raycast answers, recovery, ranking)
+"""
+import math
+
+import numpy as np
+import pytest
+
+from cuda_pf_localization_amd import GlKernels, GlobalLoc, GlParams
+from cuda_pf_localization_amd.sim import obstacle_map, room_grid
+from oracle import OracleGrid, bottom_k
+
+pytestmark = pytest.mark.gpu
+
+TAU = 2 * math.pi
+
+
+def test_generate_bitexact(gpu):
+    cells, res, org = obstacle_map(101)
+    g = GlKernels(cells, res, org[0], org[1], 50_000)
+    og = OracleGrid(cells, res, org)
+    assert g.free_cells() == int((cells == 0).sum())
+    for seed in (0, 3, 2**63 + 5):
+        g.generate(50_000, seed)
+        hyp = g.hypotheses()
+        ohyp = og.gen_hyp32(seed, 50_000)
+        assert np.array_equal(hyp, ohyp)
+        free = cells[((hyp[:, 1] - org[1]) / res).astype(int), ((hyp[:, 0] - org[0]) / res).astype(int)] == 0
+        assert free.all()
+        assert hyp[:, 2].min() >= -math.pi - 1e-6 and hyp[:, 2].max() < math.pi + 1e-6
+
+
+def test_render_bitexact(gpu):
+    cells, res, org = obstacle_map(102)
+    g = GlKernels(cells, res, org[0], org[1], 1000)
+    og = OracleGrid(cells, res, org)
+    rng = np.random.default_rng(0)
+    poses = np.column_stack([rng.random(500) * 10, rng.random(500) * 10, rng.random(500) * TAU - math.pi])
+    poses = poses.astype(np.float32)
+    for nrays, span, rmax in ((360, TAU, 8.0), (100, math.pi, 3.0), (1, TAU, 8.0), (1500, TAU, 12.0)):
+        r = g.render(poses, nrays, rmax, span)
+        o = og.render32(poses, nrays, np.float32(span), rmax)
+        assert np.array_equal(r, o), f"{np.sum(r != o)} mismatches for nrays={nrays}"
+
+
+@pytest.mark.parametrize("nrays", [360, 64, 1000, 2000])
+def test_caer_bitexact_and_topk(gpu, nrays):
+    cells, res, org = obstacle_map(103)
+    n = 20_000
+    g = GlKernels(cells, res, org[0], org[1], n)
+    og = OracleGrid(cells, res, org)
+    truth = (3.3, 6.1, 0.7)
+    scan = og.map_scan64(*truth, nrays, TAU, 8.0).astype(np.float32)
+    scan += np.random.default_rng(1).normal(size=nrays).astype(np.float32) * 0.02
+    g.generate(n, 9)
+    g.score(scan, 8.0, TAU)
+    caer = g.caer()
+    hyp = g.hypotheses()
+    ocaer = og.caer32(scan, hyp, np.float32(TAU), 8.0)
+    assert np.array_equal(caer, ocaer), f"max |d| {np.abs(caer - ocaer).max()}"
+    for k in (1, 10, 64, 4096, 5000):
+        out, kc, kidx = g.topk_ex(k)
+        want = bottom_k(ocaer, k)
+        assert np.array_equal(kidx, want.astype(np.int32)), f"k={k}"
+        assert np.array_equal(out, hyp[want])
+        assert np.array_equal(kc, ocaer[want])
+    plain = g.topk(10)
+    assert np.array_equal(plain, hyp[bottom_k(ocaer, 10)])
+
+
+def test_topk_ties_keep_index_order(gpu):
+    """All-rmax map (no obstacles inside rmax) -> many equal CAERs; stable order by index."""
+    cells = np.zeros((50, 50), np.uint8)
+    g = GlKernels(cells, 0.05, 0.0, 0.0, 3000)
+    g.generate(3000, 1)
+    scan = np.full(90, 1.0, np.float32)
+    g.score(scan, 1.0, TAU)  # every ray leaves the grid or reaches rmax -> identical CAER 0
+    caer = g.caer()
+    out, kc, kidx = g.topk_ex(100)
+    assert np.array_equal(kidx, bottom_k(caer, 100).astype(np.int32))
+    assert g.topk_ex(5000)[2].size == 3000  # k clamped to the hypothesis count
+
+
+def test_raycast_known_answers(gpu):
+    """global_loc.rs tests:11-41 through the GPU raycaster (room_grid)."""
+    cells, res, org = room_grid()
+    g = GlKernels(cells, res, org[0], org[1], 10)
+    r = g.render(np.array([[5.0, 5.0, 0.0], [5.0, 5.0, math.pi / 2], [2.5, 5.0, 0.0]], np.float32), 1, 20.0, TAU)
+    assert abs(r[0, 0] - 4.95) < 0.15 and abs(r[1, 0] - 4.95) < 0.15 and abs(r[2, 0] - 1.25) < 0.12
+    r2 = g.render(np.array([[5.0, 5.0, 0.0]], np.float32), 1, 2.0, TAU)
+    assert r2[0, 0] == np.float32(2.0)
+
+
+def test_cuda_matches_cpu_ranking(gpu):
+    """global_loc.rs tests:75-121 (30k hypotheses, truth (6, 7, 0.8), room_grid)."""
+    cells, res, org = room_grid()
+    og = OracleGrid(cells, res, org)
+    truth = (6.0, 7.0, 0.8)
+    scan64 = og.map_scan64(*truth, 360, TAU, 10.0)
+    gl = GlobalLoc(cells, res, org, GlParams(hypotheses=30_000))
+    top = gl.localize(scan64.astype(np.float32))
+    ctop = og.localize64(scan64, 30_000, 10, 42)
+    best = min(math.hypot(a[0] - b[0], a[1] - b[1]) for a in top for b in ctop)
+    assert best < 0.5
+    assert math.hypot(top[0, 0] - truth[0], top[0, 1] - truth[1]) < 0.5
+    assert np.all(np.diff(top[:, 3]) >= 0)
+
+
+def test_config2_localization_success(gpu):
+    """Config 2: obstacle map, 360 rays, 8 m, 0.02 m noise, 200k hypotheses, bottom-64:
+    the best hypothesis is within 0.5 m of the truth for most poses."""
+    from cuda_pf_localization_amd.sim import free_poses
+    cells, res, org = obstacle_map(3)
+    og = OracleGrid(cells, res, org)
+    truths = free_poses(cells, res, org, 10, seed=3, margin_cells=6)
+    gl = GlobalLoc(cells, res, org, GlParams(hypotheses=200_000, rmax=8.0, topk=64, seed=3))
+    rng = np.random.default_rng(3)
+    ok = 0
+    for t in truths:
+        scan = og.map_scan64(*t, 360, TAU, 8.0) + rng.normal(size=360) * 0.02
+        top = gl.localize(scan.astype(np.float32))
+        ok += math.hypot(top[0, 0] - t[0], top[0, 1] - t[1]) < 0.5
+    assert ok >= 8, f"{ok}/10"
