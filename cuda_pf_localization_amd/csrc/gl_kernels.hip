@@ -46,7 +46,7 @@ using namespace pfd;
 namespace glk {
 
 constexpr int kTB = 256;
-constexpr int kLdsBitsMaxWords = 12288;  // 48 KB of grid bits in LDS (393,216 cells)
+constexpr int kLdsBitsMaxWords = 6144;   // 24 KB of grid bits in LDS (196,608 cells)
 constexpr int kLdsRaysMax = 1024;        // real scan + ray table staged in LDS up to this many rays
 constexpr int kSortMax = 4096;           // winners sorted in one workgroup up to this k
 
@@ -287,26 +287,28 @@ __global__ void k_tk_pick(uint64_t* tk, unsigned* hist, int shift) {
 }
 
 __global__ __launch_bounds__(kTB) void k_tk_collect(const float* __restrict__ caer, int n, uint64_t* tk,
-                                                    uint64_t* __restrict__ win) {
+                                                    uint64_t* __restrict__ win, int k) {
     const uint64_t thr = tk[1];
     for (int i = blockIdx.x * kTB + threadIdx.x; i < n; i += gridDim.x * kTB) {
         const uint64_t c = comp_key(caer[i], i);
         if (c <= thr) {
             const unsigned long long slot = atomicAdd(reinterpret_cast<unsigned long long*>(&tk[4]), 1ull);
-            win[slot] = c;
+            if (slot < (unsigned long long)k) win[slot] = c;
         }
     }
 }
 
 // Sorts k <= kSortMax keys in one workgroup (bitonic in LDS) and gathers the poses.
-__global__ __launch_bounds__(1024) void k_tk_sort(const uint64_t* __restrict__ win, int k,
+__global__ __launch_bounds__(1024) void k_tk_sort(const uint64_t* __restrict__ win, const uint64_t* tk, int k, int n,
                                                   const float* __restrict__ hyp, const float* __restrict__ caer,
                                                   float* __restrict__ out, float* __restrict__ outcaer,
                                                   int32_t* __restrict__ outidx) {
     __shared__ uint64_t sk[kSortMax];
     int m = 1;
     while (m < k) m <<= 1;
-    for (int i = threadIdx.x; i < m; i += 1024) sk[i] = i < k ? win[i] : ~0ull;
+    const uint64_t got = tk[4];  // == k when the selection is exact
+    const int cnt = got < (uint64_t)k ? (int)got : k;
+    for (int i = threadIdx.x; i < m; i += 1024) sk[i] = i < cnt ? win[i] : ~0ull;
     __syncthreads();
     for (int size = 2; size <= m; size <<= 1)
         for (int stride = size >> 1; stride > 0; stride >>= 1) {
@@ -324,7 +326,12 @@ __global__ __launch_bounds__(1024) void k_tk_sort(const uint64_t* __restrict__ w
             __syncthreads();
         }
     for (int i = threadIdx.x; i < k; i += 1024) {
-        const int src = (int)(uint32_t)(sk[i] & 0xffffffffu);
+        const uint32_t su = (uint32_t)(sk[i] & 0xffffffffu);
+        if (su >= (uint32_t)n) {
+            outidx[i] = -1;
+            continue;
+        }
+        const int src = (int)su;
         out[3 * i] = hyp[3 * src];
         out[3 * i + 1] = hyp[3 * src + 1];
         out[3 * i + 2] = hyp[3 * src + 2];
@@ -354,13 +361,18 @@ __global__ __launch_bounds__(kTB) void k_fill_keys(const float* __restrict__ cae
     if (i < m) a[i] = i < n ? comp_key(caer[i], i) : ~0ull;
 }
 
-__global__ __launch_bounds__(kTB) void k_gather_sorted(const uint64_t* __restrict__ a, int k,
+__global__ __launch_bounds__(kTB) void k_gather_sorted(const uint64_t* __restrict__ a, int k, int n,
                                                        const float* __restrict__ hyp, const float* __restrict__ caer,
                                                        float* __restrict__ out, float* __restrict__ outcaer,
                                                        int32_t* __restrict__ outidx) {
     const int i = blockIdx.x * kTB + threadIdx.x;
     if (i >= k) return;
-    const int src = (int)(uint32_t)(a[i] & 0xffffffffu);
+    const uint32_t su = (uint32_t)(a[i] & 0xffffffffu);
+    if (su >= (uint32_t)n) {
+        outidx[i] = -1;
+        return;
+    }
+    const int src = (int)su;
     out[3 * i] = hyp[3 * src];
     out[3 * i + 1] = hyp[3 * src + 1];
     out[3 * i + 2] = hyp[3 * src + 2];
@@ -454,9 +466,9 @@ int topk_launch(GlState* s, int k) {
                                shift);
             hipLaunchKernelGGL(k_tk_pick, dim3(1), dim3(256), 0, s->stream, s->d_tk, s->d_hist, shift);
         }
-        hipLaunchKernelGGL(k_tk_collect, dim3(blocks), dim3(kTB), 0, s->stream, s->d_caer, n, s->d_tk, s->d_win);
-        hipLaunchKernelGGL(k_tk_sort, dim3(1), dim3(1024), 0, s->stream, s->d_win, k, s->d_hyp, s->d_caer, s->d_out,
-                           s->d_outcaer, s->d_outidx);
+        hipLaunchKernelGGL(k_tk_collect, dim3(blocks), dim3(kTB), 0, s->stream, s->d_caer, n, s->d_tk, s->d_win, k);
+        hipLaunchKernelGGL(k_tk_sort, dim3(1), dim3(1024), 0, s->stream, s->d_win, s->d_tk, k, n, s->d_hyp,
+                           s->d_caer, s->d_out, s->d_outcaer, s->d_outidx);
     } else {
         int m = 1;
         while (m < n) m <<= 1;
@@ -471,7 +483,7 @@ int topk_launch(GlState* s, int k) {
         for (int size = 2; size <= m; size <<= 1)
             for (int stride = size >> 1; stride > 0; stride >>= 1)
                 hipLaunchKernelGGL(k_bitonic_step, dim3(gb), dim3(kTB), 0, s->stream, s->d_sortbuf, m, size, stride);
-        hipLaunchKernelGGL(k_gather_sorted, dim3((k + kTB - 1) / kTB), dim3(kTB), 0, s->stream, s->d_sortbuf, k,
+        hipLaunchKernelGGL(k_gather_sorted, dim3((k + kTB - 1) / kTB), dim3(kTB), 0, s->stream, s->d_sortbuf, k, n,
                            s->d_hyp, s->d_caer, s->d_out, s->d_outcaer, s->d_outidx);
     }
     prof_e(s);

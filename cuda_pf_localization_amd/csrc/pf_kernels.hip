@@ -304,11 +304,12 @@ __global__ __launch_bounds__(kTB) void k_scan(Layout L, FilterDev* F, const floa
     if (threadIdx.x == 0) s_tile = (int)atomicAdd(&ctr[0], 1u);
     __syncthreads();
     const int tile = s_tile;
-    const int f = L.blk_filter[tile * (kTile / kTB)];
+    const bool in_range = tile < ntiles;  // always true while the ticket counter is consistent
+    const int f = in_range ? L.blk_filter[tile * (kTile / kTB)] : 0;
     const int off = L.f_off[f];
     const int nf = L.f_n[f];
     const int ftile0 = off / kTile;
-    const bool active = F[f].do_resample != 0;
+    const bool active = in_range && F[f].do_resample != 0;
     if (active) {
         const int base = tile * kTile + (int)threadIdx.x * 4;
         const int li0 = base - off;
@@ -345,10 +346,17 @@ __global__ __launch_bounds__(kTB) void k_scan(Layout L, FilterDev* F, const floa
                 __hip_atomic_store(&status[tile], kFlagAgg | (agg & kValMask), __ATOMIC_RELEASE,
                                    __HIP_MEMORY_SCOPE_AGENT);
                 int j = tile - 1;
-                while (true) {
+                unsigned spins = 0;
+                while (j >= ftile0) {
                     const uint64_t st = __hip_atomic_load(&status[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
                     const uint64_t fl = st & ~kValMask;
                     if (fl == 0) {
+                        // predecessors hold earlier tickets and always publish; the cap only
+                        // turns an inconsistent state into an error instead of a hang
+                        if (++spins > (1u << 24)) {
+                            ctr[2] = 1;
+                            break;
+                        }
                         __builtin_amdgcn_s_sleep(1);
                         continue;
                     }
