@@ -29,11 +29,11 @@ HBM_PEAK_GBS = 8000.0
 FP64_VALU_PEAK_TFLOPS = 78.6
 
 # Algorithmic work of k_weight per range-bearing particle-observation pair, counted on the
-# kernel's f64 formulation: d (3 sub), body vector (9 fma), r2 (3 fma), pv (3 fma),
-# reciprocal sqrt refinement (4), rn (1), er/er^2/term (4), omd/clamp/t (4),
-# acos^2 series (7 fma + 2 mul), bearing term (2), accumulate (2) -> 42 ops,
-# fma counted as 2 flops -> 69 FLOP per pair.
-WEIGHT_FLOP_PER_PAIR = 69.0
+# kernel's f64 formulation (fma = 2 FLOP): landmark offset 3, body rotation 15, |b|^2 5,
+# b.v 5, reciprocal-sqrt refinement 6, range residual 2 + square 1 + gated accumulate 2,
+# 1 - dot 2, acos^2 series 11, gated accumulate 2 -> 54 FLOP, plus one f32 rsqrt and two
+# conversions that are not counted.
+WEIGHT_FLOP_PER_PAIR = 54.0
 # Algorithmic HBM bytes per particle (SoA f32 pose 28 B, weight 4 B, log-likelihood 8 B).
 BYTES = {"predict": 56, "weight": 36, "normalize": 12, "resample_scan": 12, "resample_gather": 72,
          "est_sums": 32, "est_covs": 32}
@@ -158,6 +158,18 @@ def cpu_baseline(sc, n, budget_s=12.0):
                       f"cpu.rs, f64, OpenMP)"}
 
 
+def copy_calibration(nbytes: int) -> None:
+    """A plain copy of known size (read nbytes + write nbytes) for scaling the FETCH_SIZE /
+    WRITE_SIZE counters of the profiled kernels."""
+    import torch
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": nbytes, "launches": 3}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -167,7 +179,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-gl", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--copy-calibration", type=int, default=0,
+                    help="only run a plain device-to-device copy of this many bytes (HBM counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: the particle filter is a single-device path here "
               f"(--gpus {args.gpus} is not supported)", file=sys.stderr)

@@ -51,6 +51,21 @@ class PfhGlParams(C.Structure):
                 ("seed", u64)]
 
 
+class PfbFilterCfg(C.Structure):
+    _fields_ = [("n", i32), ("seed", u32), ("sigma_r", f32), ("sigma_b", f32), ("gate", f32),
+                ("resample_ess_ratio", f32), ("inject_ratio", f32)]
+
+
+class PfbCycleIn(C.Structure):
+    _fields_ = [("t", f32 * 3), ("q", f32 * 4), ("tstd", f32 * 3), ("rstd", f32 * 3), ("obs", C.POINTER(PfcObs)),
+                ("nobs", i32)]
+
+
+class PfbEstimate(C.Structure):
+    _fields_ = [("pos", f64 * 3), ("quat", f64 * 4), ("pos_cov", f64 * 9), ("att_cov", f64 * 9), ("ess", f64),
+                ("resampled", i32), ("valid", i32)]
+
+
 # (name, restype, argtypes)
 KERNEL_SYMBOLS = [
     # reference ABI (pf_kernels.cu:475-1025)
@@ -85,6 +100,7 @@ KERNEL_SYMBOLS = [
     ("pfx_profile_read", C.c_int, [vp, P(f64), P(C.c_int)]),
     ("pfx_particle_count", C.c_int, [vp]),
     ("pfx_sync", C.c_int, [vp]),
+    ("pfx_quat_mean", C.c_int, [P(f64), P(f64)]),
     # global-localisation extensions
     ("pfglx_topk_ex", C.c_int, [vp, C.c_int, P(f32), P(f32), P(i32)]),
     ("pfglx_set_hypotheses", C.c_int, [vp, P(f32), C.c_int]),
@@ -96,6 +112,19 @@ KERNEL_SYMBOLS = [
     ("pfglx_profile_enable", C.c_int, [vp, C.c_int]),
     ("pfglx_profile_read", C.c_int, [vp, P(f64), P(C.c_int)]),
     ("pfglx_sync", C.c_int, [vp]),
+    # batched filters
+    ("pfb_create", vp, [C.c_int, P(PfbFilterCfg), C.c_int]),
+    ("pfb_destroy", C.c_int, [vp]),
+    ("pfb_filter_count", C.c_int, [vp]),
+    ("pfb_upload_landmarks", C.c_int, [vp, C.c_int, P(f32), C.c_int]),
+    ("pfb_reinit", C.c_int, [vp, P(f32), P(f32), P(f32)]),
+    ("pfb_cycle", C.c_int, [vp, P(PfbCycleIn), P(PfbEstimate)]),
+    ("pfb_use_graph", C.c_int, [vp, C.c_int]),
+    ("pfb_autotune", C.c_int, [vp, P(PfbCycleIn)]),
+    ("pfb_get_state", C.c_int, [vp, C.c_int, P(f32), P(f32), P(f32)]),
+    ("pfb_get_ancestors", C.c_int, [vp, C.c_int, P(i32)]),
+    ("pfb_profile_enable", C.c_int, [vp, C.c_int]),
+    ("pfb_profile_read", C.c_int, [vp, P(f64), P(C.c_int)]),
 ]
 
 HOST_SYMBOLS = [

@@ -19,7 +19,7 @@ OBS_DTYPE = np.dtype([("id", "<i4"), ("mode", "<i4"), ("dx", "<f4"), ("dy", "<f4
 assert OBS_DTYPE.itemsize == 24  # ObsF / CObservation, pf_kernels.cu:79-84, cuda.rs:18-27
 
 STAGES = ["reinit", "predict", "weight", "normalize", "resample_scan", "resample_gather", "inject", "est_sums",
-          "est_covs"]
+          "est_covs", "finalize"]
 GL_STAGES = ["generate", "score", "topk"]
 
 
@@ -457,3 +457,99 @@ class GlobalLoc:
 
     def kernels_handle(self):
         return self.H.pfh_gl_kernels(self.g)
+
+
+# ------------------------------------------------------------------ batched filters
+
+
+class BatchedFilters:
+    """pfb_* ABI: nf independent filters advanced by one launch per stage."""
+
+    def __init__(self, cfgs: list, obs_cap: int = 256):
+        self.L = _lib.kernels()
+        arr = (_lib.PfbFilterCfg * len(cfgs))()
+        for i, c in enumerate(cfgs):
+            arr[i] = _lib.PfbFilterCfg(c["n"], c.get("seed", 42), c.get("sigma_r", 0.05), c.get("sigma_b", 0.02),
+                                       c.get("gate", 0.0), c.get("ratio", 0.5), c.get("inject", 0.0))
+        self.h = self.L.pfb_create(len(cfgs), arr, obs_cap)
+        if not self.h:
+            raise RuntimeError("pfb_create failed")
+        self.ns = [c["n"] for c in cfgs]
+        self.nf = len(cfgs)
+        self._keep = []
+
+    def close(self):
+        if self.h:
+            self.L.pfb_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload_landmarks(self, f: int, xyz):
+        a = np.ascontiguousarray(np.asarray(xyz, np.float32).reshape(-1))
+        _check(self.L.pfb_upload_landmarks(self.h, f, _fp(a), a.size // 3), "pfb_upload_landmarks")
+
+    def reinit(self, poses7, pos_std, rot_std):
+        p = np.ascontiguousarray(poses7, np.float32)
+        ps = np.ascontiguousarray(pos_std, np.float32)
+        rs = np.ascontiguousarray(rot_std, np.float32)
+        _check(self.L.pfb_reinit(self.h, _fp(p), _fp(ps), _fp(rs)), "pfb_reinit")
+
+    def make_inputs(self, per_filter):
+        """per_filter: list of (t3, q4, tstd3, rstd3, obs ndarray OBS_DTYPE or None)."""
+        arr = (_lib.PfbCycleIn * self.nf)()
+        keep = []
+        for f, (t, q, ts, rs, obs) in enumerate(per_filter):
+            arr[f].t[:] = [float(v) for v in t]
+            arr[f].q[:] = [float(v) for v in q]
+            arr[f].tstd[:] = [float(v) for v in ts]
+            arr[f].rstd[:] = [float(v) for v in rs]
+            if obs is not None and len(obs):
+                o = np.ascontiguousarray(obs, dtype=OBS_DTYPE)
+                keep.append(o)
+                arr[f].obs = o.ctypes.data_as(C.POINTER(_lib.PfcObs))
+                arr[f].nobs = len(o)
+            else:
+                arr[f].obs = None
+                arr[f].nobs = 0
+        return arr, keep
+
+    def cycle(self, inputs) -> C.Array:
+        out = (_lib.PfbEstimate * self.nf)()
+        _check(self.L.pfb_cycle(self.h, inputs, out), "pfb_cycle")
+        return out
+
+    def use_graph(self, on: bool = True):
+        _check(self.L.pfb_use_graph(self.h, 1 if on else 0), "pfb_use_graph")
+
+    def autotune(self, inputs) -> int:
+        r = self.L.pfb_autotune(self.h, inputs)
+        if r < 0:
+            raise RuntimeError("pfb_autotune failed")
+        return r
+
+    def get_state(self, f: int):
+        n = self.ns[f]
+        pos = np.zeros((n, 3), np.float32)
+        quat = np.zeros((n, 4), np.float32)
+        w = np.zeros(n, np.float32)
+        _check(self.L.pfb_get_state(self.h, f, _fp(pos), _fp(quat), _fp(w)), "pfb_get_state")
+        return pos, quat, w
+
+    def ancestors(self, f: int):
+        a = np.zeros(self.ns[f], np.int32)
+        _check(self.L.pfb_get_ancestors(self.h, f, _ip(a)), "pfb_get_ancestors")
+        return a
+
+    def profile(self, enable: bool = True):
+        _check(self.L.pfb_profile_enable(self.h, 1 if enable else 0), "pfb_profile_enable")
+
+    def profile_read(self):
+        ms = np.zeros(len(STAGES))
+        cnt = np.zeros(len(STAGES), np.int32)
+        _check(self.L.pfb_profile_read(self.h, _dp(ms), _ip(cnt)), "pfb_profile_read")
+        return {s: (float(ms[i]), int(cnt[i])) for i, s in enumerate(STAGES)}

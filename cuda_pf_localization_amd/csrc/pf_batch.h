@@ -28,7 +28,8 @@ struct PoseBufs {
 
 // Device-resident per-filter state (scalars of the last weight / scan / estimate passes).
 struct FilterDev {
-    double max_ll, sum_e, sum_e2, ess;
+    double max_ll, sum_e, sum_e2;  // last weight pass: max ll, sum exp(ll-max), sum exp^2
+    double sum_w2, ess;             // last normalisation: sum of squared stored weights, 1/sum
     int uniform;      // last weight pass degenerate -> weights are 1/n
     int parity;       // current pose set
     int do_resample;  // resample predicate of the current cycle
@@ -37,7 +38,6 @@ struct FilterDev {
     uint64_t lcg;     // resampling LCG state (pf_kernels.cu:116, 653)
     double sums[kEstN];
     double covs[kCovN];
-    unsigned cnt_weight, cnt_gather, cnt_sums, cnt_covs;
 };
 
 struct Layout {
@@ -67,7 +67,9 @@ struct MotionParams {
 };
 
 struct WeightParams {
-    double inv2rr, inv2bb, gate, cart_gate2, rng_gate2, brg_gate2, cart_gated, half_g2, t_gate;
+    double inv2rr, inv2bb, gate;
+    double cart_gate2, rng_gate2, brg_gate2;  // squared-residual gate thresholds (+inf: no gate)
+    double u_gate;                            // 1 - cos of the bearing gate (+inf: none)
     const ObsRec* obs;
     int nobs, pad;
 };
@@ -126,9 +128,9 @@ struct PfState {
     int32_t *inj_idx = nullptr, *inj_next = nullptr, *inj_head = nullptr;
     double* inj_jit = nullptr;
     // reduction partials / scan state
-    double *part_w = nullptr, *part_est = nullptr, *part_cov = nullptr;
-    uint64_t* scan_status = nullptr;
-    unsigned* scan_ctr = nullptr;
+    double *part_w = nullptr, *part_n = nullptr, *part_est = nullptr, *part_cov = nullptr;
+    uint64_t* tile_agg = nullptr;  // per-tile fixed-point weight sums
+    uint64_t* tile_pre = nullptr;  // exclusive prefix of the tile sums within each filter
     FilterDev* fdev = nullptr;
     // observations (reference ABI path)
     ObsRec* d_obs = nullptr;
@@ -138,6 +140,8 @@ struct PfState {
     int lm_cap = 0;
     // deterministic-draw state (single-filter ABI)
     uint64_t gseed = 42, gen = 0;
+    // launch shape of the likelihood kernel (particles per lane: 1, 2 or 4)
+    int weight_ppt = 2;
     // profiling
     bool prof_on = false;
     std::vector<ProfEvent> prof_pending;
@@ -149,7 +153,7 @@ void prof_begin(PfState* s, int stage);
 void prof_end(PfState* s);
 int launch_motion(PfState* s, ParamRef<MotionParams> A, int reinit);
 int launch_weight(PfState* s, ParamRef<WeightParams> A);
-int launch_normalize(PfState* s, int force_uniform);
+int launch_normalize(PfState* s, int force_uniform, const float* d_ratio);
 int launch_resample(PfState* s, ParamRef<InjectParams> inj, int any_inject);
 int launch_est_sums(PfState* s);
 int launch_est_covs(PfState* s, ParamRef<CovParams> A);

@@ -1,0 +1,444 @@
+// pf_batched.hip -- batched filters (pfb_* C ABI): nf independent particle filters of uneven
+// sizes share the SoA streams of one handle and advance with one launch per stage.
+//
+// A cycle (particle_filter.rs:110-124 for every filter at once):
+//   host   : per-filter odometry / weight / injection parameters and the observations joined
+//            with each filter's landmark table -> one pinned block, one H2D copy each
+//   device : motion -> likelihood (+ softmax partials) -> normalise (+ ESS of the stored
+//            weights) -> resample predicate (ESS < ratio n) -> scan -> gather -> injection ->
+//            estimate sums            [graph A when enabled]
+//   host   : Markley mean of every filter (types.rs:128-144) -> covariance parameters
+//   device : estimate covariances     [graph B when enabled]
+// The resample decision never leaves the device; filters that do not resample skip the
+// resampling kernels tile by tile.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../../include/pf_batch_api.h"
+#include "pf_batch.h"
+#include "pf_device.h"
+
+#define PB_CHECK(x)                                                                              \
+    do {                                                                                         \
+        hipError_t e_ = (x);                                                                     \
+        if (e_ != hipSuccess) {                                                                  \
+            fprintf(stderr, "pf_hip %s:%d %s: %s\n", __FILE__, __LINE__, #x, hipGetErrorString(e_)); \
+            return -1;                                                                           \
+        }                                                                                        \
+    } while (0)
+
+
+namespace {
+
+using namespace pfk;
+
+struct ParamBlock {  // one contiguous pinned / device block per cycle
+    MotionParams* motion;
+    WeightParams* weight;
+    InjectParams* inject;
+    CovParams* cov;
+};
+
+struct Batch {
+    PfState s;
+    int nf = 0, obs_cap = 0;
+    std::vector<pfb_filter_cfg> cfg;
+    std::vector<std::vector<float>> lm;
+    std::vector<int> lm_cap;
+    std::vector<uint64_t> gseed, gen;
+    // device / pinned parameter blocks
+    char* d_params = nullptr;
+    char* h_params = nullptr;
+    size_t params_bytes = 0;
+    ParamBlock dp{}, hp{};
+    float* d_ratio = nullptr;
+    ObsRec* d_obs = nullptr;
+    ObsRec* h_obs = nullptr;
+    FilterDev* h_fdev = nullptr;
+    // graphs
+    bool use_graph = false;
+    bool captured = false;
+    hipGraphExec_t gA = nullptr, gB = nullptr;
+};
+
+ParamBlock carve(char* base, int nf) {
+    ParamBlock b;
+    size_t o = 0;
+    auto take = [&](size_t sz) {
+        char* p = base + o;
+        o += (sz + 255) & ~size_t(255);
+        return p;
+    };
+    b.motion = reinterpret_cast<MotionParams*>(take(sizeof(MotionParams) * nf));
+    b.weight = reinterpret_cast<WeightParams*>(take(sizeof(WeightParams) * nf));
+    b.inject = reinterpret_cast<InjectParams*>(take(sizeof(InjectParams) * nf));
+    b.cov = reinterpret_cast<CovParams*>(take(sizeof(CovParams) * nf));
+    return b;
+}
+
+size_t params_size(int nf) {
+    auto r = [](size_t sz) { return (sz + 255) & ~size_t(255); };
+    return r(sizeof(MotionParams) * nf) + r(sizeof(WeightParams) * nf) + r(sizeof(InjectParams) * nf) +
+           r(sizeof(CovParams) * nf);
+}
+
+void free_batch(Batch* b) {
+    if (b->s.stream) (void)hipStreamSynchronize(b->s.stream);
+    if (b->gA) (void)hipGraphExecDestroy(b->gA);
+    if (b->gB) (void)hipGraphExecDestroy(b->gB);
+    state_free_particles(&b->s);
+    (void)hipFree(b->d_params);
+    (void)hipHostFree(b->h_params);
+    (void)hipFree(b->d_ratio);
+    (void)hipFree(b->d_obs);
+    (void)hipHostFree(b->h_obs);
+    (void)hipHostFree(b->h_fdev);
+    for (auto& p : b->s.prof_pending) {
+        (void)hipEventDestroy(p.a);
+        (void)hipEventDestroy(p.b);
+    }
+    if (b->s.stream) (void)hipStreamDestroy(b->s.stream);
+    delete b;
+}
+
+// Device stages up to the estimate sums (graph A).
+int stages_a(Batch* b) {
+    PfState* s = &b->s;
+    if (launch_motion(s, ParamRef<MotionParams>::array(b->dp.motion), 0) != 0) return -1;
+    if (launch_weight(s, ParamRef<WeightParams>::array(b->dp.weight)) != 0) return -1;
+    if (launch_normalize(s, 0, b->d_ratio) != 0) return -1;  // ESS + resample predicate
+    if (launch_resample(s, ParamRef<InjectParams>::array(b->dp.inject), 1) != 0) return -1;
+    if (launch_est_sums(s) != 0) return -1;
+    return 0;
+}
+
+int stages_b(Batch* b) { return launch_est_covs(&b->s, ParamRef<CovParams>::array(b->dp.cov)); }
+
+int capture(Batch* b, int (*fn)(Batch*), hipGraphExec_t* out) {
+    hipGraph_t g = nullptr;
+    PB_CHECK(hipStreamBeginCapture(b->s.stream, hipStreamCaptureModeThreadLocal));
+    const int rc = fn(b);
+    hipError_t e = hipStreamEndCapture(b->s.stream, &g);
+    if (rc != 0 || e != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        fprintf(stderr, "pf_hip graph capture failed\n");
+        return -1;
+    }
+    e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    PB_CHECK(e);
+    return 0;
+}
+
+int run_a(Batch* b) {
+    if (!b->use_graph) return stages_a(b);
+    if (!b->gA && capture(b, stages_a, &b->gA) != 0) return -1;
+    PB_CHECK(hipGraphLaunch(b->gA, b->s.stream));
+    return 0;
+}
+
+int run_b(Batch* b) {
+    if (!b->use_graph) return stages_b(b);
+    if (!b->gB && capture(b, stages_b, &b->gB) != 0) return -1;
+    PB_CHECK(hipGraphLaunch(b->gB, b->s.stream));
+    return 0;
+}
+
+void drop_graphs(Batch* b) {
+    if (b->gA) (void)hipGraphExecDestroy(b->gA);
+    if (b->gB) (void)hipGraphExecDestroy(b->gB);
+    b->gA = b->gB = nullptr;
+}
+
+// Host half of a cycle before graph A: parameters + joined observations -> device.
+int upload_inputs(Batch* b, const pfb_cycle_in* in) {
+    PfState* s = &b->s;
+    int total = 0;
+    for (int f = 0; f < b->nf; ++f) {
+        const pfb_cycle_in& c = in[f];
+        b->gseed[f] += pfd::kGenBump;  // cpu.rs:198
+        b->gen[f] += 1;                // cu:588
+        MotionParams& m = b->hp.motion[f];
+        memset(&m, 0, sizeof(m));
+        for (int k = 0; k < 3; ++k) {
+            m.t[k] = c.t[k];
+            m.s_t[k] = c.tstd[k];
+            m.s_r[k] = c.rstd[k];
+        }
+        for (int k = 0; k < 4; ++k) m.q[k] = c.q[k];
+        m.gseed = b->gseed[f];
+        const int nobs = c.nobs > b->obs_cap ? b->obs_cap : c.nobs;
+        int mobs = 0;
+        if (nobs > 0 && c.obs)
+            mobs = join_obs(c.obs, nobs, b->lm[f].data(), b->lm_cap[f], b->h_obs + total);
+        WeightParams wp = make_weight_params(b->cfg[f].sigma_r, b->cfg[f].sigma_b, b->cfg[f].gate);
+        wp.obs = b->d_obs + total;
+        wp.nobs = mobs;
+        b->hp.weight[f] = wp;
+        total += mobs;
+        InjectParams& ip = b->hp.inject[f];
+        ip.gen = b->gen[f];
+        const float r = b->cfg[f].inject_ratio;
+        int k = 0;
+        if (r > 0.f) {
+            k = (int)(r * (float)b->cfg[f].n);  // cu:667-671
+            if (k > b->cfg[f].n) k = b->cfg[f].n;
+        }
+        ip.k = k;
+        ip.pad = 0;
+    }
+    PB_CHECK(hipMemcpyAsync(b->d_params, b->h_params, b->params_bytes, hipMemcpyHostToDevice, s->stream));
+    if (total > 0)
+        PB_CHECK(hipMemcpyAsync(b->d_obs, b->h_obs, sizeof(ObsRec) * (size_t)total, hipMemcpyHostToDevice, s->stream));
+    return 0;
+}
+
+}  // namespace
+
+
+extern "C" {
+
+void* pfb_create(int nf, const pfb_filter_cfg* cfg, int obs_cap) {
+    if (nf <= 0 || !cfg) return nullptr;
+    for (int f = 0; f < nf; ++f)
+        if (cfg[f].n <= 0) return nullptr;
+    Batch* b = new (std::nothrow) Batch();
+    if (!b) return nullptr;
+    if (obs_cap < 1 || obs_cap > 4096) obs_cap = 64;
+    b->nf = nf;
+    b->obs_cap = obs_cap;
+    b->cfg.assign(cfg, cfg + nf);
+    b->lm.resize(nf);
+    b->lm_cap.assign(nf, 0);
+    b->gseed.resize(nf);
+    b->gen.assign(nf, 0);
+    std::vector<int> ns(nf);
+    std::vector<float> ratio(nf);
+    for (int f = 0; f < nf; ++f) {
+        ns[f] = cfg[f].n;
+        b->gseed[f] = cfg[f].seed ? (uint64_t)cfg[f].seed : 42ull;
+        ratio[f] = cfg[f].resample_ess_ratio;
+    }
+    b->params_bytes = params_size(nf);
+    bool ok = hipSetDevice(0) == hipSuccess &&
+              hipStreamCreateWithFlags(&b->s.stream, hipStreamNonBlocking) == hipSuccess;
+    ok = ok && state_alloc(&b->s, ns) == 0;
+    ok = ok && hipMalloc((void**)&b->d_params, b->params_bytes) == hipSuccess;
+    ok = ok && hipHostMalloc((void**)&b->h_params, b->params_bytes, hipHostMallocDefault) == hipSuccess;
+    ok = ok && hipMalloc((void**)&b->d_ratio, sizeof(float) * (size_t)nf) == hipSuccess;
+    ok = ok && hipMalloc((void**)&b->d_obs, sizeof(ObsRec) * (size_t)nf * obs_cap) == hipSuccess;
+    ok = ok && hipHostMalloc((void**)&b->h_obs, sizeof(ObsRec) * (size_t)nf * obs_cap, hipHostMallocDefault) ==
+                   hipSuccess;
+    ok = ok && hipHostMalloc((void**)&b->h_fdev, sizeof(FilterDev) * (size_t)nf, hipHostMallocDefault) == hipSuccess;
+    ok = ok && hipMemcpy(b->d_ratio, ratio.data(), sizeof(float) * (size_t)nf, hipMemcpyHostToDevice) == hipSuccess;
+    if (!ok) {
+        fprintf(stderr, "pf_hip pfb_create: device setup failed\n");
+        free_batch(b);
+        return nullptr;
+    }
+    memset(b->h_params, 0, b->params_bytes);
+    b->dp = carve(b->d_params, nf);
+    b->hp = carve(b->h_params, nf);
+    return b;
+}
+
+int pfb_destroy(void* h) {
+    if (h) free_batch(static_cast<Batch*>(h));
+    return 0;
+}
+
+int pfb_filter_count(void* h) { return h ? static_cast<Batch*>(h)->nf : -1; }
+
+int pfb_upload_landmarks(void* h, int f, const float* xyz, int count) {
+    Batch* b = static_cast<Batch*>(h);
+    if (!b || f < 0 || f >= b->nf || count < 0 || (count > 0 && !xyz)) return -1;
+    b->lm[f].assign(xyz, xyz + 3 * (size_t)count);
+    b->lm_cap[f] = count;
+    return 0;
+}
+
+int pfb_reinit(void* h, const float* pose7, const float* pos_std, const float* rot_std) {
+    Batch* b = static_cast<Batch*>(h);
+    if (!b || !pose7 || !pos_std || !rot_std) return -1;
+    for (int f = 0; f < b->nf; ++f) {
+        b->gseed[f] += pfd::kGenBump;  // cpu.rs:181
+        b->gen[f] += 1;
+        MotionParams& m = b->hp.motion[f];
+        memset(&m, 0, sizeof(m));
+        for (int k = 0; k < 3; ++k) {
+            m.t[k] = pose7[7 * f + k];
+            m.s_t[k] = pos_std[3 * f + k];
+            m.s_r[k] = rot_std[3 * f + k];
+        }
+        for (int k = 0; k < 4; ++k) m.q[k] = pose7[7 * f + 3 + k];
+        m.gseed = b->gseed[f];
+        m.winv = (float)(1.0 / (double)b->cfg[f].n);
+    }
+    PB_CHECK(hipMemcpyAsync(b->d_params, b->h_params, b->params_bytes, hipMemcpyHostToDevice, b->s.stream));
+    if (launch_motion(&b->s, ParamRef<MotionParams>::array(b->dp.motion), 1) != 0) return -1;
+    PB_CHECK(hipStreamSynchronize(b->s.stream));
+    return 0;
+}
+
+int pfb_cycle(void* h, const pfb_cycle_in* in, pfb_estimate* out) {
+    Batch* b = static_cast<Batch*>(h);
+    if (!b || !in) return -1;
+    PfState* s = &b->s;
+    if (upload_inputs(b, in) != 0) return -1;
+    if (run_a(b) != 0) return -1;
+    PB_CHECK(hipMemcpyAsync(b->h_fdev, s->fdev, sizeof(FilterDev) * (size_t)b->nf, hipMemcpyDeviceToHost, s->stream));
+    PB_CHECK(hipStreamSynchronize(s->stream));
+    for (int f = 0; f < b->nf; ++f) {
+        const FilterDev& fd = b->h_fdev[f];
+        double q[4] = {1.0, 0.0, 0.0, 0.0};
+        double M[16];
+        int e = 3;
+        for (int r = 0; r < 4; ++r)
+            for (int c = r; c < 4; ++c) {
+                M[4 * r + c] = M[4 * c + r] = fd.sums[e];
+                ++e;
+            }
+        if (pfx_quat_mean(M, q) != 0) return -1;
+        CovParams& cp = b->hp.cov[f];
+        // mean position handed back at f32 precision (cuda.rs:291-294)
+        for (int k = 0; k < 3; ++k) cp.mp[k] = (double)(float)fd.sums[k];
+        for (int k = 0; k < 4; ++k) cp.mq[k] = (double)(float)q[k];
+        if (out) {
+            for (int k = 0; k < 3; ++k) out[f].pos[k] = fd.sums[k];
+            for (int k = 0; k < 4; ++k) out[f].quat[k] = q[k];
+            out[f].ess = fd.sums[13] > 0.0 ? 1.0 / fd.sums[13] : 0.0;
+            out[f].resampled = fd.do_resample;
+            out[f].valid = 1;
+        }
+    }
+    const size_t cov_off = reinterpret_cast<char*>(b->hp.cov) - b->h_params;
+    PB_CHECK(hipMemcpyAsync(b->d_params + cov_off, b->h_params + cov_off, sizeof(CovParams) * (size_t)b->nf,
+                            hipMemcpyHostToDevice, s->stream));
+    if (run_b(b) != 0) return -1;
+    PB_CHECK(hipMemcpyAsync(b->h_fdev, s->fdev, sizeof(FilterDev) * (size_t)b->nf, hipMemcpyDeviceToHost, s->stream));
+    PB_CHECK(hipStreamSynchronize(s->stream));
+    if (out) {
+        static const int idx[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+        for (int f = 0; f < b->nf; ++f)
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 3; ++c) {
+                    out[f].pos_cov[3 * r + c] = b->h_fdev[f].covs[idx[r][c]];
+                    out[f].att_cov[3 * r + c] = b->h_fdev[f].covs[6 + idx[r][c]];
+                }
+    }
+    return 0;
+}
+
+int pfb_use_graph(void* h, int enable) {
+    Batch* b = static_cast<Batch*>(h);
+    if (!b) return -1;
+    b->use_graph = enable != 0;
+    if (!b->use_graph) drop_graphs(b);
+    if (b->use_graph) b->s.prof_on = false;  // events are not recorded inside captured graphs
+    return 0;
+}
+
+int pfb_autotune(void* h, const pfb_cycle_in* in) {
+    Batch* b = static_cast<Batch*>(h);
+    if (!b || !in) return -1;
+    PfState* s = &b->s;
+    const bool prof = s->prof_on;
+    s->prof_on = false;
+    // parameters of a cycle (without advancing the filters' draw state afterwards)
+    std::vector<uint64_t> gs = b->gseed, gn = b->gen;
+    if (upload_inputs(b, in) != 0) return -1;
+    b->gseed = gs;
+    b->gen = gn;
+    hipEvent_t e0, e1;
+    PB_CHECK(hipEventCreate(&e0));
+    PB_CHECK(hipEventCreate(&e1));
+    const int cand[3] = {1, 2, 4};
+    float best = 1e30f;
+    int best_ppt = s->weight_ppt;
+    for (int c = 0; c < 3; ++c) {
+        s->weight_ppt = cand[c];
+        if (launch_weight(s, ParamRef<WeightParams>::array(b->dp.weight)) != 0) return -1;  // warm-up
+        PB_CHECK(hipEventRecord(e0, s->stream));
+        for (int r = 0; r < 3; ++r)
+            if (launch_weight(s, ParamRef<WeightParams>::array(b->dp.weight)) != 0) return -1;
+        PB_CHECK(hipEventRecord(e1, s->stream));
+        PB_CHECK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        PB_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best) {
+            best = ms;
+            best_ppt = cand[c];
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    s->weight_ppt = best_ppt;
+    s->prof_on = prof;
+    drop_graphs(b);
+    return best_ppt;
+}
+
+int pfb_get_state(void* h, int f, float* pos, float* quat, float* w) {
+    Batch* b = static_cast<Batch*>(h);
+    if (!b || f < 0 || f >= b->nf) return -1;
+    PfState* s = &b->s;
+    PB_CHECK(hipStreamSynchronize(s->stream));
+    int par = 0;
+    PB_CHECK(hipMemcpy(&par, fdev_field<int>(s, f, offsetof(FilterDev, parity)), sizeof(int), hipMemcpyDeviceToHost));
+    const int n = s->n[f], off = s->off[f];
+    std::vector<float> tmp((size_t)n);
+    for (int c = 0; c < 7; ++c) {
+        if ((c < 3 && !pos) || (c >= 3 && !quat)) continue;
+        PB_CHECK(hipMemcpy(tmp.data(), s->pose.s[par].c[c] + off, sizeof(float) * (size_t)n, hipMemcpyDeviceToHost));
+        if (c < 3)
+            for (int i = 0; i < n; ++i) pos[3 * i + c] = tmp[i];
+        else
+            for (int i = 0; i < n; ++i) quat[4 * i + (c - 3)] = tmp[i];
+    }
+    if (w) PB_CHECK(hipMemcpy(w, s->w + off, sizeof(float) * (size_t)n, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int pfb_get_ancestors(void* h, int f, int32_t* anc) {
+    Batch* b = static_cast<Batch*>(h);
+    if (!b || f < 0 || f >= b->nf || !anc) return -1;
+    PB_CHECK(hipStreamSynchronize(b->s.stream));
+    PB_CHECK(hipMemcpy(anc, b->s.anc + b->s.off[f], sizeof(int32_t) * (size_t)b->s.n[f], hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int pfb_profile_enable(void* h, int enable) {
+    Batch* b = static_cast<Batch*>(h);
+    if (!b) return -1;
+    b->s.prof_on = enable != 0 && !b->use_graph;
+    return 0;
+}
+
+int pfb_profile_read(void* h, double* ms, int* count) {
+    Batch* b = static_cast<Batch*>(h);
+    if (!b) return -1;
+    PfState* s = &b->s;
+    PB_CHECK(hipStreamSynchronize(s->stream));
+    for (int k = 0; k < PFX_STAGE_COUNT; ++k) {
+        if (ms) ms[k] = 0.0;
+        if (count) count[k] = 0;
+    }
+    for (auto& p : s->prof_pending) {
+        float t = 0.f;
+        if (hipEventElapsedTime(&t, p.a, p.b) == hipSuccess) {
+            if (ms) ms[p.stage] += t;
+            if (count) count[p.stage] += 1;
+        }
+        (void)hipEventDestroy(p.a);
+        (void)hipEventDestroy(p.b);
+    }
+    s->prof_pending.clear();
+    return 0;
+}
+
+}  // extern "C"

@@ -125,127 +125,185 @@ __device__ inline double block_max(double v, double* sh) {
     return fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
 }
 
-// eb^2 = acos(1 - omd)^2 as a power series in t = 2*omd: eb^2 = 2 sum t^n / (n^2 C(2n,n)).
-__device__ inline double acos_sq_series(double t) {
-    double p;
-    if (t <= 0.01) {
-        p = 1.0 / 33264.0;
-        p = __fma_rn(p, t, 1.0 / 6300.0);
-        p = __fma_rn(p, t, 1.0 / 1120.0);
-        p = __fma_rn(p, t, 1.0 / 180.0);
-        p = __fma_rn(p, t, 1.0 / 24.0);
-        p = __fma_rn(p, t, 0.5);
-    } else {
-        p = 1.0 / 389398464.0;
-        p = __fma_rn(p, t, 1.0 / 85357272.0);
-        p = __fma_rn(p, t, 1.0 / 18475600.0);
-        p = __fma_rn(p, t, 1.0 / 3938220.0);
-        p = __fma_rn(p, t, 1.0 / 823680.0);
-        p = __fma_rn(p, t, 1.0 / 168168.0);
-        p = __fma_rn(p, t, 1.0 / 33264.0);
-        p = __fma_rn(p, t, 1.0 / 6300.0);
-        p = __fma_rn(p, t, 1.0 / 1120.0);
-        p = __fma_rn(p, t, 1.0 / 180.0);
-        p = __fma_rn(p, t, 1.0 / 24.0);
-        p = __fma_rn(p, t, 0.5);
-    }
-    return 2.0 * t * p;
+// Observation records are read through the constant address space: the addresses are
+// wave-uniform, so they become scalar (s_load) reads that stay off the vector memory path.
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef const ObsRec __attribute__((address_space(4)))* ConstObsPtr;
+#else
+typedef const ObsRec* ConstObsPtr;  // host pass of the single-source compile: never executed
+#endif
+
+// Per-particle body map: world -> body rotation qrot(conj(q), .) as a 3x3 matrix (exact
+// for non-unit q as well), plus the position.
+struct BodyMap {
+    double px, py, pz;
+    double m00, m01, m02, m10, m11, m12, m20, m21, m22;
+};
+
+__device__ inline BodyMap body_map(const PoseSet& S, int gi) {
+    BodyMap b;
+    b.px = (double)S.c[0][gi];
+    b.py = (double)S.c[1][gi];
+    b.pz = (double)S.c[2][gi];
+    const double qw = (double)S.c[3][gi], qx = (double)S.c[4][gi], qy = (double)S.c[5][gi],
+                 qz = (double)S.c[6][gi];
+    const double a = 1.0 - 2.0 * (qx * qx + qy * qy + qz * qz);
+    b.m00 = a + 2.0 * qx * qx;
+    b.m11 = a + 2.0 * qy * qy;
+    b.m22 = a + 2.0 * qz * qz;
+    b.m01 = 2.0 * (qx * qy + qw * qz);
+    b.m10 = 2.0 * (qx * qy - qw * qz);
+    b.m02 = 2.0 * (qx * qz - qw * qy);
+    b.m20 = 2.0 * (qx * qz + qw * qy);
+    b.m12 = 2.0 * (qy * qz + qw * qx);
+    b.m21 = 2.0 * (qy * qz - qw * qx);
+    return b;
 }
 
-// Gated log-likelihood (cpu.rs:44-97, 220-248) + fused online-softmax partials.
+// acos(1 - u)^2 as a power series in u = 1 - dot: acos(1-u)^2 = sum_n 2^(n+1) u^n / (n^2 C(2n,n))
+// (from asin(x)^2 = 1/2 sum (2x)^(2n) / (n^2 C(2n,n)) with x^2 = u/2).  Six terms hold to
+// ~1e-18 relative for u <= 0.005, twelve for u <= 0.125.
+__device__ inline double acos_sq_u6(double u) {
+    double p = 128.0 / 33264.0;
+    p = __fma_rn(p, u, 64.0 / 6300.0);
+    p = __fma_rn(p, u, 32.0 / 1120.0);
+    p = __fma_rn(p, u, 16.0 / 180.0);
+    p = __fma_rn(p, u, 8.0 / 24.0);
+    p = __fma_rn(p, u, 2.0);
+    return u * p;
+}
+__device__ inline double acos_sq_u12(double u) {
+    double p = 8192.0 / 389398464.0;
+    p = __fma_rn(p, u, 4096.0 / 85357272.0);
+    p = __fma_rn(p, u, 2048.0 / 18475600.0);
+    p = __fma_rn(p, u, 1024.0 / 3938220.0);
+    p = __fma_rn(p, u, 512.0 / 823680.0);
+    p = __fma_rn(p, u, 256.0 / 168168.0);
+    p = __fma_rn(p, u, 128.0 / 33264.0);
+    p = __fma_rn(p, u, 64.0 / 6300.0);
+    p = __fma_rn(p, u, 32.0 / 1120.0);
+    p = __fma_rn(p, u, 16.0 / 180.0);
+    p = __fma_rn(p, u, 8.0 / 24.0);
+    p = __fma_rn(p, u, 2.0);
+    return u * p;
+}
+
+// One gated log-likelihood term (cpu.rs:69-96), accumulated into acc.
+//   The reference's gate is continuous: the gated constant -g^2/2 equals -G inv2s at the
+//   threshold G = g^2 s^2, so a gated term is -min(e^2, G) inv2s (G = +inf when gate = 0).
+//   Range-bearing: rn from an f32 rsqrt seed refined once in f64 (relative ~1e-14);
+//   u = 1 - dot = 1 + pv/rn in f64; eb^2 from the series in u.  With a gate narrower than
+//   u = 0.005 the series is evaluated at min(u, u_gate) and the min() does the gating.
+template <bool kFastBearing>
+__device__ inline void ll_term(const BodyMap& b, const ObsRec& o, const WeightParams& p, double& acc) {
+    const double dx = o.lx - b.px, dy = o.ly - b.py, dz = o.lz - b.pz;
+    const double bx = __fma_rn(b.m00, dx, __fma_rn(b.m01, dy, b.m02 * dz));
+    const double by = __fma_rn(b.m10, dx, __fma_rn(b.m11, dy, b.m12 * dz));
+    const double bz = __fma_rn(b.m20, dx, __fma_rn(b.m21, dy, b.m22 * dz));
+    if (o.mode == 0) {
+        const double ex = bx - o.vx, ey = by - o.vy, ez = bz - o.vz;
+        const double r2 = __fma_rn(ex, ex, __fma_rn(ey, ey, ez * ez));
+        acc = __fma_rn(-fmin(r2, p.cart_gate2), p.inv2rr, acc);
+        return;
+    }
+    const double r2 = fmax(__fma_rn(bx, bx, __fma_rn(by, by, bz * bz)), 1e-18);  // rn >= 1e-9
+    const double pv = __fma_rn(bx, o.vx, __fma_rn(by, o.vy, bz * o.vz));
+    const double y0 = (double)__frsqrt_rn((float)r2);
+    const double hy = 0.5 * r2 * y0;
+    const double y = __fma_rn(y0, __fma_rn(-hy, y0, 0.5), y0);
+    const double er = __fma_rn(r2, y, -o.range);
+    acc = __fma_rn(-fmin(er * er, p.rng_gate2), p.inv2rr, acc);
+    double u = fmax(__fma_rn(pv, y, 1.0), 0.0);
+    double eb2;
+    if (kFastBearing) {
+        eb2 = acos_sq_u6(fmin(u, p.u_gate));
+    } else {
+        u = fmin(u, 2.0);
+        if (u <= 0.005) {
+            eb2 = acos_sq_u6(u);
+        } else if (u <= 0.125) {
+            eb2 = acos_sq_u12(u);
+        } else {
+            const double eb = acos(1.0 - u);
+            eb2 = eb * eb;
+        }
+    }
+    acc = __fma_rn(-fmin(eb2, p.brg_gate2), p.inv2bb, acc);
+}
+
+// Gated log-likelihood (cpu.rs:44-97, 220-248), kPPT particles per lane, plus fused
+// online-softmax block partials and a deterministic last-block finalise per filter.
+template <int kPPT>
 __global__ __launch_bounds__(kTB) void k_weight(Layout L, PoseBufs B, FilterDev* F, double* __restrict__ ll,
                                                 double* __restrict__ part, ParamRef<WeightParams> A) {
     __shared__ double sh[4];
-    __shared__ int s_last;
+    constexpr int kSpan = kTB * kPPT;
     const int blk = blockIdx.x;
-    const int f = L.blk_filter[blk];
+    const int f = L.blk_filter[blk * kPPT];
     const int off = L.f_off[f];
     const int nf = L.f_n[f];
-    const int li = blk * kTB + (int)threadIdx.x - off;
-    const bool valid = li < nf;
-    const WeightParams& p = A.get(f);
-    const int gi = off + (valid ? li : 0);
+    const WeightParams p = A.get(f);
     const PoseSet S = B.s[F[f].parity];
-
-    double acc = 0.0;
-    if (valid) {
-        const double px = (double)S.c[0][gi], py = (double)S.c[1][gi], pz = (double)S.c[2][gi];
-        const double qw = (double)S.c[3][gi], qx = (double)S.c[4][gi], qy = (double)S.c[5][gi],
-                     qz = (double)S.c[6][gi];
-        // Linear map of qrot(conj(q), .): world -> body (exact for non-unit q as well).
-        const double a = 1.0 - 2.0 * (qx * qx + qy * qy + qz * qz);
-        const double m00 = a + 2.0 * qx * qx, m11 = a + 2.0 * qy * qy, m22 = a + 2.0 * qz * qz;
-        const double m01 = 2.0 * (qx * qy + qw * qz), m10 = 2.0 * (qx * qy - qw * qz);
-        const double m02 = 2.0 * (qx * qz - qw * qy), m20 = 2.0 * (qx * qz + qw * qy);
-        const double m12 = 2.0 * (qy * qz + qw * qx), m21 = 2.0 * (qy * qz - qw * qx);
-        const ObsRec* __restrict__ obs = p.obs;
-        const int nobs = p.nobs;
+    int li[kPPT];
+    bool valid[kPPT];
+    BodyMap bm[kPPT];
+    double acc[kPPT];
+#pragma unroll
+    for (int e = 0; e < kPPT; ++e) {
+        li[e] = blk * kSpan + e * kTB + (int)threadIdx.x - off;
+        valid[e] = li[e] < nf;
+        bm[e] = body_map(S, off + (valid[e] ? li[e] : 0));
+        acc[e] = 0.0;
+    }
+    ConstObsPtr obs = (ConstObsPtr)p.obs;
+    const int nobs = p.nobs;
+    if (p.u_gate <= 0.005) {
         for (int k = 0; k < nobs; ++k) {
             const ObsRec o = obs[k];
-            const double dx = o.lx - px, dy = o.ly - py, dz = o.lz - pz;
-            const double bx = __fma_rn(m00, dx, __fma_rn(m01, dy, m02 * dz));
-            const double by = __fma_rn(m10, dx, __fma_rn(m11, dy, m12 * dz));
-            const double bz = __fma_rn(m20, dx, __fma_rn(m21, dy, m22 * dz));
-            if (o.mode == 0) {
-                const double ex = bx - o.vx, ey = by - o.vy, ez = bz - o.vz;
-                const double r2 = __fma_rn(ex, ex, __fma_rn(ey, ey, ez * ez));
-                acc += (p.gate > 0.0 && r2 > p.cart_gate2) ? p.cart_gated : -r2 * p.inv2rr;
-            } else {
-                const double r2 = __fma_rn(bx, bx, __fma_rn(by, by, bz * bz));
-                const double pv = __fma_rn(bx, o.vx, __fma_rn(by, o.vy, bz * o.vz));
-                double rn, rinv;
-                if (r2 > 1e-18) {
-                    double y = (double)__frsqrt_rn((float)r2);
-                    y = y * __fma_rn(-0.5 * r2 * y, y, 1.5);
-                    rn = r2 * y;
-                    rinv = y;
-                } else {
-                    rn = 1e-9;
-                    rinv = 1e9;
-                }
-                const double er = rn - o.range;
-                const double er2 = er * er;
-                acc += (p.gate > 0.0 && er2 > p.rng_gate2) ? p.half_g2 : -er2 * p.inv2rr;
-                double omd = __fma_rn(pv, rinv, 1.0);
-                omd = fmin(fmax(omd, 0.0), 2.0);
-                const double t = 2.0 * omd;
-                double bterm;
-                if (p.gate > 0.0 && t > p.t_gate) {
-                    bterm = p.half_g2;
-                } else {
-                    double eb2;
-                    if (t <= 0.25) {
-                        eb2 = acos_sq_series(t);
-                    } else {
-                        const double eb = acos(1.0 - omd);
-                        eb2 = eb * eb;
-                    }
-                    bterm = (p.gate > 0.0 && eb2 > p.brg_gate2) ? p.half_g2 : -eb2 * p.inv2bb;
-                }
-                acc += bterm;
-            }
+#pragma unroll
+            for (int e = 0; e < kPPT; ++e) ll_term<true>(bm[e], o, p, acc[e]);
         }
-        ll[gi] = acc;
+    } else {
+        for (int k = 0; k < nobs; ++k) {
+            const ObsRec o = obs[k];
+#pragma unroll
+            for (int e = 0; e < kPPT; ++e) ll_term<false>(bm[e], o, p, acc[e]);
+        }
     }
-    // Online softmax partials of this block: m_b, sum exp(ll-m_b), sum exp(2(ll-m_b)).
-    const double mb = block_max(valid ? acc : -INFINITY, sh);
-    double e = 0.0;
-    if (valid && mb > -INFINITY) e = exp(acc - mb);
-    const double sb = block_sum(e, sh);
-    const double s2b = block_sum(e * e, sh);
+    double mloc = -INFINITY;
+#pragma unroll
+    for (int e = 0; e < kPPT; ++e)
+        if (valid[e]) {
+            ll[off + li[e]] = acc[e];
+            mloc = fmax(mloc, acc[e]);
+        }
+    // online softmax partials of this block: m_b, sum exp(ll-m_b), sum exp(2(ll-m_b))
+    const double mb = block_max(mloc, sh);
+    double es = 0.0, es2 = 0.0;
+#pragma unroll
+    for (int e = 0; e < kPPT; ++e)
+        if (valid[e] && mb > -INFINITY) {
+            const double x = exp(acc[e] - mb);
+            es += x;
+            es2 += x * x;
+        }
+    const double sb = block_sum(es, sh);
+    const double s2b = block_sum(es2, sh);
     if (threadIdx.x == 0) {
         part[3 * blk] = mb;
         part[3 * blk + 1] = sb;
         part[3 * blk + 2] = s2b;
-        __threadfence();
-        const unsigned nblk = (unsigned)(L.f_padn[f] / kTB);
-        const unsigned old = atomicAdd(&F[f].cnt_weight, 1u);
-        s_last = (old == nblk - 1);
     }
-    __syncthreads();
-    if (!s_last) return;
-    __threadfence();
-    const int b0 = off / kTB, nb = L.f_padn[f] / kTB;
+}
+
+// Per filter: M = max m_b, S = sum s_b e^(m_b - M), S2 = sum s2_b e^(2(m_b - M)) over the
+// filter's likelihood blocks, in a fixed order (one block per filter; the kernel boundary
+// orders it after every partial without device-scope fences).
+__global__ __launch_bounds__(kTB) void k_weight_fin(Layout L, FilterDev* F, const double* __restrict__ part,
+                                                    int span) {
+    __shared__ double sh[4];
+    const int f = blockIdx.x;
+    const int b0 = L.f_off[f] / span, nb = L.f_padn[f] / span;
     double m = -INFINITY;
     for (int b = (int)threadIdx.x; b < nb; b += kTB) m = fmax(m, part[3 * (b0 + b)]);
     const double M = block_max(m, sh);
@@ -267,195 +325,212 @@ __global__ __launch_bounds__(kTB) void k_weight(Layout L, PoseBufs B, FilterDev*
         fd.sum_e = Ssum;
         fd.sum_e2 = S2sum;
         fd.uniform = ok ? 0 : 1;
-        fd.ess = ok ? (Ssum * Ssum) / S2sum : (double)nf;
-        fd.cnt_weight = 0;
     }
 }
 
-// w = exp(ll - max) / sum (or 1/n when the weight pass was degenerate / had no observations).
+// w = exp(ll - max) / sum (or 1/n when the weight pass was degenerate / had no observations),
+// plus block partials of sum w^2 over the stored f32 weights.
 __global__ __launch_bounds__(kTB) void k_normalize(Layout L, FilterDev* F, const double* __restrict__ ll,
-                                                   float* __restrict__ w, int force_uniform) {
+                                                   float* __restrict__ w, double* __restrict__ part, int force_uniform) {
+    __shared__ double sh[4];
     const int blk = blockIdx.x;
     const int f = L.blk_filter[blk];
     const int off = L.f_off[f];
+    const int nf = L.f_n[f];
     const int li = blk * kTB + (int)threadIdx.x - off;
-    if (li >= L.f_n[f]) return;
     const FilterDev& fd = F[f];
-    const float winv = L.f_winv[f];
-    if (force_uniform || fd.uniform) {
-        w[off + li] = winv;
-    } else {
-        w[off + li] = (float)(exp(ll[off + li] - fd.max_ll) / fd.sum_e);
+    const bool uni = force_uniform || fd.uniform;
+    double w2 = 0.0;
+    if (li < nf) {
+        const float wf = uni ? L.f_winv[f] : (float)(exp(ll[off + li] - fd.max_ll) / fd.sum_e);
+        w[off + li] = wf;
+        w2 = (double)wf * (double)wf;
+    }
+    const double bs = block_sum(w2, sh);
+    if (threadIdx.x == 0) part[blk] = bs;
+}
+
+// Per filter: ESS = 1 / sum w^2 of the stored weights -- the quantity the reference driver
+// tests (particle_filter.rs:115-117 via pfc_est_sums out[13]) -- and, when ratio is given,
+// the resample predicate ESS < ratio * n.
+__global__ __launch_bounds__(kTB) void k_norm_fin(Layout L, FilterDev* F, const double* __restrict__ part,
+                                                  const float* __restrict__ ratio) {
+    __shared__ double sh[4];
+    const int f = blockIdx.x;
+    const int b0 = L.f_off[f] / kTB, nb = L.f_padn[f] / kTB;
+    double v = 0.0;
+    for (int b = (int)threadIdx.x; b < nb; b += kTB) v += part[b0 + b];
+    const double tot = block_sum(v, sh);
+    if (threadIdx.x == 0) {
+        FilterDev& fd = F[f];
+        fd.sum_w2 = tot;
+        fd.ess = tot > 0.0 ? 1.0 / tot : 0.0;
+        if (ratio) {
+            const double r = (double)ratio[f];
+            fd.do_resample = (r > 0.0 && fd.ess < r * (double)L.f_n[f]) ? 1 : 0;
+        }
     }
 }
 
 // --------------------------------------------------------------------------- resample
 
-constexpr uint64_t kFlagAgg = 1ull << 62, kFlagIncl = 2ull << 62, kValMask = (1ull << 62) - 1;
-
-// Inclusive scan of fixed-point weights, one 1024-particle tile per block (256 x 4),
-// decoupled look-back within each filter, tiles taken in ticket order.
-__global__ __launch_bounds__(kTB) void k_scan(Layout L, FilterDev* F, const float* __restrict__ w,
-                                              uint64_t* __restrict__ cum, uint64_t* __restrict__ status,
-                                              unsigned* __restrict__ ctr, int ntiles) {
-    __shared__ int s_tile;
-    __shared__ uint64_t s_wsum[4];
-    __shared__ uint64_t s_prefix;
-    if (threadIdx.x == 0) s_tile = (int)atomicAdd(&ctr[0], 1u);
-    __syncthreads();
-    const int tile = s_tile;
-    const bool in_range = tile < ntiles;  // always true while the ticket counter is consistent
-    const int f = in_range ? L.blk_filter[tile * (kTile / kTB)] : 0;
-    const int off = L.f_off[f];
-    const int nf = L.f_n[f];
-    const int ftile0 = off / kTile;
-    const bool active = in_range && F[f].do_resample != 0;
-    if (active) {
-        const int base = tile * kTile + (int)threadIdx.x * 4;
-        const int li0 = base - off;
-        const float4 wv = *reinterpret_cast<const float4*>(w + base);
-        uint64_t q[4];
-        q[0] = (li0 + 0 < nf) ? qweight(wv.x) : 0;
-        q[1] = (li0 + 1 < nf) ? qweight(wv.y) : 0;
-        q[2] = (li0 + 2 < nf) ? qweight(wv.z) : 0;
-        q[3] = (li0 + 3 < nf) ? qweight(wv.w) : 0;
-        q[1] += q[0];
-        q[2] += q[1];
-        q[3] += q[2];
-        const uint64_t tsum = q[3];
-        // wave inclusive scan of thread sums
-        const int lane = threadIdx.x & 63, wv_id = threadIdx.x >> 6;
-        uint64_t x = tsum;
+// Block-wide inclusive scan of one u64 per thread (4 waves); returns the block total.
+__device__ inline uint64_t block_scan_u64(uint64_t x, uint64_t& incl, uint64_t* s_w) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            const uint64_t y = __shfl_up(x, d, 64);
-            if (lane >= d) x += y;
-        }
-        if (lane == 63) s_wsum[wv_id] = x;
-        __syncthreads();
-        uint64_t wpre = 0;
-        for (int k = 0; k < wv_id; ++k) wpre += s_wsum[k];
-        const uint64_t texcl = wpre + x - tsum;
-        const uint64_t agg = s_wsum[0] + s_wsum[1] + s_wsum[2] + s_wsum[3];
-        if (threadIdx.x == 0) {
-            uint64_t prefix = 0;
-            if (tile == ftile0) {
-                __hip_atomic_store(&status[tile], kFlagIncl | (agg & kValMask), __ATOMIC_RELEASE,
-                                   __HIP_MEMORY_SCOPE_AGENT);
-            } else {
-                __hip_atomic_store(&status[tile], kFlagAgg | (agg & kValMask), __ATOMIC_RELEASE,
-                                   __HIP_MEMORY_SCOPE_AGENT);
-                int j = tile - 1;
-                unsigned spins = 0;
-                while (j >= ftile0) {
-                    const uint64_t st = __hip_atomic_load(&status[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-                    const uint64_t fl = st & ~kValMask;
-                    if (fl == 0) {
-                        // predecessors hold earlier tickets and always publish; the cap only
-                        // turns an inconsistent state into an error instead of a hang
-                        if (++spins > (1u << 24)) {
-                            ctr[2] = 1;
-                            break;
-                        }
-                        __builtin_amdgcn_s_sleep(1);
-                        continue;
-                    }
-                    prefix += st & kValMask;
-                    if (fl == kFlagIncl) break;
-                    --j;
-                }
-                __hip_atomic_store(&status[tile], kFlagIncl | ((prefix + agg) & kValMask), __ATOMIC_RELEASE,
-                                   __HIP_MEMORY_SCOPE_AGENT);
-            }
-            s_prefix = prefix;
-        }
-        __syncthreads();
-        const uint64_t pre = s_prefix + texcl;
-        uint64_t* cp = cum + base;
-        cp[0] = pre + q[0];
-        cp[1] = pre + q[1];
-        cp[2] = pre + q[2];
-        cp[3] = pre + q[3];
-        // the tile holding the filter's last particle publishes the total and advances the LCG
-        const int last = nf - 1;
-        if (li0 <= last && last < li0 + 4) {
-            const uint64_t T = pre + q[last - li0];
-            FilterDev& fd = F[f];
-            fd.total = T;
-            if (T > 0) {
-                const uint64_t lcg = fd.lcg * 6364136223846793005ull + 1442695040888963407ull;  // cu:653
-                fd.lcg = lcg;
-                fd.u32 = (uint32_t)(lcg >> 32);
-            }
-        }
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint64_t y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
     }
-    // self-cleaning: the last tile to finish resets statuses and tickets for the next launch
     __syncthreads();
+    if (lane == 63) s_w[wv] = x;
+    __syncthreads();
+    uint64_t pre = 0;
+    for (int k = 0; k < wv; ++k) pre += s_w[k];
+    incl = pre + x;
+    return s_w[0] + s_w[1] + s_w[2] + s_w[3];
+}
+
+// Fixed-point weights of the 4 particles of this lane in a 1024-particle tile.
+__device__ inline void tile_q(const float* __restrict__ w, int tile, int off, int nf, uint64_t (&q)[4]) {
+    const int base = tile * kTile + (int)threadIdx.x * 4;
+    const int li0 = base - off;
+    const float4 wv = *reinterpret_cast<const float4*>(w + base);
+    q[0] = (li0 + 0 < nf) ? qweight(wv.x) : 0;
+    q[1] = (li0 + 1 < nf) ? qweight(wv.y) : 0;
+    q[2] = (li0 + 2 < nf) ? qweight(wv.z) : 0;
+    q[3] = (li0 + 3 < nf) ? qweight(wv.w) : 0;
+}
+
+// Reduce-then-scan of the fixed-point weights (integer sums: any association gives the same
+// bits).  Pass 1: one aggregate per 1024-particle tile.
+__global__ __launch_bounds__(kTB) void k_scan_tiles(Layout L, FilterDev* F, const float* __restrict__ w,
+                                                    uint64_t* __restrict__ tile_agg) {
+    __shared__ uint64_t s_w[4];
+    const int tile = blockIdx.x;
+    const int f = L.blk_filter[tile * (kTile / kTB)];
+    if (!F[f].do_resample) return;
+    uint64_t q[4];
+    tile_q(w, tile, L.f_off[f], L.f_n[f], q);
+    uint64_t incl;
+    const uint64_t tot = block_scan_u64(q[0] + q[1] + q[2] + q[3], incl, s_w);
+    if (threadIdx.x == 0) tile_agg[tile] = tot;
+}
+
+// Pass 2 (one block per filter): exclusive prefix of the filter's tile aggregates, the
+// total T and the advanced LCG (pf_kernels.cu:647-657: only when T > 0).
+__global__ __launch_bounds__(kTB) void k_scan_top(Layout L, FilterDev* F, const uint64_t* __restrict__ tile_agg,
+                                                  uint64_t* __restrict__ tile_pre) {
+    __shared__ uint64_t s_w[4];
+    const int f = blockIdx.x;
+    FilterDev& fd = F[f];
+    if (!fd.do_resample) return;
+    const int t0 = L.f_off[f] / kTile, nt = L.f_padn[f] / kTile;
+    uint64_t carry = 0;
+    for (int c = 0; c < nt; c += kTB) {
+        const int t = c + (int)threadIdx.x;
+        const uint64_t a = t < nt ? tile_agg[t0 + t] : 0;
+        uint64_t incl;
+        const uint64_t tot = block_scan_u64(a, incl, s_w);
+        if (t < nt) tile_pre[t0 + t] = carry + incl - a;
+        carry += tot;
+    }
     if (threadIdx.x == 0) {
-        __threadfence();
-        const unsigned done = atomicAdd(&ctr[1], 1u);
-        s_tile = (done == (unsigned)ntiles - 1) ? -1 : 0;
-    }
-    __syncthreads();
-    if (s_tile == -1) {
-        __threadfence();
-        for (int t = (int)threadIdx.x; t < ntiles; t += kTB) status[t] = 0;
-        if (threadIdx.x == 0) {
-            ctr[0] = 0;
-            ctr[1] = 0;
+        fd.total = carry;
+        if (carry > 0) {
+            const uint64_t lcg = fd.lcg * 6364136223846793005ull + 1442695040888963407ull;  // cu:653
+            fd.lcg = lcg;
+            fd.u32 = (uint32_t)(lcg >> 32);
         }
     }
 }
 
-// Systematic pick: ancestor(i) = first j with cum_j * n 2^32 >= T (U32 + i 2^32); gather poses
-// from the current set into the other set, weights -> 1/n, last block flips the parity.
+// Pass 3: inclusive cumulative fixed-point weight of every particle.
+__global__ __launch_bounds__(kTB) void k_scan_apply(Layout L, FilterDev* F, const float* __restrict__ w,
+                                                    const uint64_t* __restrict__ tile_pre, uint64_t* __restrict__ cum) {
+    __shared__ uint64_t s_w[4];
+    const int tile = blockIdx.x;
+    const int f = L.blk_filter[tile * (kTile / kTB)];
+    if (!F[f].do_resample) return;
+    uint64_t q[4];
+    tile_q(w, tile, L.f_off[f], L.f_n[f], q);
+    q[1] += q[0];
+    q[2] += q[1];
+    q[3] += q[2];
+    uint64_t incl;
+    (void)block_scan_u64(q[3], incl, s_w);
+    const uint64_t pre = tile_pre[tile] + incl - q[3];
+    uint64_t* cp = cum + tile * kTile + (int)threadIdx.x * 4;
+    cp[0] = pre + q[0];
+    cp[1] = pre + q[1];
+    cp[2] = pre + q[2];
+    cp[3] = pre + q[3];
+}
+
+// Systematic pick: ancestor(i) = first j with cum_j * n 2^32 >= T (U32 + i 2^32) -- the
+// pointer t_i = (U + i) T / n of cpu.rs:276-285 compared exactly in 128 bits.
+__device__ inline int first_at_or_above(const uint64_t* c, int lo, int hi, uint64_t A, uint64_t T, uint64_t Bi) {
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (mul128_lt(c[mid], A, T, Bi))
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+constexpr int kGatherLds = 2048;  // cumulative weights staged per block (16 KB)
+
+// Merge-path gather: each block owns 256 consecutive outputs; two global searches bound
+// their ancestors to [jlo, jhi], that slice of the scan is staged in LDS and every lane
+// searches it locally.  Poses move from the current set to the other set (no copies);
+// weights -> 1/n; k_flip makes the other set current after the injection.
 __global__ __launch_bounds__(kTB) void k_gather(Layout L, FilterDev* F, PoseBufs B, const uint64_t* __restrict__ cum,
                                                 float* __restrict__ w, int32_t* __restrict__ anc) {
-    __shared__ int s_last;
+    __shared__ int s_lo, s_hi;
+    __shared__ uint64_t s_cum[kGatherLds];
     const int blk = blockIdx.x;
     const int f = L.blk_filter[blk];
-    FilterDev& fd = F[f];
+    const FilterDev& fd = F[f];
     if (!fd.do_resample) return;
     const int off = L.f_off[f];
     const int nf = L.f_n[f];
-    const int li = blk * kTB + (int)threadIdx.x - off;
+    const int i0 = blk * kTB - off;
+    const int li = i0 + (int)threadIdx.x;
     const uint64_t T = fd.total;
     const int par = fd.parity;
-    if (li < nf) {
-        if (T == 0) {
-            w[off + li] = L.f_winv[f];  // cu:647-650: degenerate total -> uniform weights only
-        } else {
-            const uint64_t A = (uint64_t)nf << 32;
-            const uint64_t Bi = (uint64_t)fd.u32 + ((uint64_t)li << 32);
-            const uint64_t* c = cum + off;
-            int lo = 0, hi = nf - 1;
-            while (lo < hi) {
-                const int mid = (lo + hi) >> 1;
-                if (mul128_lt(c[mid], A, T, Bi))
-                    lo = mid + 1;
-                else
-                    hi = mid;
-            }
+    if (T == 0) {
+        if (li < nf) w[off + li] = L.f_winv[f];  // cu:647-650: degenerate total -> uniform weights only
+        return;
+    }
+    const uint64_t A = (uint64_t)nf << 32;
+    const uint64_t U = (uint64_t)fd.u32;
+    const uint64_t* c = cum + off;
+    if (i0 < nf) {
+        const int i1 = min(i0 + kTB - 1, nf - 1);
+        if (threadIdx.x == 0) s_lo = first_at_or_above(c, 0, nf - 1, A, T, U + ((uint64_t)i0 << 32));
+        if (threadIdx.x == 64) s_hi = first_at_or_above(c, 0, nf - 1, A, T, U + ((uint64_t)i1 << 32));
+        __syncthreads();
+        const int jlo = s_lo, jhi = s_hi;
+        const int span = jhi - jlo + 1;
+        const bool staged = span <= kGatherLds;
+        if (staged)
+            for (int t = (int)threadIdx.x; t < span; t += kTB) s_cum[t] = c[jlo + t];
+        __syncthreads();
+        if (li < nf) {
+            const uint64_t Bi = U + ((uint64_t)li << 32);
+            int j;
+            if (staged)
+                j = jlo + first_at_or_above(s_cum, 0, span - 1, A, T, Bi);
+            else
+                j = first_at_or_above(c, jlo, jhi, A, T, Bi);
             const PoseSet S = B.s[par], D = B.s[par ^ 1];
 #pragma unroll
-            for (int k = 0; k < 7; ++k) D.c[k][off + li] = S.c[k][off + lo];
-            anc[off + li] = lo;
+            for (int k = 0; k < 7; ++k) D.c[k][off + li] = S.c[k][off + j];
+            anc[off + li] = j;
             w[off + li] = L.f_winv[f];
         }
-    }
-    if (T == 0) return;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        const unsigned nblk = (unsigned)(L.f_padn[f] / kTB);
-        const unsigned old = atomicAdd(&fd.cnt_gather, 1u);
-        s_last = (old == nblk - 1);
-    }
-    __syncthreads();
-    if (s_last && threadIdx.x == 0) {
-        fd.parity = par ^ 1;
-        fd.cnt_gather = 0;
-        __threadfence();
     }
 }
 
@@ -513,7 +588,7 @@ __global__ __launch_bounds__(kTB) void k_inj_apply(Layout L, FilterDev* F, PoseB
     if (!fd.do_resample || fd.total == 0 || j >= p.k) return;
     const int t = idx[off + j];
     if (head[off + t] != j) return;
-    const PoseSet S = B.s[fd.parity];
+    const PoseSet S = B.s[fd.parity ^ 1];  // the gathered set (k_flip runs afterwards)
     double ax = (double)S.c[0][off + t], ay = (double)S.c[1][off + t], az = (double)S.c[2][off + t];
     // ascending traversal of the (unordered) list: repeatedly take the smallest pick > prev
     int prev = -1;
@@ -532,40 +607,45 @@ __global__ __launch_bounds__(kTB) void k_inj_apply(Layout L, FilterDev* F, PoseB
     S.c[2][off + t] = (float)az;
 }
 
+// The gathered set becomes current for filters that resampled with a positive total.
+__global__ void k_flip(FilterDev* F, int nfilt) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f < nfilt && F[f].do_resample && F[f].total > 0) F[f].parity ^= 1;
+}
+
 // ---------------------------------------------------------------------------- estimate
 
-// Writes the block partial of NV values and, in the filter's last block, their
-// deterministic sum to out (fixed thread->block assignment + fixed tree order).
+// Block partial of NV values: wave butterflies and one LDS exchange.
 template <int NV>
-__device__ inline void reduce_partials(double (&acc)[NV], double* __restrict__ part, int blk, int b0, int nb,
-                                       unsigned* cnt, double* out, double* sh) {
-    __shared__ int s_last;
-    double tot[NV];
+__device__ inline void block_partials(double (&acc)[NV], double* __restrict__ part, int blk) {
+    __shared__ double shv[4][NV];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
-    for (int k = 0; k < NV; ++k) tot[k] = block_sum(acc[k], sh);
-    if (threadIdx.x == 0) {
+    for (int k = 0; k < NV; ++k) acc[k] = wave_sum(acc[k]);
+    if (lane == 0)
 #pragma unroll
-        for (int k = 0; k < NV; ++k) part[(size_t)NV * blk + k] = tot[k];
-        __threadfence();
-        const unsigned old = atomicAdd(cnt, 1u);
-        s_last = (old == (unsigned)nb - 1);
-    }
+        for (int k = 0; k < NV; ++k) shv[wv][k] = acc[k];
     __syncthreads();
-    if (!s_last) return;
-    __threadfence();
-    double v[NV];
-#pragma unroll
-    for (int k = 0; k < NV; ++k) v[k] = 0.0;
-    for (int b = (int)threadIdx.x; b < nb; b += kTB)
-#pragma unroll
-        for (int k = 0; k < NV; ++k) v[k] += part[(size_t)NV * (b0 + b) + k];
-#pragma unroll
-    for (int k = 0; k < NV; ++k) tot[k] = block_sum(v[k], sh);
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int k = 0; k < NV; ++k) out[k] = tot[k];
-        *cnt = 0;
+    if ((int)threadIdx.x < NV) {
+        const int k = threadIdx.x;
+        part[(size_t)NV * blk + k] = (shv[0][k] + shv[1][k]) + (shv[2][k] + shv[3][k]);
     }
+}
+
+// Per filter (one block each): 16 lanes per value sum the filter's block partials in a
+// fixed strided order, then a fixed 16-lane tree.  which = 0 -> sums, 1 -> covariances.
+template <int NV>
+__global__ __launch_bounds__(kTB) void k_est_fin(Layout L, FilterDev* F, const double* __restrict__ part, int which) {
+    static_assert(NV * 16 <= kTB, "one 16-lane group per value");
+    const int f = blockIdx.x;
+    const int b0 = L.f_off[f] / kTile, nb = L.f_padn[f] / kTile;
+    const int k = threadIdx.x >> 4, sub = threadIdx.x & 15;
+    double v = 0.0;
+    if (k < NV)
+        for (int b = sub; b < nb; b += 16) v += part[(size_t)NV * (b0 + b) + k];
+#pragma unroll
+    for (int d = 8; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
+    if (k < NV && sub == 0) (which == 0 ? F[f].sums : F[f].covs)[k] = v;
 }
 
 // est layout (cu:354-404): [0..2] sum w p, [3..12] sum w q_r q_c (r<=c), [13] sum w^2.
@@ -611,7 +691,7 @@ __global__ __launch_bounds__(kTB) void k_est_sums(Layout L, FilterDev* F, PoseBu
             }
         }
     }
-    reduce_partials<kEstN>(acc, part, blk, off / kTile, L.f_padn[f] / kTile, &F[f].cnt_sums, F[f].sums, sh);
+    block_partials<kEstN>(acc, part, blk);
 }
 
 // cov layout (cu:406-471): [0..5] position, [6..11] attitude tangent (axis-angle of qm^-1 q,
@@ -682,7 +762,7 @@ __global__ __launch_bounds__(kTB) void k_est_covs(Layout L, FilterDev* F, PoseBu
             }
         }
     }
-    reduce_partials<kCovN>(acc, part, blk, off / kTile, L.f_padn[f] / kTile, &F[f].cnt_covs, F[f].covs, sh);
+    block_partials<kCovN>(acc, part, blk);
 }
 
 // Sets the per-filter resample predicate: force (ABI call) or ESS < ratio * n (fused cycle).
@@ -737,12 +817,11 @@ int state_alloc(PfState* s, const std::vector<int>& ns) {
     PF_CHECK(hipMalloc((void**)&s->inj_jit, sizeof(double) * 3 * (size_t)NP));
     const int nb256 = NP / kTB, ntile = NP / kTile;
     PF_CHECK(hipMalloc((void**)&s->part_w, sizeof(double) * 3 * (size_t)nb256));
+    PF_CHECK(hipMalloc((void**)&s->part_n, sizeof(double) * (size_t)nb256));
     PF_CHECK(hipMalloc((void**)&s->part_est, sizeof(double) * kEstN * (size_t)ntile));
     PF_CHECK(hipMalloc((void**)&s->part_cov, sizeof(double) * kCovN * (size_t)ntile));
-    PF_CHECK(hipMalloc((void**)&s->scan_status, sizeof(uint64_t) * (size_t)ntile));
-    PF_CHECK(hipMemsetAsync(s->scan_status, 0, sizeof(uint64_t) * (size_t)ntile, s->stream));
-    PF_CHECK(hipMalloc((void**)&s->scan_ctr, sizeof(unsigned) * 4));
-    PF_CHECK(hipMemsetAsync(s->scan_ctr, 0, sizeof(unsigned) * 4, s->stream));
+    PF_CHECK(hipMalloc((void**)&s->tile_agg, sizeof(uint64_t) * (size_t)ntile));
+    PF_CHECK(hipMalloc((void**)&s->tile_pre, sizeof(uint64_t) * (size_t)ntile));
     // layout tables
     std::vector<int> blk_filter(nb256);
     for (int f = 0; f < nf; ++f)
@@ -785,8 +864,8 @@ void state_free_particles(PfState* s) {
         }
     void** ptrs[] = {(void**)&s->w,        (void**)&s->ll,          (void**)&s->cum,       (void**)&s->anc,
                      (void**)&s->inj_idx,  (void**)&s->inj_next,    (void**)&s->inj_head,  (void**)&s->inj_jit,
-                     (void**)&s->part_w,   (void**)&s->part_est,    (void**)&s->part_cov,  (void**)&s->scan_status,
-                     (void**)&s->scan_ctr, (void**)&s->d_blk_filter, (void**)&s->d_fn,     (void**)&s->d_foff,
+                     (void**)&s->part_w,   (void**)&s->part_n,  (void**)&s->part_est,    (void**)&s->part_cov,  (void**)&s->tile_agg,
+                     (void**)&s->tile_pre, (void**)&s->d_blk_filter, (void**)&s->d_fn,     (void**)&s->d_foff,
                      (void**)&s->d_fpadn,  (void**)&s->d_fwinv,     (void**)&s->fdev};
     for (void** p : ptrs) {
         (void)hipFree(*p);
@@ -820,17 +899,33 @@ int launch_motion(PfState* s, ParamRef<MotionParams> A, int reinit) {
 
 int launch_weight(PfState* s, ParamRef<WeightParams> A) {
     prof_begin(s, PFX_STAGE_WEIGHT);
-    hipLaunchKernelGGL(k_weight, dim3(s->NP / kTB), dim3(kTB), 0, s->stream, s->L, s->pose, s->fdev, s->ll, s->part_w,
-                       A);
+    const int ppt = s->weight_ppt;
+    if (ppt == 1)
+        hipLaunchKernelGGL(k_weight<1>, dim3(s->NP / kTB), dim3(kTB), 0, s->stream, s->L, s->pose, s->fdev, s->ll,
+                           s->part_w, A);
+    else if (ppt == 4)
+        hipLaunchKernelGGL(k_weight<4>, dim3(s->NP / (kTB * 4)), dim3(kTB), 0, s->stream, s->L, s->pose, s->fdev,
+                           s->ll, s->part_w, A);
+    else
+        hipLaunchKernelGGL(k_weight<2>, dim3(s->NP / (kTB * 2)), dim3(kTB), 0, s->stream, s->L, s->pose, s->fdev,
+                           s->ll, s->part_w, A);
+    prof_end(s);
+    PF_LAUNCH_CHECK();
+    prof_begin(s, PFX_STAGE_FINALIZE);
+    hipLaunchKernelGGL(k_weight_fin, dim3(s->nf), dim3(kTB), 0, s->stream, s->L, s->fdev, s->part_w, kTB * ppt);
     prof_end(s);
     PF_LAUNCH_CHECK();
     return 0;
 }
 
-int launch_normalize(PfState* s, int force_uniform) {
+int launch_normalize(PfState* s, int force_uniform, const float* d_ratio) {
     prof_begin(s, PFX_STAGE_NORMALIZE);
     hipLaunchKernelGGL(k_normalize, dim3(s->NP / kTB), dim3(kTB), 0, s->stream, s->L, s->fdev, s->ll, s->w,
-                       force_uniform);
+                       s->part_n, force_uniform);
+    prof_end(s);
+    PF_LAUNCH_CHECK();
+    prof_begin(s, PFX_STAGE_FINALIZE);
+    hipLaunchKernelGGL(k_norm_fin, dim3(s->nf), dim3(kTB), 0, s->stream, s->L, s->fdev, s->part_n, d_ratio);
     prof_end(s);
     PF_LAUNCH_CHECK();
     return 0;
@@ -839,8 +934,9 @@ int launch_normalize(PfState* s, int force_uniform) {
 int launch_resample(PfState* s, ParamRef<InjectParams> inj, int any_inject) {
     const int ntile = s->NP / kTile;
     prof_begin(s, PFX_STAGE_RESAMPLE_SCAN);
-    hipLaunchKernelGGL(k_scan, dim3(ntile), dim3(kTB), 0, s->stream, s->L, s->fdev, s->w, s->cum, s->scan_status,
-                       s->scan_ctr, ntile);
+    hipLaunchKernelGGL(k_scan_tiles, dim3(ntile), dim3(kTB), 0, s->stream, s->L, s->fdev, s->w, s->tile_agg);
+    hipLaunchKernelGGL(k_scan_top, dim3(s->nf), dim3(kTB), 0, s->stream, s->L, s->fdev, s->tile_agg, s->tile_pre);
+    hipLaunchKernelGGL(k_scan_apply, dim3(ntile), dim3(kTB), 0, s->stream, s->L, s->fdev, s->w, s->tile_pre, s->cum);
     prof_end(s);
     PF_LAUNCH_CHECK();
     prof_begin(s, PFX_STAGE_RESAMPLE_GATHER);
@@ -858,6 +954,10 @@ int launch_resample(PfState* s, ParamRef<InjectParams> inj, int any_inject) {
         prof_end(s);
         PF_LAUNCH_CHECK();
     }
+    prof_begin(s, PFX_STAGE_FINALIZE);
+    hipLaunchKernelGGL(k_flip, dim3((s->nf + 63) / 64), dim3(64), 0, s->stream, s->fdev, s->nf);
+    prof_end(s);
+    PF_LAUNCH_CHECK();
     return 0;
 }
 
@@ -867,6 +967,10 @@ int launch_est_sums(PfState* s) {
                        s->part_est);
     prof_end(s);
     PF_LAUNCH_CHECK();
+    prof_begin(s, PFX_STAGE_FINALIZE);
+    hipLaunchKernelGGL(k_est_fin<kEstN>, dim3(s->nf), dim3(kTB), 0, s->stream, s->L, s->fdev, s->part_est, 0);
+    prof_end(s);
+    PF_LAUNCH_CHECK();
     return 0;
 }
 
@@ -874,6 +978,10 @@ int launch_est_covs(PfState* s, ParamRef<CovParams> A) {
     prof_begin(s, PFX_STAGE_EST_COVS);
     hipLaunchKernelGGL(k_est_covs, dim3(s->NP / kTile), dim3(kTB), 0, s->stream, s->L, s->fdev, s->pose, s->w,
                        s->part_cov, A);
+    prof_end(s);
+    PF_LAUNCH_CHECK();
+    prof_begin(s, PFX_STAGE_FINALIZE);
+    hipLaunchKernelGGL(k_est_fin<kCovN>, dim3(s->nf), dim3(kTB), 0, s->stream, s->L, s->fdev, s->part_cov, 1);
     prof_end(s);
     PF_LAUNCH_CHECK();
     return 0;
@@ -894,14 +1002,15 @@ WeightParams make_weight_params(double sigma_r, double sigma_b, double gate) {
     p.inv2rr = 1.0 / (2.0 * sr * sr);
     p.inv2bb = 1.0 / (2.0 * sb * sb);
     p.gate = gate;
-    p.cart_gate2 = g2 * 3.0 * sr * sr;
-    p.rng_gate2 = g2 * sr * sr;
-    p.brg_gate2 = g2 * sb * sb;
-    p.cart_gated = -0.5 * gate * gate * 3.0;
-    p.half_g2 = -0.5 * gate * gate;
-    // t = 2(1 - cos eb) beyond which the bearing term is certainly gated
+    // squared-residual thresholds (cpu.rs:63-65); +inf without a gate (cpu.rs:72, 85, 90)
+    const bool gated = gate > 0.0;
+    p.cart_gate2 = gated ? g2 * 3.0 * sr * sr : INFINITY;
+    p.rng_gate2 = gated ? g2 * sr * sr : INFINITY;
+    p.brg_gate2 = gated ? g2 * sb * sb : INFINITY;
+    // u = 1 - cos(eb) beyond which the bearing term is certainly gated (slightly above the
+    // exact threshold: the series stays monotone there and min() with brg_gate2 gates)
     const double gb = gate * sb;
-    p.t_gate = (gb >= M_PI) ? 1e300 : 2.0 * (1.0 - cos(gb)) * (1.0 + 1e-6) + 1e-300;
+    p.u_gate = (gated && gb < 0.5) ? (1.0 - cos(gb)) * (1.0 + 1e-6) + 1e-300 : INFINITY;
     return p;
 }
 
@@ -1040,18 +1149,18 @@ int pfc_weight(void* h, const void* obs, int nobs, float sigma_r, float sigma_b,
     PfState* s = static_cast<PfState*>(h);
     if (!s || s->nf != 1) return -1;
     nobs = nobs > s->obs_cap ? s->obs_cap : nobs;  // cu:604
-    if (nobs <= 0 || !obs) return launch_normalize(s, 1);
+    if (nobs <= 0 || !obs) return launch_normalize(s, 1, nullptr);
     // the staging buffer may still feed the previous call's copy
     PF_CHECK(hipEventSynchronize(s->obs_ev));
     const int m = join_obs(static_cast<const pfc_obs*>(obs), nobs, s->lm.data(), s->lm_cap, s->h_obs);
-    if (m == 0) return launch_normalize(s, 1);  // every id unknown: ll = 0 for all -> uniform
+    if (m == 0) return launch_normalize(s, 1, nullptr);  // every id unknown: ll = 0 for all -> uniform
     PF_CHECK(hipMemcpyAsync(s->d_obs, s->h_obs, sizeof(ObsRec) * (size_t)m, hipMemcpyHostToDevice, s->stream));
     PF_CHECK(hipEventRecord(s->obs_ev, s->stream));
     WeightParams p = make_weight_params(sigma_r, sigma_b, gate);
     p.obs = s->d_obs;
     p.nobs = m;
     if (launch_weight(s, ParamRef<WeightParams>::value(p)) != 0) return -1;
-    return launch_normalize(s, 0);
+    return launch_normalize(s, 0, nullptr);
 }
 
 int pfc_resample(void* h, float random_inject_ratio) {
@@ -1245,6 +1354,60 @@ int pfx_profile_read(void* h, double* ms, int* count) {
 int pfx_particle_count(void* h) {
     PfState* s = static_cast<PfState*>(h);
     return (s && s->nf == 1) ? s->n[0] : 0;
+}
+
+// Largest-eigenvalue unit eigenvector of (M + M^T)/2 by cyclic Jacobi rotations,
+// sign chosen so that w >= 0 (types.rs:139-142).
+int pfx_quat_mean(const double* M, double* q) {
+    if (!M || !q) return -1;
+    double a[4][4], v[4][4];
+    for (int r = 0; r < 4; ++r)
+        for (int c = 0; c < 4; ++c) {
+            a[r][c] = 0.5 * (M[4 * r + c] + M[4 * c + r]);
+            v[r][c] = r == c ? 1.0 : 0.0;
+        }
+    for (int sweep = 0; sweep < 50; ++sweep) {
+        double off = 0.0, diag = 0.0;
+        for (int r = 0; r < 4; ++r) {
+            diag += a[r][r] * a[r][r];
+            for (int c = r + 1; c < 4; ++c) off += a[r][c] * a[r][c];
+        }
+        if (off <= 1e-32 * diag || off == 0.0) break;
+        for (int p = 0; p < 3; ++p)
+            for (int r = p + 1; r < 4; ++r) {
+                const double apr = a[p][r];
+                if (apr == 0.0) continue;
+                // rotation angle zeroing a[p][r]: tan(2 phi) = 2 a_pr / (a_rr - a_pp)
+                const double tau = (a[r][r] - a[p][p]) / (2.0 * apr);
+                const double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+                const double c = 1.0 / sqrt(1.0 + t * t), s = t * c;
+                for (int k = 0; k < 4; ++k) {
+                    const double kp = a[k][p], kr = a[k][r];
+                    a[k][p] = c * kp - s * kr;
+                    a[k][r] = s * kp + c * kr;
+                }
+                for (int k = 0; k < 4; ++k) {
+                    const double pk = a[p][k], rk = a[r][k];
+                    a[p][k] = c * pk - s * rk;
+                    a[r][k] = s * pk + c * rk;
+                }
+                for (int k = 0; k < 4; ++k) {
+                    const double kp = v[k][p], kr = v[k][r];
+                    v[k][p] = c * kp - s * kr;
+                    v[k][r] = s * kp + c * kr;
+                }
+            }
+    }
+    int best = 0;
+    for (int i = 1; i < 4; ++i)
+        if (a[i][i] > a[best][best]) best = i;
+    double nrm = 0.0;
+    for (int r = 0; r < 4; ++r) nrm += v[r][best] * v[r][best];
+    nrm = sqrt(nrm);
+    if (!(nrm > 0.0)) return -1;
+    const double sg = v[0][best] < 0.0 ? -1.0 : 1.0;
+    for (int r = 0; r < 4; ++r) q[r] = sg * v[r][best] / nrm;
+    return 0;
 }
 
 int pfx_sync(void* h) {

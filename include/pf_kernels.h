@@ -129,7 +129,8 @@ enum {
     PFX_STAGE_INJECT = 6,
     PFX_STAGE_EST_SUMS = 7,
     PFX_STAGE_EST_COVS = 8,
-    PFX_STAGE_COUNT = 9
+    PFX_STAGE_FINALIZE = 9, /* one-block-per-filter reductions (weight, ESS, scan top, estimate) */
+    PFX_STAGE_COUNT = 10
 };
 
 /* Overwrites the particle set (pos 3n xyz, quat 4n wxyz, w n; w may be NULL = 1/n).
@@ -155,6 +156,10 @@ int pfx_profile_enable(void *h, int enable);
 int pfx_profile_read(void *h, double *ms, int *count);
 /* Number of particles of the handle (0 before pfc_init). */
 int pfx_particle_count(void *h);
+/* Host utility: Markley mean of a 4x4 moment matrix M (row-major; symmetrised), i.e. the
+ * unit eigenvector of the largest eigenvalue with w >= 0 (types.rs:128-144), by cyclic
+ * Jacobi rotations.  Shared by the host layer and the batched path. */
+int pfx_quat_mean(const double *M16, double *q4);
 /* Blocks until every queued operation on the handle's stream has finished. */
 int pfx_sync(void *h);
 

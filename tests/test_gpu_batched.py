@@ -1,0 +1,135 @@
+"""Batched filters (pfb_*): every filter of a batch reproduces the single-filter path.
+
+Filter f of a batch runs the same kernels on the same inputs as one pfc_* handle driven by
+the host ParticleFilter (particle_filter.rs:110-124, fast mode), so poses, weights and
+estimates must be bit-identical; the device-side resample predicate must make the same
+decisions.  Graph replay must reproduce eager launches exactly, over several replays with
+eager work and synchronisations in between.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from cuda_pf_localization_amd import OBS_DTYPE, BatchedFilters, Observation, OdomDelta, ParticleFilter, PfConfig
+from cuda_pf_localization_amd.sim import helix_scenario
+
+pytestmark = pytest.mark.gpu
+
+
+def scenarios(sizes, lms, cycles, seed):
+    out = []
+    for f, (n, nl) in enumerate(zip(sizes, lms)):
+        sc = helix_scenario(nl, (20.0, 20.0, 5.0), cycles, seed + f, phase=0.7 * f)
+        out.append(sc)
+    return out
+
+
+def batch_inputs(bf, scs, k):
+    per = []
+    for sc in scs:
+        st = sc.steps[k]
+        obs = np.array([(i, m, *v, r) for (i, m, v, r) in st.obs], OBS_DTYPE)
+        per.append((st.odom_t, st.odom_q, st.trans_noise, st.rot_noise, obs))
+    return bf.make_inputs(per)
+
+
+def make_batch(sizes, scs, inject=0.0):
+    cfgs = [dict(n=n, seed=11 + f, sigma_r=0.05, sigma_b=0.02, gate=3.0, ratio=0.5, inject=inject)
+            for f, n in enumerate(sizes)]
+    bf = BatchedFilters(cfgs, obs_cap=256)
+    for f, sc in enumerate(scs):
+        bf.upload_landmarks(f, np.array([sc.landmarks[i] for i in range(len(sc.landmarks))], np.float32))
+    poses = np.array([[*sc.prior_pos, *sc.prior_q] for sc in scs], np.float32)
+    bf.reinit(poses, np.full((len(sizes), 3), 0.5, np.float32), np.full((len(sizes), 3), 0.1, np.float32))
+    return bf
+
+
+@pytest.mark.parametrize("inject", [0.0, 0.05])
+def test_batched_equals_single_filters(gpu, inject):
+    sizes = [1000, 5000, 20_000, 777, 4096]
+    lms = [12, 40, 200, 4, 97]
+    cycles = 12
+    scs = scenarios(sizes, lms, cycles, 100)
+    bf = make_batch(sizes, scs, inject)
+    singles = []
+    for f, (n, sc) in enumerate(zip(sizes, scs)):
+        pf = ParticleFilter(PfConfig(particle_count=n, seed=11 + f, range_noise=0.05, bearing_noise=0.02,
+                                     gate_sigma=3.0, resample_ess_ratio=0.5, random_inject_ratio=inject, fast=True))
+        pf.set_map(sc.landmarks)
+        pf.reinitialize([*sc.prior_pos, *sc.prior_q], [0.5] * 3, [0.1] * 3)
+        singles.append(pf)
+    n_rs = 0
+    for k in range(cycles):
+        inp, keep = batch_inputs(bf, scs, k)
+        out = bf.cycle(inp)
+        for f, (pf, sc) in enumerate(zip(singles, scs)):
+            st = sc.steps[k]
+            f32 = lambda v: [float(np.float32(x)) for x in v]  # the batch ABI takes f32 odometry
+            e = pf.update(OdomDelta(f32(st.odom_t), f32(st.odom_q), f32(st.trans_noise), f32(st.rot_noise)),
+                          [Observation(i, m, v, r) for (i, m, v, r) in st.obs])
+            assert bool(out[f].resampled) == e.resampled, f"filter {f} cycle {k}"
+            n_rs += e.resampled
+            assert np.array_equal(np.array(out[f].pos[:]), e.position)
+            assert np.array_equal(np.array(out[f].quat[:]), e.attitude)
+            assert np.array_equal(np.array(out[f].pos_cov[:]).reshape(3, 3), e.pos_cov)
+            assert np.array_equal(np.array(out[f].att_cov[:]).reshape(3, 3), e.att_cov)
+    for f, pf in enumerate(singles):
+        p, q, w = bf.get_state(f)
+        sp, sq, sw = pf.snapshot()
+        assert np.array_equal(p.astype(np.float64), sp)
+    assert n_rs > 0
+
+
+def test_graph_replay_matches_eager(gpu):
+    sizes = [3000, 1024, 9000]
+    lms = [20, 8, 64]
+    cycles = 8
+    scs = scenarios(sizes, lms, cycles, 200)
+    eager = make_batch(sizes, scs, 0.02)
+    graph = make_batch(sizes, scs, 0.02)
+    graph.use_graph(True)
+    for k in range(cycles):
+        inp_e, ke = batch_inputs(eager, scs, k)
+        inp_g, kg = batch_inputs(graph, scs, k)
+        oe = eager.cycle(inp_e)
+        og = graph.cycle(inp_g)
+        for f in range(len(sizes)):
+            assert np.array_equal(np.array(oe[f].pos[:]), np.array(og[f].pos[:]))
+            assert np.array_equal(np.array(oe[f].att_cov[:]), np.array(og[f].att_cov[:]))
+            assert oe[f].resampled == og[f].resampled
+        # eager work + synchronisation between replays
+        for f in range(len(sizes)):
+            pe = eager.get_state(f)[0]
+            pg = graph.get_state(f)[0]
+            assert np.array_equal(pe, pg)
+
+
+def test_autotune_keeps_results(gpu):
+    sizes = [50_000, 2000]
+    lms = [128, 16]
+    scs = scenarios(sizes, lms, 3, 300)
+    a = make_batch(sizes, scs)
+    b = make_batch(sizes, scs)
+    inp, keep = batch_inputs(b, scs, 0)
+    ppt = b.autotune(inp)
+    assert ppt in (1, 2, 4)
+    for k in range(3):
+        ia, ka = batch_inputs(a, scs, k)
+        ib, kb = batch_inputs(b, scs, k)
+        oa = a.cycle(ia)
+        ob = b.cycle(ib)
+        for f in range(len(sizes)):
+            assert np.array_equal(np.array(oa[f].pos[:]), np.array(ob[f].pos[:]))
+
+
+def test_filters_without_observations_coast(gpu):
+    sizes = [2048, 1500]
+    bf = BatchedFilters([dict(n=n, seed=5, ratio=0.5) for n in sizes])
+    bf.reinit(np.array([[1, 2, 3, 1, 0, 0, 0], [0, 0, 0, 1, 0, 0, 0]], np.float32),
+              np.full((2, 3), 0.1, np.float32), np.full((2, 3), 0.01, np.float32))
+    for _ in range(20):
+        inp, keep = bf.make_inputs([((1.0, 0, 0), (1, 0, 0, 0), (0.01,) * 3, (0.005,) * 3, None)] * 2)
+        out = bf.cycle(inp)
+    assert abs(out[0].pos[0] - 21.0) < 0.5 and abs(out[1].pos[0] - 20.0) < 0.5
+    assert out[0].ess > 0.99 * sizes[0]
