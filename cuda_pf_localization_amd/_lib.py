@@ -51,6 +51,18 @@ class PfhGlParams(C.Structure):
                 ("seed", u64)]
 
 
+class PfhLearnedParams(C.Structure):
+    _fields_ = [("n_mix", C.c_int32), ("n_uniform", C.c_int32), ("fallback_hypotheses", C.c_int32),
+                ("fallback_caer_per_ray", C.c_double), ("seed", C.c_uint64), ("refine_rounds", C.c_int32),
+                ("refine_parents", C.c_int32), ("refine_children", C.c_int32), ("refine_sigma_xy", C.c_double),
+                ("refine_sigma_yaw", C.c_double), ("refine_shrink", C.c_double)]
+
+
+class PfhLearnedInfo(C.Structure):
+    _fields_ = [("used_fallback", C.c_int32), ("proposal_best_caer_per_ray", C.c_double),
+                ("best_caer_per_ray", C.c_double)]
+
+
 class PfbFilterCfg(C.Structure):
     _fields_ = [("n", i32), ("seed", u32), ("sigma_r", f32), ("sigma_b", f32), ("gate", f32),
                 ("resample_ess_ratio", f32), ("inject_ratio", f32)]
@@ -125,6 +137,28 @@ KERNEL_SYMBOLS = [
     ("pfb_get_ancestors", C.c_int, [vp, C.c_int, P(i32)]),
     ("pfb_profile_enable", C.c_int, [vp, C.c_int]),
     ("pfb_profile_read", C.c_int, [vp, P(f64), P(C.c_int)]),
+    # learned proposal
+    ("pfl_create", vp, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, u64]),
+    ("pfl_destroy", C.c_int, [vp]),
+    ("pfl_param_count", C.c_int64, [vp]),
+    ("pfl_get_params", C.c_int, [vp, P(f32)]),
+    ("pfl_set_params", C.c_int, [vp, P(f32)]),
+    ("pfl_forward", C.c_int, [vp, P(f32), C.c_int, P(f32)]),
+    ("pfl_grad", C.c_int, [vp, P(f32), P(f32), C.c_int, P(f32), P(f64)]),
+    ("pfl_train_step", C.c_int, [vp, P(f32), P(f32), C.c_int, f32, P(f64)]),
+    ("pfl_train_on_map", C.c_int, [vp, vp, C.c_int64, C.c_int, f32, C.c_int, f32, f32, f32, u64, P(f64)]),
+    ("pfl_propose", C.c_int, [vp, vp, P(f32), C.c_int, f32, C.c_int, C.c_int, u64]),
+    ("pfl_last_propose_ms", f64, [vp]),
+    ("pfglx_sample_poses_device", C.c_int, [vp, C.c_int, u64, vp, vp]),
+    ("pfglx_generate_range", C.c_int, [vp, C.c_int, C.c_int, u64]),
+    ("pfglx_hypotheses_device", vp, [vp]),
+    ("pfglx_map_info", C.c_int, [vp, P(f64)]),
+    ("pfglx_set_count", C.c_int, [vp, C.c_int]),
+    ("pfglx_stream", vp, [vp]),
+    ("pfglx_topk_device", C.c_int, [vp, C.c_int]),
+    ("pfglx_set_scan", C.c_int, [vp, P(f32), C.c_int, f32]),
+    ("pfglx_score_async", C.c_int, [vp, C.c_int, f32]),
+    ("pfglx_refine", C.c_int, [vp, C.c_int, C.c_int, f32, f32, u64]),
 ]
 
 HOST_SYMBOLS = [
@@ -142,6 +176,7 @@ HOST_SYMBOLS = [
     ("pfh_gl_localize", C.c_int, [vp, P(f32), C.c_int, P(f64)]),
     ("pfh_gl_set_seed", C.c_int, [vp, u64]),
     ("pfh_gl_kernels", vp, [vp]),
+    ("pfh_gl_localize_learned", C.c_int, [vp, vp, P(f32), C.c_int, vp, P(f64), vp]),
 ]
 
 OPTIONAL_KERNEL_SYMBOLS: list = []  # filled by modules that add entry points (batched, learned)

@@ -458,6 +458,120 @@ class GlobalLoc:
     def kernels_handle(self):
         return self.H.pfh_gl_kernels(self.g)
 
+    def localize_learned(self, proposal: "LearnedProposal", scan, lp: "LearnedParams"):
+        """Learned proposal with uniform fallback; returns (rows x, y, theta, caer; info dict)."""
+        s = np.ascontiguousarray(scan, np.float32)
+        out = np.zeros((self.params.topk, 4))
+        p = _lib.PfhLearnedParams(lp.n_mix, lp.n_uniform, lp.fallback_hypotheses, lp.fallback_caer_per_ray, lp.seed,
+                                  lp.refine_rounds, lp.refine_parents, lp.refine_children, lp.refine_sigma_xy,
+                                  lp.refine_sigma_yaw, lp.refine_shrink)
+        info = _lib.PfhLearnedInfo()
+        k = self.H.pfh_gl_localize_learned(self.g, proposal.h, _fp(s), s.size, C.byref(p), _dp(out), C.byref(info))
+        if k <= 0:
+            raise RuntimeError("localize_learned failed")
+        return out[:k], dict(used_fallback=bool(info.used_fallback),
+                             proposal_best_caer_per_ray=info.proposal_best_caer_per_ray,
+                             best_caer_per_ray=info.best_caer_per_ray)
+
+
+# ------------------------------------------------------------------ learned proposal
+
+
+@dataclass
+class LearnedParams:
+    """Learned-proposal localisation: mixture + uniform hypotheses, uniform fallback."""
+    n_mix: int = 8000
+    n_uniform: int = 2000
+    fallback_hypotheses: int = 200_000
+    fallback_caer_per_ray: float = 0.05
+    seed: int = 1
+    refine_rounds: int = 3
+    refine_parents: int = 32
+    refine_children: int = 64
+    refine_sigma_xy: float = 0.12
+    refine_sigma_yaw: float = 0.06
+    refine_shrink: float = 0.35
+
+
+class LearnedProposal:
+    """pfl_* handle: per-map MLP scan -> Gaussian mixture over (x, y, sin yaw, cos yaw)."""
+
+    def __init__(self, in_dim: int = 360, hid: int = 512, n_hidden: int = 3, ncomp: int = 16,
+                 max_batch: int = 4096, seed: int = 0):
+        self.L = _lib.kernels()
+        self.dims = (in_dim, hid, n_hidden, ncomp)
+        self.nout = 9 * ncomp
+        self.max_batch = max_batch
+        self.h = self.L.pfl_create(in_dim, hid, n_hidden, ncomp, max_batch, seed)
+        if not self.h:
+            raise RuntimeError("pfl_create failed")
+
+    def close(self):
+        if self.h:
+            self.L.pfl_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def param_count(self) -> int:
+        return int(self.L.pfl_param_count(self.h))
+
+    def get_params(self) -> np.ndarray:
+        p = np.zeros(self.param_count(), np.float32)
+        _check(self.L.pfl_get_params(self.h, _fp(p)), "pfl_get_params")
+        return p
+
+    def set_params(self, p):
+        a = np.ascontiguousarray(p, np.float32)
+        if a.size != self.param_count():
+            raise ValueError("parameter vector has the wrong length")
+        _check(self.L.pfl_set_params(self.h, _fp(a)), "pfl_set_params")
+
+    def _x(self, x):
+        a = np.ascontiguousarray(x, np.float32).reshape(-1, self.dims[0])
+        if not 0 < len(a) <= self.max_batch:
+            raise ValueError("batch size outside [1, max_batch]")
+        return a
+
+    def forward(self, x) -> np.ndarray:
+        a = self._x(x)
+        out = np.zeros((len(a), self.nout), np.float32)
+        _check(self.L.pfl_forward(self.h, _fp(a), len(a), _fp(out)), "pfl_forward")
+        return out
+
+    def grad(self, x, y):
+        a = self._x(x)
+        t = np.ascontiguousarray(y, np.float32).reshape(len(a), 4)
+        g = np.zeros(self.param_count(), np.float32)
+        loss = C.c_double()
+        _check(self.L.pfl_grad(self.h, _fp(a), _fp(t), len(a), _fp(g), C.byref(loss)), "pfl_grad")
+        return loss.value, g
+
+    def train_step(self, x, y, lr: float = 1e-3) -> float:
+        a = self._x(x)
+        t = np.ascontiguousarray(y, np.float32).reshape(len(a), 4)
+        loss = C.c_double()
+        _check(self.L.pfl_train_step(self.h, _fp(a), _fp(t), len(a), lr, C.byref(loss)), "pfl_train_step")
+        return loss.value
+
+    def train_on_map(self, gl_handle, n_samples: int = 1_000_000, batch: int = 4096, lr: float = 1e-3,
+                     rmax: float = 8.0, ang_span: float = 2 * np.pi, noise_sigma: float = 0.02, seed: int = 1):
+        """Trains on scans rendered on the GPU; returns (mean loss of the first 5 % of steps,
+        of the last 5 %)."""
+        ml = np.zeros(2)
+        _check(self.L.pfl_train_on_map(self.h, gl_handle, n_samples, batch, lr, self.dims[0], rmax, ang_span,
+                                       noise_sigma, seed, _dp(ml)), "pfl_train_on_map")
+        return float(ml[0]), float(ml[1])
+
+    def propose(self, gl_handle, scan, rmax: float, n_mix: int, n_uniform: int, seed: int):
+        s = np.ascontiguousarray(scan, np.float32)
+        _check(self.L.pfl_propose(self.h, gl_handle, _fp(s), s.size, rmax, n_mix, n_uniform, seed), "pfl_propose")
+        return self.L.pfl_last_propose_ms(self.h)
+
 
 # ------------------------------------------------------------------ batched filters
 

@@ -19,7 +19,7 @@
 #include <new>
 #include <vector>
 
-#include "../../include/pf_gl.h"
+#include "../../include/pf_learned.h"
 #include "pf_device.h"
 
 using namespace pfd;
@@ -64,6 +64,7 @@ struct GlState {
     int nfree = 0;
     int cap_hyp = 0;
     int nh = 0;
+    int last_k = 0;  // rows of d_out written by the last top-k
     float* d_hyp = nullptr;
     float* d_caer = nullptr;
     // scan + ray table
@@ -73,6 +74,11 @@ struct GlState {
     int ray_cap = 0;
     int tab_nrays = -1;
     float tab_span = 0.f;
+    int scan_nrays = 0;  // rays of the scan resident in d_scan
+    float* h_scan = nullptr;  // pinned staging of the scan upload
+    int h_scan_cap = 0;
+    hipEvent_t scan_ev = nullptr;  // recorded after each upload from h_scan
+    bool scan_ev_live = false;
     // top-k
     uint64_t* d_tk = nullptr;  // [0] prefix, [1] threshold, [2] remaining|done, [3] count
     unsigned* d_hist = nullptr;
@@ -453,12 +459,165 @@ int score_launch(GlState* s, int nrays, float rmax) {
     return 0;
 }
 
+// Bottom-k for n <= kTopkSmallN, k <= kTopkSmallK in one workgroup: radix select of the
+// k-th smallest unique key (CAER bits, index) starting below the bits shared by every key,
+// stopping early once the selected digit bucket holds a single key; then the k winners are
+// bitonic-sorted in LDS.
+constexpr int kTopkSmallN = 1 << 16;
+constexpr int kTopkSmallK = 1024;
+
+__device__ inline uint64_t shfl_xor_u64(uint64_t v, int m) {
+    const unsigned lo = (unsigned)__shfl_xor((int)(uint32_t)v, m, 64);
+    const unsigned hi = (unsigned)__shfl_xor((int)(uint32_t)(v >> 32), m, 64);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+__global__ __launch_bounds__(1024) void k_topk_small(const float* __restrict__ caer, int n, int k,
+                                                     const float* __restrict__ hyp, float* __restrict__ out,
+                                                     float* __restrict__ outcaer, int32_t* __restrict__ outidx) {
+    __shared__ unsigned hist[256];
+    __shared__ uint64_t red[2][16];
+    __shared__ uint64_t win[kTopkSmallK];
+    __shared__ uint64_t s_prefix;
+    __shared__ int s_d, s_rem, s_done, s_cnt;
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    uint64_t mn = ~0ull, mx = 0;
+    for (int i = t; i < n; i += 1024) {
+        const uint64_t key = comp_key(caer[i], i);
+        mn = key < mn ? key : mn;
+        mx = key > mx ? key : mx;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        const uint64_t a = shfl_xor_u64(mn, o), b = shfl_xor_u64(mx, o);
+        mn = a < mn ? a : mn;
+        mx = b > mx ? b : mx;
+    }
+    if (lane == 0) {
+        red[0][wv] = mn;
+        red[1][wv] = mx;
+    }
+    __syncthreads();
+    if (t == 0) {
+        uint64_t a = red[0][0], b = red[1][0];
+        for (int w = 1; w < 16; ++w) {
+            a = red[0][w] < a ? red[0][w] : a;
+            b = red[1][w] > b ? red[1][w] : b;
+        }
+        const int L = a == b ? 64 : __clzll((long long)(a ^ b));
+        s_d = L;
+        s_prefix = L == 64 ? a : (L == 0 ? 0ull : (a & ~(~0ull >> L)));
+        s_rem = k;
+        s_done = L == 64 ? 1 : 0;
+        s_cnt = 0;
+    }
+    __syncthreads();
+    while (s_done == 0) {
+        const int d = s_d, rem = s_rem;
+        const uint64_t prefix = s_prefix;
+        const int w = 64 - d < 8 ? 64 - d : 8;
+        const int shift = 64 - d - w;
+        const unsigned dmask = (1u << w) - 1u;
+        if (t < 256) hist[t] = 0;
+        __syncthreads();
+        for (int i = t; i < n; i += 1024) {
+            const uint64_t key = comp_key(caer[i], i);
+            if (d == 0 || (key >> (64 - d)) == (prefix >> (64 - d))) atomicAdd(&hist[(key >> shift) & dmask], 1u);
+        }
+        __syncthreads();
+        if (wv == 0) {
+            unsigned c[4], sum = 0;
+            for (int j = 0; j < 4; ++j) {
+                c[j] = hist[4 * lane + j];
+                sum += c[j];
+            }
+            unsigned incl = sum;
+            for (int o = 1; o < 64; o <<= 1) {
+                const unsigned y = (unsigned)__shfl_up((int)incl, o, 64);
+                if (lane >= o) incl += y;
+            }
+            const unsigned excl = incl - sum;
+            if (excl < (unsigned)rem && (unsigned)rem <= incl) {
+                unsigned run = excl;
+                int b = 4 * lane + 3;
+                unsigned cb = c[3];
+                for (int j = 0; j < 4; ++j) {
+                    if (run + c[j] >= (unsigned)rem) {
+                        b = 4 * lane + j;
+                        cb = c[j];
+                        break;
+                    }
+                    run += c[j];
+                }
+                s_prefix = prefix | ((uint64_t)b << shift);
+                s_rem = rem - (int)run;
+                s_d = d + w;
+                s_done = d + w == 64 ? 1 : (cb == 1 ? 2 : 0);
+            }
+        }
+        __syncthreads();
+    }
+    if (s_done == 2) {  // the chosen bucket holds exactly one key: find it
+        const int d = s_d;
+        const uint64_t prefix = s_prefix;
+        __syncthreads();
+        for (int i = t; i < n; i += 1024) {
+            const uint64_t key = comp_key(caer[i], i);
+            if ((key >> (64 - d)) == (prefix >> (64 - d))) s_prefix = key;
+        }
+        __syncthreads();
+    }
+    const uint64_t kth = s_prefix;
+    for (int i = t; i < n; i += 1024) {
+        const uint64_t key = comp_key(caer[i], i);
+        if (key <= kth) {
+            const int slot = atomicAdd(&s_cnt, 1);
+            if (slot < kTopkSmallK) win[slot] = key;
+        }
+    }
+    __syncthreads();
+    int m = 1;
+    while (m < k) m <<= 1;
+    const int filled = s_cnt < k ? s_cnt : k;
+    for (int i = t; i < m; i += 1024)
+        if (i >= filled) win[i] = ~0ull;
+    __syncthreads();
+    for (int size = 2; size <= m; size <<= 1)
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            if (t < m) {
+                const int p = t ^ stride;
+                if (p > t) {
+                    const uint64_t a = win[t], b = win[p];
+                    const bool up = (t & size) == 0;
+                    if ((a > b) == up) {
+                        win[t] = b;
+                        win[p] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    if (t < k) {
+        const uint64_t key = win[t];
+        const uint32_t idx = (uint32_t)key;
+        if (idx < (uint32_t)n) {
+            out[3 * t] = hyp[3 * idx];
+            out[3 * t + 1] = hyp[3 * idx + 1];
+            out[3 * t + 2] = hyp[3 * idx + 2];
+            outcaer[t] = caer[idx];
+            outidx[t] = (int32_t)idx;
+        }
+    }
+}
+
 int topk_launch(GlState* s, int k) {
     const int n = s->nh;
     int blocks = (n + kTB - 1) / kTB;
     if (blocks > 1024) blocks = 1024;
     prof_b(s, PFGLX_STAGE_TOPK);
-    if (k <= kSortMax) {
+    if (n <= kTopkSmallN && k <= kTopkSmallK) {
+        hipLaunchKernelGGL(k_topk_small, dim3(1), dim3(1024), 0, s->stream, s->d_caer, n, k, s->d_hyp, s->d_out,
+                           s->d_outcaer, s->d_outidx);
+    } else if (k <= kSortMax) {
         hipLaunchKernelGGL(k_tk_init, dim3(1), dim3(256), 0, s->stream, s->d_tk, s->d_hist, k);
         for (int pass = 0; pass < 8; ++pass) {
             const int shift = 56 - 8 * pass;
@@ -488,6 +647,51 @@ int topk_launch(GlState* s, int k) {
     }
     prof_e(s);
     GL_LAUNCH_CHECK();
+    s->last_k = k;
+    return 0;
+}
+
+// Local refinement: `per` children per parent (the last top-k rows); child 0 is the parent
+// itself, the others add N(0, sxy^2) to x, y and N(0, syaw^2) to yaw (wrapped to [-pi, pi)),
+// clamped to the map rectangle.
+__global__ __launch_bounds__(kTB) void k_gl_refine(const float* __restrict__ parents, int k, int per, float sxy,
+                                                   float syaw, uint64_t seed, float x0, float y0, float x1, float y1,
+                                                   float* __restrict__ hyp) {
+    const int h = blockIdx.x * kTB + threadIdx.x;
+    if (h >= k * per) return;
+    const int p = h / per, j = h - p * per;
+    float x = parents[3 * p], y = parents[3 * p + 1], t = parents[3 * p + 2];
+    if (j > 0) {
+        uint64_t st = sm_final(seed * kPhi + (uint64_t)h * kC1);
+        const uint64_t z0 = sm_next(st), z1 = sm_next(st), z2 = sm_next(st), z3 = sm_next(st);
+        const float u1 = ((float)(z0 >> 40) + 1.0f) * 0x1.0p-24f, u2 = (float)(z1 >> 40) * 0x1.0p-24f;
+        const float u3 = ((float)(z2 >> 40) + 1.0f) * 0x1.0p-24f, u4 = (float)(z3 >> 40) * 0x1.0p-24f;
+        const float r1 = sqrtf(-2.0f * logf(u1)), r2 = sqrtf(-2.0f * logf(u3));
+        x = fminf(fmaxf(x + sxy * r1 * cosf(6.2831853f * u2), x0), x1);
+        y = fminf(fmaxf(y + sxy * r1 * sinf(6.2831853f * u2), y0), y1);
+        t += syaw * r2 * cosf(6.2831853f * u4);
+        if (t >= 3.14159265f) t -= 6.28318531f;
+        if (t < -3.14159265f) t += 6.28318531f;
+    }
+    hyp[3 * h] = x;
+    hyp[3 * h + 1] = y;
+    hyp[3 * h + 2] = t;
+}
+
+int upload_scan(GlState* s, const float* scan, int nrays, float span) {
+    if (ensure_rays(s, nrays, span) != 0) return -1;
+    if (s->scan_ev_live) GL_CHECK(hipEventSynchronize(s->scan_ev));  // staging buffer free again
+    if (nrays > s->h_scan_cap) {
+        if (s->h_scan) GL_CHECK(hipHostFree(s->h_scan));
+        s->h_scan = nullptr;
+        GL_CHECK(hipHostMalloc((void**)&s->h_scan, sizeof(float) * (size_t)nrays, hipHostMallocDefault));
+        s->h_scan_cap = nrays;
+    }
+    memcpy(s->h_scan, scan, sizeof(float) * (size_t)nrays);
+    GL_CHECK(hipMemcpyAsync(s->d_scan, s->h_scan, sizeof(float) * (size_t)nrays, hipMemcpyHostToDevice, s->stream));
+    GL_CHECK(hipEventRecord(s->scan_ev, s->stream));
+    s->scan_ev_live = true;
+    s->scan_nrays = nrays;
     return 0;
 }
 
@@ -523,6 +727,7 @@ void* pfgl_create(int W, int H, float res, float ox, float oy, const unsigned ch
     s->g.ox = ox;
     s->g.oy = oy;
     bool ok = hipSetDevice(0) == hipSuccess && hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) == hipSuccess;
+    ok = ok && hipEventCreateWithFlags(&s->scan_ev, hipEventDisableTiming) == hipSuccess;
     ok = ok && hipMalloc((void**)&s->d_bits, sizeof(uint32_t) * (size_t)nwords) == hipSuccess;
     ok = ok && hipMalloc((void**)&s->d_free, sizeof(int32_t) * (freexy.size() + 2)) == hipSuccess;
     ok = ok && hipMalloc((void**)&s->d_hyp, sizeof(float) * 3 * (size_t)cap_hyp) == hipSuccess;
@@ -553,6 +758,8 @@ int pfgl_destroy(void* h) {
                     s->d_sphi, s->d_tk,      s->d_hist,     s->d_win,    s->d_sortbuf, s->d_out,
                     s->d_outcaer, s->d_outidx, s->d_poses, s->d_ranges};
     for (void* p : ptrs) (void)hipFree(p);
+    if (s->h_scan) (void)hipHostFree(s->h_scan);
+    if (s->scan_ev) (void)hipEventDestroy(s->scan_ev);
     for (auto& e : s->prof) {
         (void)hipEventDestroy(e.a);
         (void)hipEventDestroy(e.b);
@@ -583,11 +790,22 @@ int pfgl_generate(void* h, int n, unsigned long long seed) {
 int pfgl_score(void* h, const float* real_scan, int nrays, float rmax, float ang_span) {
     GlState* s = static_cast<GlState*>(h);
     if (!s || s->nh <= 0 || !real_scan || nrays <= 0) return -1;
-    if (ensure_rays(s, nrays, ang_span) != 0) return -1;
-    GL_CHECK(hipMemcpyAsync(s->d_scan, real_scan, sizeof(float) * (size_t)nrays, hipMemcpyHostToDevice, s->stream));
+    if (upload_scan(s, real_scan, nrays, ang_span) != 0) return -1;
     if (score_launch(s, nrays, rmax) != 0) return -1;
-    GL_CHECK(hipStreamSynchronize(s->stream));  // real_scan is borrowed for the call only
+    GL_CHECK(hipStreamSynchronize(s->stream));
     return 0;
+}
+
+int pfglx_set_scan(void* h, const float* scan, int nrays, float ang_span) {
+    GlState* s = static_cast<GlState*>(h);
+    if (!s || !scan || nrays <= 0) return -1;
+    return upload_scan(s, scan, nrays, ang_span);
+}
+
+int pfglx_score_async(void* h, int nrays, float rmax) {
+    GlState* s = static_cast<GlState*>(h);
+    if (!s || s->nh <= 0 || nrays <= 0 || nrays != s->scan_nrays) return -1;
+    return score_launch(s, nrays, rmax);
 }
 
 int pfglx_topk_ex(void* h, int k, float* out3k, float* caer_k, int32_t* idx_k) {
@@ -684,6 +902,79 @@ int pfglx_render_device(void* h, const float* d_poses3n, int n, int nrays, float
                        d_out);
     GL_LAUNCH_CHECK();
     return 0;
+}
+
+int pfglx_sample_poses_device(void* h, int n, uint64_t seed, float* d_out, void* stream) {
+    GlState* s = static_cast<GlState*>(h);
+    if (!s || s->nfree <= 0 || n <= 0 || !d_out) return -1;
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : s->stream;
+    hipLaunchKernelGGL(k_gl_gen, dim3((n + kTB - 1) / kTB), dim3(kTB), 0, st, s->d_free, s->nfree, s->g.res, s->g.ox,
+                       s->g.oy, seed, n, d_out);
+    GL_LAUNCH_CHECK();
+    return 0;
+}
+
+int pfglx_generate_range(void* h, int offset, int n, uint64_t seed) {
+    GlState* s = static_cast<GlState*>(h);
+    if (!s || s->nfree <= 0 || offset < 0 || n < 0 || offset + n > s->cap_hyp) return -1;
+    if (n > 0) {
+        hipLaunchKernelGGL(k_gl_gen, dim3((n + kTB - 1) / kTB), dim3(kTB), 0, s->stream, s->d_free, s->nfree,
+                           s->g.res, s->g.ox, s->g.oy, seed, n, s->d_hyp + 3 * (size_t)offset);
+        GL_LAUNCH_CHECK();
+    }
+    s->nh = offset + n;
+    return 0;
+}
+
+int pfglx_topk_device(void* h, int k) {
+    GlState* s = static_cast<GlState*>(h);
+    if (!s || k <= 0 || s->nh <= 0) return -1;
+    if (k > s->nh) k = s->nh;
+    if (topk_launch(s, k) != 0) return -1;
+    return k;
+}
+
+int pfglx_refine(void* h, int k_parents, int per_parent, float sigma_xy, float sigma_yaw, uint64_t seed) {
+    GlState* s = static_cast<GlState*>(h);
+    if (!s || k_parents <= 0 || per_parent <= 0 || k_parents > s->last_k) return -1;
+    if ((int64_t)k_parents * per_parent > s->cap_hyp) return -1;
+    const int n = k_parents * per_parent;
+    const float x0 = s->g.ox, y0 = s->g.oy;
+    const float x1 = s->g.ox + s->g.W * s->g.res - 1e-4f, y1 = s->g.oy + s->g.H * s->g.res - 1e-4f;
+    hipLaunchKernelGGL(k_gl_refine, dim3((n + kTB - 1) / kTB), dim3(kTB), 0, s->stream, s->d_out, k_parents,
+                       per_parent, sigma_xy, sigma_yaw, seed, x0, y0, x1, y1, s->d_hyp);
+    GL_LAUNCH_CHECK();
+    s->nh = n;
+    return 0;
+}
+
+float* pfglx_hypotheses_device(void* h) {
+    GlState* s = static_cast<GlState*>(h);
+    return s ? s->d_hyp : nullptr;
+}
+
+int pfglx_map_info(void* h, double* out6) {
+    GlState* s = static_cast<GlState*>(h);
+    if (!s || !out6) return -1;
+    out6[0] = s->g.W;
+    out6[1] = s->g.H;
+    out6[2] = s->g.res;
+    out6[3] = s->g.ox;
+    out6[4] = s->g.oy;
+    out6[5] = s->nfree;
+    return 0;
+}
+
+int pfglx_set_count(void* h, int n) {
+    GlState* s = static_cast<GlState*>(h);
+    if (!s || n < 0 || n > s->cap_hyp) return -1;
+    s->nh = n;
+    return 0;
+}
+
+void* pfglx_stream(void* h) {
+    GlState* s = static_cast<GlState*>(h);
+    return s ? (void*)s->stream : nullptr;
 }
 
 int pfglx_profile_enable(void* h, int enable) {

@@ -19,6 +19,7 @@
 #include "pf_gl.h"
 #include "pf_host.h"
 #include "pf_kernels.h"
+#include "pf_learned.h"
 
 namespace pfh {
 
@@ -263,6 +264,51 @@ class GlobalLoc {
         }
         return k;
     }
+    // Learned proposal first, the reference's uniform dispersal as fallback.
+    int localize_learned(void* mlp, const float* scan, int nrays, const pfh_learned_params& lp, double* out4k,
+                         pfh_learned_info* info) {
+        const float rmax = (float)p_.rmax, span = (float)p_.ang_span;
+        // device-resident chain: one scan upload, no waits until the final top-k
+        if (pfglx_set_scan(h_, scan, nrays, span) != 0) return -1;
+        if (pfl_propose(mlp, h_, scan, nrays, rmax, lp.n_mix, lp.n_uniform, lp.seed) != 0) return -1;
+        if (pfglx_score_async(h_, nrays, rmax) != 0) return -1;
+        double sxy = lp.refine_sigma_xy, syaw = lp.refine_sigma_yaw;
+        for (int r = 0; r < lp.refine_rounds; ++r) {
+            const int kp = pfglx_topk_device(h_, lp.refine_parents);
+            if (kp <= 0) return -1;
+            if (pfglx_refine(h_, kp, lp.refine_children, (float)sxy, (float)syaw,
+                             lp.seed * 0x9E3779B97F4A7C15ull + (uint64_t)r + 1) != 0)
+                return -1;
+            if (pfglx_score_async(h_, nrays, rmax) != 0) return -1;
+            sxy *= lp.refine_shrink;
+            syaw *= lp.refine_shrink;
+        }
+        std::vector<float> xyz((size_t)p_.topk * 3), caer((size_t)p_.topk);
+        int k = pfglx_topk_ex(h_, p_.topk, xyz.data(), caer.data(), nullptr);
+        if (k <= 0) return -1;
+        const double first = (double)caer[0] / nrays;
+        int fell_back = 0;
+        if (first > lp.fallback_caer_per_ray) {
+            fell_back = 1;
+            if (pfgl_generate(h_, lp.fallback_hypotheses, (unsigned long long)(lp.seed ^ 0xD1B54A32D192ED03ull)) != 0)
+                return -1;
+            if (pfgl_score(h_, scan, nrays, rmax, span) != 0) return -1;
+            k = pfglx_topk_ex(h_, p_.topk, xyz.data(), caer.data(), nullptr);
+            if (k <= 0) return -1;
+        }
+        for (int i = 0; i < k; ++i) {
+            out4k[4 * i] = xyz[3 * i];
+            out4k[4 * i + 1] = xyz[3 * i + 1];
+            out4k[4 * i + 2] = xyz[3 * i + 2];
+            out4k[4 * i + 3] = caer[i];
+        }
+        if (info) {
+            info->used_fallback = fell_back;
+            info->proposal_best_caer_per_ray = first;
+            info->best_caer_per_ray = (double)caer[0] / nrays;
+        }
+        return k;
+    }
     void set_seed(uint64_t s) { p_.seed = s; }
     void* handle() const { return h_; }
 
@@ -379,6 +425,13 @@ int pfh_gl_set_seed(void* gp, uint64_t seed) {
     if (!g) return -1;
     g->set_seed(seed);
     return 0;
+}
+
+int pfh_gl_localize_learned(void* gp, void* mlp, const float* scan, int nrays, const pfh_learned_params* lp,
+                            double* out4k, pfh_learned_info* info) {
+    GlobalLoc* g = static_cast<GlobalLoc*>(gp);
+    if (!g || !mlp || !scan || !lp || !out4k || nrays <= 0) return -1;
+    return g->localize_learned(mlp, scan, nrays, *lp, out4k, info);
 }
 
 void* pfh_gl_kernels(void* gp) {

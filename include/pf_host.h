@@ -76,6 +76,28 @@ int pfh_gl_localize(void *g, const float *scan, int nrays, double *out4k);
 int pfh_gl_set_seed(void *g, uint64_t seed);
 void *pfh_gl_kernels(void *g);
 
+/* Learned-proposal localisation (no reference counterpart; pfl_* in pf_learned.h):
+ * n_mix mixture + n_uniform uniform hypotheses are scored by CAER, then refine_rounds
+ * rounds of local refinement (refine_children per each of the refine_parents best, jitter
+ * sigma_xy / sigma_yaw shrinking by refine_shrink per round) are scored; if the best CAER
+ * per ray still exceeds fallback_caer_per_ray, fallback_hypotheses uniform ones are scored
+ * instead (the reference's dispersal, global_loc.rs:306-327).  The pfh_gl handle's
+ * hypothesis capacity must cover every count. */
+typedef struct pfh_learned_params {
+    int32_t n_mix, n_uniform, fallback_hypotheses;
+    double fallback_caer_per_ray;
+    uint64_t seed;
+    int32_t refine_rounds, refine_parents, refine_children;
+    double refine_sigma_xy, refine_sigma_yaw, refine_shrink;
+} pfh_learned_params;
+typedef struct pfh_learned_info {
+    int32_t used_fallback;
+    double proposal_best_caer_per_ray; /* best CAER / nrays of the learned path (after refinement) */
+    double best_caer_per_ray;          /* of the returned winner */
+} pfh_learned_info;
+int pfh_gl_localize_learned(void *g, void *mlp, const float *scan, int nrays, const pfh_learned_params *lp,
+                            double *out4k, pfh_learned_info *info);
+
 #ifdef __cplusplus
 }
 #endif

@@ -62,6 +62,12 @@ _SIGS = [
                                C.c_int, P(f32)]),
     ("or_render32", None, [C.c_int, C.c_int, f32, f32, f32, P(C.c_uint8), C.c_int, f32, f32, P(f32), C.c_int,
                            P(f32)]),
+    # learned proposal network (mlp_oracle.c)
+    ("om_param_count", C.c_long, [C.c_int, C.c_int, C.c_int, C.c_int]),
+    ("om_forward", C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, P(f32), P(f32), C.c_int, C.c_int, P(f64)]),
+    ("om_head", f64, [P(f64), P(f64), C.c_int, C.c_int, P(f64)]),
+    ("om_loss_grad", C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, P(f32), P(f32), P(f32), C.c_int, C.c_int,
+                               P(f64), P(f64)]),
 ]
 
 _L = None
@@ -310,3 +316,55 @@ class OracleGrid:
 def bottom_k(caer, k):
     """Stable ascending order by (caer, index), first k indices."""
     return np.lexsort((np.arange(len(caer)), caer))[:k]
+
+
+# ------------------------------------------------------------------ learned proposal network
+
+
+class OracleMlp:
+    """f64 restatement of the pfl_* network (mlp_oracle.c); `bf16=True` rounds the operands
+    the GPU rounds to bfloat16."""
+
+    def __init__(self, in_dim, hid, n_hidden, ncomp):
+        self.dims = (in_dim, hid, n_hidden, ncomp)
+        self.nout = 9 * ncomp
+
+    def param_count(self) -> int:
+        return lib().om_param_count(*self.dims)
+
+    def forward(self, params, x, bf16=True):
+        x = np.ascontiguousarray(x, np.float32)
+        B = x.shape[0]
+        out = np.zeros((B, self.nout))
+        p = np.ascontiguousarray(params, np.float32)
+        if lib().om_forward(*self.dims, _f(p), _f(x), B, int(bf16), _d(out)) != 0:
+            raise ValueError("om_forward")
+        return out
+
+    def loss_grad(self, params, x, y, bf16=True):
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.ascontiguousarray(y, np.float32)
+        p = np.ascontiguousarray(params, np.float32)
+        g = np.zeros(p.size)
+        loss = C.c_double()
+        if lib().om_loss_grad(*self.dims, _f(p), _f(x), _f(y), x.shape[0], int(bf16), _d(g), C.byref(loss)) != 0:
+            raise ValueError("om_loss_grad")
+        return loss.value, g
+
+
+def mixture_nll(out, y, ncomp, with_grad=False):
+    """Mean mixture NLL of raw output rows (mlp_oracle.c om_head) and optionally d/d(out)."""
+    o = np.ascontiguousarray(out, np.float64)
+    yy = np.ascontiguousarray(y, np.float64)
+    d = np.zeros_like(o) if with_grad else None
+    v = lib().om_head(_d(o), _d(yy), o.shape[0], ncomp, _d(d) if with_grad else None)
+    return (v, d) if with_grad else v
+
+
+def adam_step(p, m, v, g, t, lr, b1=0.9, b2=0.999, eps=1e-8):
+    """Textbook Adam (Kingma & Ba 2015, Algorithm 1) in f64; returns (p, m, v)."""
+    m = b1 * m + (1 - b1) * g
+    v = b2 * v + (1 - b2) * g * g
+    mh = m / (1 - b1 ** t)
+    vh = v / (1 - b2 ** t)
+    return p - lr * mh / (np.sqrt(vh) + eps), m, v

@@ -73,6 +73,25 @@ def test_caer_bitexact_and_topk(gpu, nrays):
     assert np.array_equal(plain, hyp[bottom_k(ocaer, 10)])
 
 
+@pytest.mark.parametrize("n", [1, 700, 65_536, 65_537, 150_000])
+def test_topk_paths_match_stable_sort(gpu, n):
+    """Single-workgroup selection (n <= 65536, k <= 1024) and the multi-workgroup paths give
+    the stable ascending order by (CAER, index) of the CAERs the GPU computed."""
+    cells, res, org = obstacle_map(104)
+    g = GlKernels(cells, res, org[0], org[1], n)
+    og = OracleGrid(cells, res, org)
+    scan = og.map_scan64(4.1, 4.9, -0.4, 120, TAU, 8.0).astype(np.float32)
+    g.generate(n, 5)
+    g.score(scan, 8.0, TAU)
+    caer = g.caer()
+    hyp = g.hypotheses()
+    for k in (1, 32, 1023, 1024, 1025, 4096):
+        out, kc, kidx = g.topk_ex(k)
+        want = bottom_k(caer, k)
+        assert np.array_equal(kidx, want.astype(np.int32)), f"n={n} k={k}"
+        assert np.array_equal(out, hyp[want]) and np.array_equal(kc, caer[want])
+
+
 def test_topk_ties_keep_index_order(gpu):
     """All-rmax map (no obstacles inside rmax) -> many equal CAERs; stable order by index."""
     cells = np.zeros((50, 50), np.uint8)

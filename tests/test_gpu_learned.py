@@ -1,0 +1,189 @@
+"""Learned proposal (pfl_*): BF16 MFMA network against the f64 oracle, Adam, training on
+GPU-rendered scans, and the learned-vs-uniform localisation A/B.
+
+Tolerances: the GPU multiplies bf16 operands exactly and accumulates in f32, the oracle
+rounds the same operands to bf16 and accumulates in f64.  Rare differences in which way a
+hidden activation rounds to bf16 (its f32 and f64 pre-activations differ by accumulation
+error) perturb single elements by one bf16 ulp, so forward outputs are compared at
+2e-3 absolute + 1e-3 relative and gradients by relative L2 error (5e-3) plus an elementwise
+bound of 2 % of the layer's largest gradient.
+"""
+import math
+import time
+
+import numpy as np
+import pytest
+
+from cuda_pf_localization_amd import GlKernels, GlobalLoc, GlParams, LearnedParams, LearnedProposal
+from cuda_pf_localization_amd.sim import free_poses, obstacle_map
+from oracle import OracleGrid, OracleMlp, adam_step
+
+pytestmark = pytest.mark.gpu
+
+FULL = (360, 512, 3, 16)
+RAGGED = (37, 100, 2, 3)
+
+
+def rand_batch(dims, B, seed):
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(0.0, 1.2, (B, dims[0])).astype(np.float32)
+    th = rng.uniform(-np.pi, np.pi, B)
+    y = np.stack([rng.uniform(-0.5, 0.5, B), rng.uniform(-0.5, 0.5, B), np.sin(th), np.cos(th)], 1).astype(np.float32)
+    return x, y
+
+
+def layer_slices(dims):
+    in_dim, hid, nh, nc = dims
+    o, out = 0, []
+    for l in range(nh + 1):
+        di = in_dim if l == 0 else hid
+        do = 9 * nc if l == nh else hid
+        out.append(slice(o, o + do * di))
+        o += do * di
+        out.append(slice(o, o + do))
+        o += do
+    return out
+
+
+@pytest.mark.parametrize("dims,B", [(FULL, 64), (RAGGED, 70), (FULL, 1)])
+def test_forward_matches_oracle(gpu, dims, B):
+    net = LearnedProposal(*dims, max_batch=128, seed=5)
+    p = net.get_params()
+    om = OracleMlp(*dims)
+    assert net.param_count() == om.param_count() == p.size
+    x, _ = rand_batch(dims, B, 1)
+    got = net.forward(x)
+    ref = om.forward(p, x, bf16=True)
+    np.testing.assert_allclose(got, ref, rtol=1e-3, atol=2e-3)
+
+
+def test_set_params_roundtrip(gpu):
+    net = LearnedProposal(*RAGGED, max_batch=16, seed=1)
+    rng = np.random.default_rng(2)
+    p = rng.normal(size=net.param_count()).astype(np.float32) * 0.1
+    net.set_params(p)
+    assert np.array_equal(net.get_params(), p)
+    x, _ = rand_batch(RAGGED, 16, 3)
+    np.testing.assert_allclose(net.forward(x), OracleMlp(*RAGGED).forward(p, x), rtol=1e-3, atol=2e-3)
+
+
+@pytest.mark.parametrize("dims,B", [(FULL, 64), (RAGGED, 70)])
+def test_gradients_match_oracle(gpu, dims, B):
+    net = LearnedProposal(*dims, max_batch=128, seed=7)
+    p = net.get_params()
+    x, y = rand_batch(dims, B, 2)
+    loss, g = net.grad(x, y)
+    rl, rg = OracleMlp(*dims).loss_grad(p, x, y, bf16=True)
+    assert abs(loss - rl) < 1e-4 * max(1.0, abs(rl))
+    assert np.linalg.norm(g - rg) < 5e-3 * np.linalg.norm(rg)
+    for sl in layer_slices(dims):
+        gm = np.abs(rg[sl]).max()
+        assert np.abs(g[sl] - rg[sl]).max() <= 2e-2 * gm + 1e-7, sl
+    # parameters untouched by pfl_grad
+    assert np.array_equal(net.get_params(), p)
+
+
+def test_adam_steps_match_textbook(gpu):
+    dims = RAGGED
+    net = LearnedProposal(*dims, max_batch=64, seed=9)
+    x, y = rand_batch(dims, 64, 4)
+    p = net.get_params().astype(np.float64)
+    m = np.zeros_like(p)
+    v = np.zeros_like(p)
+    for t in (1, 2, 3):
+        _, g = net.grad(x, y)
+        net.train_step(x, y, lr=1e-3)
+        p, m, v = adam_step(p, m, v, g.astype(np.float64), t, 1e-3)
+        np.testing.assert_allclose(net.get_params(), p, rtol=0, atol=2e-6)
+
+
+def test_train_on_map_reduces_loss(gpu):
+    cells, res, org = obstacle_map(100)
+    gk = GlKernels(cells, res, *org, 1024)
+    net = LearnedProposal(360, 256, 2, 8, max_batch=1024, seed=3)
+    first, last = net.train_on_map(gk.h, n_samples=256 * 1024, batch=1024, lr=1e-3, rmax=8.0, seed=5)
+    print(f"loss first {first:.3f} last {last:.3f}")
+    assert np.isfinite(first) and np.isfinite(last)
+    assert last < first - 1.0
+
+
+def test_propose_fills_hypotheses(gpu):
+    cells, res, org = obstacle_map(101)
+    gk = GlKernels(cells, res, *org, 4000)
+    net = LearnedProposal(360, 64, 1, 4, max_batch=16, seed=3)
+    og = OracleGrid(cells, res, org)
+    scan = og.map_scan64(5.0, 5.0, 0.3, 360, 2 * math.pi, 8.0).astype(np.float32)
+    ms = net.propose(gk.h, scan, 8.0, 3000, 1000, seed=11)
+    assert ms > 0
+    hyp = gk.hypotheses()
+    assert hyp.shape == (4000, 3) and np.all(np.isfinite(hyp))
+    assert hyp[:, 0].min() >= 0 and hyp[:, 0].max() < 10.0 and hyp[:, 1].min() >= 0 and hyp[:, 1].max() < 10.0
+    assert np.all(np.abs(hyp[:, 2]) <= math.pi + 1e-6)
+    # the uniform tail is the pfgl_generate stream for that seed
+    gk2 = GlKernels(cells, res, *org, 1000)
+    gk2.generate(1000, 11 ^ 0x9E3779B97F4A7C15)
+    assert np.array_equal(hyp[3000:], gk2.hypotheses())
+    with pytest.raises(RuntimeError):
+        net.propose(gk.h, scan, 8.0, 3500, 1000, seed=1)  # beyond the handle's capacity
+
+
+def test_refine_children(gpu):
+    cells, res, org = obstacle_map(102)
+    gk = GlKernels(cells, res, *org, 5000)
+    L = gk.L
+    og = OracleGrid(cells, res, org)
+    scan = og.map_scan64(4.0, 6.0, -1.0, 360, 2 * math.pi, 8.0).astype(np.float32)
+    gk.generate(5000, 3)
+    gk.score(scan, 8.0, 2 * math.pi)
+    top, caer, idx = gk.topk_ex(8)
+    assert L.pfglx_refine(gk.h, 9, 100, 0.05, 0.02, 1) != 0  # more parents than the last top-k
+    assert L.pfglx_refine(gk.h, 8, 700, 0.05, 0.02, 1) != 0  # beyond capacity
+    assert L.pfglx_refine(gk.h, 8, 500, 0.05, 0.02, 1) == 0
+    hyp = gk.hypotheses().reshape(8, 500, 3)
+    assert np.array_equal(hyp[:, 0], top)
+    d = hyp[:, 1:, :2] - top[:, None, :2]
+    assert 0.03 < d.std() < 0.07
+    assert np.all(np.abs(hyp[..., 2]) <= math.pi + 1e-6)
+    assert hyp[..., :2].min() >= 0 and hyp[..., :2].max() < 10.0
+    # a refined set scores at least as well as its parents
+    gk.score(scan, 8.0, 2 * math.pi)
+    assert gk.topk_ex(1)[1][0] <= caer[0]
+
+
+def test_learned_localization_ab(gpu):
+    """Config-4 style A/B on one map: trained proposal (8k mixture + 2k uniform, uniform
+    fallback) against 200k uniform hypotheses, same scans, same success criterion."""
+    cells, res, org = obstacle_map(100)
+    gl = GlobalLoc(cells, res, org, GlParams(hypotheses=200_000, rmax=8.0, topk=64, seed=3))
+    net = LearnedProposal(*FULL, max_batch=4096, seed=1)
+    t0 = time.perf_counter()
+    first, last = net.train_on_map(gl.kernels_handle(), n_samples=1_000_000, batch=4096, lr=1e-3, rmax=8.0, seed=2)
+    train_s = time.perf_counter() - t0
+    og = OracleGrid(cells, res, org)
+    truths = free_poses(cells, res, org, 20, seed=100, margin_cells=6)
+    rng = np.random.default_rng(100)
+    scans = [(og.map_scan64(*t, 360, 2 * math.pi, 8.0) + rng.normal(size=360) * 0.02).astype(np.float32)
+             for t in truths]
+    lp = LearnedParams(seed=4)
+    gl.localize_learned(net, scans[0], lp)  # warm-up
+    gl.localize(scans[0])
+    ok_l = ok_u = fb = 0
+    tl = tu = 0.0
+    for i, (t, s) in enumerate(zip(truths, scans)):
+        lp.seed = 1000 + i
+        a = time.perf_counter()
+        top, info = gl.localize_learned(net, s, lp)
+        tl += time.perf_counter() - a
+        ok_l += math.hypot(top[0, 0] - t[0], top[0, 1] - t[1]) < 0.5
+        fb += info["used_fallback"]
+        print(f"pose {i}: err {math.hypot(top[0, 0] - t[0], top[0, 1] - t[1]):.3f} m, learned best "
+              f"{info['proposal_best_caer_per_ray']:.4f} m/ray, fallback {info['used_fallback']}")
+        gl.set_seed(2000 + i)
+        a = time.perf_counter()
+        top = gl.localize(s)
+        tu += time.perf_counter() - a
+        ok_u += math.hypot(top[0, 0] - t[0], top[0, 1] - t[1]) < 0.5
+    print(f"train {train_s:.2f}s loss {first:.3f}->{last:.3f}; learned {ok_l}/20 in {tl * 50:.3f} ms/fix "
+          f"({fb} fallbacks); uniform {ok_u}/20 in {tu * 50:.3f} ms/fix")
+    assert last < first
+    assert ok_l >= 18
