@@ -39,6 +39,22 @@ BYTES = {"predict": 56, "weight": 36, "normalize": 12, "resample_scan": 12, "res
          "est_sums": 32, "est_covs": 32}
 
 
+PROFILE_PF = os.path.join(ROOT, "profiles", "pf_cycle_1M_particles_256_obs.txt")
+
+
+def profiled_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    summary (copy-calibrated), or None when the summary is absent."""
+    try:
+        with open(PROFILE_PF) as f:
+            for line in f:
+                if line.startswith(f"{kernel} traffic_bytes_per_launch:"):
+                    return int(line.split(":")[1])
+    except OSError:
+        return None
+    return None
+
+
 def build_cycle_inputs(sc):
     from cuda_pf_localization_amd import _lib
     steps = []
@@ -133,6 +149,144 @@ def run_gl(n_hyp, n_poses, seed=3):
             "stage_ms": {"generate": msv[0], "score": msv[1], "topk": msv[2]}}
 
 
+PFH_OBS_DTYPE = np.dtype([("id", "<i4"), ("mode", "<i4"), ("vx", "<f8"), ("vy", "<f8"), ("vz", "<f8"),
+                          ("range", "<f8")])
+
+
+def run_batched(nf=64, cycles=100, warmup=2, seed=11):
+    """64 independent filters: N log-uniform in [1k, 500k], L uniform in {4..256}, 20 x 20 x 5 m
+    boxes, random helix phase, 0.05 m / 0.02 rad observation noise, 100 cycles.  One batched
+    cycle (pfb_cycle: one launch per stage for all filters, hipGraph replay, autotuned weight
+    tile) against the same 64 filters run one after another through the plain path."""
+    import ctypes as C
+    from cuda_pf_localization_amd import OBS_DTYPE, BatchedFilters, ParticleFilter, PfConfig, _lib
+    from cuda_pf_localization_amd.sim import helix_arrays
+    rng = np.random.default_rng(seed)
+    ns = np.exp(rng.uniform(math.log(1000), math.log(500_000), nf)).astype(int)
+    ls = rng.integers(4, 257, nf)
+    phases = rng.uniform(0.0, 2 * math.pi, nf)
+    hs = [helix_arrays(int(ls[f]), (20.0, 20.0, 5.0), cycles, seed * 1000 + f, phase=float(phases[f]))
+          for f in range(nf)]
+    obs32, obs64 = [], []
+    for h in hs:
+        L = len(h.landmarks)
+        a = np.zeros((cycles, L), OBS_DTYPE)
+        b = np.zeros((cycles, L), PFH_OBS_DTYPE)
+        a["id"] = b["id"] = np.arange(L)
+        a["mode"] = b["mode"] = 1
+        for j, (k32, k64) in enumerate((("dx", "vx"), ("dy", "vy"), ("dz", "vz"))):
+            a[k32] = h.obs_v[:, :, j]
+            b[k64] = h.obs_v[:, :, j]
+        a["range"] = h.obs_r
+        b["range"] = h.obs_r
+        obs32.append(a)
+        obs64.append(b)
+    cfgs = [dict(n=int(ns[f]), seed=seed + f, sigma_r=0.05, sigma_b=0.02, gate=3.0, ratio=0.5, inject=0.0)
+            for f in range(nf)]
+    bf = BatchedFilters(cfgs, obs_cap=256)
+    for f, h in enumerate(hs):
+        bf.upload_landmarks(f, h.landmarks.astype(np.float32))
+    bf.reinit(np.array([[*h.prior_pos, *h.prior_q] for h in hs], np.float32), np.full((nf, 3), 0.5, np.float32),
+              np.full((nf, 3), 0.1, np.float32))
+    inputs = [bf.make_inputs([(h.odom_t[k], h.odom_q[k], (h.odo_pos,) * 3, (h.odo_rot,) * 3, obs32[f][k])
+                              for f, h in enumerate(hs)]) for k in range(cycles)]
+    bf.use_graph(True)
+    ppt = bf.autotune(inputs[0][0])
+    for k in range(warmup):
+        bf.cycle(inputs[k][0])
+    t0 = time.perf_counter()
+    for k in range(warmup, cycles):
+        out = bf.cycle(inputs[k][0])
+    t_b = (time.perf_counter() - t0) * 1000.0 / (cycles - warmup)
+    err_b = float(np.mean([math.dist(out[f].pos[:], hs[f].truth[-1]) for f in range(nf)]))
+    bf.close()
+    # plain path, one filter after another
+    t_p = 0.0
+    err_p = []
+    est = _lib.PfhEstimate()
+    for f, h in enumerate(hs):
+        pf = ParticleFilter(PfConfig(particle_count=int(ns[f]), seed=seed + f, range_noise=0.05, bearing_noise=0.02,
+                                     gate_sigma=3.0, resample_ess_ratio=0.5, random_inject_ratio=0.0, fast=True))
+        pf.set_map({i: tuple(p) for i, p in enumerate(h.landmarks)})
+        pf.reinitialize([*h.prior_pos, *h.prior_q], [0.5] * 3, [0.1] * 3)
+        odo = []
+        for k in range(cycles):
+            o = _lib.PfhOdom()
+            o.t[:] = list(h.odom_t[k])
+            o.q[:] = list(h.odom_q[k])
+            o.trans_noise[:] = [h.odo_pos] * 3
+            o.rot_noise[:] = [h.odo_rot] * 3
+            odo.append(o)
+        ptrs = [obs64[f][k].ctypes.data_as(C.POINTER(_lib.PfhObs)) for k in range(cycles)]
+        for k in range(warmup):
+            pf.update_raw(odo[k], ptrs[k], len(h.landmarks), est)
+        t0 = time.perf_counter()
+        for k in range(warmup, cycles):
+            pf.update_raw(odo[k], ptrs[k], len(h.landmarks), est)
+        t_p += time.perf_counter() - t0
+        err_p.append(math.dist(est.pos[:], h.truth[-1]))
+        pf.close()
+    t_p = t_p * 1000.0 / (cycles - warmup)
+    return {"workload": "64 filters, N log-uniform [1k, 500k], L uniform [4, 256], 100 cycles, seed 11",
+            "particles_total": int(ns.sum()), "landmarks_total": int(ls.sum()),
+            "batched_ms_per_cycle": round(t_b, 4), "sequential_plain_ms_per_cycle": round(t_p, 4),
+            "speedup": round(t_p / t_b, 2), "particle_cycles_per_s": round(float(ns.sum()) / (t_b * 1e-3), 1),
+            "weight_particles_per_lane": int(ppt), "graph": True,
+            "mean_final_err_m": {"batched": round(err_b, 4), "plain": round(float(np.mean(err_p)), 4)}}
+
+
+def run_learned(maps=tuple(range(100, 110)), n_poses=20):
+    """Learned proposal on a 10-map family: per map, train the MLP on 1M GPU-rendered scans
+    (Adam 1e-3, batch 4096), then localise n_poses scans with the learned proposal (8k
+    mixture + 2k uniform, refinement, uniform fallback) and with 200k uniform hypotheses."""
+    from cuda_pf_localization_amd import GlobalLoc, GlParams, LearnedParams, LearnedProposal
+    from cuda_pf_localization_amd.sim import f64_scan, free_poses, obstacle_map
+    ok_l = ok_u = fb = 0
+    t_l = t_u = t_train = 0.0
+    losses = []
+    params = None
+    for ms in maps:
+        cells, res, org = obstacle_map(ms)
+        gl = GlobalLoc(cells, res, org, GlParams(hypotheses=200_000, rmax=8.0, topk=64, seed=ms))
+        net = LearnedProposal(360, 512, 3, 16, max_batch=4096, seed=ms)
+        params = net.param_count()
+        t0 = time.perf_counter()
+        first, last = net.train_on_map(gl.kernels_handle(), n_samples=1_000_000, batch=4096, lr=1e-3, rmax=8.0,
+                                       ang_span=2 * math.pi, noise_sigma=0.02, seed=ms)
+        t_train += time.perf_counter() - t0
+        losses.append((round(first, 3), round(last, 3)))
+        truths = free_poses(cells, res, org, n_poses, seed=ms, margin_cells=6)
+        rng = np.random.default_rng(ms)
+        scans = [(f64_scan(cells, res, org, t, 360, 2 * math.pi, 8.0) + rng.normal(size=360) * 0.02)
+                 .astype(np.float32) for t in truths]
+        lp = LearnedParams(seed=ms)
+        gl.localize_learned(net, scans[0], lp)  # warm-up
+        gl.localize(scans[0])
+        for i, (t, s) in enumerate(zip(truths, scans)):
+            lp.seed = ms * 1000 + i
+            a = time.perf_counter()
+            top, info = gl.localize_learned(net, s, lp)
+            t_l += time.perf_counter() - a
+            ok_l += math.hypot(top[0, 0] - t[0], top[0, 1] - t[1]) < 0.5
+            fb += info["used_fallback"]
+            gl.set_seed(ms * 1000 + i)
+            a = time.perf_counter()
+            top = gl.localize(s)
+            t_u += time.perf_counter() - a
+            ok_u += math.hypot(top[0, 0] - t[0], top[0, 1] - t[1]) < 0.5
+        net.close()
+        gl.close()
+    n = len(maps) * n_poses
+    train_flop = 6.0 * params * 1_000_000 * len(maps)
+    return {"workload": f"{len(maps)} obstacle maps (seeds {maps[0]}..{maps[-1]}), {n_poses} poses each, 360 rays, "
+                        "rmax 8 m, noise 0.02 m",
+            "learned_ms_per_fix": round(t_l * 1000.0 / n, 4), "uniform_ms_per_fix": round(t_u * 1000.0 / n, 4),
+            "speedup": round(t_u / t_l, 2), "learned_success": f"{ok_l}/{n}", "uniform_success": f"{ok_u}/{n}",
+            "fallbacks": int(fb), "train_s_per_map": round(t_train / len(maps), 3),
+            "train_tflops_end_to_end": round(train_flop / t_train / 1e12, 2), "params": int(params),
+            "loss_first_last": losses}
+
+
 def cpu_baseline(sc, n, budget_s=12.0):
     """The oracle (C restatement of cpu.rs, f64, OpenMP like rayon) on this host, same config-1
     workload at full particle count, timed over as many cycles as fit ~budget_s."""
@@ -178,6 +332,8 @@ def main():
     ap.add_argument("--particles", type=int, default=1_000_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-gl", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="only run a plain device-to-device copy of this many bytes (HBM counter calibration)")
@@ -229,11 +385,16 @@ def main():
         "roofline": {"kernel": "k_weight", "bound": "valu", "achieved": round(achieved_tflops, 2) if achieved_tflops
                      else None, "peak": FP64_VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved_tflops / FP64_VALU_PEAK_TFLOPS, 3) if achieved_tflops else None,
-                     "traffic": None, "ms_avg": round(w["ms_avg"], 5),
+                     "traffic": profiled_traffic("k_weight"), "traffic_algorithmic": 36 * n,
+                     "ms_avg": round(w["ms_avg"], 5),
                      "flop_per_pair": WEIGHT_FLOP_PER_PAIR, "pairs_per_launch": pairs},
     }
     if not args.no_gl:
         line["gl"] = run_gl(200_000, 20)
+    if not args.no_batched:
+        line["batched"] = run_batched()
+    if not args.no_learned:
+        line["learned"] = run_learned()
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(sc, n, args.cpu_budget)
     print(json.dumps(line), flush=True)

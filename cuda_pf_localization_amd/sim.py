@@ -349,3 +349,73 @@ def _cast_ray64(cells, W, H, res, origin, px, py, dx, dy, rmax):
         if cells[cy, cx] == 1:
             return min(dist, rmax)
     return rmax
+
+
+# ------------------------------------------------------------------ vectorised helix (batched workloads)
+
+
+def _qrot_rows(q, v):
+    """Rotates each row of v[n, 3] by the unit quaternion q (wxyz)."""
+    w, x, y, z = q
+    u = np.array([x, y, z])
+    t = 2.0 * np.cross(u[None, :], v)
+    return v + w * t + np.cross(u[None, :], t)
+
+
+@dataclass
+class HelixArrays:
+    """One helix run as arrays: landmarks[L, 3]; odometry t[C, 3], q[C, 4]; observations
+    v[C, L, 3] (unit landmark->sensor, body frame) and r[C, L]; truth pos[C, 3]."""
+    landmarks: np.ndarray
+    prior_pos: tuple
+    prior_q: tuple
+    odom_t: np.ndarray
+    odom_q: np.ndarray
+    obs_v: np.ndarray
+    obs_r: np.ndarray
+    truth: np.ndarray
+    odo_pos: float
+    odo_rot: float
+
+
+def helix_arrays(n_landmarks: int, box, cycles: int, seed: int, phase: float = 0.0, sigma_r: float = 0.05,
+                 sigma_b: float = 0.02, odo_pos: float = 0.02, odo_rot: float = 0.01, radius: float = 5.0,
+                 pitch: float = 0.5, cycles_per_rev: int = 100) -> HelixArrays:
+    """The helix_scenario workload generated with numpy (PCG64 draws, not the splitmix64
+    stream): same geometry, noise model and observation convention, fast enough for
+    batches of many filters."""
+    rng = np.random.default_rng(seed)
+    bx, by, bz = box
+    lm = rng.uniform(0.0, 1.0, (n_landmarks, 3)) * np.array([bx, by, bz])
+    cx, cy, z0 = bx / 2.0, by / 2.0, bz / 2.0
+
+    def pose_at(k):
+        th = phase + 2.0 * math.pi * k / cycles_per_rev
+        return (np.array([cx + radius * math.cos(th), cy + radius * math.sin(th), z0 + pitch * k / cycles_per_rev]),
+                yaw_quat(th + 0.5 * math.pi))
+
+    pos, att = pose_at(0)
+    ot = np.zeros((cycles, 3))
+    oq = np.zeros((cycles, 4))
+    ov = np.zeros((cycles, n_landmarks, 3))
+    orr = np.zeros((cycles, n_landmarks))
+    truth = np.zeros((cycles, 3))
+    for k in range(1, cycles + 1):
+        npos, natt = pose_at(k)
+        ot[k - 1] = _qrot_rows(qconj(att), (npos - pos)[None, :])[0] + odo_pos * rng.normal(size=3)
+        oq[k - 1] = qmul(qconj(att), natt)
+        pos, att = npos, natt
+        body = _qrot_rows(qconj(att), lm - pos)
+        d = np.linalg.norm(body, axis=1)
+        v = -body / d[:, None]
+        a = rng.normal(size=(n_landmarks, 3))
+        a -= np.sum(a * v, axis=1)[:, None] * v
+        a /= np.maximum(np.linalg.norm(a, axis=1), 1e-300)[:, None]
+        ang = sigma_b * rng.normal(size=n_landmarks)
+        v = v * np.cos(ang)[:, None] + np.cross(a, v) * np.sin(ang)[:, None]
+        v /= np.linalg.norm(v, axis=1)[:, None]
+        ov[k - 1] = v
+        orr[k - 1] = d + sigma_r * rng.normal(size=n_landmarks)
+        truth[k - 1] = npos
+    p0, q0 = pose_at(0)
+    return HelixArrays(lm, tuple(p0), q0, ot, oq, ov, orr, truth, odo_pos, odo_rot)

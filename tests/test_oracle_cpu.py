@@ -409,3 +409,20 @@ def test_sim_fixtures_invariants():
     g = sim.Gauss(7)
     v = [g.normal() for _ in range(20000)]
     assert abs(np.mean(v)) < 0.02 and abs(np.std(v) - 0.707) < 0.02  # sim.rs:38-41 triangle, std ~0.707
+
+
+def test_helix_arrays_observation_convention():
+    """Noise-free helix_arrays observations are the unit landmark->sensor vectors in the
+    body frame and the exact ranges, like helix_scenario's."""
+    h = sim.helix_arrays(9, (20.0, 20.0, 5.0), 5, 3, phase=0.4, sigma_r=0.0, sigma_b=0.0, odo_pos=0.0)
+    s = sim.helix_scenario(9, (20.0, 20.0, 5.0), 5, 3, phase=0.4, sigma_r=0.0, sigma_b=0.0, odo_pos=0.0)
+    for k in range(5):
+        pos = np.array(s.steps[k].truth_pos)
+        assert np.allclose(h.truth[k], pos)
+        att = s.steps[k].truth_q
+        for i in range(9):
+            body = np.array(sim.qrot(sim.qconj(att), tuple(h.landmarks[i] - pos)))
+            assert np.allclose(h.obs_v[k, i], -body / np.linalg.norm(body), atol=1e-12)
+            assert abs(h.obs_r[k, i] - np.linalg.norm(body)) < 1e-12
+        assert np.allclose(h.odom_t[k], s.steps[k].odom_t, atol=1e-12)
+        assert np.allclose(h.odom_q[k], s.steps[k].odom_q, atol=1e-12)
