@@ -112,6 +112,7 @@ KERNEL_SYMBOLS = [
     ("pfx_profile_read", C.c_int, [vp, P(f64), P(C.c_int)]),
     ("pfx_particle_count", C.c_int, [vp]),
     ("pfx_sync", C.c_int, [vp]),
+    ("pfx_set_weight_variant", C.c_int, [vp, C.c_int]),
     ("pfx_quat_mean", C.c_int, [P(f64), P(f64)]),
     # global-localisation extensions
     ("pfglx_topk_ex", C.c_int, [vp, C.c_int, P(f32), P(f32), P(i32)]),

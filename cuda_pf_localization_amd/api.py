@@ -75,6 +75,10 @@ class PfKernels:
         except Exception:
             pass
 
+    def set_weight_variant(self, variant: int):
+        """Likelihood kernel launch shape: 1, 2 or 4 particles per lane."""
+        _check(self.L.pfx_set_weight_variant(self.h, variant), "pfx_set_weight_variant")
+
     def init(self, n: int, seed: int):
         _check(self.L.pfc_init(self.h, n, seed & 0xFFFFFFFF), "pfc_init")
         self.n = n

@@ -207,21 +207,31 @@ __global__ __launch_bounds__(kTB) void k_gl_score(Grid g, const float* __restric
 }
 
 // Virtual scans for poses[3n] -> out[n][nrays] (training data / tests).
+// Renders n poses (ranges[h * nrays + r]): one lane per ray over the whole batch, so a
+// batch of a few thousand scans still fills every CU; the bit grid is staged in LDS when it
+// fits.  Directions are computed exactly as in k_gl_score (per-pose det_sincos, rotated
+// f64 table entry rounded to f32).
+template <bool kLdsGrid>
 __global__ __launch_bounds__(kTB) void k_gl_render(Grid g, const float* __restrict__ poses, int n,
                                                    const double* __restrict__ cphi, const double* __restrict__ sphi,
                                                    int nrays, float rmax, float* __restrict__ out) {
-    const int lane = threadIdx.x & 63;
-    const int waves_total = gridDim.x * (kTB / 64);
-    for (int h = blockIdx.x * (kTB / 64) + (threadIdx.x >> 6); h < n; h += waves_total) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    uint32_t* sbits = reinterpret_cast<uint32_t*>(smem);
+    if (kLdsGrid) {
+        for (int i = threadIdx.x; i < g.nwords; i += kTB) sbits[i] = g.bits[i];
+        __syncthreads();
+    }
+    const uint32_t* bits = kLdsGrid ? sbits : g.bits;
+    const size_t total = (size_t)n * (size_t)nrays;
+    for (size_t t = (size_t)blockIdx.x * kTB + threadIdx.x; t < total; t += (size_t)gridDim.x * kTB) {
+        const int h = (int)(t / (size_t)nrays), r = (int)(t - (size_t)h * nrays);
         const float x = poses[3 * h], y = poses[3 * h + 1];
         double st, ct;
         det_sincos((double)poses[3 * h + 2], st, ct);
-        for (int r = lane; r < nrays; r += 64) {
-            const double cp = cphi[r], sp = sphi[r];
-            const float dx = (float)(ct * cp - st * sp);
-            const float dy = (float)(st * cp + ct * sp);
-            out[(size_t)h * nrays + r] = cast_ray(g.bits, g.W, g.H, g.res, g.ox, g.oy, x, y, dx, dy, rmax);
-        }
+        const double cp = cphi[r], sp = sphi[r];
+        const float dx = (float)(ct * cp - st * sp);
+        const float dy = (float)(st * cp + ct * sp);
+        out[t] = cast_ray(bits, g.W, g.H, g.res, g.ox, g.oy, x, y, dx, dy, rmax);
     }
 }
 
@@ -678,6 +688,21 @@ __global__ __launch_bounds__(kTB) void k_gl_refine(const float* __restrict__ par
     hyp[3 * h + 2] = t;
 }
 
+int render_launch(GlState* s, const Grid& g, const float* poses, int n, int nrays, float rmax, float* out,
+                  hipStream_t st, int /*unused*/) {
+    const size_t total = (size_t)n * (size_t)nrays;
+    size_t want = (total + kTB - 1) / kTB;
+    const int blocks = (int)(want < 16384 ? (want < 1 ? 1 : want) : 16384);
+    if (g.nwords <= kLdsBitsMaxWords)
+        hipLaunchKernelGGL(k_gl_render<true>, dim3(blocks), dim3(kTB), 4 * (size_t)g.nwords, st, g, poses, n,
+                           s->d_cphi, s->d_sphi, nrays, rmax, out);
+    else
+        hipLaunchKernelGGL(k_gl_render<false>, dim3(blocks), dim3(kTB), 0, st, g, poses, n, s->d_cphi, s->d_sphi,
+                           nrays, rmax, out);
+    GL_LAUNCH_CHECK();
+    return 0;
+}
+
 int upload_scan(GlState* s, const float* scan, int nrays, float span) {
     if (ensure_rays(s, nrays, span) != 0) return -1;
     if (s->scan_ev_live) GL_CHECK(hipEventSynchronize(s->scan_ev));  // staging buffer free again
@@ -879,9 +904,7 @@ int pfglx_render(void* h, const float* poses3n, int n, int nrays, float rmax, fl
     g.bits = s->d_bits;
     int blocks = (n + 3) / 4;
     if (blocks > 8192) blocks = 8192;
-    hipLaunchKernelGGL(k_gl_render, dim3(blocks), dim3(kTB), 0, s->stream, g, s->d_poses, n, s->d_cphi, s->d_sphi,
-                       nrays, rmax, s->d_ranges);
-    GL_LAUNCH_CHECK();
+    if (render_launch(s, g, s->d_poses, n, nrays, rmax, s->d_ranges, s->stream, blocks) != 0) return -1;
     GL_CHECK(hipMemcpyAsync(out, s->d_ranges, sizeof(float) * need * (size_t)nrays, hipMemcpyDeviceToHost, s->stream));
     GL_CHECK(hipStreamSynchronize(s->stream));
     return 0;
@@ -898,9 +921,7 @@ int pfglx_render_device(void* h, const float* d_poses3n, int n, int nrays, float
     int blocks = (n + 3) / 4;
     if (blocks > 8192) blocks = 8192;
     hipStream_t st = stream ? static_cast<hipStream_t>(stream) : s->stream;
-    hipLaunchKernelGGL(k_gl_render, dim3(blocks), dim3(kTB), 0, st, g, d_poses3n, n, s->d_cphi, s->d_sphi, nrays, rmax,
-                       d_out);
-    GL_LAUNCH_CHECK();
+    if (render_launch(s, g, d_poses3n, n, nrays, rmax, d_out, st, blocks) != 0) return -1;
     return 0;
 }
 

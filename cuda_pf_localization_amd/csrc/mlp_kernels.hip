@@ -74,6 +74,8 @@ struct GemmArgs {
     long ldh;
     const float* Zp;  // EPI_DGRAD: pre-activation of the previous layer
     long ldz;
+    int kchunk;  // K range per workgroup along gridDim.z (set by the launcher)
+    float* ws;   // split-K partials [gridDim.z][M][N]
 };
 
 constexpr int kBM = 64, kBN = 64, kBK = 32, kPad = 8;
@@ -98,28 +100,85 @@ __device__ inline void epilogue(const GemmArgs& g, int m, int n, float v) {
     }
 }
 
-// C[M x N] = A[M x K] B[K x N] on BF16 MFMA; 64 x 64 tile per 4-wave block, 32 x 32 per wave.
-template <class TA, class TB, int EPI>
+// 8 consecutive elements of one operand row -> bf16 (round to nearest even); `cnt` of them
+// are in bounds, the rest become 0.  16-byte vector loads when the run is whole and aligned.
+__device__ inline void load8(const uint16_t* p, int cnt, uint16_t out[8]) {
+    if (cnt == 8 && ((uintptr_t)p & 15) == 0) {
+        const uint4 v = *reinterpret_cast<const uint4*>(p);
+        out[0] = (uint16_t)v.x; out[1] = (uint16_t)(v.x >> 16);
+        out[2] = (uint16_t)v.y; out[3] = (uint16_t)(v.y >> 16);
+        out[4] = (uint16_t)v.z; out[5] = (uint16_t)(v.z >> 16);
+        out[6] = (uint16_t)v.w; out[7] = (uint16_t)(v.w >> 16);
+        return;
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) out[j] = j < cnt ? p[j] : (uint16_t)0;
+}
+__device__ inline void load8(const float* p, int cnt, uint16_t out[8]) {
+    if (cnt == 8 && ((uintptr_t)p & 15) == 0) {
+        const float4 a = *reinterpret_cast<const float4*>(p);
+        const float4 b = *reinterpret_cast<const float4*>(p + 4);
+        out[0] = f2bf(a.x); out[1] = f2bf(a.y); out[2] = f2bf(a.z); out[3] = f2bf(a.w);
+        out[4] = f2bf(b.x); out[5] = f2bf(b.y); out[6] = f2bf(b.z); out[7] = f2bf(b.w);
+        return;
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) out[j] = j < cnt ? f2bf(p[j]) : (uint16_t)0;
+}
+
+// Stages a 64 (rows) x 32 (k) tile of an operand into LDS as [row][k] bf16.
+//   kKContig: element (r, k) at X[r * ld + k]  (thread: one row, 8 consecutive k)
+//   else:     element (r, k) at X[k * ld + r]  (thread: one k, 8 consecutive rows)
+template <bool kKContig, class T>
+__device__ inline void stage_tile(const T* __restrict__ X, long ld, int R, int K, int r0, int k0, int kend, int tid,
+                                  uint16_t (*Xs)[kBK + kPad]) {
+    uint16_t v[8];
+    if (kKContig) {
+        const int r = tid >> 2, kc = (tid & 3) * 8;
+        int cnt = kend - (k0 + kc);
+        cnt = r0 + r < R ? (cnt < 0 ? 0 : (cnt > 8 ? 8 : cnt)) : 0;
+        load8(X + (long)(r0 + r) * ld + (k0 + kc), cnt, v);
+        uint4 pk;
+        pk.x = (uint32_t)v[0] | ((uint32_t)v[1] << 16);
+        pk.y = (uint32_t)v[2] | ((uint32_t)v[3] << 16);
+        pk.z = (uint32_t)v[4] | ((uint32_t)v[5] << 16);
+        pk.w = (uint32_t)v[6] | ((uint32_t)v[7] << 16);
+        *reinterpret_cast<uint4*>(&Xs[r][kc]) = pk;
+    } else {
+        const int k = tid >> 3, rc = (tid & 7) * 8;
+        int cnt = R - (r0 + rc);
+        cnt = k0 + k < kend ? (cnt < 0 ? 0 : (cnt > 8 ? 8 : cnt)) : 0;
+        load8(X + (long)(k0 + k) * ld + (r0 + rc), cnt, v);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) Xs[rc + j][k] = v[j];
+    }
+}
+
+// C[M x N] = A[M x K] B[K x N] on BF16 MFMA (v_mfma_f32_32x32x16_bf16, f32 accumulate):
+// 64 x 64 tile per 4-wave workgroup, 32 x 32 per wave, K in steps of 32 through LDS.
+// kAKc: A is k-contiguous (A[m * sam + k]); else m-contiguous (A[k * sak + m]).
+// kBKc: B is k-contiguous (B[n * sbn + k]); else n-contiguous (B[k * sbk + n]).
+// gridDim.z > 1 splits K: workgroup z covers [z * kchunk, (z + 1) * kchunk) and writes its
+// raw partial to ws[z][M][N]; k_splitk_sum then adds the partials in z order.
+template <class TA, class TB, int EPI, bool kAKc, bool kBKc>
 __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
     __shared__ __align__(16) uint16_t As[kBM][kBK + kPad];
     __shared__ __align__(16) uint16_t Bs[kBN][kBK + kPad];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int wm = (wv >> 1) * 32, wn = (wv & 1) * 32;
     const int m0 = blockIdx.y * kBM, n0 = blockIdx.x * kBN;
+    const int kbeg = blockIdx.z * g.kchunk;
+    const int kend = kbeg + g.kchunk < g.K ? kbeg + g.kchunk : g.K;
     const TA* A = static_cast<const TA*>(g.A);
     const TB* Bm = static_cast<const TB*>(g.B);
+    const long lda = kAKc ? g.sam : g.sak;
+    const long ldb = kBKc ? g.sbn : g.sbk;
     f32x16 acc;
 #pragma unroll
     for (int i = 0; i < 16; ++i) acc[i] = 0.f;
-    const int lr = tid >> 2, lc = (tid & 3) * 8;
-    for (int k0 = 0; k0 < g.K; k0 += kBK) {
-        const int ma = m0 + lr, nb = n0 + lr;
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            const int k = k0 + lc + j;
-            As[lr][lc + j] = (ma < g.M && k < g.K) ? as_bf(A[(long)ma * g.sam + (long)k * g.sak]) : (uint16_t)0;
-            Bs[lr][lc + j] = (nb < g.N && k < g.K) ? as_bf(Bm[(long)k * g.sbk + (long)nb * g.sbn]) : (uint16_t)0;
-        }
+    for (int k0 = kbeg; k0 < kend; k0 += kBK) {
+        stage_tile<kAKc>(A, lda, g.M, g.K, m0, k0, kend, tid, As);
+        stage_tile<kBKc>(Bm, ldb, g.N, g.K, n0, k0, kend, tid, Bs);
         __syncthreads();
 #pragma unroll
         for (int kh = 0; kh < 2; ++kh) {
@@ -135,8 +194,22 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
     for (int i = 0; i < 16; ++i) {
         const int m = m0 + wm + (i >> 2) * 8 + (lane >> 5) * 4 + (i & 3);
         const int n = n0 + wn + (lane & 31);
-        if (m < g.M && n < g.N) epilogue<EPI>(g, m, n, acc[i]);
+        if (m >= g.M || n >= g.N) continue;
+        if (gridDim.z > 1)
+            g.ws[((long)blockIdx.z * g.M + m) * g.N + n] = acc[i];
+        else
+            epilogue<EPI>(g, m, n, acc[i]);
     }
+}
+
+// Split-K reduction: C = sum over z of ws[z] in z order (weight-gradient epilogue).
+__global__ __launch_bounds__(256) void k_splitk_sum(const float* __restrict__ ws, int S, int M, int N,
+                                                    float* __restrict__ C, long ldc) {
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (long)M * N) return;
+    float v = 0.f;
+    for (int z = 0; z < S; ++z) v += ws[(long)z * M * N + i];
+    C[(i / N) * ldc + (i % N)] = v;
 }
 
 // Small-batch forward (M <= kGemvMaxB, bf16 operands): one wave per output column, lanes
@@ -229,14 +302,28 @@ __global__ void k_loss_reduce(const double* __restrict__ part, int nparts, doubl
     }
 }
 
-// gb[n] = sum_m dZ[m, n] (one lane per column, sequential over rows).
-__global__ __launch_bounds__(256) void k_colsum(const float* __restrict__ dZ, int M, int N, long ld,
-                                                float* __restrict__ gb) {
+// gb[n] = sum_m dZ[m, n] in two passes: part[s][n] sums rows [s * rows, (s + 1) * rows),
+// then the kColsumSplit partials are added in s order.
+constexpr int kColsumSplit = 128;
+__global__ __launch_bounds__(256) void k_colsum_part(const float* __restrict__ dZ, int M, int N, long ld, int rows,
+                                                     float* __restrict__ part) {
     const int n = blockIdx.x * 256 + threadIdx.x;
     if (n >= N) return;
+    const int m0 = blockIdx.y * rows;
+    const int m1 = m0 + rows < M ? m0 + rows : M;
     float s = 0.f;
-    for (int m = 0; m < M; ++m) s += dZ[(long)m * ld + n];
-    gb[n] = s;
+    for (int m = m0; m < m1; ++m) s += dZ[(long)m * ld + n];
+    part[(long)blockIdx.y * N + n] = s;
+}
+// One wave per column: lane l adds partials l, l + 64, ...; fixed shuffle tree.
+__global__ __launch_bounds__(256) void k_colsum_fin(const float* __restrict__ part, int S, int N,
+                                                    float* __restrict__ gb) {
+    const int n = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (n >= N) return;
+    float s = 0.f;
+    for (int z = lane; z < S; z += 64) s += part[(long)z * N + n];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if (lane == 0) gb[n] = s;
 }
 
 __global__ __launch_bounds__(256) void k_adam(float* __restrict__ p, float* __restrict__ mo, float* __restrict__ ve,
@@ -351,6 +438,9 @@ struct Mlp {
     std::vector<uint16_t*> H;
     float *O = nullptr, *dO = nullptr, *dA = nullptr, *dB = nullptr, *Y = nullptr;
     double* loss_part = nullptr;
+    float* ws = nullptr;  // split-K partials of the weight-gradient GEMMs
+    size_t ws_cap = 0;    // floats
+    float* cs_part = nullptr;  // column-sum partials [kColsumSplit][max out]
     double* step_loss = nullptr;
     int step_loss_cap = 0;
     float *poses = nullptr, *ranges = nullptr;
@@ -367,7 +457,7 @@ struct Mlp {
 };
 
 template <class TA, class TB, int EPI>
-int gemm(hipStream_t st, const GemmArgs& g) {
+int gemm(hipStream_t st, const GemmArgs& g, float* ws = nullptr, size_t ws_cap = 0) {
     if constexpr (std::is_same<TA, uint16_t>::value && std::is_same<TB, uint16_t>::value &&
                   (EPI == EPI_HIDDEN || EPI == EPI_OUT)) {
         if (g.M <= kGemvMaxB) {
@@ -376,9 +466,41 @@ int gemm(hipStream_t st, const GemmArgs& g) {
             return 0;
         }
     }
-    const dim3 grid((g.N + kBN - 1) / kBN, (g.M + kBM - 1) / kBM);
-    hipLaunchKernelGGL((k_gemm<TA, TB, EPI>), grid, dim3(256), 0, st, g);
+    GemmArgs a = g;
+    const int gn = (g.N + kBN - 1) / kBN, gm = (g.M + kBM - 1) / kBM;
+    int S = 1;
+    if (EPI == EPI_WGRAD && ws != nullptr && gm * gn < 256) {
+        // split K so that the grid has >= 512 workgroups, each with >= 4 K steps
+        S = (512 + gm * gn - 1) / (gm * gn);
+        const int max_s = (g.K + 4 * kBK - 1) / (4 * kBK);
+        S = S < max_s ? S : max_s;
+        while (S > 1 && (size_t)S * g.M * g.N > ws_cap) --S;
+        S = S < 1 ? 1 : S;
+    }
+    a.kchunk = ((g.K + S - 1) / S + kBK - 1) / kBK * kBK;
+    S = (g.K + a.kchunk - 1) / a.kchunk;
+    a.ws = ws;
+    const dim3 grid(gn, gm, S);
+    const bool ak = g.sak == 1, bk = g.sbk == 1;
+    if ((!ak && g.sam != 1) || (!bk && g.sbn != 1)) {
+        fprintf(stderr, "pf_hip gemm: operands must be contiguous along one dimension\n");
+        return -1;
+    }
+    if (ak && bk)
+        hipLaunchKernelGGL((k_gemm<TA, TB, EPI, true, true>), grid, dim3(256), 0, st, a);
+    else if (ak)
+        hipLaunchKernelGGL((k_gemm<TA, TB, EPI, true, false>), grid, dim3(256), 0, st, a);
+    else if (bk)
+        hipLaunchKernelGGL((k_gemm<TA, TB, EPI, false, true>), grid, dim3(256), 0, st, a);
+    else
+        hipLaunchKernelGGL((k_gemm<TA, TB, EPI, false, false>), grid, dim3(256), 0, st, a);
     ML_LAUNCH();
+    if (S > 1) {
+        const long mn = (long)g.M * g.N;
+        hipLaunchKernelGGL(k_splitk_sum, dim3((unsigned)((mn + 255) / 256)), dim3(256), 0, st, ws, S, g.M, g.N, g.C,
+                           g.ldc);
+        ML_LAUNCH();
+    }
     return 0;
 }
 
@@ -430,8 +552,12 @@ int backward(Mlp* n, int B, hipStream_t st) {
         g.sbn = 1;
         g.C = ly.gW;
         g.ldc = ly.in;
-        if (gemm<float, uint16_t, EPI_WGRAD>(st, g) != 0) return -1;
-        hipLaunchKernelGGL(k_colsum, dim3((ly.out + 255) / 256), dim3(256), 0, st, dcur, B, ly.out, (long)ly.out, ly.gb);
+        if (gemm<float, uint16_t, EPI_WGRAD>(st, g, n->ws, n->ws_cap) != 0) return -1;
+        const int rows = (B + kColsumSplit - 1) / kColsumSplit;
+        hipLaunchKernelGGL(k_colsum_part, dim3((ly.out + 255) / 256, kColsumSplit), dim3(256), 0, st, dcur, B, ly.out,
+                           (long)ly.out, rows, n->cs_part);
+        hipLaunchKernelGGL(k_colsum_fin, dim3((ly.out + 3) / 4), dim3(256), 0, st, n->cs_part, kColsumSplit, ly.out,
+                           ly.gb);
         ML_LAUNCH();
         if (l == 0) break;
         // data gradient into the previous layer's pre-activation
@@ -523,7 +649,8 @@ void free_mlp(Mlp* n) {
     }
     for (auto* p : n->Z) (void)hipFree(p);
     for (auto* p : n->H) (void)hipFree(p);
-    void* ps[] = {n->X, n->O, n->dO, n->dA, n->dB, n->Y, n->loss_part, n->step_loss, n->poses, n->ranges};
+    void* ps[] = {n->X,         n->O,         n->dO,    n->dA,     n->dB,   n->Y,
+                  n->loss_part, n->step_loss, n->poses, n->ranges, n->ws,   n->cs_part};
     for (void* p : ps) (void)hipFree(p);
     if (n->ev0) (void)hipEventDestroy(n->ev0);
     if (n->ev1) (void)hipEventDestroy(n->ev1);
@@ -595,6 +722,15 @@ void* pfl_create(int in_dim, int hid, int n_hidden, int ncomp, int max_batch, ui
     ok = ok && hipMalloc((void**)&n->Y, 4 * mb * 4) == hipSuccess;
     ok = ok && hipMalloc((void**)&n->loss_part, sizeof(double) * ((mb + 255) / 256)) == hipSuccess;
     ok = ok && hipMalloc((void**)&n->poses, 4 * mb * 3) == hipSuccess;
+    size_t max_w = 0;
+    int max_out = 0;
+    for (auto& ly : n->L) {
+        max_w = std::max(max_w, (size_t)ly.in * ly.out);
+        max_out = std::max(max_out, ly.out);
+    }
+    n->ws_cap = 8 * max_w;
+    ok = ok && hipMalloc((void**)&n->ws, sizeof(float) * n->ws_cap) == hipSuccess;
+    ok = ok && hipMalloc((void**)&n->cs_part, sizeof(float) * kColsumSplit * (size_t)max_out) == hipSuccess;
     if (ok) ok = refresh_bf16(n) == 0 && hipStreamSynchronize(n->stream) == hipSuccess;
     if (!ok) {
         fprintf(stderr, "pf_hip pfl_create: device setup failed\n");

@@ -71,7 +71,8 @@ struct WeightParams {
     double cart_gate2, rng_gate2, brg_gate2;  // squared-residual gate thresholds (+inf: no gate)
     double u_gate;                            // 1 - cos of the bearing gate (+inf: none)
     const ObsRec* obs;
-    int nobs, pad;
+    int nobs;  // joined records
+    int nrb;   // of which range-bearing (they come first)
 };
 
 struct InjectParams {
@@ -141,7 +142,7 @@ struct PfState {
     // deterministic-draw state (single-filter ABI)
     uint64_t gseed = 42, gen = 0;
     // launch shape of the likelihood kernel (particles per lane: 1, 2 or 4)
-    int weight_ppt = 2;
+    int weight_ppt = 1;
     // profiling
     bool prof_on = false;
     std::vector<ProfEvent> prof_pending;
@@ -159,7 +160,9 @@ int launch_est_sums(PfState* s);
 int launch_est_covs(PfState* s, ParamRef<CovParams> A);
 int launch_predicate(PfState* s, const float* d_ratio, int force);
 WeightParams make_weight_params(double sigma_r, double sigma_b, double gate);
-int join_obs(const struct pfc_obs* obs, int nobs, const float* lm, int lm_cap, ObsRec* out);
+// Joins observations with their landmarks (unknown ids dropped); range-bearing records
+// first, then Cartesian; *nrb = number of range-bearing records.
+int join_obs(const struct pfc_obs* obs, int nobs, const float* lm, int lm_cap, ObsRec* out, int* nrb);
 
 // Device address of a FilterDev member of filter f.
 template <class M>

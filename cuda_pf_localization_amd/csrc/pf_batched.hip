@@ -173,12 +173,13 @@ int upload_inputs(Batch* b, const pfb_cycle_in* in) {
         for (int k = 0; k < 4; ++k) m.q[k] = c.q[k];
         m.gseed = b->gseed[f];
         const int nobs = c.nobs > b->obs_cap ? b->obs_cap : c.nobs;
-        int mobs = 0;
+        int mobs = 0, nrb = 0;
         if (nobs > 0 && c.obs)
-            mobs = join_obs(c.obs, nobs, b->lm[f].data(), b->lm_cap[f], b->h_obs + total);
+            mobs = join_obs(c.obs, nobs, b->lm[f].data(), b->lm_cap[f], b->h_obs + total, &nrb);
         WeightParams wp = make_weight_params(b->cfg[f].sigma_r, b->cfg[f].sigma_b, b->cfg[f].gate);
         wp.obs = b->d_obs + total;
         wp.nobs = mobs;
+        wp.nrb = nrb;
         b->hp.weight[f] = wp;
         total += mobs;
         InjectParams& ip = b->hp.inject[f];
@@ -357,7 +358,7 @@ int pfb_autotune(void* h, const pfb_cycle_in* in) {
     hipEvent_t e0, e1;
     PB_CHECK(hipEventCreate(&e0));
     PB_CHECK(hipEventCreate(&e1));
-    const int cand[3] = {1, 2, 4};
+    const int cand[3] = {1, 2, 4};  // particles per lane of the likelihood kernel
     float best = 1e30f;
     int best_ppt = s->weight_ppt;
     for (int c = 0; c < 3; ++c) {

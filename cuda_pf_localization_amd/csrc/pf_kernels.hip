@@ -138,6 +138,7 @@ typedef const ObsRec* ConstObsPtr;  // host pass of the single-source compile: n
 struct BodyMap {
     double px, py, pz;
     double m00, m01, m02, m10, m11, m12, m20, m21, m22;
+    double mpx, mpy, mpz;  // M p, so that b = M l - M p takes three FMAs per component
 };
 
 __device__ inline BodyMap body_map(const PoseSet& S, int gi) {
@@ -157,6 +158,9 @@ __device__ inline BodyMap body_map(const PoseSet& S, int gi) {
     b.m20 = 2.0 * (qx * qz + qw * qy);
     b.m12 = 2.0 * (qy * qz + qw * qx);
     b.m21 = 2.0 * (qy * qz - qw * qx);
+    b.mpx = __fma_rn(b.m00, b.px, __fma_rn(b.m01, b.py, b.m02 * b.pz));
+    b.mpy = __fma_rn(b.m10, b.px, __fma_rn(b.m11, b.py, b.m12 * b.pz));
+    b.mpz = __fma_rn(b.m20, b.px, __fma_rn(b.m21, b.py, b.m22 * b.pz));
     return b;
 }
 
@@ -191,28 +195,26 @@ __device__ inline double acos_sq_u12(double u) {
 // One gated log-likelihood term (cpu.rs:69-96), accumulated into acc.
 //   The reference's gate is continuous: the gated constant -g^2/2 equals -G inv2s at the
 //   threshold G = g^2 s^2, so a gated term is -min(e^2, G) inv2s (G = +inf when gate = 0).
+//   b = M (l - p) is formed as M l - M p (three FMAs per component).
 //   Range-bearing: rn from an f32 rsqrt seed refined once in f64 (relative ~1e-14);
 //   u = 1 - dot = 1 + pv/rn in f64; eb^2 from the series in u.  With a gate narrower than
 //   u = 0.005 the series is evaluated at min(u, u_gate) and the min() does the gating.
+__device__ inline void body_offset(const BodyMap& b, const ObsRec& o, double& bx, double& by, double& bz) {
+    bx = __fma_rn(b.m00, o.lx, __fma_rn(b.m01, o.ly, __fma_rn(b.m02, o.lz, -b.mpx)));
+    by = __fma_rn(b.m10, o.lx, __fma_rn(b.m11, o.ly, __fma_rn(b.m12, o.lz, -b.mpy)));
+    bz = __fma_rn(b.m20, o.lx, __fma_rn(b.m21, o.ly, __fma_rn(b.m22, o.lz, -b.mpz)));
+}
+
 template <bool kFastBearing>
-__device__ inline void ll_term(const BodyMap& b, const ObsRec& o, const WeightParams& p, double& acc) {
-    const double dx = o.lx - b.px, dy = o.ly - b.py, dz = o.lz - b.pz;
-    const double bx = __fma_rn(b.m00, dx, __fma_rn(b.m01, dy, b.m02 * dz));
-    const double by = __fma_rn(b.m10, dx, __fma_rn(b.m11, dy, b.m12 * dz));
-    const double bz = __fma_rn(b.m20, dx, __fma_rn(b.m21, dy, b.m22 * dz));
-    if (o.mode == 0) {
-        const double ex = bx - o.vx, ey = by - o.vy, ez = bz - o.vz;
-        const double r2 = __fma_rn(ex, ex, __fma_rn(ey, ey, ez * ez));
-        acc = __fma_rn(-fmin(r2, p.cart_gate2), p.inv2rr, acc);
-        return;
-    }
+__device__ inline void rb_from_b(double bx, double by, double bz, double vx, double vy, double vz, double range,
+                                 double irr, double ibb, const WeightParams& p, double& acc) {
     const double r2 = fmax(__fma_rn(bx, bx, __fma_rn(by, by, bz * bz)), 1e-18);  // rn >= 1e-9
-    const double pv = __fma_rn(bx, o.vx, __fma_rn(by, o.vy, bz * o.vz));
+    const double pv = __fma_rn(bx, vx, __fma_rn(by, vy, bz * vz));
     const double y0 = (double)__frsqrt_rn((float)r2);
     const double hy = 0.5 * r2 * y0;
     const double y = __fma_rn(y0, __fma_rn(-hy, y0, 0.5), y0);
-    const double er = __fma_rn(r2, y, -o.range);
-    acc = __fma_rn(-fmin(er * er, p.rng_gate2), p.inv2rr, acc);
+    const double er = __fma_rn(r2, y, -range);
+    acc = __fma_rn(-fmin(er * er, p.rng_gate2), irr, acc);
     double u = fmax(__fma_rn(pv, y, 1.0), 0.0);
     double eb2;
     if (kFastBearing) {
@@ -228,7 +230,14 @@ __device__ inline void ll_term(const BodyMap& b, const ObsRec& o, const WeightPa
             eb2 = eb * eb;
         }
     }
-    acc = __fma_rn(-fmin(eb2, p.brg_gate2), p.inv2bb, acc);
+    acc = __fma_rn(-fmin(eb2, p.brg_gate2), ibb, acc);
+}
+
+__device__ inline void cart_from_b(double bx, double by, double bz, double vx, double vy, double vz, double irr,
+                                   const WeightParams& p, double& acc) {
+    const double ex = bx - vx, ey = by - vy, ez = bz - vz;
+    const double r2 = __fma_rn(ex, ex, __fma_rn(ey, ey, ez * ez));
+    acc = __fma_rn(-fmin(r2, p.cart_gate2), irr, acc);
 }
 
 // Gated log-likelihood (cpu.rs:44-97, 220-248), kPPT particles per lane, plus fused
@@ -256,18 +265,35 @@ __global__ __launch_bounds__(kTB) void k_weight(Layout L, PoseBufs B, FilterDev*
         acc[e] = 0.0;
     }
     ConstObsPtr obs = (ConstObsPtr)p.obs;
-    const int nobs = p.nobs;
+    const int nobs = p.nobs, nrb = p.nrb;  // range-bearing records first, then Cartesian
     if (p.u_gate <= 0.005) {
-        for (int k = 0; k < nobs; ++k) {
+        for (int k = 0; k < nrb; ++k) {
             const ObsRec o = obs[k];
 #pragma unroll
-            for (int e = 0; e < kPPT; ++e) ll_term<true>(bm[e], o, p, acc[e]);
+            for (int e = 0; e < kPPT; ++e) {
+                double bx, by, bz;
+                body_offset(bm[e], o, bx, by, bz);
+                rb_from_b<true>(bx, by, bz, o.vx, o.vy, o.vz, o.range, p.inv2rr, p.inv2bb, p, acc[e]);
+            }
         }
     } else {
-        for (int k = 0; k < nobs; ++k) {
+        for (int k = 0; k < nrb; ++k) {
             const ObsRec o = obs[k];
 #pragma unroll
-            for (int e = 0; e < kPPT; ++e) ll_term<false>(bm[e], o, p, acc[e]);
+            for (int e = 0; e < kPPT; ++e) {
+                double bx, by, bz;
+                body_offset(bm[e], o, bx, by, bz);
+                rb_from_b<false>(bx, by, bz, o.vx, o.vy, o.vz, o.range, p.inv2rr, p.inv2bb, p, acc[e]);
+            }
+        }
+    }
+    for (int k = nrb; k < nobs; ++k) {
+        const ObsRec o = obs[k];
+#pragma unroll
+        for (int e = 0; e < kPPT; ++e) {
+            double bx, by, bz;
+            body_offset(bm[e], o, bx, by, bz);
+            cart_from_b(bx, by, bz, o.vx, o.vy, o.vz, p.inv2rr, p, acc[e]);
         }
     }
     double mloc = -INFINITY;
@@ -900,6 +926,7 @@ int launch_motion(PfState* s, ParamRef<MotionParams> A, int reinit) {
 int launch_weight(PfState* s, ParamRef<WeightParams> A) {
     prof_begin(s, PFX_STAGE_WEIGHT);
     const int ppt = s->weight_ppt;
+    const int span = kTB * ppt;
     if (ppt == 1)
         hipLaunchKernelGGL(k_weight<1>, dim3(s->NP / kTB), dim3(kTB), 0, s->stream, s->L, s->pose, s->fdev, s->ll,
                            s->part_w, A);
@@ -912,7 +939,7 @@ int launch_weight(PfState* s, ParamRef<WeightParams> A) {
     prof_end(s);
     PF_LAUNCH_CHECK();
     prof_begin(s, PFX_STAGE_FINALIZE);
-    hipLaunchKernelGGL(k_weight_fin, dim3(s->nf), dim3(kTB), 0, s->stream, s->L, s->fdev, s->part_w, kTB * ppt);
+    hipLaunchKernelGGL(k_weight_fin, dim3(s->nf), dim3(kTB), 0, s->stream, s->L, s->fdev, s->part_w, span);
     prof_end(s);
     PF_LAUNCH_CHECK();
     return 0;
@@ -1015,22 +1042,28 @@ WeightParams make_weight_params(double sigma_r, double sigma_b, double gate) {
 }
 
 // Joins observations with the landmark table (skips ids outside [0, lm_cap), cu:242).
-int join_obs(const pfc_obs* obs, int nobs, const float* lm, int lm_cap, ObsRec* out) {
-    int m = 0;
-    for (int k = 0; k < nobs; ++k) {
-        const pfc_obs& o = obs[k];
-        if (o.id < 0 || o.id >= lm_cap) continue;
-        ObsRec& r = out[m++];
-        r.lx = lm[3 * o.id];
-        r.ly = lm[3 * o.id + 1];
-        r.lz = lm[3 * o.id + 2];
-        r.vx = o.dx;
-        r.vy = o.dy;
-        r.vz = o.dz;
-        r.range = o.range;
-        r.mode = o.mode == 0 ? 0 : 1;
-        r.pad = 0;
+int join_obs(const pfc_obs* obs, int nobs, const float* lm, int lm_cap, ObsRec* out, int* nrb) {
+    // range-bearing records first (in input order), then Cartesian ones (in input order)
+    int m = 0, rb = 0;
+    for (int pass = 0; pass < 2; ++pass) {
+        for (int k = 0; k < nobs; ++k) {
+            const pfc_obs& o = obs[k];
+            if (o.id < 0 || o.id >= lm_cap) continue;
+            if ((o.mode == 0) != (pass == 1)) continue;
+            ObsRec& r = out[m++];
+            r.lx = lm[3 * o.id];
+            r.ly = lm[3 * o.id + 1];
+            r.lz = lm[3 * o.id + 2];
+            r.vx = o.dx;
+            r.vy = o.dy;
+            r.vz = o.dz;
+            r.range = o.range;
+            r.mode = o.mode == 0 ? 0 : 1;
+            r.pad = 0;
+        }
+        if (pass == 0) rb = m;
     }
+    if (nrb) *nrb = rb;
     return m;
 }
 
@@ -1152,13 +1185,15 @@ int pfc_weight(void* h, const void* obs, int nobs, float sigma_r, float sigma_b,
     if (nobs <= 0 || !obs) return launch_normalize(s, 1, nullptr);
     // the staging buffer may still feed the previous call's copy
     PF_CHECK(hipEventSynchronize(s->obs_ev));
-    const int m = join_obs(static_cast<const pfc_obs*>(obs), nobs, s->lm.data(), s->lm_cap, s->h_obs);
+    int nrb = 0;
+    const int m = join_obs(static_cast<const pfc_obs*>(obs), nobs, s->lm.data(), s->lm_cap, s->h_obs, &nrb);
     if (m == 0) return launch_normalize(s, 1, nullptr);  // every id unknown: ll = 0 for all -> uniform
     PF_CHECK(hipMemcpyAsync(s->d_obs, s->h_obs, sizeof(ObsRec) * (size_t)m, hipMemcpyHostToDevice, s->stream));
     PF_CHECK(hipEventRecord(s->obs_ev, s->stream));
     WeightParams p = make_weight_params(sigma_r, sigma_b, gate);
     p.obs = s->d_obs;
     p.nobs = m;
+    p.nrb = nrb;
     if (launch_weight(s, ParamRef<WeightParams>::value(p)) != 0) return -1;
     return launch_normalize(s, 0, nullptr);
 }
@@ -1407,6 +1442,13 @@ int pfx_quat_mean(const double* M, double* q) {
     if (!(nrm > 0.0)) return -1;
     const double sg = v[0][best] < 0.0 ? -1.0 : 1.0;
     for (int r = 0; r < 4; ++r) q[r] = sg * v[r][best] / nrm;
+    return 0;
+}
+
+int pfx_set_weight_variant(void* h, int variant) {
+    PfState* s = static_cast<PfState*>(h);
+    if (!s || (variant != 1 && variant != 2 && variant != 4)) return -1;
+    s->weight_ppt = variant;
     return 0;
 }
 

@@ -56,7 +56,7 @@ int pfb_cycle(void *h, const pfb_cycle_in *in, pfb_estimate *out);
 /* 1: replay the device stages from captured hipGraphs (captured on the next cycle). */
 int pfb_use_graph(void *h, int enable);
 /* Times the tunable launch shapes on the current state and keeps the fastest;
- * returns the chosen particles-per-lane of the likelihood kernel. */
+ * returns the chosen particles-per-lane of the likelihood kernel (1, 2 or 4). */
 int pfb_autotune(void *h, const pfb_cycle_in *in);
 /* Particles of filter f (pos 3n, quat 4n, weights n; any may be NULL). */
 int pfb_get_state(void *h, int f, float *pos3n, float *quat4n, float *w);

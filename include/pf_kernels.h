@@ -160,6 +160,9 @@ int pfx_particle_count(void *h);
  * unit eigenvector of the largest eigenvalue with w >= 0 (types.rs:128-144), by cyclic
  * Jacobi rotations.  Shared by the host layer and the batched path. */
 int pfx_quat_mean(const double *M16, double *q4);
+/* Likelihood kernel launch shape: 1, 2 or 4 particles per lane (default 1).  All shapes give
+ * bit-identical results. */
+int pfx_set_weight_variant(void *h, int variant);
 /* Blocks until every queued operation on the handle's stream has finished. */
 int pfx_sync(void *h);
 

@@ -236,6 +236,51 @@ def test_weight_obs_cap_truncates(gpu):
     check_weights(k, o)
 
 
+def _weight_case(case, seed):
+    q_t = (math.cos(0.4), 0.0, 0.0, math.sin(0.4))
+    truth = np.array([10.0, 10.0, 2.5])
+    if case == "config1":
+        lm = make_landmarks(256, seed, box=(50.0, 50.0, 10.0))
+        truth = np.array([25.0, 25.0, 5.0])
+        return lm, rb_obs_for(truth, q_t, lm, seed + 1, outliers=0.01), truth, q_t, 3.0, 0.5
+    lm = make_landmarks(37, seed)
+    rb = rb_obs_for(truth, q_t, lm, seed + 1)
+    ca = cart_obs_for(truth, q_t, lm, seed + 2)
+    if case == "rb_gated":
+        return lm, rb, truth, q_t, 3.0, 1.0
+    if case == "rb_wide":  # ungated, wide cloud: bearing residuals beyond the 6-term series
+        return lm, rb, truth, q_t, 0.0, 3.0
+    if case == "cart":
+        return lm, ca, truth, q_t, 3.0, 0.5
+    mixed = np.concatenate([ca[:5], rb[5:20], ca[20:], rb[:5], np.array([(99, 1, 0, 0, 1, 3)], OBS_DTYPE)])
+    return lm, mixed, truth, q_t, 0.0, 0.5
+
+
+@pytest.mark.parametrize("variant", [1, 2, 4])
+@pytest.mark.parametrize("case", ["rb_gated", "rb_wide", "cart", "mixed", "config1"])
+def test_weight_variants_match_oracle(gpu, variant, case):
+    """Every launch shape of the likelihood kernel (1, 2, 4 particles per lane) against the
+    oracle, for range-bearing (gated and ungated, wide clouds beyond the short bearing
+    series), Cartesian and mixed observation lists."""
+    n = 30_000 if case == "config1" else 5000
+    lm, obs, truth, q_t, gate, spread = _weight_case(case, 11)
+    k, o = pair(n, 21)
+    k.set_weight_variant(variant)
+    pose = f32([*truth, *q_t])
+    k.reinit(pose, f32([spread] * 3), f32([0.1, 0.1, 0.3]))
+    o.reinit(pose, f32([spread] * 3), f32([0.1, 0.1, 0.3]))
+    k.upload_landmarks(lm)
+    k.weight(obs, 0.05, 0.02, gate)
+    o.weight(obs, lm, float(np.float32(0.05)), float(np.float32(0.02)), float(np.float32(gate)))
+    ll = k.get_ll()
+    _, _, ow, oll = o.state()
+    dll = np.abs(ll - oll)
+    print(f"variant {variant} {case}: max |dll| {dll.max():.3e} (|ll| max {np.abs(oll).max():.3e})")
+    assert np.all(dll <= 1e-9 + 1e-12 * np.abs(oll)), f"max |dll| {dll.max():.3e}"
+    _, _, w = k.get_state()
+    assert np.abs(w.astype(np.float64) - ow).max() <= 1e-6 * ow.max()
+
+
 def weighted_state(n, seed, conc=1.0):
     pos, quat = random_state(n, seed)
     rng = np.random.default_rng(seed + 1)
