@@ -38,6 +38,8 @@ struct FilterDev {
     uint64_t lcg;     // resampling LCG state (pf_kernels.cu:116, 653)
     double sums[kEstN];
     double covs[kCovN];
+    double qmean[4];  // device Markley mean of the last estimate (wxyz, w >= 0)
+    int qvalid, pad;  // 0: degenerate moment matrix (qmean = identity)
 };
 
 struct Layout {
@@ -158,6 +160,8 @@ int launch_normalize(PfState* s, int force_uniform, const float* d_ratio);
 int launch_resample(PfState* s, ParamRef<InjectParams> inj, int any_inject);
 int launch_est_sums(PfState* s);
 int launch_est_covs(PfState* s, ParamRef<CovParams> A);
+// Markley mean of every filter's estimate sums on the device -> FilterDev.qmean, d_cov[f].
+int launch_quat_mean(PfState* s, CovParams* d_cov);
 int launch_predicate(PfState* s, const float* d_ratio, int force);
 WeightParams make_weight_params(double sigma_r, double sigma_b, double gate);
 // Joins observations with their landmarks (unknown ids dropped); range-bearing records

@@ -63,7 +63,7 @@ struct Batch {
     // graphs
     bool use_graph = false;
     bool captured = false;
-    hipGraphExec_t gA = nullptr, gB = nullptr;
+    hipGraphExec_t gA = nullptr;  // the captured cycle
 };
 
 ParamBlock carve(char* base, int nf) {
@@ -90,7 +90,6 @@ size_t params_size(int nf) {
 void free_batch(Batch* b) {
     if (b->s.stream) (void)hipStreamSynchronize(b->s.stream);
     if (b->gA) (void)hipGraphExecDestroy(b->gA);
-    if (b->gB) (void)hipGraphExecDestroy(b->gB);
     state_free_particles(&b->s);
     (void)hipFree(b->d_params);
     (void)hipHostFree(b->h_params);
@@ -106,18 +105,19 @@ void free_batch(Batch* b) {
     delete b;
 }
 
-// Device stages up to the estimate sums (graph A).
-int stages_a(Batch* b) {
+// Every device stage of a cycle (one graph): predict, weight, normalise (ESS + resample
+// predicate), resample (skipped on the device when the predicate is false), estimate sums,
+// Markley mean, covariances.
+int stages_all(Batch* b) {
     PfState* s = &b->s;
     if (launch_motion(s, ParamRef<MotionParams>::array(b->dp.motion), 0) != 0) return -1;
     if (launch_weight(s, ParamRef<WeightParams>::array(b->dp.weight)) != 0) return -1;
-    if (launch_normalize(s, 0, b->d_ratio) != 0) return -1;  // ESS + resample predicate
+    if (launch_normalize(s, 0, b->d_ratio) != 0) return -1;
     if (launch_resample(s, ParamRef<InjectParams>::array(b->dp.inject), 1) != 0) return -1;
     if (launch_est_sums(s) != 0) return -1;
-    return 0;
+    if (launch_quat_mean(s, b->dp.cov) != 0) return -1;
+    return launch_est_covs(s, ParamRef<CovParams>::array(b->dp.cov));
 }
-
-int stages_b(Batch* b) { return launch_est_covs(&b->s, ParamRef<CovParams>::array(b->dp.cov)); }
 
 int capture(Batch* b, int (*fn)(Batch*), hipGraphExec_t* out) {
     hipGraph_t g = nullptr;
@@ -135,27 +135,19 @@ int capture(Batch* b, int (*fn)(Batch*), hipGraphExec_t* out) {
     return 0;
 }
 
-int run_a(Batch* b) {
-    if (!b->use_graph) return stages_a(b);
-    if (!b->gA && capture(b, stages_a, &b->gA) != 0) return -1;
+int run_cycle(Batch* b) {
+    if (!b->use_graph) return stages_all(b);
+    if (!b->gA && capture(b, stages_all, &b->gA) != 0) return -1;
     PB_CHECK(hipGraphLaunch(b->gA, b->s.stream));
-    return 0;
-}
-
-int run_b(Batch* b) {
-    if (!b->use_graph) return stages_b(b);
-    if (!b->gB && capture(b, stages_b, &b->gB) != 0) return -1;
-    PB_CHECK(hipGraphLaunch(b->gB, b->s.stream));
     return 0;
 }
 
 void drop_graphs(Batch* b) {
     if (b->gA) (void)hipGraphExecDestroy(b->gA);
-    if (b->gB) (void)hipGraphExecDestroy(b->gB);
-    b->gA = b->gB = nullptr;
+    b->gA = nullptr;
 }
 
-// Host half of a cycle before graph A: parameters + joined observations -> device.
+// Host half of a cycle before the device stages: parameters + joined observations -> device.
 int upload_inputs(Batch* b, const pfb_cycle_in* in) {
     PfState* s = &b->s;
     int total = 0;
@@ -291,46 +283,24 @@ int pfb_cycle(void* h, const pfb_cycle_in* in, pfb_estimate* out) {
     if (!b || !in) return -1;
     PfState* s = &b->s;
     if (upload_inputs(b, in) != 0) return -1;
-    if (run_a(b) != 0) return -1;
-    PB_CHECK(hipMemcpyAsync(b->h_fdev, s->fdev, sizeof(FilterDev) * (size_t)b->nf, hipMemcpyDeviceToHost, s->stream));
-    PB_CHECK(hipStreamSynchronize(s->stream));
-    for (int f = 0; f < b->nf; ++f) {
-        const FilterDev& fd = b->h_fdev[f];
-        double q[4] = {1.0, 0.0, 0.0, 0.0};
-        double M[16];
-        int e = 3;
-        for (int r = 0; r < 4; ++r)
-            for (int c = r; c < 4; ++c) {
-                M[4 * r + c] = M[4 * c + r] = fd.sums[e];
-                ++e;
-            }
-        if (pfx_quat_mean(M, q) != 0) return -1;
-        CovParams& cp = b->hp.cov[f];
-        // mean position handed back at f32 precision (cuda.rs:291-294)
-        for (int k = 0; k < 3; ++k) cp.mp[k] = (double)(float)fd.sums[k];
-        for (int k = 0; k < 4; ++k) cp.mq[k] = (double)(float)q[k];
-        if (out) {
-            for (int k = 0; k < 3; ++k) out[f].pos[k] = fd.sums[k];
-            for (int k = 0; k < 4; ++k) out[f].quat[k] = q[k];
-            out[f].ess = fd.sums[13] > 0.0 ? 1.0 / fd.sums[13] : 0.0;
-            out[f].resampled = fd.do_resample;
-            out[f].valid = 1;
-        }
-    }
-    const size_t cov_off = reinterpret_cast<char*>(b->hp.cov) - b->h_params;
-    PB_CHECK(hipMemcpyAsync(b->d_params + cov_off, b->h_params + cov_off, sizeof(CovParams) * (size_t)b->nf,
-                            hipMemcpyHostToDevice, s->stream));
-    if (run_b(b) != 0) return -1;
+    if (run_cycle(b) != 0) return -1;
     PB_CHECK(hipMemcpyAsync(b->h_fdev, s->fdev, sizeof(FilterDev) * (size_t)b->nf, hipMemcpyDeviceToHost, s->stream));
     PB_CHECK(hipStreamSynchronize(s->stream));
     if (out) {
         static const int idx[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
-        for (int f = 0; f < b->nf; ++f)
+        for (int f = 0; f < b->nf; ++f) {
+            const FilterDev& fd = b->h_fdev[f];
+            for (int k = 0; k < 3; ++k) out[f].pos[k] = fd.sums[k];
+            for (int k = 0; k < 4; ++k) out[f].quat[k] = fd.qmean[k];
             for (int r = 0; r < 3; ++r)
                 for (int c = 0; c < 3; ++c) {
-                    out[f].pos_cov[3 * r + c] = b->h_fdev[f].covs[idx[r][c]];
-                    out[f].att_cov[3 * r + c] = b->h_fdev[f].covs[6 + idx[r][c]];
+                    out[f].pos_cov[3 * r + c] = fd.covs[idx[r][c]];
+                    out[f].att_cov[3 * r + c] = fd.covs[6 + idx[r][c]];
                 }
+            out[f].ess = fd.sums[13] > 0.0 ? 1.0 / fd.sums[13] : 0.0;
+            out[f].resampled = fd.do_resample;
+            out[f].valid = fd.qvalid;
+        }
     }
     return 0;
 }

@@ -11,6 +11,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <cmath>
+
 namespace pfd {
 
 constexpr uint64_t kPhi = 0x9E3779B97F4A7C15ull;
@@ -260,6 +262,61 @@ __device__ inline uint64_t qweight(float wf) {
     if (!(w > 0.0)) return 0;
     if (w >= 64.0) return (uint64_t)1 << 62;
     return (uint64_t)(w * 0x1.0p56);
+}
+
+// Dominant unit eigenvector of (M + M^T)/2 (4x4, row-major) by cyclic Jacobi rotations, sign
+// chosen so that w >= 0: the Markley quaternion mean (types.rs:128-144).  Host and device
+// run this same function; it uses only IEEE-exact operations (+ - * / sqrt), so both give
+// the same bits under -ffp-contract=off.  Returns -1 for a zero eigenvector.
+__host__ __device__ inline int quat_mean4(const double* M, double* q) {
+    double a[4][4], v[4][4];
+    for (int r = 0; r < 4; ++r)
+        for (int c = 0; c < 4; ++c) {
+            a[r][c] = 0.5 * (M[4 * r + c] + M[4 * c + r]);
+            v[r][c] = r == c ? 1.0 : 0.0;
+        }
+    for (int sweep = 0; sweep < 50; ++sweep) {
+        double off = 0.0, diag = 0.0;
+        for (int r = 0; r < 4; ++r) {
+            diag += a[r][r] * a[r][r];
+            for (int c = r + 1; c < 4; ++c) off += a[r][c] * a[r][c];
+        }
+        if (off <= 1e-32 * diag || off == 0.0) break;
+        for (int p = 0; p < 3; ++p)
+            for (int r = p + 1; r < 4; ++r) {
+                const double apr = a[p][r];
+                if (apr == 0.0) continue;
+                // rotation angle zeroing a[p][r]: tan(2 phi) = 2 a_pr / (a_rr - a_pp)
+                const double tau = (a[r][r] - a[p][p]) / (2.0 * apr);
+                const double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+                const double c = 1.0 / sqrt(1.0 + t * t), s = t * c;
+                for (int k = 0; k < 4; ++k) {
+                    const double kp = a[k][p], kr = a[k][r];
+                    a[k][p] = c * kp - s * kr;
+                    a[k][r] = s * kp + c * kr;
+                }
+                for (int k = 0; k < 4; ++k) {
+                    const double pk = a[p][k], rk = a[r][k];
+                    a[p][k] = c * pk - s * rk;
+                    a[r][k] = s * pk + c * rk;
+                }
+                for (int k = 0; k < 4; ++k) {
+                    const double kp = v[k][p], kr = v[k][r];
+                    v[k][p] = c * kp - s * kr;
+                    v[k][r] = s * kp + c * kr;
+                }
+            }
+    }
+    int best = 0;
+    for (int i = 1; i < 4; ++i)
+        if (a[i][i] > a[best][best]) best = i;
+    double nrm = 0.0;
+    for (int r = 0; r < 4; ++r) nrm += v[r][best] * v[r][best];
+    nrm = sqrt(nrm);
+    if (!(nrm > 0.0)) return -1;
+    const double sg = v[0][best] < 0.0 ? -1.0 : 1.0;
+    for (int r = 0; r < 4; ++r) q[r] = sg * v[r][best] / nrm;
+    return 0;
 }
 
 }  // namespace pfd

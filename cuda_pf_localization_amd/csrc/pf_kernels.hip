@@ -322,6 +322,34 @@ __global__ __launch_bounds__(kTB) void k_weight(Layout L, PoseBufs B, FilterDev*
     }
 }
 
+// Markley mean on the device (types.rs:128-144): one thread per filter builds the moment
+// matrix from the estimate sums (upper triangle, cuda.rs:277-286), takes its dominant unit
+// eigenvector with quat_mean4 (the host's function, same IEEE operation sequence), and
+// writes the covariance pass parameters with the mean handed over at f32 precision
+// (cuda.rs:291-294).  A degenerate moment matrix yields the identity and valid = 0.
+__global__ void k_quat_mean(FilterDev* F, int nfilt, CovParams* cov) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nfilt) return;
+    FilterDev& fd = F[f];
+    double M[16];
+    int e = 3;
+    for (int r = 0; r < 4; ++r)
+        for (int c = r; c < 4; ++c) {
+            M[4 * r + c] = M[4 * c + r] = fd.sums[e];
+            ++e;
+        }
+    double q[4] = {1.0, 0.0, 0.0, 0.0};
+    const int rc = quat_mean4(M, q);
+    if (rc != 0) {
+        q[0] = 1.0;
+        q[1] = q[2] = q[3] = 0.0;
+    }
+    for (int k = 0; k < 4; ++k) fd.qmean[k] = q[k];
+    fd.qvalid = rc == 0 ? 1 : 0;
+    for (int k = 0; k < 3; ++k) cov[f].mp[k] = (double)(float)fd.sums[k];
+    for (int k = 0; k < 4; ++k) cov[f].mq[k] = (double)(float)q[k];
+}
+
 // Per filter: M = max m_b, S = sum s_b e^(m_b - M), S2 = sum s2_b e^(2(m_b - M)) over the
 // filter's likelihood blocks, in a fixed order (one block per filter; the kernel boundary
 // orders it after every partial without device-scope fences).
@@ -661,17 +689,16 @@ __device__ inline void block_partials(double (&acc)[NV], double* __restrict__ pa
 // Per filter (one block each): 16 lanes per value sum the filter's block partials in a
 // fixed strided order, then a fixed 16-lane tree.  which = 0 -> sums, 1 -> covariances.
 template <int NV>
-__global__ __launch_bounds__(kTB) void k_est_fin(Layout L, FilterDev* F, const double* __restrict__ part, int which) {
-    static_assert(NV * 16 <= kTB, "one 16-lane group per value");
+__global__ __launch_bounds__(1024) void k_est_fin(Layout L, FilterDev* F, const double* __restrict__ part, int which) {
+    static_assert(NV <= 16, "one wave per value");
     const int f = blockIdx.x;
     const int b0 = L.f_off[f] / kTile, nb = L.f_padn[f] / kTile;
-    const int k = threadIdx.x >> 4, sub = threadIdx.x & 15;
+    const int k = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (k >= NV) return;
     double v = 0.0;
-    if (k < NV)
-        for (int b = sub; b < nb; b += 16) v += part[(size_t)NV * (b0 + b) + k];
-#pragma unroll
-    for (int d = 8; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
-    if (k < NV && sub == 0) (which == 0 ? F[f].sums : F[f].covs)[k] = v;
+    for (int b = lane; b < nb; b += 64) v += part[(size_t)NV * (b0 + b) + k];
+    v = wave_sum(v);
+    if (lane == 0) (which == 0 ? F[f].sums : F[f].covs)[k] = v;
 }
 
 // est layout (cu:354-404): [0..2] sum w p, [3..12] sum w q_r q_c (r<=c), [13] sum w^2.
@@ -995,7 +1022,15 @@ int launch_est_sums(PfState* s) {
     prof_end(s);
     PF_LAUNCH_CHECK();
     prof_begin(s, PFX_STAGE_FINALIZE);
-    hipLaunchKernelGGL(k_est_fin<kEstN>, dim3(s->nf), dim3(kTB), 0, s->stream, s->L, s->fdev, s->part_est, 0);
+    hipLaunchKernelGGL(k_est_fin<kEstN>, dim3(s->nf), dim3(1024), 0, s->stream, s->L, s->fdev, s->part_est, 0);
+    prof_end(s);
+    PF_LAUNCH_CHECK();
+    return 0;
+}
+
+int launch_quat_mean(PfState* s, CovParams* d_cov) {
+    prof_begin(s, PFX_STAGE_FINALIZE);
+    hipLaunchKernelGGL(k_quat_mean, dim3((s->nf + 63) / 64), dim3(64), 0, s->stream, s->fdev, s->nf, d_cov);
     prof_end(s);
     PF_LAUNCH_CHECK();
     return 0;
@@ -1008,7 +1043,7 @@ int launch_est_covs(PfState* s, ParamRef<CovParams> A) {
     prof_end(s);
     PF_LAUNCH_CHECK();
     prof_begin(s, PFX_STAGE_FINALIZE);
-    hipLaunchKernelGGL(k_est_fin<kCovN>, dim3(s->nf), dim3(kTB), 0, s->stream, s->L, s->fdev, s->part_cov, 1);
+    hipLaunchKernelGGL(k_est_fin<kCovN>, dim3(s->nf), dim3(1024), 0, s->stream, s->L, s->fdev, s->part_cov, 1);
     prof_end(s);
     PF_LAUNCH_CHECK();
     return 0;
@@ -1391,58 +1426,9 @@ int pfx_particle_count(void* h) {
     return (s && s->nf == 1) ? s->n[0] : 0;
 }
 
-// Largest-eigenvalue unit eigenvector of (M + M^T)/2 by cyclic Jacobi rotations,
-// sign chosen so that w >= 0 (types.rs:139-142).
 int pfx_quat_mean(const double* M, double* q) {
     if (!M || !q) return -1;
-    double a[4][4], v[4][4];
-    for (int r = 0; r < 4; ++r)
-        for (int c = 0; c < 4; ++c) {
-            a[r][c] = 0.5 * (M[4 * r + c] + M[4 * c + r]);
-            v[r][c] = r == c ? 1.0 : 0.0;
-        }
-    for (int sweep = 0; sweep < 50; ++sweep) {
-        double off = 0.0, diag = 0.0;
-        for (int r = 0; r < 4; ++r) {
-            diag += a[r][r] * a[r][r];
-            for (int c = r + 1; c < 4; ++c) off += a[r][c] * a[r][c];
-        }
-        if (off <= 1e-32 * diag || off == 0.0) break;
-        for (int p = 0; p < 3; ++p)
-            for (int r = p + 1; r < 4; ++r) {
-                const double apr = a[p][r];
-                if (apr == 0.0) continue;
-                // rotation angle zeroing a[p][r]: tan(2 phi) = 2 a_pr / (a_rr - a_pp)
-                const double tau = (a[r][r] - a[p][p]) / (2.0 * apr);
-                const double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
-                const double c = 1.0 / sqrt(1.0 + t * t), s = t * c;
-                for (int k = 0; k < 4; ++k) {
-                    const double kp = a[k][p], kr = a[k][r];
-                    a[k][p] = c * kp - s * kr;
-                    a[k][r] = s * kp + c * kr;
-                }
-                for (int k = 0; k < 4; ++k) {
-                    const double pk = a[p][k], rk = a[r][k];
-                    a[p][k] = c * pk - s * rk;
-                    a[r][k] = s * pk + c * rk;
-                }
-                for (int k = 0; k < 4; ++k) {
-                    const double kp = v[k][p], kr = v[k][r];
-                    v[k][p] = c * kp - s * kr;
-                    v[k][r] = s * kp + c * kr;
-                }
-            }
-    }
-    int best = 0;
-    for (int i = 1; i < 4; ++i)
-        if (a[i][i] > a[best][best]) best = i;
-    double nrm = 0.0;
-    for (int r = 0; r < 4; ++r) nrm += v[r][best] * v[r][best];
-    nrm = sqrt(nrm);
-    if (!(nrm > 0.0)) return -1;
-    const double sg = v[0][best] < 0.0 ? -1.0 : 1.0;
-    for (int r = 0; r < 4; ++r) q[r] = sg * v[r][best] / nrm;
-    return 0;
+    return quat_mean4(M, q);
 }
 
 int pfx_set_weight_variant(void* h, int variant) {
