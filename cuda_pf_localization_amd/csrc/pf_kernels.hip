@@ -167,13 +167,13 @@ __device__ inline BodyMap body_map(const PoseSet& S, int gi) {
 // acos(1 - u)^2 as a power series in u = 1 - dot: acos(1-u)^2 = sum_n 2^(n+1) u^n / (n^2 C(2n,n))
 // (from asin(x)^2 = 1/2 sum (2x)^(2n) / (n^2 C(2n,n)) with x^2 = u/2).  Six terms hold to
 // ~1e-18 relative for u <= 0.005, twelve for u <= 0.125.
-__device__ inline double acos_sq_u6(double u) {
+__host__ __device__ inline double acos_sq_u6(double u) {
     double p = 128.0 / 33264.0;
-    p = __fma_rn(p, u, 64.0 / 6300.0);
-    p = __fma_rn(p, u, 32.0 / 1120.0);
-    p = __fma_rn(p, u, 16.0 / 180.0);
-    p = __fma_rn(p, u, 8.0 / 24.0);
-    p = __fma_rn(p, u, 2.0);
+    p = fma(p, u, 64.0 / 6300.0);
+    p = fma(p, u, 32.0 / 1120.0);
+    p = fma(p, u, 16.0 / 180.0);
+    p = fma(p, u, 8.0 / 24.0);
+    p = fma(p, u, 2.0);
     return u * p;
 }
 __device__ inline double acos_sq_u12(double u) {
@@ -198,7 +198,10 @@ __device__ inline double acos_sq_u12(double u) {
 //   b = M (l - p) is formed as M l - M p (three FMAs per component).
 //   Range-bearing: rn from an f32 rsqrt seed refined once in f64 (relative ~1e-14);
 //   u = 1 - dot = 1 + pv/rn in f64; eb^2 from the series in u.  With a gate narrower than
-//   u = 0.005 the series is evaluated at min(u, u_gate) and the min() does the gating.
+//   u = 0.005 the series is evaluated at min(u, u_gate), u_gate being the largest u whose
+//   series value does not exceed the gate's g^2 sb^2: that min() alone does the gating.  u is
+//   clamped at 0 (an f32 direction of norm 1 + 1e-7 can put it 1e-7 below), as the
+//   reference clamps the cosine.
 __device__ inline void body_offset(const BodyMap& b, const ObsRec& o, double& bx, double& by, double& bz) {
     bx = __fma_rn(b.m00, o.lx, __fma_rn(b.m01, o.ly, __fma_rn(b.m02, o.lz, -b.mpx)));
     by = __fma_rn(b.m10, o.lx, __fma_rn(b.m11, o.ly, __fma_rn(b.m12, o.lz, -b.mpy)));
@@ -215,20 +218,19 @@ __device__ inline void rb_from_b(double bx, double by, double bz, double vx, dou
     const double y = __fma_rn(y0, __fma_rn(-hy, y0, 0.5), y0);
     const double er = __fma_rn(r2, y, -range);
     acc = __fma_rn(-fmin(er * er, p.rng_gate2), irr, acc);
-    double u = fmax(__fma_rn(pv, y, 1.0), 0.0);
-    double eb2;
     if (kFastBearing) {
-        eb2 = acos_sq_u6(fmin(u, p.u_gate));
+        acc = __fma_rn(-acos_sq_u6(fmin(fmax(__fma_rn(pv, y, 1.0), 0.0), p.u_gate)), ibb, acc);
+        return;
+    }
+    const double u = fmin(fmax(__fma_rn(pv, y, 1.0), 0.0), 2.0);
+    double eb2;
+    if (u <= 0.005) {
+        eb2 = acos_sq_u6(u);
+    } else if (u <= 0.125) {
+        eb2 = acos_sq_u12(u);
     } else {
-        u = fmin(u, 2.0);
-        if (u <= 0.005) {
-            eb2 = acos_sq_u6(u);
-        } else if (u <= 0.125) {
-            eb2 = acos_sq_u12(u);
-        } else {
-            const double eb = acos(1.0 - u);
-            eb2 = eb * eb;
-        }
+        const double eb = acos(1.0 - u);
+        eb2 = eb * eb;
     }
     acc = __fma_rn(-fmin(eb2, p.brg_gate2), ibb, acc);
 }
@@ -267,6 +269,7 @@ __global__ __launch_bounds__(kTB) void k_weight(Layout L, PoseBufs B, FilterDev*
     ConstObsPtr obs = (ConstObsPtr)p.obs;
     const int nobs = p.nobs, nrb = p.nrb;  // range-bearing records first, then Cartesian
     if (p.u_gate <= 0.005) {
+#pragma unroll 2
         for (int k = 0; k < nrb; ++k) {
             const ObsRec o = obs[k];
 #pragma unroll
@@ -1069,10 +1072,25 @@ WeightParams make_weight_params(double sigma_r, double sigma_b, double gate) {
     p.cart_gate2 = gated ? g2 * 3.0 * sr * sr : INFINITY;
     p.rng_gate2 = gated ? g2 * sr * sr : INFINITY;
     p.brg_gate2 = gated ? g2 * sb * sb : INFINITY;
-    // u = 1 - cos(eb) beyond which the bearing term is certainly gated (slightly above the
-    // exact threshold: the series stays monotone there and min() with brg_gate2 gates)
+    // Bearing gate in u = 1 - cos(eb): the largest u whose 6-term series value is <= g^2 sb^2
+    // (bisection over doubles with the kernel's own series), so that series(min(u, u_gate))
+    // is the gated term; +inf when ungated.
     const double gb = gate * sb;
-    p.u_gate = (gated && gb < 0.5) ? (1.0 - cos(gb)) * (1.0 + 1e-6) + 1e-300 : INFINITY;
+    p.u_gate = INFINITY;
+    if (gated && gb < 0.5) {
+        const double target = p.brg_gate2;
+        double lo = 0.0, hi = 2.0 * (1.0 - cos(gb)) + 1e-300;
+        while (acos_sq_u6(hi) <= target) hi *= 2.0;
+        for (int it = 0; it < 2000; ++it) {
+            const double mid = 0.5 * (lo + hi);
+            if (!(mid > lo && mid < hi)) break;
+            if (acos_sq_u6(mid) <= target)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        p.u_gate = lo;
+    }
     return p;
 }
 
