@@ -119,22 +119,21 @@ __device__ inline float cast_ray(const uint32_t* bits, int W, int H, float res, 
     float tmy = ady > 1e-9f ? ((diry > 0.f ? 1.f - fry : fry) * tdy) : INFINITY;
     int idx = cy * W + cx;
     const int dxi = sx, dyi = sy * W;
+    // Branch-free step: both candidate boundary times are formed and one is selected, which
+    // rounds exactly as the reference's if/else; the grid word is read only in bounds.
     for (int step = 0; step < 8192; ++step) {
-        float dist;
-        if (tmx <= tmy) {
-            dist = tmx;
-            tmx += tdx;
-            cx += sx;
-            idx += dxi;
-        } else {
-            dist = tmy;
-            tmy += tdy;
-            cy += sy;
-            idx += dyi;
-        }
-        if (dist > rmax) return rmax;
-        if ((unsigned)cx >= (unsigned)W || (unsigned)cy >= (unsigned)H) return rmax;
-        if (occ_bit(bits, idx)) return dist;
+        const bool xs = tmx <= tmy;
+        const float dist = xs ? tmx : tmy;
+        const float tmx1 = tmx + tdx, tmy1 = tmy + tdy;
+        tmx = xs ? tmx1 : tmx;
+        tmy = xs ? tmy : tmy1;
+        cx += xs ? sx : 0;
+        cy += xs ? 0 : sy;
+        idx += xs ? dxi : dyi;
+        const bool inb = (unsigned)cx < (unsigned)W && (unsigned)cy < (unsigned)H;
+        const bool occ = occ_bit(bits, inb ? idx : 0) && inb;
+        if (dist > rmax || !inb) return rmax;
+        if (occ) return dist;
     }
     return rmax;
 }
