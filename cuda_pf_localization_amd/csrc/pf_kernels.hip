@@ -167,6 +167,15 @@ __device__ inline BodyMap body_map(const PoseSet& S, int gi) {
 // acos(1 - u)^2 as a power series in u = 1 - dot: acos(1-u)^2 = sum_n 2^(n+1) u^n / (n^2 C(2n,n))
 // (from asin(x)^2 = 1/2 sum (2x)^(2n) / (n^2 C(2n,n)) with x^2 = u/2).  Six terms hold to
 // ~1e-18 relative for u <= 0.005, twelve for u <= 0.125.
+// Four terms for the gated fast path (u <= u_gate <= 0.005): the first omitted term,
+// (64/6300) u^5, is below 3.2e-14 there, i.e. < 1e-11 relative.
+__host__ __device__ inline double acos_sq_u4(double u) {
+    double p = 32.0 / 1120.0;
+    p = fma(p, u, 16.0 / 180.0);
+    p = fma(p, u, 8.0 / 24.0);
+    p = fma(p, u, 2.0);
+    return u * p;
+}
 __host__ __device__ inline double acos_sq_u6(double u) {
     double p = 128.0 / 33264.0;
     p = fma(p, u, 64.0 / 6300.0);
@@ -196,7 +205,7 @@ __device__ inline double acos_sq_u12(double u) {
 //   The reference's gate is continuous: the gated constant -g^2/2 equals -G inv2s at the
 //   threshold G = g^2 s^2, so a gated term is -min(e^2, G) inv2s (G = +inf when gate = 0).
 //   b = M (l - p) is formed as M l - M p (three FMAs per component).
-//   Range-bearing: rn from an f32 rsqrt seed refined once in f64 (relative ~1e-14);
+//   Range-bearing: rn from the v_rsq_f64 seed refined once by Newton (relative ~1e-16);
 //   u = 1 - dot = 1 + pv/rn in f64; eb^2 from the series in u.  With a gate narrower than
 //   u = 0.005 the series is evaluated at min(u, u_gate), u_gate being the largest u whose
 //   series value does not exceed the gate's g^2 sb^2: that min() alone does the gating.  u is
@@ -213,13 +222,13 @@ __device__ inline void rb_from_b(double bx, double by, double bz, double vx, dou
                                  double irr, double ibb, const WeightParams& p, double& acc) {
     const double r2 = fmax(__fma_rn(bx, bx, __fma_rn(by, by, bz * bz)), 1e-18);  // rn >= 1e-9
     const double pv = __fma_rn(bx, vx, __fma_rn(by, vy, bz * vz));
-    const double y0 = (double)__frsqrt_rn((float)r2);
+    const double y0 = __builtin_amdgcn_rsq(r2);  // v_rsq_f64 seed, refined once below
     const double hy = 0.5 * r2 * y0;
     const double y = __fma_rn(y0, __fma_rn(-hy, y0, 0.5), y0);
     const double er = __fma_rn(r2, y, -range);
     acc = __fma_rn(-fmin(er * er, p.rng_gate2), irr, acc);
     if (kFastBearing) {
-        acc = __fma_rn(-acos_sq_u6(fmin(fmax(__fma_rn(pv, y, 1.0), 0.0), p.u_gate)), ibb, acc);
+        acc = __fma_rn(-acos_sq_u4(fmin(fmax(__fma_rn(pv, y, 1.0), 0.0), p.u_gate)), ibb, acc);
         return;
     }
     const double u = fmin(fmax(__fma_rn(pv, y, 1.0), 0.0), 2.0);
@@ -1072,7 +1081,7 @@ WeightParams make_weight_params(double sigma_r, double sigma_b, double gate) {
     p.cart_gate2 = gated ? g2 * 3.0 * sr * sr : INFINITY;
     p.rng_gate2 = gated ? g2 * sr * sr : INFINITY;
     p.brg_gate2 = gated ? g2 * sb * sb : INFINITY;
-    // Bearing gate in u = 1 - cos(eb): the largest u whose 6-term series value is <= g^2 sb^2
+    // Bearing gate in u = 1 - cos(eb): the largest u whose 4-term series value is <= g^2 sb^2
     // (bisection over doubles with the kernel's own series), so that series(min(u, u_gate))
     // is the gated term; +inf when ungated.
     const double gb = gate * sb;
@@ -1080,11 +1089,11 @@ WeightParams make_weight_params(double sigma_r, double sigma_b, double gate) {
     if (gated && gb < 0.5) {
         const double target = p.brg_gate2;
         double lo = 0.0, hi = 2.0 * (1.0 - cos(gb)) + 1e-300;
-        while (acos_sq_u6(hi) <= target) hi *= 2.0;
+        while (acos_sq_u4(hi) <= target) hi *= 2.0;
         for (int it = 0; it < 2000; ++it) {
             const double mid = 0.5 * (lo + hi);
             if (!(mid > lo && mid < hi)) break;
-            if (acos_sq_u6(mid) <= target)
+            if (acos_sq_u4(mid) <= target)
                 lo = mid;
             else
                 hi = mid;
