@@ -107,6 +107,7 @@ KERNEL_SYMBOLS = [
     ("pfx_set_lcg", C.c_int, [vp, u64]),
     ("pfx_get_lcg", C.c_int, [vp, P(u64)]),
     ("pfx_get_rng", C.c_int, [vp, P(u64), P(u64)]),
+    ("pfx_set_rng", C.c_int, [vp, u64, u64]),
     ("pfx_weight_scalars", C.c_int, [vp, P(f64)]),
     ("pfx_profile_enable", C.c_int, [vp, C.c_int]),
     ("pfx_profile_read", C.c_int, [vp, P(f64), P(C.c_int)]),
@@ -178,6 +179,12 @@ HOST_SYMBOLS = [
     ("pfh_gl_set_seed", C.c_int, [vp, u64]),
     ("pfh_gl_kernels", vp, [vp]),
     ("pfh_gl_localize_learned", C.c_int, [vp, vp, P(f32), C.c_int, vp, P(f64), vp]),
+    ("pfh_map_from_pgm_yaml", C.c_int, [P(C.c_uint8), C.c_size_t, C.c_char_p, P(C.c_int), P(C.c_int), P(f64),
+                                        P(f64), P(f64), P(C.c_uint8), C.c_size_t, C.c_char_p, C.c_int]),
+    ("pfh_landmarks_from_text", C.c_int, [C.c_char_p, P(i32), P(f64), C.c_int, C.c_char_p, C.c_int]),
+    ("pfh_filter_reinit_from_gl", C.c_int, [vp, P(f64), f64, P(f64), P(f64)]),
+    ("pfh_filter_save", C.c_int, [vp, C.c_char_p]),
+    ("pfh_filter_load", C.c_int, [vp, C.c_char_p]),
 ]
 
 OPTIONAL_KERNEL_SYMBOLS: list = []  # filled by modules that add entry points (batched, learned)

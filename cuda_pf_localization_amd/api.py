@@ -40,6 +40,55 @@ def _check(rc, what):
         raise RuntimeError(f"{what} failed (rc={rc})")
 
 
+def _text(x) -> str:
+    """x itself when it looks like file content (has a newline or a ':'), else the file it names."""
+    if isinstance(x, (bytes, bytearray)):
+        return bytes(x).decode()
+    if "\n" in x or ":" in x:
+        return x
+    with open(x) as f:
+        return f.read()
+
+
+def load_map_pgm_yaml(pgm, yaml):
+    """ROS map_server pair (global_loc.rs:32-130) -> (cells[H, W] uint8, res, (ox, oy)); row 0 is
+    the bottom of the map.  `pgm` is a path or the file bytes, `yaml` a path or the text."""
+    H = _lib.host()
+    if isinstance(pgm, (bytes, bytearray)):
+        pb = bytes(pgm)
+    else:
+        with open(pgm, "rb") as f:
+            pb = f.read()
+    yt = _text(yaml)
+    buf = np.frombuffer(bytes(pb), np.uint8)
+    w, h = C.c_int(), C.c_int()
+    res, ox, oy = C.c_double(), C.c_double(), C.c_double()
+    err = C.create_string_buffer(256)
+    args = (buf.ctypes.data_as(C.POINTER(C.c_uint8)), buf.size, yt.encode(), C.byref(w), C.byref(h), C.byref(res),
+            C.byref(ox), C.byref(oy))
+    if H.pfh_map_from_pgm_yaml(*args, None, 0, err, 256) != 0:
+        raise ValueError(err.value.decode())
+    cells = np.zeros((h.value, w.value), np.uint8)
+    if H.pfh_map_from_pgm_yaml(*args, cells.ctypes.data_as(C.POINTER(C.c_uint8)), cells.size, err, 256) != 0:
+        raise ValueError(err.value.decode())
+    return cells, res.value, (ox.value, oy.value)
+
+
+def load_landmarks(text_or_path) -> dict:
+    """Landmark text (landmark_map.rs:59-80): `id: [x, y, z]` or `id x y z` lines -> {id: xyz};
+    a repeated id replaces the earlier entry."""
+    text = _text(text_or_path)
+    H = _lib.host()
+    cap = text.count("\n") + 1
+    ids = np.zeros(cap, np.int32)
+    xyz = np.zeros((cap, 3))
+    err = C.create_string_buffer(256)
+    n = H.pfh_landmarks_from_text(text.encode(), _ip(ids), _dp(xyz), cap, err, 256)
+    if n < 0:
+        raise ValueError(err.value.decode())
+    return {int(ids[i]): tuple(float(v) for v in xyz[i]) for i in range(n)}
+
+
 def device_available() -> bool:
     return _lib.kernels().pfc_device_available() == 1
 
@@ -368,6 +417,22 @@ class ParticleFilter:
         ids = np.array(sorted(landmarks), np.int32)
         xyz = np.array([landmarks[i] for i in ids], np.float64).reshape(-1, 3)
         _check(self.H.pfh_filter_set_map(self.f, _ip(ids), _dp(xyz), len(ids)), "pfh_filter_set_map")
+
+    def reinit_from_gl(self, xyt, z: float = 0.0, pos_std=(0.25, 0.25, 0.1), rot_std=(0.02, 0.02, 0.1)):
+        """GL -> PF hand-off: reinitialise about a global-localisation result (x, y, yaw)."""
+        a = np.ascontiguousarray(np.asarray(xyt, np.float64)[:3])
+        ps = np.ascontiguousarray(pos_std, np.float64)
+        rs = np.ascontiguousarray(rot_std, np.float64)
+        _check(self.H.pfh_filter_reinit_from_gl(self.f, _dp(a), float(z), _dp(ps), _dp(rs)),
+               "pfh_filter_reinit_from_gl")
+
+    def save(self, path: str):
+        """Checkpoint: particles, stored weights and draw state."""
+        _check(self.H.pfh_filter_save(self.f, str(path).encode()), "pfh_filter_save")
+
+    def load(self, path: str):
+        """Resume from a checkpoint of a filter with the same particle count."""
+        _check(self.H.pfh_filter_load(self.f, str(path).encode()), "pfh_filter_load")
 
     def reinitialize(self, pose7, pos_std, rot_std):
         p = np.asarray(pose7, np.float64)

@@ -15,6 +15,11 @@
 #include <array>
 #include <optional>
 #include <vector>
+#include <sstream>
+#include <string>
+#include <cctype>
+#include <climits>
+#include <algorithm>
 
 #include "pf_gl.h"
 #include "pf_host.h"
@@ -394,6 +399,266 @@ int pfh_filter_snapshot(void* fp, double* pos3n, double* quat4n, double* w) {
 void* pfh_filter_kernels(void* fp) {
     ParticleFilter* f = static_cast<ParticleFilter*>(fp);
     return f ? f->backend().handle() : nullptr;
+}
+
+int pfh_filter_reinit_from_gl(void* fp, const double* xyt, double z, const double* pos_std, const double* rot_std) {
+    ParticleFilter* f = static_cast<ParticleFilter*>(fp);
+    if (!f || !xyt || !pos_std || !rot_std) return -1;
+    const double half = 0.5 * xyt[2];
+    const double pose7[7] = {xyt[0], xyt[1], z, std::cos(half), 0.0, 0.0, std::sin(half)};
+    return f->reinitialize(pose7, pos_std, rot_std);
+}
+
+namespace {
+constexpr char kCkptMagic[8] = {'P', 'F', 'H', 'C', 'K', 'P', 'T', '1'};
+}
+
+int pfh_filter_save(void* fp, const char* path) {
+    ParticleFilter* f = static_cast<ParticleFilter*>(fp);
+    if (!f || !path) return -1;
+    void* h = f->backend().handle();
+    const int64_t n = f->backend().n();
+    std::vector<float> pos((size_t)n * 3), quat((size_t)n * 4), w((size_t)n);
+    uint64_t gseed = 0, gen = 0, lcg = 0;
+    if (pfx_get_state(h, pos.data(), quat.data(), w.data()) != 0 || pfx_get_rng(h, &gseed, &gen) != 0 ||
+        pfx_get_lcg(h, &lcg) != 0)
+        return -1;
+    FILE* fo = fopen(path, "wb");
+    if (!fo) {
+        fprintf(stderr, "pf_host: cannot write %s\n", path);
+        return -1;
+    }
+    bool ok = fwrite(kCkptMagic, 1, 8, fo) == 8 && fwrite(&n, sizeof(n), 1, fo) == 1 &&
+              fwrite(&gseed, 8, 1, fo) == 1 && fwrite(&gen, 8, 1, fo) == 1 && fwrite(&lcg, 8, 1, fo) == 1;
+    ok = ok && fwrite(pos.data(), 4, pos.size(), fo) == pos.size() &&
+         fwrite(quat.data(), 4, quat.size(), fo) == quat.size() && fwrite(w.data(), 4, w.size(), fo) == w.size();
+    ok = (fclose(fo) == 0) && ok;
+    return ok ? 0 : -1;
+}
+
+int pfh_filter_load(void* fp, const char* path) {
+    ParticleFilter* f = static_cast<ParticleFilter*>(fp);
+    if (!f || !path) return -1;
+    FILE* fi = fopen(path, "rb");
+    if (!fi) {
+        fprintf(stderr, "pf_host: cannot read %s\n", path);
+        return -1;
+    }
+    char magic[8];
+    int64_t n = 0;
+    uint64_t gseed = 0, gen = 0, lcg = 0;
+    bool ok = fread(magic, 1, 8, fi) == 8 && memcmp(magic, kCkptMagic, 8) == 0 && fread(&n, sizeof(n), 1, fi) == 1 &&
+              fread(&gseed, 8, 1, fi) == 1 && fread(&gen, 8, 1, fi) == 1 && fread(&lcg, 8, 1, fi) == 1;
+    if (!ok || n != f->backend().n()) {
+        fclose(fi);
+        fprintf(stderr, "pf_host: %s is not a checkpoint of a %lld-particle filter\n", path,
+                (long long)f->backend().n());
+        return -1;
+    }
+    std::vector<float> pos((size_t)n * 3), quat((size_t)n * 4), w((size_t)n);
+    ok = fread(pos.data(), 4, pos.size(), fi) == pos.size() && fread(quat.data(), 4, quat.size(), fi) == quat.size() &&
+         fread(w.data(), 4, w.size(), fi) == w.size();
+    fclose(fi);
+    if (!ok) return -1;
+    void* h = f->backend().handle();
+    if (pfx_set_state(h, pos.data(), quat.data(), w.data()) != 0 || pfx_set_rng(h, gseed, gen) != 0 ||
+        pfx_set_lcg(h, lcg) != 0)
+        return -1;
+    return 0;
+}
+
+namespace {
+void set_err(char* err, int len, const std::string& msg) {
+    if (err && len > 0) {
+        const size_t k = std::min((size_t)len - 1, msg.size());
+        memcpy(err, msg.data(), k);
+        err[k] = 0;
+    }
+}
+
+// Last value of `key: <number>` in the YAML text (comments stripped), global_loc.rs:35-45.
+bool yaml_number(const std::string& yaml, const std::string& key, double* out) {
+    bool hit = false;
+    std::istringstream in(yaml);
+    std::string line;
+    while (std::getline(in, line)) {
+        std::string t = line.substr(0, line.find('#'));
+        const size_t b = t.find_first_not_of(" \t\r");
+        if (b == std::string::npos) continue;
+        t = t.substr(b);
+        t.erase(t.find_last_not_of(" \t\r") + 1);
+        if (t.compare(0, key.size(), key) != 0) continue;
+        std::string rest = t.substr(key.size());
+        const size_t c = rest.find_first_not_of(" \t");
+        if (c == std::string::npos || rest[c] != ':') continue;
+        rest = rest.substr(c + 1);
+        const size_t v0 = rest.find_first_not_of(" \t");
+        if (v0 == std::string::npos) continue;
+        rest = rest.substr(v0);
+        char* end = nullptr;
+        const double v = strtod(rest.c_str(), &end);
+        if (end && *end == 0 && end != rest.c_str()) {
+            *out = v;
+            hit = true;
+        } else {
+            hit = false;  // the reference keeps the last matching line and fails its parse
+        }
+    }
+    return hit;
+}
+}  // namespace
+
+int pfh_map_from_pgm_yaml(const uint8_t* pgm, size_t pgm_len, const char* yaml_c, int* W, int* H, double* res,
+                          double* ox, double* oy, uint8_t* cells, size_t cells_cap, char* err, int err_len) {
+    if (!pgm || !yaml_c || !W || !H || !res || !ox || !oy) return -1;
+    const std::string yaml(yaml_c);
+    if (!yaml_number(yaml, "resolution", res)) {
+        set_err(err, err_len, "yaml: no resolution");
+        return -1;
+    }
+    // origin: first line starting with "origin", values after '['
+    {
+        std::istringstream in(yaml);
+        std::string line;
+        bool found = false;
+        while (std::getline(in, line)) {
+            const size_t b = line.find_first_not_of(" \t");
+            if (b == std::string::npos || line.compare(b, 6, "origin") != 0) continue;
+            const size_t br = line.find('[');
+            if (br == std::string::npos) break;
+            std::string body = line.substr(br + 1);
+            const size_t close = body.find(']');
+            if (close != std::string::npos) body = body.substr(0, close);
+            std::vector<double> vals;
+            std::istringstream vs(body);
+            std::string tok;
+            while (std::getline(vs, tok, ',')) {
+                const size_t t0 = tok.find_first_not_of(" \t");
+                if (t0 == std::string::npos) continue;
+                const size_t t1 = tok.find_last_not_of(" \t\r");
+                tok = tok.substr(t0, t1 - t0 + 1);
+                char* end = nullptr;
+                const double v = strtod(tok.c_str(), &end);
+                if (end && *end == 0 && end != tok.c_str()) vals.push_back(v);
+            }
+            *ox = vals.size() > 0 ? vals[0] : 0.0;
+            *oy = vals.size() > 1 ? vals[1] : 0.0;
+            found = true;
+            break;
+        }
+        if (!found) {
+            set_err(err, err_len, "yaml: no origin");
+            return -1;
+        }
+    }
+    double occ_th = 0.65;
+    (void)yaml_number(yaml, "occupied_thresh", &occ_th);
+    // P5 header tokens (whitespace separated, '#' comments), one whitespace after maxval
+    size_t idx = 0;
+    auto token = [&](std::string& out) -> bool {
+        for (;;) {
+            while (idx < pgm_len && isspace(pgm[idx])) ++idx;
+            if (idx < pgm_len && pgm[idx] == '#') {
+                while (idx < pgm_len && pgm[idx] != '\n') ++idx;
+                continue;
+            }
+            break;
+        }
+        const size_t start = idx;
+        while (idx < pgm_len && !isspace(pgm[idx])) ++idx;
+        if (start == idx) return false;
+        out.assign(reinterpret_cast<const char*>(pgm + start), idx - start);
+        return true;
+    };
+    std::string tok;
+    if (!token(tok)) {
+        set_err(err, err_len, "pgm: truncated");
+        return -1;
+    }
+    if (tok != "P5") {
+        set_err(err, err_len, "pgm: expected P5, got " + tok);
+        return -1;
+    }
+    long dims[3];
+    const char* what[3] = {"pgm: bad width", "pgm: bad height", "pgm: bad maxval"};
+    for (int k = 0; k < 3; ++k) {
+        if (!token(tok)) {
+            set_err(err, err_len, "pgm: truncated");
+            return -1;
+        }
+        char* end = nullptr;
+        dims[k] = strtol(tok.c_str(), &end, 10);
+        if (!end || *end != 0 || dims[k] < 0) {
+            set_err(err, err_len, what[k]);
+            return -1;
+        }
+    }
+    idx += 1;
+    const size_t w = (size_t)dims[0], h = (size_t)dims[1], need = w * h;
+    if (pgm_len < idx + need) {
+        set_err(err, err_len,
+                "pgm: need " + std::to_string(need) + " px, have " + std::to_string(pgm_len > idx ? pgm_len - idx : 0));
+        return -1;
+    }
+    *W = (int)w;
+    *H = (int)h;
+    if (!cells) return 0;
+    if (cells_cap < need) {
+        set_err(err, err_len, "output buffer too small");
+        return -1;
+    }
+    for (size_t y = 0; y < h; ++y)
+        for (size_t x = 0; x < w; ++x) {
+            const double occ_p = (double)(255 - pgm[idx + y * w + x]) / 255.0;
+            cells[(h - 1 - y) * w + x] = occ_p > occ_th ? 1 : 0;  // rows top-down in the file
+        }
+    return 0;
+}
+
+int pfh_landmarks_from_text(const char* text, int32_t* ids, double* xyz, int cap, char* err, int err_len) {
+    if (!text) return -1;
+    std::istringstream in(text);
+    std::string line;
+    int count = 0, lineno = 0;
+    while (std::getline(in, line)) {
+        ++lineno;
+        line = line.substr(0, line.find('#'));
+        for (char& c : line)
+            if (c == '[' || c == ']' || c == ':' || c == ',') c = ' ';
+        std::istringstream ls(line);
+        std::vector<std::string> nums;
+        std::string t;
+        while (ls >> t) nums.push_back(t);
+        if (nums.empty()) continue;
+        if (nums.size() != 4) {
+            set_err(err, err_len, "line " + std::to_string(lineno) + ": expected 'id: [x, y, z]'");
+            return -1;
+        }
+        double v[4];
+        for (int k = 0; k < 4; ++k) {
+            char* end = nullptr;
+            v[k] = strtod(nums[k].c_str(), &end);
+            if (!end || *end != 0) {
+                set_err(err, err_len, "line " + std::to_string(lineno) + ": invalid float literal");
+                return -1;
+            }
+        }
+        // `as i32` of the parsed f64: truncation toward zero, saturating, NaN -> 0
+        const double iv = v[0] != v[0] ? 0.0 : std::trunc(v[0]);
+        const int32_t id = iv >= 2147483647.0 ? INT32_MAX : (iv <= -2147483648.0 ? INT32_MIN : (int32_t)iv);
+        if (id < 0) {
+            set_err(err, err_len, "line " + std::to_string(lineno) + ": landmark ids must be non-negative");
+            return -1;
+        }
+        if (count < cap && ids && xyz) {
+            ids[count] = id;
+            xyz[3 * count] = v[1];
+            xyz[3 * count + 1] = v[2];
+            xyz[3 * count + 2] = v[3];
+        }
+        ++count;
+    }
+    return count;
 }
 
 int pfh_quat_mean(const double M[16], double q[4]) {

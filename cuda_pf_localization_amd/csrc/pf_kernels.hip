@@ -1407,6 +1407,14 @@ int pfx_get_rng(void* h, uint64_t* gseed, uint64_t* gen) {
     return 0;
 }
 
+int pfx_set_rng(void* h, uint64_t gseed, uint64_t gen) {
+    PfState* s = static_cast<PfState*>(h);
+    if (!s) return -1;
+    s->gseed = gseed;
+    s->gen = gen;
+    return 0;
+}
+
 int pfx_weight_scalars(void* h, double* out4) {
     PfState* s = static_cast<PfState*>(h);
     if (!s || s->nf < 1 || !out4) return -1;

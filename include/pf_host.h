@@ -98,6 +98,28 @@ typedef struct pfh_learned_info {
 int pfh_gl_localize_learned(void *g, void *mlp, const float *scan, int nrays, const pfh_learned_params *lp,
                             double *out4k, pfh_learned_info *info);
 
+/* ---- on-disk formats ----
+ * ROS map_server pair (global_loc.rs:32-130): P5 PGM bytes + YAML text (resolution, origin
+ * [x, y, ...], occupied_thresh default 0.65).  A cell is occupied (1) when
+ * (255 - pixel) / 255 > occupied_thresh; rows are flipped so that row 0 is the bottom.
+ * cells may be NULL to query W and H.  Returns 0, or -1 with a message in err. */
+int pfh_map_from_pgm_yaml(const uint8_t *pgm, size_t pgm_len, const char *yaml, int *W, int *H, double *res,
+                          double *ox, double *oy, uint8_t *cells, size_t cells_cap, char *err, int err_len);
+/* Landmark text (landmark_map.rs:59-80): lines `id: [x, y, z]` or `id x y z`, '#' starts a
+ * comment.  Returns the number of entries written to ids[] / xyz[3 i ..] (at most cap), or
+ * -1 with "line N: ..." in err. */
+int pfh_landmarks_from_text(const char *text, int32_t *ids, double *xyz, int cap, char *err, int err_len);
+
+/* GL -> PF hand-off (README:84-86): reinitialise the filter about (x, y, z) with the yaw of
+ * the localisation result as a rotation about z. */
+int pfh_filter_reinit_from_gl(void *f, const double *xyt3, double z, const double *pos_std3,
+                              const double *rot_std3);
+/* Checkpoint / resume: particles with their stored weights and the draw state (generation
+ * seed, generation, resampling LCG); a resumed filter continues bit-identically.  Load needs
+ * a filter of the same particle count. */
+int pfh_filter_save(void *f, const char *path);
+int pfh_filter_load(void *f, const char *path);
+
 #ifdef __cplusplus
 }
 #endif

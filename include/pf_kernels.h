@@ -145,8 +145,10 @@ int pfx_get_ancestors(void *h, int32_t *anc);
 /* Sets / reads the resampling LCG state (the u0 and injection stream). */
 int pfx_set_lcg(void *h, uint64_t lcg);
 int pfx_get_lcg(void *h, uint64_t *lcg);
-/* Generation counter and per-generation seed (deterministic-draw mode state). */
+/* Generation counter and per-generation seed (deterministic-draw mode state); set restores
+ * a checkpoint. */
 int pfx_get_rng(void *h, uint64_t *gseed, uint64_t *gen);
+int pfx_set_rng(void *h, uint64_t gseed, uint64_t gen);
 /* Device scalars of the last weight call: [max ll, sum exp(ll-max), sum exp^2, ESS]. */
 int pfx_weight_scalars(void *h, double *out4);
 /* Per-stage device time: enable=1 starts recording HIP events on the library stream. */

@@ -1,0 +1,80 @@
+"""GL -> PF hand-off (README:84-86) and checkpoint / resume through the host layer."""
+import math
+
+import numpy as np
+import pytest
+
+from cuda_pf_localization_amd import (GlobalLoc, GlParams, Observation, OdomDelta, ParticleFilter, PfConfig,
+                                      quat_mean)
+from cuda_pf_localization_amd.sim import f64_scan, free_poses, helix_scenario, obstacle_map
+
+pytestmark = pytest.mark.gpu
+
+
+def test_gl_to_pf_handoff(gpu):
+    cells, res, org = obstacle_map(5)
+    truth = free_poses(cells, res, org, 1, seed=8, margin_cells=6)[0]
+    scan = (f64_scan(cells, res, org, truth, 360, 2 * math.pi, 8.0)
+            + np.random.default_rng(2).normal(size=360) * 0.02).astype(np.float32)
+    gl = GlobalLoc(cells, res, org, GlParams(hypotheses=200_000, rmax=8.0, topk=8, seed=4))
+    top = gl.localize(scan)
+    assert math.hypot(top[0, 0] - truth[0], top[0, 1] - truth[1]) < 0.5
+    pf = ParticleFilter(PfConfig(particle_count=50_000, seed=3))
+    pf.reinit_from_gl(top[0, :3], z=1.25, pos_std=(0.05, 0.05, 0.02), rot_std=(0.01, 0.01, 0.02))
+    pos, quat, w = pf.snapshot()
+    m = pos.mean(axis=0)
+    assert abs(m[0] - top[0, 0]) < 2e-3 and abs(m[1] - top[0, 1]) < 2e-3 and abs(m[2] - 1.25) < 1e-3
+    q = quat_mean(quat.T @ quat / len(quat))
+    yaw = 2.0 * math.atan2(q[3], q[0])
+    d = (yaw - top[0, 2] + math.pi) % (2 * math.pi) - math.pi
+    assert abs(d) < 2e-3 and abs(q[1]) < 2e-3 and abs(q[2]) < 2e-3
+
+
+def _steps(sc, k0, k1):
+    out = []
+    for st in sc.steps[k0:k1]:
+        f32 = lambda v: [float(np.float32(x)) for x in v]
+        out.append((OdomDelta(f32(st.odom_t), f32(st.odom_q), f32(st.trans_noise), f32(st.rot_noise)),
+                    [Observation(i, m, v, r) for (i, m, v, r) in st.obs]))
+    return out
+
+
+def test_checkpoint_resume_is_bit_identical(gpu, tmp_path):
+    sc = helix_scenario(24, (20.0, 20.0, 5.0), 12, 15)
+    cfg = PfConfig(particle_count=30_000, seed=9, range_noise=0.05, bearing_noise=0.02, gate_sigma=3.0,
+                   resample_ess_ratio=0.5, random_inject_ratio=0.02, fast=True)
+    a = ParticleFilter(cfg)
+    a.set_map(sc.landmarks)
+    a.reinitialize([*sc.prior_pos, *sc.prior_q], [0.5] * 3, [0.1] * 3)
+    for od, obs in _steps(sc, 0, 6):
+        a.update(od, obs)
+    path = tmp_path / "filter.ckpt"
+    a.save(path)
+    ra = [a.update(od, obs) for od, obs in _steps(sc, 6, 12)]
+    b = ParticleFilter(cfg)
+    b.set_map(sc.landmarks)
+    b.reinitialize([0, 0, 0, 1, 0, 0, 0], [1.0] * 3, [1.0] * 3)  # overwritten by the checkpoint
+    b.load(path)
+    rb = [b.update(od, obs) for od, obs in _steps(sc, 6, 12)]
+    assert any(e.resampled for e in ra)
+    for ea, eb in zip(ra, rb):
+        assert np.array_equal(ea.position, eb.position) and np.array_equal(ea.attitude, eb.attitude)
+        assert np.array_equal(ea.pos_cov, eb.pos_cov) and np.array_equal(ea.att_cov, eb.att_cov)
+        assert ea.resampled == eb.resampled
+    small = ParticleFilter(PfConfig(particle_count=1000, seed=9))
+    with pytest.raises(RuntimeError):
+        small.load(path)
+
+
+def test_pf_bench_cli_tables(gpu, capsys):
+    """The pf-bench restatement prints the reference's table layout; the HIP filter tracks the
+    circle scenario (bench:21-60) and the GL table localises the room_grid truth."""
+    from cuda_pf_localization_amd import pf_bench
+    ms, err = pf_bench.run_pf(10_000, steps_n=40)
+    assert ms > 0 and err < 0.5
+    ms, err, ok = pf_bench.run_gl(5_000, runs=3)
+    assert ms > 0 and ok >= 2
+    pf_bench.main(["--csv"])
+    out = capsys.readouterr().out.splitlines()
+    assert out[0] == "n,backend,ms_per_cycle,mean_track_err_m"
+    assert [line.split(",")[0] for line in out[1:]] == ["1000", "10000", "100000", "500000"]
