@@ -130,6 +130,8 @@ struct PfState {
     int32_t* anc = nullptr;
     int32_t *inj_idx = nullptr, *inj_next = nullptr, *inj_head = nullptr;
     double* inj_jit = nullptr;
+    int32_t* d_inj_blk = nullptr;  // particle-layout blocks that can hold injection picks
+    int inj_nblk = 0;
     // reduction partials / scan state
     double *part_w = nullptr, *part_n = nullptr, *part_est = nullptr, *part_cov = nullptr;
     uint64_t* tile_agg = nullptr;  // per-tile fixed-point weight sums
@@ -157,11 +159,13 @@ void prof_end(PfState* s);
 int launch_motion(PfState* s, ParamRef<MotionParams> A, int reinit);
 int launch_weight(PfState* s, ParamRef<WeightParams> A);
 int launch_normalize(PfState* s, int force_uniform, const float* d_ratio);
-int launch_resample(PfState* s, ParamRef<InjectParams> inj, int any_inject);
-int launch_est_sums(PfState* s);
+int launch_resample(PfState* s, ParamRef<InjectParams> inj, const int32_t* inj_tab, int inj_nblk);
+// Block table of the injection kernels from each filter's largest pick count.
+int set_inject_blocks(PfState* s, const std::vector<int>& kmax);
+// Estimate sums; with d_cov non-null also the Markley mean of every filter's sums on the
+// device -> FilterDev.qmean and the covariance pass parameters d_cov[f].
+int launch_est_sums(PfState* s, CovParams* d_cov);
 int launch_est_covs(PfState* s, ParamRef<CovParams> A);
-// Markley mean of every filter's estimate sums on the device -> FilterDev.qmean, d_cov[f].
-int launch_quat_mean(PfState* s, CovParams* d_cov);
 int launch_predicate(PfState* s, const float* d_ratio, int force);
 WeightParams make_weight_params(double sigma_r, double sigma_b, double gate);
 // Joins observations with their landmarks (unknown ids dropped); range-bearing records

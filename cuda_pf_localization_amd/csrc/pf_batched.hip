@@ -57,8 +57,9 @@ struct Batch {
     size_t params_bytes = 0;
     ParamBlock dp{}, hp{};
     float* d_ratio = nullptr;
-    ObsRec* d_obs = nullptr;
-    ObsRec* h_obs = nullptr;
+    ObsRec* d_obs = nullptr;  // inside the d_params allocation, after the parameter blocks
+    ObsRec* h_obs = nullptr;  // inside h_params
+    hipEvent_t ev_done = nullptr;  // end of a cycle's copy-back (polled)
     FilterDev* h_fdev = nullptr;
     // graphs
     bool use_graph = false;
@@ -91,11 +92,10 @@ void free_batch(Batch* b) {
     if (b->s.stream) (void)hipStreamSynchronize(b->s.stream);
     if (b->gA) (void)hipGraphExecDestroy(b->gA);
     state_free_particles(&b->s);
-    (void)hipFree(b->d_params);
+    (void)hipFree(b->d_params);  // parameter blocks and observations share one allocation
     (void)hipHostFree(b->h_params);
+    if (b->ev_done) (void)hipEventDestroy(b->ev_done);
     (void)hipFree(b->d_ratio);
-    (void)hipFree(b->d_obs);
-    (void)hipHostFree(b->h_obs);
     (void)hipHostFree(b->h_fdev);
     for (auto& p : b->s.prof_pending) {
         (void)hipEventDestroy(p.a);
@@ -106,16 +106,15 @@ void free_batch(Batch* b) {
 }
 
 // Every device stage of a cycle (one graph): predict, weight, normalise (ESS + resample
-// predicate), resample (skipped on the device when the predicate is false), estimate sums,
-// Markley mean, covariances.
+// predicate), resample (skipped on the device when the predicate is false), estimate sums with
+// the Markley mean, covariances.
 int stages_all(Batch* b) {
     PfState* s = &b->s;
     if (launch_motion(s, ParamRef<MotionParams>::array(b->dp.motion), 0) != 0) return -1;
     if (launch_weight(s, ParamRef<WeightParams>::array(b->dp.weight)) != 0) return -1;
     if (launch_normalize(s, 0, b->d_ratio) != 0) return -1;
-    if (launch_resample(s, ParamRef<InjectParams>::array(b->dp.inject), 1) != 0) return -1;
-    if (launch_est_sums(s) != 0) return -1;
-    if (launch_quat_mean(s, b->dp.cov) != 0) return -1;
+    if (launch_resample(s, ParamRef<InjectParams>::array(b->dp.inject), s->d_inj_blk, s->inj_nblk) != 0) return -1;
+    if (launch_est_sums(s, b->dp.cov) != 0) return -1;
     return launch_est_covs(s, ParamRef<CovParams>::array(b->dp.cov));
 }
 
@@ -147,6 +146,14 @@ void drop_graphs(Batch* b) {
     b->gA = nullptr;
 }
 
+// Injection picks per resample of a filter (cu:667-671); fixed by the filter's configuration.
+int inject_count(const pfb_filter_cfg& c) {
+    const float r = c.inject_ratio;
+    if (!(r > 0.f)) return 0;
+    const int k = (int)(r * (float)c.n);
+    return k > c.n ? c.n : k;
+}
+
 // Host half of a cycle before the device stages: parameters + joined observations -> device.
 int upload_inputs(Batch* b, const pfb_cycle_in* in) {
     PfState* s = &b->s;
@@ -176,18 +183,11 @@ int upload_inputs(Batch* b, const pfb_cycle_in* in) {
         total += mobs;
         InjectParams& ip = b->hp.inject[f];
         ip.gen = b->gen[f];
-        const float r = b->cfg[f].inject_ratio;
-        int k = 0;
-        if (r > 0.f) {
-            k = (int)(r * (float)b->cfg[f].n);  // cu:667-671
-            if (k > b->cfg[f].n) k = b->cfg[f].n;
-        }
-        ip.k = k;
+        ip.k = inject_count(b->cfg[f]);
         ip.pad = 0;
     }
-    PB_CHECK(hipMemcpyAsync(b->d_params, b->h_params, b->params_bytes, hipMemcpyHostToDevice, s->stream));
-    if (total > 0)
-        PB_CHECK(hipMemcpyAsync(b->d_obs, b->h_obs, sizeof(ObsRec) * (size_t)total, hipMemcpyHostToDevice, s->stream));
+    PB_CHECK(hipMemcpyAsync(b->d_params, b->h_params, b->params_bytes + sizeof(ObsRec) * (size_t)total,
+                            hipMemcpyHostToDevice, s->stream));
     return 0;
 }
 
@@ -210,10 +210,11 @@ void* pfb_create(int nf, const pfb_filter_cfg* cfg, int obs_cap) {
     b->lm_cap.assign(nf, 0);
     b->gseed.resize(nf);
     b->gen.assign(nf, 0);
-    std::vector<int> ns(nf);
+    std::vector<int> ns(nf), kinj(nf);
     std::vector<float> ratio(nf);
     for (int f = 0; f < nf; ++f) {
         ns[f] = cfg[f].n;
+        kinj[f] = inject_count(cfg[f]);
         b->gseed[f] = cfg[f].seed ? (uint64_t)cfg[f].seed : 42ull;
         ratio[f] = cfg[f].resample_ess_ratio;
     }
@@ -221,12 +222,18 @@ void* pfb_create(int nf, const pfb_filter_cfg* cfg, int obs_cap) {
     bool ok = hipSetDevice(0) == hipSuccess &&
               hipStreamCreateWithFlags(&b->s.stream, hipStreamNonBlocking) == hipSuccess;
     ok = ok && state_alloc(&b->s, ns) == 0;
-    ok = ok && hipMalloc((void**)&b->d_params, b->params_bytes) == hipSuccess;
-    ok = ok && hipHostMalloc((void**)&b->h_params, b->params_bytes, hipHostMallocDefault) == hipSuccess;
+    ok = ok && set_inject_blocks(&b->s, kinj) == 0;
+    // one staging allocation per side: parameter blocks, then the joined observations, so a
+    // cycle's inputs go over in one copy
+    const size_t stage_bytes = b->params_bytes + sizeof(ObsRec) * (size_t)nf * obs_cap;
+    ok = ok && hipMalloc((void**)&b->d_params, stage_bytes) == hipSuccess;
+    ok = ok && hipHostMalloc((void**)&b->h_params, stage_bytes, hipHostMallocDefault) == hipSuccess;
     ok = ok && hipMalloc((void**)&b->d_ratio, sizeof(float) * (size_t)nf) == hipSuccess;
-    ok = ok && hipMalloc((void**)&b->d_obs, sizeof(ObsRec) * (size_t)nf * obs_cap) == hipSuccess;
-    ok = ok && hipHostMalloc((void**)&b->h_obs, sizeof(ObsRec) * (size_t)nf * obs_cap, hipHostMallocDefault) ==
-                   hipSuccess;
+    ok = ok && hipEventCreateWithFlags(&b->ev_done, hipEventDisableTiming) == hipSuccess;
+    if (ok) {
+        b->d_obs = reinterpret_cast<ObsRec*>(b->d_params + b->params_bytes);
+        b->h_obs = reinterpret_cast<ObsRec*>(b->h_params + b->params_bytes);
+    }
     ok = ok && hipHostMalloc((void**)&b->h_fdev, sizeof(FilterDev) * (size_t)nf, hipHostMallocDefault) == hipSuccess;
     ok = ok && hipMemcpy(b->d_ratio, ratio.data(), sizeof(float) * (size_t)nf, hipMemcpyHostToDevice) == hipSuccess;
     if (!ok) {
@@ -285,7 +292,12 @@ int pfb_cycle(void* h, const pfb_cycle_in* in, pfb_estimate* out) {
     if (upload_inputs(b, in) != 0) return -1;
     if (run_cycle(b) != 0) return -1;
     PB_CHECK(hipMemcpyAsync(b->h_fdev, s->fdev, sizeof(FilterDev) * (size_t)b->nf, hipMemcpyDeviceToHost, s->stream));
-    PB_CHECK(hipStreamSynchronize(s->stream));
+    PB_CHECK(hipEventRecord(b->ev_done, s->stream));
+    // poll rather than block: the cycle is short and a sleeping wait adds wake-up latency
+    hipError_t q;
+    while ((q = hipEventQuery(b->ev_done)) == hipErrorNotReady) {
+    }
+    PB_CHECK(q);
     if (out) {
         static const int idx[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
         for (int f = 0; f < b->nf; ++f) {

@@ -269,20 +269,28 @@ __device__ inline uint64_t qweight(float wf) {
 // run this same function; it uses only IEEE-exact operations (+ - * / sqrt), so both give
 // the same bits under -ffp-contract=off.  Returns -1 for a zero eigenvector.
 __host__ __device__ inline int quat_mean4(const double* M, double* q) {
+    // Every loop below is unrolled so that all indices are compile-time constants and the
+    // 4x4 arrays stay in registers on the device (one thread runs this per filter).
     double a[4][4], v[4][4];
+#pragma unroll
     for (int r = 0; r < 4; ++r)
+#pragma unroll
         for (int c = 0; c < 4; ++c) {
             a[r][c] = 0.5 * (M[4 * r + c] + M[4 * c + r]);
             v[r][c] = r == c ? 1.0 : 0.0;
         }
     for (int sweep = 0; sweep < 50; ++sweep) {
         double off = 0.0, diag = 0.0;
+#pragma unroll
         for (int r = 0; r < 4; ++r) {
             diag += a[r][r] * a[r][r];
+#pragma unroll
             for (int c = r + 1; c < 4; ++c) off += a[r][c] * a[r][c];
         }
         if (off <= 1e-32 * diag || off == 0.0) break;
+#pragma unroll
         for (int p = 0; p < 3; ++p)
+#pragma unroll
             for (int r = p + 1; r < 4; ++r) {
                 const double apr = a[p][r];
                 if (apr == 0.0) continue;
@@ -290,16 +298,19 @@ __host__ __device__ inline int quat_mean4(const double* M, double* q) {
                 const double tau = (a[r][r] - a[p][p]) / (2.0 * apr);
                 const double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
                 const double c = 1.0 / sqrt(1.0 + t * t), s = t * c;
+#pragma unroll
                 for (int k = 0; k < 4; ++k) {
                     const double kp = a[k][p], kr = a[k][r];
                     a[k][p] = c * kp - s * kr;
                     a[k][r] = s * kp + c * kr;
                 }
+#pragma unroll
                 for (int k = 0; k < 4; ++k) {
                     const double pk = a[p][k], rk = a[r][k];
                     a[p][k] = c * pk - s * rk;
                     a[r][k] = s * pk + c * rk;
                 }
+#pragma unroll
                 for (int k = 0; k < 4; ++k) {
                     const double kp = v[k][p], kr = v[k][r];
                     v[k][p] = c * kp - s * kr;
@@ -307,15 +318,24 @@ __host__ __device__ inline int quat_mean4(const double* M, double* q) {
                 }
             }
     }
-    int best = 0;
-    for (int i = 1; i < 4; ++i)
-        if (a[i][i] > a[best][best]) best = i;
+    // column of the largest diagonal entry (first one on ties), picked by selects
+    double amax = a[0][0];
+    double vb[4] = {v[0][0], v[1][0], v[2][0], v[3][0]};
+#pragma unroll
+    for (int i = 1; i < 4; ++i) {
+        const bool take = a[i][i] > amax;
+        amax = take ? a[i][i] : amax;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) vb[r] = take ? v[r][i] : vb[r];
+    }
     double nrm = 0.0;
-    for (int r = 0; r < 4; ++r) nrm += v[r][best] * v[r][best];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) nrm += vb[r] * vb[r];
     nrm = sqrt(nrm);
     if (!(nrm > 0.0)) return -1;
-    const double sg = v[0][best] < 0.0 ? -1.0 : 1.0;
-    for (int r = 0; r < 4; ++r) q[r] = sg * v[r][best] / nrm;
+    const double sg = vb[0] < 0.0 ? -1.0 : 1.0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) q[r] = sg * vb[r] / nrm;
     return 0;
 }
 

@@ -334,20 +334,19 @@ __global__ __launch_bounds__(kTB) void k_weight(Layout L, PoseBufs B, FilterDev*
     }
 }
 
-// Markley mean on the device (types.rs:128-144): one thread per filter builds the moment
-// matrix from the estimate sums (upper triangle, cuda.rs:277-286), takes its dominant unit
-// eigenvector with quat_mean4 (the host's function, same IEEE operation sequence), and
-// writes the covariance pass parameters with the mean handed over at f32 precision
-// (cuda.rs:291-294).  A degenerate moment matrix yields the identity and valid = 0.
-__global__ void k_quat_mean(FilterDev* F, int nfilt, CovParams* cov) {
-    const int f = blockIdx.x * blockDim.x + threadIdx.x;
-    if (f >= nfilt) return;
-    FilterDev& fd = F[f];
+// Markley mean on the device (types.rs:128-144): the moment matrix from the estimate sums
+// (upper triangle, cuda.rs:277-286), its dominant unit eigenvector with quat_mean4 (the host's
+// function, same IEEE operation sequence), and the covariance pass parameters with the mean
+// handed over at f32 precision (cuda.rs:291-294).  A degenerate moment matrix yields the
+// identity and valid = 0.
+__device__ inline void markley_mean(FilterDev& fd, const double* sums, CovParams& cov) {
     double M[16];
     int e = 3;
+#pragma unroll
     for (int r = 0; r < 4; ++r)
+#pragma unroll
         for (int c = r; c < 4; ++c) {
-            M[4 * r + c] = M[4 * c + r] = fd.sums[e];
+            M[4 * r + c] = M[4 * c + r] = sums[e];
             ++e;
         }
     double q[4] = {1.0, 0.0, 0.0, 0.0};
@@ -358,8 +357,8 @@ __global__ void k_quat_mean(FilterDev* F, int nfilt, CovParams* cov) {
     }
     for (int k = 0; k < 4; ++k) fd.qmean[k] = q[k];
     fd.qvalid = rc == 0 ? 1 : 0;
-    for (int k = 0; k < 3; ++k) cov[f].mp[k] = (double)(float)fd.sums[k];
-    for (int k = 0; k < 4; ++k) cov[f].mq[k] = (double)(float)q[k];
+    for (int k = 0; k < 3; ++k) cov.mp[k] = (double)(float)sums[k];
+    for (int k = 0; k < 4; ++k) cov.mq[k] = (double)(float)q[k];
 }
 
 // Per filter: M = max m_b, S = sum s_b e^(m_b - M), S2 = sum s2_b e^(2(m_b - M)) over the
@@ -600,11 +599,14 @@ __global__ __launch_bounds__(kTB) void k_gather(Layout L, FilterDev* F, PoseBufs
     }
 }
 
-// Injection picks (cu:333-352 formula): target index and f64 jitter of pick j.
-__global__ __launch_bounds__(kTB) void k_inj_pick(Layout L, FilterDev* F, ParamRef<InjectParams> A,
+// Injection picks (cu:333-352 formula): target index and f64 jitter of pick j.  The three
+// injection kernels run over the particle-layout blocks that can hold picks: tab lists them
+// (blocks whose first local index is below the filter's largest pick count), or, when tab is
+// null, the grid is the first ceil(k / 256) blocks of a single filter.
+__global__ __launch_bounds__(kTB) void k_inj_pick(Layout L, FilterDev* F, ParamRef<InjectParams> A, const int32_t* __restrict__ tab,
                                                   int32_t* __restrict__ idx, double* __restrict__ jit,
                                                   int32_t* __restrict__ head) {
-    const int blk = blockIdx.x;
+    const int blk = tab ? tab[blockIdx.x] : (int)blockIdx.x;
     const int f = L.blk_filter[blk];
     const FilterDev& fd = F[f];
     const InjectParams& p = A.get(f);
@@ -626,10 +628,10 @@ __global__ __launch_bounds__(kTB) void k_inj_pick(Layout L, FilterDev* F, ParamR
     head[off + t] = -1;
 }
 
-__global__ __launch_bounds__(kTB) void k_inj_link(Layout L, FilterDev* F, ParamRef<InjectParams> A,
+__global__ __launch_bounds__(kTB) void k_inj_link(Layout L, FilterDev* F, ParamRef<InjectParams> A, const int32_t* __restrict__ tab,
                                                   const int32_t* __restrict__ idx, int32_t* __restrict__ head,
                                                   int32_t* __restrict__ next) {
-    const int blk = blockIdx.x;
+    const int blk = tab ? tab[blockIdx.x] : (int)blockIdx.x;
     const int f = L.blk_filter[blk];
     const FilterDev& fd = F[f];
     const InjectParams& p = A.get(f);
@@ -641,11 +643,11 @@ __global__ __launch_bounds__(kTB) void k_inj_link(Layout L, FilterDev* F, ParamR
 }
 
 // The pick that ended up as list head applies every pick of its target in ascending pick order.
-__global__ __launch_bounds__(kTB) void k_inj_apply(Layout L, FilterDev* F, PoseBufs B, ParamRef<InjectParams> A,
+__global__ __launch_bounds__(kTB) void k_inj_apply(Layout L, FilterDev* F, PoseBufs B, ParamRef<InjectParams> A, const int32_t* __restrict__ tab,
                                                    const int32_t* __restrict__ idx, const double* __restrict__ jit,
                                                    const int32_t* __restrict__ head,
                                                    const int32_t* __restrict__ next) {
-    const int blk = blockIdx.x;
+    const int blk = tab ? tab[blockIdx.x] : (int)blockIdx.x;
     const int f = L.blk_filter[blk];
     const FilterDev& fd = F[f];
     const InjectParams& p = A.get(f);
@@ -698,19 +700,30 @@ __device__ inline void block_partials(double (&acc)[NV], double* __restrict__ pa
     }
 }
 
-// Per filter (one block each): 16 lanes per value sum the filter's block partials in a
-// fixed strided order, then a fixed 16-lane tree.  which = 0 -> sums, 1 -> covariances.
+// Per filter (one block each): one wave per value sums the filter's block partials in a fixed
+// strided order, then a fixed butterfly.  which = 0 -> sums, 1 -> covariances.  With cov
+// non-null (sums only) thread 0 then takes the Markley mean of the finished sums and writes the
+// covariance pass parameters.
 template <int NV>
-__global__ __launch_bounds__(1024) void k_est_fin(Layout L, FilterDev* F, const double* __restrict__ part, int which) {
+__global__ __launch_bounds__(1024) void k_est_fin(Layout L, FilterDev* F, const double* __restrict__ part, int which,
+                                                  CovParams* cov) {
     static_assert(NV <= 16, "one wave per value");
+    __shared__ double sv[NV];
     const int f = blockIdx.x;
     const int b0 = L.f_off[f] / kTile, nb = L.f_padn[f] / kTile;
     const int k = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    if (k >= NV) return;
-    double v = 0.0;
-    for (int b = lane; b < nb; b += 64) v += part[(size_t)NV * (b0 + b) + k];
-    v = wave_sum(v);
-    if (lane == 0) (which == 0 ? F[f].sums : F[f].covs)[k] = v;
+    if (k < NV) {
+        double v = 0.0;
+        for (int b = lane; b < nb; b += 64) v += part[(size_t)NV * (b0 + b) + k];
+        v = wave_sum(v);
+        if (lane == 0) {
+            (which == 0 ? F[f].sums : F[f].covs)[k] = v;
+            sv[k] = v;
+        }
+    }
+    if (!cov) return;
+    __syncthreads();
+    if (threadIdx.x == 0) markley_mean(F[f], sv, cov[f]);
 }
 
 // est layout (cu:354-404): [0..2] sum w p, [3..12] sum w q_r q_c (r<=c), [13] sum w^2.
@@ -931,13 +944,30 @@ void state_free_particles(PfState* s) {
                      (void**)&s->inj_idx,  (void**)&s->inj_next,    (void**)&s->inj_head,  (void**)&s->inj_jit,
                      (void**)&s->part_w,   (void**)&s->part_n,  (void**)&s->part_est,    (void**)&s->part_cov,  (void**)&s->tile_agg,
                      (void**)&s->tile_pre, (void**)&s->d_blk_filter, (void**)&s->d_fn,     (void**)&s->d_foff,
-                     (void**)&s->d_fpadn,  (void**)&s->d_fwinv,     (void**)&s->fdev};
+                     (void**)&s->d_fpadn,  (void**)&s->d_fwinv,     (void**)&s->fdev,      (void**)&s->d_inj_blk};
     for (void** p : ptrs) {
         (void)hipFree(*p);
         *p = nullptr;
     }
     s->nf = 0;
     s->NP = 0;
+    s->inj_nblk = 0;
+}
+
+int set_inject_blocks(PfState* s, const std::vector<int>& kmax) {
+    std::vector<int32_t> tab;
+    for (int f = 0; f < s->nf; ++f) {
+        const int k = std::min(std::max(kmax[f], 0), s->n[f]);
+        const int b0 = s->off[f] / kTB;
+        for (int b = 0; b < (k + kTB - 1) / kTB; ++b) tab.push_back(b0 + b);
+    }
+    (void)hipFree(s->d_inj_blk);
+    s->d_inj_blk = nullptr;
+    s->inj_nblk = (int)tab.size();
+    if (tab.empty()) return 0;
+    PF_CHECK(hipMalloc((void**)&s->d_inj_blk, sizeof(int32_t) * tab.size()));
+    PF_CHECK(hipMemcpy(s->d_inj_blk, tab.data(), sizeof(int32_t) * tab.size(), hipMemcpyHostToDevice));
+    return 0;
 }
 
 // ---- profiling helpers
@@ -997,7 +1027,7 @@ int launch_normalize(PfState* s, int force_uniform, const float* d_ratio) {
     return 0;
 }
 
-int launch_resample(PfState* s, ParamRef<InjectParams> inj, int any_inject) {
+int launch_resample(PfState* s, ParamRef<InjectParams> inj, const int32_t* inj_tab, int inj_nblk) {
     const int ntile = s->NP / kTile;
     prof_begin(s, PFX_STAGE_RESAMPLE_SCAN);
     hipLaunchKernelGGL(k_scan_tiles, dim3(ntile), dim3(kTB), 0, s->stream, s->L, s->fdev, s->w, s->tile_agg);
@@ -1010,13 +1040,15 @@ int launch_resample(PfState* s, ParamRef<InjectParams> inj, int any_inject) {
                        s->anc);
     prof_end(s);
     PF_LAUNCH_CHECK();
-    if (any_inject) {
+    if (inj_nblk > 0) {
         prof_begin(s, PFX_STAGE_INJECT);
-        const dim3 g(s->NP / kTB), b(kTB);
-        hipLaunchKernelGGL(k_inj_pick, g, b, 0, s->stream, s->L, s->fdev, inj, s->inj_idx, s->inj_jit, s->inj_head);
-        hipLaunchKernelGGL(k_inj_link, g, b, 0, s->stream, s->L, s->fdev, inj, s->inj_idx, s->inj_head, s->inj_next);
-        hipLaunchKernelGGL(k_inj_apply, g, b, 0, s->stream, s->L, s->fdev, s->pose, inj, s->inj_idx, s->inj_jit,
-                           s->inj_head, s->inj_next);
+        const dim3 g(inj_nblk), b(kTB);
+        hipLaunchKernelGGL(k_inj_pick, g, b, 0, s->stream, s->L, s->fdev, inj, inj_tab, s->inj_idx, s->inj_jit,
+                           s->inj_head);
+        hipLaunchKernelGGL(k_inj_link, g, b, 0, s->stream, s->L, s->fdev, inj, inj_tab, s->inj_idx, s->inj_head,
+                           s->inj_next);
+        hipLaunchKernelGGL(k_inj_apply, g, b, 0, s->stream, s->L, s->fdev, s->pose, inj, inj_tab, s->inj_idx,
+                           s->inj_jit, s->inj_head, s->inj_next);
         prof_end(s);
         PF_LAUNCH_CHECK();
     }
@@ -1027,22 +1059,15 @@ int launch_resample(PfState* s, ParamRef<InjectParams> inj, int any_inject) {
     return 0;
 }
 
-int launch_est_sums(PfState* s) {
+int launch_est_sums(PfState* s, CovParams* d_cov) {
     prof_begin(s, PFX_STAGE_EST_SUMS);
     hipLaunchKernelGGL(k_est_sums, dim3(s->NP / kTile), dim3(kTB), 0, s->stream, s->L, s->fdev, s->pose, s->w,
                        s->part_est);
     prof_end(s);
     PF_LAUNCH_CHECK();
     prof_begin(s, PFX_STAGE_FINALIZE);
-    hipLaunchKernelGGL(k_est_fin<kEstN>, dim3(s->nf), dim3(1024), 0, s->stream, s->L, s->fdev, s->part_est, 0);
-    prof_end(s);
-    PF_LAUNCH_CHECK();
-    return 0;
-}
-
-int launch_quat_mean(PfState* s, CovParams* d_cov) {
-    prof_begin(s, PFX_STAGE_FINALIZE);
-    hipLaunchKernelGGL(k_quat_mean, dim3((s->nf + 63) / 64), dim3(64), 0, s->stream, s->fdev, s->nf, d_cov);
+    hipLaunchKernelGGL(k_est_fin<kEstN>, dim3(s->nf), dim3(1024), 0, s->stream, s->L, s->fdev, s->part_est, 0,
+                       d_cov);
     prof_end(s);
     PF_LAUNCH_CHECK();
     return 0;
@@ -1055,7 +1080,8 @@ int launch_est_covs(PfState* s, ParamRef<CovParams> A) {
     prof_end(s);
     PF_LAUNCH_CHECK();
     prof_begin(s, PFX_STAGE_FINALIZE);
-    hipLaunchKernelGGL(k_est_fin<kCovN>, dim3(s->nf), dim3(1024), 0, s->stream, s->L, s->fdev, s->part_cov, 1);
+    hipLaunchKernelGGL(k_est_fin<kCovN>, dim3(s->nf), dim3(1024), 0, s->stream, s->L, s->fdev, s->part_cov, 1,
+                       nullptr);
     prof_end(s);
     PF_LAUNCH_CHECK();
     return 0;
@@ -1272,13 +1298,13 @@ int pfc_resample(void* h, float random_inject_ratio) {
         if (k > s->n[0]) k = s->n[0];
     }
     ip.k = k;
-    return launch_resample(s, ParamRef<InjectParams>::value(ip), k > 0);
+    return launch_resample(s, ParamRef<InjectParams>::value(ip), nullptr, (k + kTB - 1) / kTB);
 }
 
 int pfc_est_sums(void* h, double* out) {
     PfState* s = static_cast<PfState*>(h);
     if (!s || s->nf != 1 || !out) return -1;
-    if (launch_est_sums(s) != 0) return -1;
+    if (launch_est_sums(s, nullptr) != 0) return -1;
     PF_CHECK(hipMemcpyAsync(out, fdev_field<double>(s, 0, offsetof(FilterDev, sums)), sizeof(double) * kEstN,
                             hipMemcpyDeviceToHost, s->stream));
     PF_CHECK(hipStreamSynchronize(s->stream));
