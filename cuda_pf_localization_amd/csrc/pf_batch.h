@@ -70,7 +70,8 @@ struct MotionParams {
 
 struct WeightParams {
     double inv2rr, inv2bb, gate;
-    double cart_gate2, rng_gate2, brg_gate2;  // squared-residual gate thresholds (+inf: no gate)
+    double cart_gate2, brg_gate2;  // squared-residual gate thresholds (+inf: no gate)
+    double rng_gate;               // |range residual| gate g sr (+inf: no gate)
     double u_gate;                            // 1 - cos of the bearing gate (+inf: none)
     const ObsRec* obs;
     int nobs;  // joined records

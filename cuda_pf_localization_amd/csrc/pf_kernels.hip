@@ -169,13 +169,13 @@ __device__ inline BodyMap body_map(const PoseSet& S, int gi) {
 // ~1e-18 relative for u <= 0.005, twelve for u <= 0.125.
 // Four terms for the gated fast path (u <= u_gate <= 0.005): the first omitted term,
 // (64/6300) u^5, is below 3.2e-14 there, i.e. < 1e-11 relative.
-__host__ __device__ inline double acos_sq_u4(double u) {
+__host__ __device__ inline double acos_sq_p4(double u) {  // the series divided by u
     double p = 32.0 / 1120.0;
     p = fma(p, u, 16.0 / 180.0);
     p = fma(p, u, 8.0 / 24.0);
-    p = fma(p, u, 2.0);
-    return u * p;
+    return fma(p, u, 2.0);
 }
+__host__ __device__ inline double acos_sq_u4(double u) { return u * acos_sq_p4(u); }
 __host__ __device__ inline double acos_sq_u6(double u) {
     double p = 128.0 / 33264.0;
     p = fma(p, u, 64.0 / 6300.0);
@@ -201,16 +201,20 @@ __device__ inline double acos_sq_u12(double u) {
     return u * p;
 }
 
-// One gated log-likelihood term (cpu.rs:69-96), accumulated into acc.
+// One gated log-likelihood term (cpu.rs:69-96).  The kernel keeps two sums per particle, of
+// gated squared range (and Cartesian) residuals and of gated squared bearing errors, and forms
+// ll = -(sum_r inv2rr + sum_b inv2bb) once at the end.
 //   The reference's gate is continuous: the gated constant -g^2/2 equals -G inv2s at the
-//   threshold G = g^2 s^2, so a gated term is -min(e^2, G) inv2s (G = +inf when gate = 0).
+//   threshold G = g^2 s^2, so a gated term is -min(e^2, G) inv2s (G = +inf when gate = 0); for
+//   the range it is taken as min(|e|, g s)^2, one min and one FMA.
 //   b = M (l - p) is formed as M l - M p (three FMAs per component).
-//   Range-bearing: rn from the v_rsq_f64 seed refined once by Newton (relative ~1e-16);
-//   u = 1 - dot = 1 + pv/rn in f64; eb^2 from the series in u.  With a gate narrower than
-//   u = 0.005 the series is evaluated at min(u, u_gate), u_gate being the largest u whose
-//   series value does not exceed the gate's g^2 sb^2: that min() alone does the gating.  u is
-//   clamped at 0 (an f32 direction of norm 1 + 1e-7 can put it 1e-7 below), as the
-//   reference clamps the cosine.
+//   Range-bearing: |b|^2 + 1e-18 (rn >= 1e-9), rn from the v_rsq_f64 seed refined once by
+//   Newton (relative ~1e-16); u = 1 - dot = 1 + pv/rn in f64; eb^2 from the series in u.  With
+//   a gate narrower than u = 0.005 the series is evaluated at min(u, u_gate), u_gate being the
+//   largest u whose series value does not exceed the gate's g^2 sb^2: that min() alone does
+//   the gating, and the series' last factor u folds into the accumulating FMA.  u is clamped at
+//   0 (an f32 direction of norm 1 + 1e-7 can put it 1e-7 below), as the reference clamps the
+//   cosine.
 __device__ inline void body_offset(const BodyMap& b, const ObsRec& o, double& bx, double& by, double& bz) {
     bx = __fma_rn(b.m00, o.lx, __fma_rn(b.m01, o.ly, __fma_rn(b.m02, o.lz, -b.mpx)));
     by = __fma_rn(b.m10, o.lx, __fma_rn(b.m11, o.ly, __fma_rn(b.m12, o.lz, -b.mpy)));
@@ -219,16 +223,17 @@ __device__ inline void body_offset(const BodyMap& b, const ObsRec& o, double& bx
 
 template <bool kFastBearing>
 __device__ inline void rb_from_b(double bx, double by, double bz, double vx, double vy, double vz, double range,
-                                 double irr, double ibb, const WeightParams& p, double& acc) {
-    const double r2 = fmax(__fma_rn(bx, bx, __fma_rn(by, by, bz * bz)), 1e-18);  // rn >= 1e-9
+                                 const WeightParams& p, double& accr, double& accb) {
+    const double r2 = __fma_rn(bx, bx, __fma_rn(by, by, __fma_rn(bz, bz, 1e-18)));
     const double pv = __fma_rn(bx, vx, __fma_rn(by, vy, bz * vz));
     const double y0 = __builtin_amdgcn_rsq(r2);  // v_rsq_f64 seed, refined once below
     const double hy = 0.5 * r2 * y0;
     const double y = __fma_rn(y0, __fma_rn(-hy, y0, 0.5), y0);
-    const double er = __fma_rn(r2, y, -range);
-    acc = __fma_rn(-fmin(er * er, p.rng_gate2), irr, acc);
+    const double ec = fmin(fabs(__fma_rn(r2, y, -range)), p.rng_gate);
+    accr = __fma_rn(ec, ec, accr);
     if (kFastBearing) {
-        acc = __fma_rn(-acos_sq_u4(fmin(fmax(__fma_rn(pv, y, 1.0), 0.0), p.u_gate)), ibb, acc);
+        const double u = fmin(fmax(__fma_rn(pv, y, 1.0), 0.0), p.u_gate);
+        accb = __fma_rn(u, acos_sq_p4(u), accb);
         return;
     }
     const double u = fmin(fmax(__fma_rn(pv, y, 1.0), 0.0), 2.0);
@@ -241,14 +246,14 @@ __device__ inline void rb_from_b(double bx, double by, double bz, double vx, dou
         const double eb = acos(1.0 - u);
         eb2 = eb * eb;
     }
-    acc = __fma_rn(-fmin(eb2, p.brg_gate2), ibb, acc);
+    accb += fmin(eb2, p.brg_gate2);
 }
 
-__device__ inline void cart_from_b(double bx, double by, double bz, double vx, double vy, double vz, double irr,
-                                   const WeightParams& p, double& acc) {
+__device__ inline void cart_from_b(double bx, double by, double bz, double vx, double vy, double vz,
+                                   const WeightParams& p, double& accr) {
     const double ex = bx - vx, ey = by - vy, ez = bz - vz;
     const double r2 = __fma_rn(ex, ex, __fma_rn(ey, ey, ez * ez));
-    acc = __fma_rn(-fmin(r2, p.cart_gate2), irr, acc);
+    accr += fmin(r2, p.cart_gate2);
 }
 
 // Gated log-likelihood (cpu.rs:44-97, 220-248), kPPT particles per lane, plus fused
@@ -267,13 +272,14 @@ __global__ __launch_bounds__(kTB) void k_weight(Layout L, PoseBufs B, FilterDev*
     int li[kPPT];
     bool valid[kPPT];
     BodyMap bm[kPPT];
-    double acc[kPPT];
+    double accr[kPPT], accb[kPPT], acc[kPPT];
 #pragma unroll
     for (int e = 0; e < kPPT; ++e) {
         li[e] = blk * kSpan + e * kTB + (int)threadIdx.x - off;
         valid[e] = li[e] < nf;
         bm[e] = body_map(S, off + (valid[e] ? li[e] : 0));
-        acc[e] = 0.0;
+        accr[e] = 0.0;
+        accb[e] = 0.0;
     }
     ConstObsPtr obs = (ConstObsPtr)p.obs;
     const int nobs = p.nobs, nrb = p.nrb;  // range-bearing records first, then Cartesian
@@ -285,7 +291,7 @@ __global__ __launch_bounds__(kTB) void k_weight(Layout L, PoseBufs B, FilterDev*
             for (int e = 0; e < kPPT; ++e) {
                 double bx, by, bz;
                 body_offset(bm[e], o, bx, by, bz);
-                rb_from_b<true>(bx, by, bz, o.vx, o.vy, o.vz, o.range, p.inv2rr, p.inv2bb, p, acc[e]);
+                rb_from_b<true>(bx, by, bz, o.vx, o.vy, o.vz, o.range, p, accr[e], accb[e]);
             }
         }
     } else {
@@ -295,7 +301,7 @@ __global__ __launch_bounds__(kTB) void k_weight(Layout L, PoseBufs B, FilterDev*
             for (int e = 0; e < kPPT; ++e) {
                 double bx, by, bz;
                 body_offset(bm[e], o, bx, by, bz);
-                rb_from_b<false>(bx, by, bz, o.vx, o.vy, o.vz, o.range, p.inv2rr, p.inv2bb, p, acc[e]);
+                rb_from_b<false>(bx, by, bz, o.vx, o.vy, o.vz, o.range, p, accr[e], accb[e]);
             }
         }
     }
@@ -305,9 +311,11 @@ __global__ __launch_bounds__(kTB) void k_weight(Layout L, PoseBufs B, FilterDev*
         for (int e = 0; e < kPPT; ++e) {
             double bx, by, bz;
             body_offset(bm[e], o, bx, by, bz);
-            cart_from_b(bx, by, bz, o.vx, o.vy, o.vz, p.inv2rr, p, acc[e]);
+            cart_from_b(bx, by, bz, o.vx, o.vy, o.vz, p, accr[e]);
         }
     }
+#pragma unroll
+    for (int e = 0; e < kPPT; ++e) acc[e] = -__fma_rn(accr[e], p.inv2rr, accb[e] * p.inv2bb);
     double mloc = -INFINITY;
 #pragma unroll
     for (int e = 0; e < kPPT; ++e)
@@ -545,11 +553,31 @@ __device__ inline int first_at_or_above(const uint64_t* c, int lo, int hi, uint6
     return lo;
 }
 
+// The same search by one wave (result wave-uniform): a 64-ary search in which lane l samples
+// lo + l * step; the samples below the target form a prefix of the lanes, whose length bounds
+// the answer to one step.  About log64(hi - lo) rounds of loads instead of log2(hi - lo)
+// dependent ones.
+__device__ inline int wave_first_at_or_above(const uint64_t* c, int lo, int hi, uint64_t A, uint64_t T, uint64_t Bi) {
+    const int lane = threadIdx.x & 63;
+    while (hi - lo > 64) {
+        const int step = (hi - lo + 63) >> 6;
+        const int j = lo + lane * step;
+        const bool below = j < hi && mul128_lt(c[j], A, T, Bi);
+        const int k = __popcll(__ballot(below));
+        const int nlo = k > 0 ? lo + (k - 1) * step + 1 : lo;
+        hi = min(lo + k * step, hi);
+        lo = nlo;
+    }
+    const int j = lo + lane;
+    const bool below = j < hi && mul128_lt(c[j], A, T, Bi);
+    return lo + __popcll(__ballot(below));
+}
+
 constexpr int kGatherLds = 2048;  // cumulative weights staged per block (16 KB)
 
-// Merge-path gather: each block owns 256 consecutive outputs; two global searches bound
-// their ancestors to [jlo, jhi], that slice of the scan is staged in LDS and every lane
-// searches it locally.  Poses move from the current set to the other set (no copies);
+// Merge-path gather: each block owns 256 consecutive outputs; two global searches (one wave
+// each) bound their ancestors to [jlo, jhi], that slice of the scan is staged in LDS and every
+// lane searches it locally.  Poses move from the current set to the other set (no copies);
 // weights -> 1/n; k_flip makes the other set current after the injection.
 __global__ __launch_bounds__(kTB) void k_gather(Layout L, FilterDev* F, PoseBufs B, const uint64_t* __restrict__ cum,
                                                 float* __restrict__ w, int32_t* __restrict__ anc) {
@@ -574,8 +602,13 @@ __global__ __launch_bounds__(kTB) void k_gather(Layout L, FilterDev* F, PoseBufs
     const uint64_t* c = cum + off;
     if (i0 < nf) {
         const int i1 = min(i0 + kTB - 1, nf - 1);
-        if (threadIdx.x == 0) s_lo = first_at_or_above(c, 0, nf - 1, A, T, U + ((uint64_t)i0 << 32));
-        if (threadIdx.x == 64) s_hi = first_at_or_above(c, 0, nf - 1, A, T, U + ((uint64_t)i1 << 32));
+        if (threadIdx.x < 64) {
+            const int r = wave_first_at_or_above(c, 0, nf - 1, A, T, U + ((uint64_t)i0 << 32));
+            if (threadIdx.x == 0) s_lo = r;
+        } else if (threadIdx.x < 128) {
+            const int r = wave_first_at_or_above(c, 0, nf - 1, A, T, U + ((uint64_t)i1 << 32));
+            if (threadIdx.x == 64) s_hi = r;
+        }
         __syncthreads();
         const int jlo = s_lo, jhi = s_hi;
         const int span = jhi - jlo + 1;
@@ -1105,7 +1138,7 @@ WeightParams make_weight_params(double sigma_r, double sigma_b, double gate) {
     // squared-residual thresholds (cpu.rs:63-65); +inf without a gate (cpu.rs:72, 85, 90)
     const bool gated = gate > 0.0;
     p.cart_gate2 = gated ? g2 * 3.0 * sr * sr : INFINITY;
-    p.rng_gate2 = gated ? g2 * sr * sr : INFINITY;
+    p.rng_gate = gated ? fabs(gate) * sr : INFINITY;
     p.brg_gate2 = gated ? g2 * sb * sb : INFINITY;
     // Bearing gate in u = 1 - cos(eb): the largest u whose 4-term series value is <= g^2 sb^2
     // (bisection over doubles with the kernel's own series), so that series(min(u, u_gate))
