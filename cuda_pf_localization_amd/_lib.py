@@ -114,6 +114,7 @@ KERNEL_SYMBOLS = [
     ("pfx_particle_count", C.c_int, [vp]),
     ("pfx_sync", C.c_int, [vp]),
     ("pfx_set_weight_variant", C.c_int, [vp, C.c_int]),
+    ("pfx_set_scan_spin_limit", C.c_int, [vp, C.c_int]),
     ("pfx_quat_mean", C.c_int, [P(f64), P(f64)]),
     # global-localisation extensions
     ("pfglx_topk_ex", C.c_int, [vp, C.c_int, P(f32), P(f32), P(i32)]),

@@ -128,6 +128,11 @@ class PfKernels:
         """Likelihood kernel launch shape: 1, 2 or 4 particles per lane."""
         _check(self.L.pfx_set_weight_variant(self.h, variant), "pfx_set_weight_variant")
 
+    def set_scan_spin_limit(self, limit: int):
+        """Reads of a predecessor tile's status in the resampling scan before the tile sums the
+        predecessors' weights itself (same bits either way)."""
+        _check(self.L.pfx_set_scan_spin_limit(self.h, limit), "pfx_set_scan_spin_limit")
+
     def init(self, n: int, seed: int):
         _check(self.L.pfc_init(self.h, n, seed & 0xFFFFFFFF), "pfc_init")
         self.n = n

@@ -56,6 +56,34 @@ namespace pfk {
 
 // ============================================================================ kernels
 
+// Predict (cpu.rs:197-218) of local particle li stored at gi: the pose is read from S, moved,
+// rounded to f32 and written back; v receives the stored values (x y z w qx qy qz).
+__device__ inline void predict_pose(const MotionParams& p, const PoseSet& S, int gi, int li, float (&v)[7]) {
+    double nn[6];
+    particle_normals(p.gseed, (uint64_t)li, nn);
+    const dquat q{(double)S.c[3][gi], (double)S.c[4][gi], (double)S.c[5][gi], (double)S.c[6][gi]};
+    const double bx = p.t[0] + p.s_t[0] * nn[0];
+    const double by = p.t[1] + p.s_t[1] * nn[1];
+    const double bz = p.t[2] + p.s_t[2] * nn[2];
+    double wx, wy, wz;
+    qrot(q, bx, by, bz, wx, wy, wz);
+    const double x = (double)S.c[0][gi] + wx;
+    const double y = (double)S.c[1][gi] + wy;
+    const double z = (double)S.c[2][gi] + wz;
+    const dquat qo{p.q[0], p.q[1], p.q[2], p.q[3]};
+    const dquat dq = noise_dq(p.s_r[0], p.s_r[1], p.s_r[2], nn + 3);
+    const dquat qn = qmul(qmul(q, qo), dq);
+    v[0] = (float)x;
+    v[1] = (float)y;
+    v[2] = (float)z;
+    v[3] = (float)qn.w;
+    v[4] = (float)qn.x;
+    v[5] = (float)qn.y;
+    v[6] = (float)qn.z;
+#pragma unroll
+    for (int c = 0; c < 7; ++c) S.c[c][gi] = v[c];
+}
+
 // Reinit (cpu.rs:180-195) and predict (cpu.rs:197-218), one particle per lane.
 __global__ __launch_bounds__(kTB) void k_motion(Layout L, PoseBufs B, FilterDev* F, float* __restrict__ w,
                                                 ParamRef<MotionParams> A, int reinit) {
@@ -67,9 +95,9 @@ __global__ __launch_bounds__(kTB) void k_motion(Layout L, PoseBufs B, FilterDev*
     const MotionParams& p = A.get(f);
     const int gi = off + li;
     const PoseSet S = B.s[F[f].parity];
-    double nn[6];
-    particle_normals(p.gseed, (uint64_t)li, nn);
     if (reinit) {
+        double nn[6];
+        particle_normals(p.gseed, (uint64_t)li, nn);
         const double x = p.t[0] + p.s_t[0] * nn[0];
         const double y = p.t[1] + p.s_t[1] * nn[1];
         const double z = p.t[2] + p.s_t[2] * nn[2];
@@ -85,25 +113,8 @@ __global__ __launch_bounds__(kTB) void k_motion(Layout L, PoseBufs B, FilterDev*
         S.c[6][gi] = (float)q.z;
         w[gi] = p.winv;
     } else {
-        const dquat q{(double)S.c[3][gi], (double)S.c[4][gi], (double)S.c[5][gi], (double)S.c[6][gi]};
-        const double bx = p.t[0] + p.s_t[0] * nn[0];
-        const double by = p.t[1] + p.s_t[1] * nn[1];
-        const double bz = p.t[2] + p.s_t[2] * nn[2];
-        double wx, wy, wz;
-        qrot(q, bx, by, bz, wx, wy, wz);
-        const double x = (double)S.c[0][gi] + wx;
-        const double y = (double)S.c[1][gi] + wy;
-        const double z = (double)S.c[2][gi] + wz;
-        const dquat qo{p.q[0], p.q[1], p.q[2], p.q[3]};
-        const dquat dq = noise_dq(p.s_r[0], p.s_r[1], p.s_r[2], nn + 3);
-        const dquat qn = qmul(qmul(q, qo), dq);
-        S.c[0][gi] = (float)x;
-        S.c[1][gi] = (float)y;
-        S.c[2][gi] = (float)z;
-        S.c[3][gi] = (float)qn.w;
-        S.c[4][gi] = (float)qn.x;
-        S.c[5][gi] = (float)qn.y;
-        S.c[6][gi] = (float)qn.z;
+        float v[7];
+        predict_pose(p, S, gi, li, v);
     }
 }
 
@@ -125,6 +136,29 @@ __device__ inline double block_max(double v, double* sh) {
     return fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
 }
 
+// The same reductions over a 1024-thread block (16 waves), in a fixed sequential wave order.
+constexpr int kFinTB = 1024;
+__device__ inline double block16_sum(double v, double* sh) {
+    v = wave_sum(v);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double r = sh[0];
+#pragma unroll
+    for (int k = 1; k < kFinTB / 64; ++k) r += sh[k];
+    return r;
+}
+__device__ inline double block16_max(double v, double* sh) {
+    v = wave_max(v);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double r = sh[0];
+#pragma unroll
+    for (int k = 1; k < kFinTB / 64; ++k) r = fmax(r, sh[k]);
+    return r;
+}
+
 // Observation records are read through the constant address space: the addresses are
 // wave-uniform, so they become scalar (s_load) reads that stay off the vector memory path.
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -141,13 +175,12 @@ struct BodyMap {
     double mpx, mpy, mpz;  // M p, so that b = M l - M p takes three FMAs per component
 };
 
-__device__ inline BodyMap body_map(const PoseSet& S, int gi) {
+__device__ inline BodyMap body_map(const float (&v)[7]) {
     BodyMap b;
-    b.px = (double)S.c[0][gi];
-    b.py = (double)S.c[1][gi];
-    b.pz = (double)S.c[2][gi];
-    const double qw = (double)S.c[3][gi], qx = (double)S.c[4][gi], qy = (double)S.c[5][gi],
-                 qz = (double)S.c[6][gi];
+    b.px = (double)v[0];
+    b.py = (double)v[1];
+    b.pz = (double)v[2];
+    const double qw = (double)v[3], qx = (double)v[4], qy = (double)v[5], qz = (double)v[6];
     const double a = 1.0 - 2.0 * (qx * qx + qy * qy + qz * qz);
     b.m00 = a + 2.0 * qx * qx;
     b.m11 = a + 2.0 * qy * qy;
@@ -162,6 +195,12 @@ __device__ inline BodyMap body_map(const PoseSet& S, int gi) {
     b.mpy = __fma_rn(b.m10, b.px, __fma_rn(b.m11, b.py, b.m12 * b.pz));
     b.mpz = __fma_rn(b.m20, b.px, __fma_rn(b.m21, b.py, b.m22 * b.pz));
     return b;
+}
+__device__ inline BodyMap body_map(const PoseSet& S, int gi) {
+    float v[7];
+#pragma unroll
+    for (int c = 0; c < 7; ++c) v[c] = S.c[c][gi];
+    return body_map(v);
 }
 
 // acos(1 - u)^2 as a power series in u = 1 - dot: acos(1-u)^2 = sum_n 2^(n+1) u^n / (n^2 C(2n,n))
@@ -370,18 +409,18 @@ __device__ inline void markley_mean(FilterDev& fd, const double* sums, CovParams
 }
 
 // Per filter: M = max m_b, S = sum s_b e^(m_b - M), S2 = sum s2_b e^(2(m_b - M)) over the
-// filter's likelihood blocks, in a fixed order (one block per filter; the kernel boundary
-// orders it after every partial without device-scope fences).
-__global__ __launch_bounds__(kTB) void k_weight_fin(Layout L, FilterDev* F, const double* __restrict__ part,
-                                                    int span) {
-    __shared__ double sh[4];
+// filter's likelihood blocks, in a fixed order (one 1024-thread block per filter, so that each
+// lane has few partials to load; the kernel boundary orders it after every partial).
+__global__ __launch_bounds__(kFinTB) void k_weight_fin(Layout L, FilterDev* F, const double* __restrict__ part,
+                                                       int span) {
+    __shared__ double sh[kFinTB / 64];
     const int f = blockIdx.x;
     const int b0 = L.f_off[f] / span, nb = L.f_padn[f] / span;
     double m = -INFINITY;
-    for (int b = (int)threadIdx.x; b < nb; b += kTB) m = fmax(m, part[3 * (b0 + b)]);
-    const double M = block_max(m, sh);
+    for (int b = (int)threadIdx.x; b < nb; b += kFinTB) m = fmax(m, part[3 * (b0 + b)]);
+    const double M = block16_max(m, sh);
     double s = 0.0, s2 = 0.0;
-    for (int b = (int)threadIdx.x; b < nb; b += kTB) {
+    for (int b = (int)threadIdx.x; b < nb; b += kFinTB) {
         const double mbb = part[3 * (b0 + b)];
         if (mbb > -INFINITY) {
             const double sc = exp(mbb - M);
@@ -389,8 +428,8 @@ __global__ __launch_bounds__(kTB) void k_weight_fin(Layout L, FilterDev* F, cons
             s2 += part[3 * (b0 + b) + 2] * (sc * sc);
         }
     }
-    const double Ssum = block_sum(s, sh);
-    const double S2sum = block_sum(s2, sh);
+    const double Ssum = block16_sum(s, sh);
+    const double S2sum = block16_sum(s2, sh);
     if (threadIdx.x == 0) {
         FilterDev& fd = F[f];
         const bool ok = (Ssum > 0.0) && isfinite(Ssum) && isfinite(M);
@@ -407,15 +446,17 @@ __global__ __launch_bounds__(kTB) void k_normalize(Layout L, FilterDev* F, const
                                                    float* __restrict__ w, double* __restrict__ part, int force_uniform) {
     __shared__ double sh[4];
     const int blk = blockIdx.x;
+    const int gi = blk * kTB + (int)threadIdx.x;
+    const double llv = ll[gi];  // issued before the filter lookup (gi < NP always)
     const int f = L.blk_filter[blk];
     const int off = L.f_off[f];
     const int nf = L.f_n[f];
-    const int li = blk * kTB + (int)threadIdx.x - off;
+    const int li = gi - off;
     const FilterDev& fd = F[f];
     const bool uni = force_uniform || fd.uniform;
     double w2 = 0.0;
     if (li < nf) {
-        const float wf = uni ? L.f_winv[f] : (float)(exp(ll[off + li] - fd.max_ll) / fd.sum_e);
+        const float wf = uni ? L.f_winv[f] : (float)(exp(llv - fd.max_ll) / fd.sum_e);
         w[off + li] = wf;
         w2 = (double)wf * (double)wf;
     }
@@ -426,14 +467,14 @@ __global__ __launch_bounds__(kTB) void k_normalize(Layout L, FilterDev* F, const
 // Per filter: ESS = 1 / sum w^2 of the stored weights -- the quantity the reference driver
 // tests (particle_filter.rs:115-117 via pfc_est_sums out[13]) -- and, when ratio is given,
 // the resample predicate ESS < ratio * n.
-__global__ __launch_bounds__(kTB) void k_norm_fin(Layout L, FilterDev* F, const double* __restrict__ part,
-                                                  const float* __restrict__ ratio) {
-    __shared__ double sh[4];
+__global__ __launch_bounds__(kFinTB) void k_norm_fin(Layout L, FilterDev* F, const double* __restrict__ part,
+                                                     const float* __restrict__ ratio) {
+    __shared__ double sh[kFinTB / 64];
     const int f = blockIdx.x;
     const int b0 = L.f_off[f] / kTB, nb = L.f_padn[f] / kTB;
     double v = 0.0;
-    for (int b = (int)threadIdx.x; b < nb; b += kTB) v += part[b0 + b];
-    const double tot = block_sum(v, sh);
+    for (int b = (int)threadIdx.x; b < nb; b += kFinTB) v += part[b0 + b];
+    const double tot = block16_sum(v, sh);
     if (threadIdx.x == 0) {
         FilterDev& fd = F[f];
         fd.sum_w2 = tot;
@@ -465,74 +506,185 @@ __device__ inline uint64_t block_scan_u64(uint64_t x, uint64_t& incl, uint64_t* 
 }
 
 // Fixed-point weights of the 4 particles of this lane in a 1024-particle tile.
-__device__ inline void tile_q(const float* __restrict__ w, int tile, int off, int nf, uint64_t (&q)[4]) {
-    const int base = tile * kTile + (int)threadIdx.x * 4;
-    const int li0 = base - off;
-    const float4 wv = *reinterpret_cast<const float4*>(w + base);
+__device__ inline void tile_q(const float4 wv, int tile, int off, int nf, uint64_t (&q)[4]) {
+    const int li0 = tile * kTile + (int)threadIdx.x * 4 - off;
     q[0] = (li0 + 0 < nf) ? qweight(wv.x) : 0;
     q[1] = (li0 + 1 < nf) ? qweight(wv.y) : 0;
     q[2] = (li0 + 2 < nf) ? qweight(wv.z) : 0;
     q[3] = (li0 + 3 < nf) ? qweight(wv.w) : 0;
 }
-
-// Reduce-then-scan of the fixed-point weights (integer sums: any association gives the same
-// bits).  Pass 1: one aggregate per 1024-particle tile.
-__global__ __launch_bounds__(kTB) void k_scan_tiles(Layout L, FilterDev* F, const float* __restrict__ w,
-                                                    uint64_t* __restrict__ tile_agg) {
-    __shared__ uint64_t s_w[4];
-    const int tile = blockIdx.x;
-    const int f = L.blk_filter[tile * (kTile / kTB)];
-    if (!F[f].do_resample) return;
-    uint64_t q[4];
-    tile_q(w, tile, L.f_off[f], L.f_n[f], q);
-    uint64_t incl;
-    const uint64_t tot = block_scan_u64(q[0] + q[1] + q[2] + q[3], incl, s_w);
-    if (threadIdx.x == 0) tile_agg[tile] = tot;
+__device__ inline float4 tile_w(const float* __restrict__ w, int tile) {
+    return *reinterpret_cast<const float4*>(w + tile * kTile + (int)threadIdx.x * 4);
 }
 
-// Pass 2 (one block per filter): exclusive prefix of the filter's tile aggregates, the
-// total T and the advanced LCG (pf_kernels.cu:647-657: only when T > 0).
-__global__ __launch_bounds__(kTB) void k_scan_top(Layout L, FilterDev* F, const uint64_t* __restrict__ tile_agg,
-                                                  uint64_t* __restrict__ tile_pre) {
+// Single-pass scan of the fixed-point weights with decoupled look-back (integer sums: any
+// association gives the same bits).
+//
+// Tiles are 1024 particles, one block each.  Each tile publishes its aggregate, looks back over
+// its filter's predecessors until it meets an inclusive prefix, and publishes its own inclusive
+// prefix; the first tile of a filter publishes its prefix directly.  The whole block looks
+// back, 1024 predecessors per window (four per thread, read together, so one memory round trip
+// resolves up to 1024 tiles).  A published 64-bit value occupies two status words, each
+// (status << 32) | one 32-bit half, written and read as device-scope atomics; status =
+// epoch << 2 | kind, so words from an earlier launch never match (k_flip, which ends every
+// resample, advances the epoch).  Tiles of filters that do not resample refresh their words
+// with kind 0, so every word is at most one launch old.
+//
+// A tile waits only on lower-numbered tiles.  When every tile of the launch fits on the device
+// at once the tile is the block index (workgroups are dispatched in index order); otherwise the
+// tile comes from a ticket counter taken when the block starts, so its predecessors' blocks are
+// already running, and the last ticket re-arms the counter.
+//
+// A wait is bounded (spin_limit reads per predecessor); a tile whose look-back does not complete
+// within it sums its predecessors' weights itself, which is slower but gives the same bits, so
+// the kernel ends whatever the scheduling.  The filter's last tile stores the total T and
+// advances the LCG (pf_kernels.cu:647-657: only when T > 0).
+constexpr uint32_t kLbAgg = 1, kLbPre = 2;
+constexpr int kLbPer = 4, kLbWin = kTB * kLbPer;  // predecessors per thread and per look-back window
+
+__device__ inline void lb_publish(uint64_t* st, int tile, uint32_t status, uint64_t v) {
+    __hip_atomic_store(&st[2 * tile], ((uint64_t)status << 32) | (v & 0xffffffffull), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&st[2 * tile + 1], ((uint64_t)status << 32) | (v >> 32), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Kind of the value tile has published in this epoch (0: none yet, or the halves of two
+// different publications were read).
+__device__ inline uint32_t lb_read(uint64_t* st, int tile, uint32_t epoch, uint64_t& v) {
+    const uint64_t lo = __hip_atomic_load(&st[2 * tile], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint64_t hi = __hip_atomic_load(&st[2 * tile + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t sl = (uint32_t)(lo >> 32), sh = (uint32_t)(hi >> 32);
+    if (sl != sh || (sl >> 2) != epoch) return 0;
+    v = (hi << 32) | (lo & 0xffffffffull);
+    return sl & 3u;
+}
+
+__device__ inline uint64_t wave_sum_u64(uint64_t v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    return v;
+}
+
+// Block sum of one u64 per thread (4 waves); any association gives the same bits.
+__device__ inline uint64_t block_sum_u64(uint64_t v, uint64_t* s_w) {
+    v = wave_sum_u64(v);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = v;
+    __syncthreads();
+    return s_w[0] + s_w[1] + s_w[2] + s_w[3];
+}
+
+__global__ __launch_bounds__(kTB) void k_scan(Layout L, FilterDev* F, const float* __restrict__ w,
+                                              uint64_t* __restrict__ cum, uint64_t* st, unsigned long long* ctl,
+                                              int ntile, int use_ticket, int spin_limit) {
     __shared__ uint64_t s_w[4];
-    const int f = blockIdx.x;
-    FilterDev& fd = F[f];
-    if (!fd.do_resample) return;
-    const int t0 = L.f_off[f] / kTile, nt = L.f_padn[f] / kTile;
-    uint64_t carry = 0;
-    for (int c = 0; c < nt; c += kTB) {
-        const int t = c + (int)threadIdx.x;
-        const uint64_t a = t < nt ? tile_agg[t0 + t] : 0;
-        uint64_t incl;
-        const uint64_t tot = block_scan_u64(a, incl, s_w);
-        if (t < nt) tile_pre[t0 + t] = carry + incl - a;
-        carry += tot;
-    }
+    __shared__ int s_tile, s_failed, s_dmin;
+    __shared__ uint64_t s_excl;
     if (threadIdx.x == 0) {
-        fd.total = carry;
-        if (carry > 0) {
-            const uint64_t lcg = fd.lcg * 6364136223846793005ull + 1442695040888963407ull;  // cu:653
-            fd.lcg = lcg;
-            fd.u32 = (uint32_t)(lcg >> 32);
+        int tk = (int)blockIdx.x;
+        if (use_ticket) {
+            tk = (int)atomicAdd(&ctl[1], 1ull);
+            if (tk == ntile - 1) atomicExch(&ctl[1], 0ull);
         }
+        s_tile = tk;
     }
-}
-
-// Pass 3: inclusive cumulative fixed-point weight of every particle.
-__global__ __launch_bounds__(kTB) void k_scan_apply(Layout L, FilterDev* F, const float* __restrict__ w,
-                                                    const uint64_t* __restrict__ tile_pre, uint64_t* __restrict__ cum) {
-    __shared__ uint64_t s_w[4];
-    const int tile = blockIdx.x;
+    __syncthreads();
+    const int tile = s_tile;
+    const float4 wv = tile_w(w, tile);  // issued before the filter lookup
+    const uint32_t epoch = (uint32_t)ctl[0] & 0x3fffffffu;
     const int f = L.blk_filter[tile * (kTile / kTB)];
-    if (!F[f].do_resample) return;
+    FilterDev& fd = F[f];
+    if (!fd.do_resample) {
+        if (threadIdx.x == 0) lb_publish(st, tile, epoch << 2, 0);
+        return;
+    }
+    const int off = L.f_off[f], nf = L.f_n[f];
+    const int t0 = off / kTile, nt = L.f_padn[f] / kTile;
     uint64_t q[4];
-    tile_q(w, tile, L.f_off[f], L.f_n[f], q);
+    tile_q(wv, tile, off, nf, q);
     q[1] += q[0];
     q[2] += q[1];
     q[3] += q[2];
     uint64_t incl;
-    (void)block_scan_u64(q[3], incl, s_w);
-    const uint64_t pre = tile_pre[tile] + incl - q[3];
+    const uint64_t agg = block_scan_u64(q[3], incl, s_w);
+    if (threadIdx.x == 0) {
+        lb_publish(st, tile, (epoch << 2) | (tile == t0 ? kLbPre : kLbAgg), agg);
+        s_failed = 0;
+        s_excl = 0;
+    }
+    __syncthreads();
+    if (tile != t0) {
+        // windows of kLbWin predecessors: thread i holds the ones at distance i + 256 j, j < kLbPer
+        const int tid = threadIdx.x;
+        uint64_t ex = 0;
+        for (int base = tile - 1; base >= t0; base -= kLbWin) {
+            uint64_t v[kLbPer];
+            uint32_t kind[kLbPer];
+            bool in[kLbPer];
+#pragma unroll
+            for (int j = 0; j < kLbPer; ++j) {
+                in[j] = base - tid - kTB * j >= t0;
+                kind[j] = 0;
+                v[j] = 0;
+            }
+            for (int it = 0;; ++it) {
+                bool all = true;
+#pragma unroll
+                for (int j = 0; j < kLbPer; ++j)
+                    if (in[j] && kind[j] == 0) {
+                        kind[j] = lb_read(st, base - tid - kTB * j, epoch, v[j]);
+                        all = all && kind[j] != 0;
+                    }
+                if (all || it >= spin_limit) break;
+                __builtin_amdgcn_s_sleep(1);
+            }
+            int bad = 0, dmin = INT_MAX;
+#pragma unroll
+            for (int j = 0; j < kLbPer; ++j) {
+                bad |= (in[j] && kind[j] == 0) ? 1 : 0;
+                if (in[j] && kind[j] == kLbPre) dmin = min(dmin, tid + kTB * j);
+            }
+            if (tid == 0) s_dmin = INT_MAX;
+            if (__syncthreads_or(bad)) {
+                if (tid == 0) s_failed = 1;
+                break;
+            }
+            if (dmin != INT_MAX) atomicMin(&s_dmin, dmin);
+            __syncthreads();
+            const int dstar = s_dmin;  // nearest predecessor holding an inclusive prefix
+            uint64_t part = 0;
+#pragma unroll
+            for (int j = 0; j < kLbPer; ++j)
+                if (in[j] && tid + kTB * j <= dstar) part += v[j];
+            ex += block_sum_u64(part, s_w);
+            if (dstar != INT_MAX) break;
+        }
+        if (tid == 0) s_excl = ex;
+    }
+    __syncthreads();
+    uint64_t excl = s_excl;
+    if (s_failed) {  // look-back did not complete in time: sum the predecessors directly
+        excl = 0;
+        for (int t = t0; t < tile; ++t) {
+            uint64_t qq[4], inc2;
+            tile_q(tile_w(w, t), t, off, nf, qq);
+            excl += block_scan_u64(qq[0] + qq[1] + qq[2] + qq[3], inc2, s_w);
+        }
+    }
+    if (threadIdx.x == 0) {
+        if (tile != t0) lb_publish(st, tile, (epoch << 2) | kLbPre, excl + agg);
+        if (tile == t0 + nt - 1) {
+            const uint64_t T = excl + agg;
+            fd.total = T;
+            if (T > 0) {
+                const uint64_t lcg = fd.lcg * 6364136223846793005ull + 1442695040888963407ull;  // cu:653
+                fd.lcg = lcg;
+                fd.u32 = (uint32_t)(lcg >> 32);
+            }
+        }
+    }
+    const uint64_t pre = excl + incl - q[3];
     uint64_t* cp = cum + tile * kTile + (int)threadIdx.x * 4;
     cp[0] = pre + q[0];
     cp[1] = pre + q[1];
@@ -708,10 +860,12 @@ __global__ __launch_bounds__(kTB) void k_inj_apply(Layout L, FilterDev* F, PoseB
     S.c[2][off + t] = (float)az;
 }
 
-// The gathered set becomes current for filters that resampled with a positive total.
-__global__ void k_flip(FilterDev* F, int nfilt) {
+// The gathered set becomes current for filters that resampled with a positive total; the
+// scan's status epoch advances for the next resample.
+__global__ void k_flip(FilterDev* F, int nfilt, unsigned long long* scan_ctl) {
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f < nfilt && F[f].do_resample && F[f].total > 0) F[f].parity ^= 1;
+    if (f == 0) scan_ctl[0] = (scan_ctl[0] + 1ull) & 0x3fffffffull;
 }
 
 // ---------------------------------------------------------------------------- estimate
@@ -738,9 +892,9 @@ __device__ inline void block_partials(double (&acc)[NV], double* __restrict__ pa
 // non-null (sums only) thread 0 then takes the Markley mean of the finished sums and writes the
 // covariance pass parameters.
 template <int NV>
-__global__ __launch_bounds__(1024) void k_est_fin(Layout L, FilterDev* F, const double* __restrict__ part, int which,
-                                                  CovParams* cov) {
-    static_assert(NV <= 16, "one wave per value");
+__global__ __launch_bounds__(kFinTB) void k_est_fin(Layout L, FilterDev* F, const double* __restrict__ part,
+                                                    int which, CovParams* cov) {
+    static_assert(NV <= kFinTB / 64, "one wave per value");
     __shared__ double sv[NV];
     const int f = blockIdx.x;
     const int b0 = L.f_off[f] / kTile, nb = L.f_padn[f] / kTile;
@@ -931,8 +1085,19 @@ int state_alloc(PfState* s, const std::vector<int>& ns) {
     PF_CHECK(hipMalloc((void**)&s->part_n, sizeof(double) * (size_t)nb256));
     PF_CHECK(hipMalloc((void**)&s->part_est, sizeof(double) * kEstN * (size_t)ntile));
     PF_CHECK(hipMalloc((void**)&s->part_cov, sizeof(double) * kCovN * (size_t)ntile));
-    PF_CHECK(hipMalloc((void**)&s->tile_agg, sizeof(uint64_t) * (size_t)ntile));
-    PF_CHECK(hipMalloc((void**)&s->tile_pre, sizeof(uint64_t) * (size_t)ntile));
+    PF_CHECK(hipMalloc((void**)&s->scan_st, sizeof(uint64_t) * 2 * (size_t)ntile));
+    PF_CHECK(hipMemsetAsync(s->scan_st, 0, sizeof(uint64_t) * 2 * (size_t)ntile, s->stream));
+    PF_CHECK(hipMalloc((void**)&s->scan_ctl, 2 * sizeof(unsigned long long)));
+    PF_CHECK(hipMemsetAsync(s->scan_ctl, 0, 2 * sizeof(unsigned long long), s->stream));
+    {
+        // scan tiles that can be resident at once (block index order suffices up to this many)
+        int per_cu = 0, dev = 0;
+        hipDeviceProp_t prop;
+        PF_CHECK(hipGetDevice(&dev));
+        PF_CHECK(hipGetDeviceProperties(&prop, dev));
+        PF_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_scan, kTB, 0));
+        s->scan_resident = per_cu * prop.multiProcessorCount;
+    }
     // layout tables
     std::vector<int> blk_filter(nb256);
     for (int f = 0; f < nf; ++f)
@@ -975,8 +1140,8 @@ void state_free_particles(PfState* s) {
         }
     void** ptrs[] = {(void**)&s->w,        (void**)&s->ll,          (void**)&s->cum,       (void**)&s->anc,
                      (void**)&s->inj_idx,  (void**)&s->inj_next,    (void**)&s->inj_head,  (void**)&s->inj_jit,
-                     (void**)&s->part_w,   (void**)&s->part_n,  (void**)&s->part_est,    (void**)&s->part_cov,  (void**)&s->tile_agg,
-                     (void**)&s->tile_pre, (void**)&s->d_blk_filter, (void**)&s->d_fn,     (void**)&s->d_foff,
+                     (void**)&s->part_w,   (void**)&s->part_n,  (void**)&s->part_est,    (void**)&s->part_cov,  (void**)&s->scan_st,
+                     (void**)&s->scan_ctl, (void**)&s->d_blk_filter, (void**)&s->d_fn,     (void**)&s->d_foff,
                      (void**)&s->d_fpadn,  (void**)&s->d_fwinv,     (void**)&s->fdev,      (void**)&s->d_inj_blk};
     for (void** p : ptrs) {
         (void)hipFree(*p);
@@ -1028,7 +1193,6 @@ int launch_motion(PfState* s, ParamRef<MotionParams> A, int reinit) {
 int launch_weight(PfState* s, ParamRef<WeightParams> A) {
     prof_begin(s, PFX_STAGE_WEIGHT);
     const int ppt = s->weight_ppt;
-    const int span = kTB * ppt;
     if (ppt == 1)
         hipLaunchKernelGGL(k_weight<1>, dim3(s->NP / kTB), dim3(kTB), 0, s->stream, s->L, s->pose, s->fdev, s->ll,
                            s->part_w, A);
@@ -1041,7 +1205,8 @@ int launch_weight(PfState* s, ParamRef<WeightParams> A) {
     prof_end(s);
     PF_LAUNCH_CHECK();
     prof_begin(s, PFX_STAGE_FINALIZE);
-    hipLaunchKernelGGL(k_weight_fin, dim3(s->nf), dim3(kTB), 0, s->stream, s->L, s->fdev, s->part_w, span);
+    hipLaunchKernelGGL(k_weight_fin, dim3(s->nf), dim3(kFinTB), 0, s->stream, s->L, s->fdev, s->part_w,
+                       kTB * ppt);
     prof_end(s);
     PF_LAUNCH_CHECK();
     return 0;
@@ -1054,7 +1219,7 @@ int launch_normalize(PfState* s, int force_uniform, const float* d_ratio) {
     prof_end(s);
     PF_LAUNCH_CHECK();
     prof_begin(s, PFX_STAGE_FINALIZE);
-    hipLaunchKernelGGL(k_norm_fin, dim3(s->nf), dim3(kTB), 0, s->stream, s->L, s->fdev, s->part_n, d_ratio);
+    hipLaunchKernelGGL(k_norm_fin, dim3(s->nf), dim3(kFinTB), 0, s->stream, s->L, s->fdev, s->part_n, d_ratio);
     prof_end(s);
     PF_LAUNCH_CHECK();
     return 0;
@@ -1063,9 +1228,8 @@ int launch_normalize(PfState* s, int force_uniform, const float* d_ratio) {
 int launch_resample(PfState* s, ParamRef<InjectParams> inj, const int32_t* inj_tab, int inj_nblk) {
     const int ntile = s->NP / kTile;
     prof_begin(s, PFX_STAGE_RESAMPLE_SCAN);
-    hipLaunchKernelGGL(k_scan_tiles, dim3(ntile), dim3(kTB), 0, s->stream, s->L, s->fdev, s->w, s->tile_agg);
-    hipLaunchKernelGGL(k_scan_top, dim3(s->nf), dim3(kTB), 0, s->stream, s->L, s->fdev, s->tile_agg, s->tile_pre);
-    hipLaunchKernelGGL(k_scan_apply, dim3(ntile), dim3(kTB), 0, s->stream, s->L, s->fdev, s->w, s->tile_pre, s->cum);
+    hipLaunchKernelGGL(k_scan, dim3(ntile), dim3(kTB), 0, s->stream, s->L, s->fdev, s->w, s->cum, s->scan_st,
+                       s->scan_ctl, ntile, ntile > s->scan_resident ? 1 : 0, s->scan_spin_limit);
     prof_end(s);
     PF_LAUNCH_CHECK();
     prof_begin(s, PFX_STAGE_RESAMPLE_GATHER);
@@ -1086,7 +1250,7 @@ int launch_resample(PfState* s, ParamRef<InjectParams> inj, const int32_t* inj_t
         PF_LAUNCH_CHECK();
     }
     prof_begin(s, PFX_STAGE_FINALIZE);
-    hipLaunchKernelGGL(k_flip, dim3((s->nf + 63) / 64), dim3(64), 0, s->stream, s->fdev, s->nf);
+    hipLaunchKernelGGL(k_flip, dim3((s->nf + 63) / 64), dim3(64), 0, s->stream, s->fdev, s->nf, s->scan_ctl);
     prof_end(s);
     PF_LAUNCH_CHECK();
     return 0;
@@ -1099,7 +1263,7 @@ int launch_est_sums(PfState* s, CovParams* d_cov) {
     prof_end(s);
     PF_LAUNCH_CHECK();
     prof_begin(s, PFX_STAGE_FINALIZE);
-    hipLaunchKernelGGL(k_est_fin<kEstN>, dim3(s->nf), dim3(1024), 0, s->stream, s->L, s->fdev, s->part_est, 0,
+    hipLaunchKernelGGL(k_est_fin<kEstN>, dim3(s->nf), dim3(kFinTB), 0, s->stream, s->L, s->fdev, s->part_est, 0,
                        d_cov);
     prof_end(s);
     PF_LAUNCH_CHECK();
@@ -1113,7 +1277,7 @@ int launch_est_covs(PfState* s, ParamRef<CovParams> A) {
     prof_end(s);
     PF_LAUNCH_CHECK();
     prof_begin(s, PFX_STAGE_FINALIZE);
-    hipLaunchKernelGGL(k_est_fin<kCovN>, dim3(s->nf), dim3(1024), 0, s->stream, s->L, s->fdev, s->part_cov, 1,
+    hipLaunchKernelGGL(k_est_fin<kCovN>, dim3(s->nf), dim3(kFinTB), 0, s->stream, s->L, s->fdev, s->part_cov, 1,
                        nullptr);
     prof_end(s);
     PF_LAUNCH_CHECK();
@@ -1529,6 +1693,13 @@ int pfx_set_weight_variant(void* h, int variant) {
     PfState* s = static_cast<PfState*>(h);
     if (!s || (variant != 1 && variant != 2 && variant != 4)) return -1;
     s->weight_ppt = variant;
+    return 0;
+}
+
+int pfx_set_scan_spin_limit(void* h, int limit) {
+    PfState* s = static_cast<PfState*>(h);
+    if (!s || limit < 0) return -1;
+    s->scan_spin_limit = limit;
     return 0;
 }
 

@@ -311,6 +311,43 @@ def test_resample_bitexact(gpu, n, ratio):
     assert np.all(wn == np.float32(1.0 / n))
 
 
+@pytest.mark.parametrize("n", [4097, 65_537, 1_000_000])
+def test_resample_scan_direct_route_bitexact(gpu, n):
+    """The look-back scan with a zero wait budget: every tile that finds a predecessor
+    unpublished sums the predecessors itself.  Same ancestors as the oracle, twice in a row
+    (the second launch runs under the next epoch of the status words)."""
+    pos, quat, w = weighted_state(n, n + 3)
+    k, o = pair(n, 5)
+    k.set_scan_spin_limit(0)
+    for rep in range(2):
+        k.set_state(pos, quat, w)
+        o.set_state(pos.astype(np.float64), quat.astype(np.float64), w.astype(np.float64))
+        lcg = 0x0F1E_2D3C_4B5A_6978 ^ (n + rep)
+        k.set_lcg(lcg)
+        o.set_rng(o.rng()[0], 0, lcg)
+        k.resample(0.0)
+        o.resample(0.0)
+        assert np.array_equal(k.ancestors(), o.ancestors())
+        assert k.get_lcg() == o.rng()[2]
+
+
+def test_resample_bitexact_more_tiles_than_resident(gpu):
+    """4.2M particles: more scan tiles than the device holds at once, so tiles come from the
+    ticket counter; ancestors still bit-exact, over two launches."""
+    n = 4_200_000
+    pos, quat, w = weighted_state(n, 77)
+    k, o = pair(n, 5)
+    for rep in range(2):
+        k.set_state(pos, quat, w)
+        o.set_state(pos.astype(np.float64), quat.astype(np.float64), w.astype(np.float64))
+        lcg = 0x5555_AAAA_1234_8765 + rep
+        k.set_lcg(lcg)
+        o.set_rng(o.rng()[0], 0, lcg)
+        k.resample(0.0)
+        o.resample(0.0)
+        assert np.array_equal(k.ancestors(), o.ancestors())
+
+
 def test_resample_copies_property(gpu):
     n = 100_000
     pos, quat, w = weighted_state(n, 21, conc=0.5)
