@@ -51,7 +51,10 @@ constexpr int kLdsRaysMax = 1024;        // real scan + ray table staged in LDS 
 constexpr int kSortMax = 4096;           // winners sorted in one workgroup up to this k
 
 struct Grid {
-    const uint32_t* bits;  // row-major bit cy*W+cx, 1 = occupied
+    // Occupancy bits of the map framed by one extra cell on every side, row-major over
+    // (W + 2) x (H + 2): bit (cy + 1) * (W + 2) + cx + 1 is map cell (cx, cy), 1 = occupied.
+    // The frame cells are set, so a ray stops on the frame where it would leave the map.
+    const uint32_t* bits;
     int W, H, nwords;
     float res, ox, oy;
 };
@@ -104,11 +107,13 @@ struct GlState {
 __device__ inline bool occ_bit(const uint32_t* bits, int idx) { return (bits[idx >> 5] >> (idx & 31)) & 1u; }
 
 // f32 DDA with the operation sequence of pf_kernels.cu:784-808 (distance to the first
-// occupied cell along a unit direction; rmax on exit, miss or beyond rmax).
+// occupied cell along a unit direction; rmax on exit, miss or beyond rmax).  The walk tracks
+// only the framed linear index: leaving the map means entering the set frame, so the loop
+// tests one bit per step and the frame is told apart from an obstacle once, at the end.
 __device__ inline float cast_ray(const uint32_t* bits, int W, int H, float res, float ox, float oy, float px, float py,
                                  float dirx, float diry, float rmax) {
     const float fx = (px - ox) / res, fy = (py - oy) / res;
-    int cx = (int)floorf(fx), cy = (int)floorf(fy);
+    const int cx = (int)floorf(fx), cy = (int)floorf(fy);
     if (!(cx >= 0 && cy >= 0 && cx < W && cy < H)) return rmax;
     const int sx = dirx > 0.f ? 1 : -1, sy = diry > 0.f ? 1 : -1;
     const float adx = fabsf(dirx), ady = fabsf(diry);
@@ -117,25 +122,33 @@ __device__ inline float cast_ray(const uint32_t* bits, int W, int H, float res, 
     const float frx = fx - (float)cx, fry = fy - (float)cy;
     float tmx = adx > 1e-9f ? ((dirx > 0.f ? 1.f - frx : frx) * tdx) : INFINITY;
     float tmy = ady > 1e-9f ? ((diry > 0.f ? 1.f - fry : fry) * tdy) : INFINITY;
-    int idx = cy * W + cx;
-    const int dxi = sx, dyi = sy * W;
+    const int Wp = W + 2;
+    int idx = (cy + 1) * Wp + cx + 1;
+    const int dxi = sx, dyi = sy * Wp;
     // Branch-free step: both candidate boundary times are formed and one is selected, which
-    // rounds exactly as the reference's if/else; the grid word is read only in bounds.
+    // rounds exactly as the reference's if/else.
+    float hit = rmax;
+    int hidx = -1;
     for (int step = 0; step < 8192; ++step) {
         const bool xs = tmx <= tmy;
         const float dist = xs ? tmx : tmy;
         const float tmx1 = tmx + tdx, tmy1 = tmy + tdy;
         tmx = xs ? tmx1 : tmx;
         tmy = xs ? tmy : tmy1;
-        cx += xs ? sx : 0;
-        cy += xs ? 0 : sy;
         idx += xs ? dxi : dyi;
-        const bool inb = (unsigned)cx < (unsigned)W && (unsigned)cy < (unsigned)H;
-        const bool occ = occ_bit(bits, inb ? idx : 0) && inb;
-        if (dist > rmax || !inb) return rmax;
-        if (occ) return dist;
+        const bool occ = occ_bit(bits, idx);
+        if (dist > rmax) break;
+        if (occ) {
+            hit = dist;
+            hidx = idx;
+            break;
+        }
     }
-    return rmax;
+    if (hidx >= 0) {  // a frame cell: the ray left the map
+        const int qy = hidx / Wp, qx = hidx - qy * Wp;
+        if (qx == 0 || qy == 0 || qx == Wp - 1 || qy == H + 1) hit = rmax;
+    }
+    return hit;
 }
 
 // Hypothesis h (deterministic-draw mode): cell = floor(u53 * nfree); x/y offsets and yaw
@@ -726,22 +739,28 @@ using namespace glk;
 extern "C" {
 
 void* pfgl_create(int W, int H, float res, float ox, float oy, const unsigned char* occ, int cap_hyp) {
-    if (W <= 0 || H <= 0 || cap_hyp <= 0 || !occ || (long long)W * H > (1ll << 30)) return nullptr;
+    if (W <= 0 || H <= 0 || cap_hyp <= 0 || !occ || (long long)(W + 2) * (H + 2) > (1ll << 30)) return nullptr;
     GlState* s = new (std::nothrow) GlState();
     if (!s) return nullptr;
     const long long ncell = (long long)W * H;
-    const int nwords = (int)((ncell + 31) / 32);
+    const long long Wp = W + 2, Hp = H + 2;  // framed grid (frame cells set)
+    const int nwords = (int)((Wp * Hp + 31) / 32);
     std::vector<uint32_t> bits((size_t)nwords, 0u);
     std::vector<int32_t> freexy;
     freexy.reserve((size_t)ncell * 2);
-    for (long long i = 0; i < ncell; ++i) {
-        if (occ[i]) {
-            bits[(size_t)(i >> 5)] |= 1u << (i & 31);
-        } else {
+    for (long long y = 0; y < Hp; ++y)
+        for (long long x = 0; x < Wp; ++x) {
+            const bool frame = x == 0 || y == 0 || x == Wp - 1 || y == Hp - 1;
+            if (frame || occ[(y - 1) * W + (x - 1)]) {
+                const long long i = y * Wp + x;
+                bits[(size_t)(i >> 5)] |= 1u << (i & 31);
+            }
+        }
+    for (long long i = 0; i < ncell; ++i)
+        if (!occ[i]) {
             freexy.push_back((int32_t)(i % W));
             freexy.push_back((int32_t)(i / W));
         }
-    }
     s->nfree = (int)(freexy.size() / 2);
     s->cap_hyp = cap_hyp;
     s->g.W = W;
