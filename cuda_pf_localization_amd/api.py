@@ -261,6 +261,11 @@ class GlKernels:
         except Exception:
             pass
 
+    def set_score_mode(self, mode: int):
+        """0: by hypothesis count, 1: one wave per hypothesis, 2: one lane per ray, errors summed
+        in a second pass (same CAER bits)."""
+        _check(self.L.pfglx_set_score_mode(self.h, mode), "pfglx_set_score_mode")
+
     def free_cells(self) -> int:
         return self.L.pfgl_free_cells(self.h)
 

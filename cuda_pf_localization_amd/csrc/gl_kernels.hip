@@ -48,6 +48,10 @@ namespace glk {
 constexpr int kTB = 256;
 constexpr int kLdsBitsMaxWords = 6144;   // 24 KB of grid bits in LDS (196,608 cells)
 constexpr int kLdsRaysMax = 1024;        // real scan + ray table staged in LDS up to this many rays
+constexpr size_t kLdsScoreBytesMax = 64 * 1024;  // byte grid + ray tables per scoring block (two per CU)
+constexpr int kScoreBytesMinHyp = 16384;         // below this, 256-thread bit-grid blocks fill the chip better
+constexpr int kScoreBlockMaxHyp = 65536;         // below this, one lane per ray in two passes (mode 2)
+constexpr size_t kRayDiffsMax = (size_t)32 << 20;  // per-ray |diff| scratch of the two-pass route (128 MB)
 constexpr int kSortMax = 4096;           // winners sorted in one workgroup up to this k
 
 struct Grid {
@@ -55,7 +59,8 @@ struct Grid {
     // (W + 2) x (H + 2): bit (cy + 1) * (W + 2) + cx + 1 is map cell (cx, cy), 1 = occupied.
     // The frame cells are set, so a ray stops on the frame where it would leave the map.
     const uint32_t* bits;
-    int W, H, nwords;
+    const uint8_t* bytes;  // the same framed grid, one byte per cell (nbytes, a multiple of 16)
+    int W, H, nwords, nbytes;
     float res, ox, oy;
 };
 
@@ -63,6 +68,7 @@ struct GlState {
     hipStream_t stream = nullptr;
     Grid g{};
     uint32_t* d_bits = nullptr;
+    uint8_t* d_bytes = nullptr;
     int32_t* d_free = nullptr;  // (cx, cy) pairs, row-major order
     int nfree = 0;
     int cap_hyp = 0;
@@ -78,6 +84,9 @@ struct GlState {
     int tab_nrays = -1;
     float tab_span = 0.f;
     int scan_nrays = 0;  // rays of the scan resident in d_scan
+    int score_mode = 0;  // 0 by hypothesis count, 1 one wave per hypothesis, 2 one lane per ray (two passes)
+    float* d_diff = nullptr;  // per-ray |scan - cast| of the two-pass scoring
+    size_t diff_cap = 0;
     float* h_scan = nullptr;  // pinned staging of the scan upload
     int h_scan_cap = 0;
     hipEvent_t scan_ev = nullptr;  // recorded after each upload from h_scan
@@ -106,11 +115,23 @@ struct GlState {
 
 __device__ inline bool occ_bit(const uint32_t* bits, int idx) { return (bits[idx >> 5] >> (idx & 31)) & 1u; }
 
+// Cell lookups for the raycaster: packed bits (small footprint) or one byte per cell (one LDS
+// read and one compare per step).
+struct OccBits {
+    const uint32_t* bits;
+    __device__ bool operator()(int idx) const { return occ_bit(bits, idx); }
+};
+struct OccBytes {
+    const uint8_t* bytes;
+    __device__ bool operator()(int idx) const { return bytes[idx] != 0; }
+};
+
 // f32 DDA with the operation sequence of pf_kernels.cu:784-808 (distance to the first
 // occupied cell along a unit direction; rmax on exit, miss or beyond rmax).  The walk tracks
 // only the framed linear index: leaving the map means entering the set frame, so the loop
 // tests one bit per step and the frame is told apart from an obstacle once, at the end.
-__device__ inline float cast_ray(const uint32_t* bits, int W, int H, float res, float ox, float oy, float px, float py,
+template <class Occ>
+__device__ inline float cast_ray(Occ occ_at, int W, int H, float res, float ox, float oy, float px, float py,
                                  float dirx, float diry, float rmax) {
     const float fx = (px - ox) / res, fy = (py - oy) / res;
     const int cx = (int)floorf(fx), cy = (int)floorf(fy);
@@ -136,7 +157,7 @@ __device__ inline float cast_ray(const uint32_t* bits, int W, int H, float res, 
         tmx = xs ? tmx1 : tmx;
         tmy = xs ? tmy : tmy1;
         idx += xs ? dxi : dyi;
-        const bool occ = occ_bit(bits, idx);
+        const bool occ = occ_at(idx);
         if (dist > rmax) break;
         if (occ) {
             hit = dist;
@@ -174,34 +195,41 @@ __global__ __launch_bounds__(kTB) void k_gl_gen(const int32_t* __restrict__ free
     hyp[3 * h + 2] = -3.14159265f + tt;
 }
 
-// CAER scoring.  LDS layout: [grid bits][real scan f32][cos phi f64][sin phi f64].
-template <bool kLdsGrid, bool kLdsRays>
-__global__ __launch_bounds__(kTB) void k_gl_score(Grid g, const float* __restrict__ hyp, int nh,
+// CAER scoring.  LDS layout: [grid][real scan f32][cos phi f64][sin phi f64]; the grid is
+// staged as bytes (kBytes, 1024-thread blocks, two per CU) or as bits (256-thread blocks).
+template <int kBT, bool kBytes, bool kLdsGrid, bool kLdsRays>
+__global__ __launch_bounds__(kBT) void k_gl_score(Grid g, const float* __restrict__ hyp, int nh,
                                                   const float* __restrict__ scan, const double* __restrict__ cphi,
                                                   const double* __restrict__ sphi, int nrays, float rmax,
                                                   float* __restrict__ caer) {
     extern __shared__ __align__(16) unsigned char smem[];
-    uint32_t* sbits = reinterpret_cast<uint32_t*>(smem);
-    const int gw = kLdsGrid ? ((g.nwords + 3) & ~3) : 0;
-    double* scphi = reinterpret_cast<double*>(smem + 4 * (size_t)gw);
+    const int gbytes = kLdsGrid ? (kBytes ? g.nbytes : 4 * ((g.nwords + 3) & ~3)) : 0;
+    double* scphi = reinterpret_cast<double*>(smem + gbytes);
     double* ssphi = scphi + (kLdsRays ? nrays : 0);
     float* sscan = reinterpret_cast<float*>(ssphi + (kLdsRays ? nrays : 0));
-    if (kLdsGrid)
-        for (int i = threadIdx.x; i < g.nwords; i += kTB) sbits[i] = g.bits[i];
+    if (kLdsGrid) {
+        if (kBytes) {
+            const uint4* src = reinterpret_cast<const uint4*>(g.bytes);
+            uint4* dst = reinterpret_cast<uint4*>(smem);
+            for (int i = threadIdx.x; i < g.nbytes / 16; i += kBT) dst[i] = src[i];
+        } else {
+            uint32_t* sbits = reinterpret_cast<uint32_t*>(smem);
+            for (int i = threadIdx.x; i < g.nwords; i += kBT) sbits[i] = g.bits[i];
+        }
+    }
     if (kLdsRays)
-        for (int i = threadIdx.x; i < nrays; i += kTB) {
+        for (int i = threadIdx.x; i < nrays; i += kBT) {
             scphi[i] = cphi[i];
             ssphi[i] = sphi[i];
             sscan[i] = scan[i];
         }
     __syncthreads();
-    const uint32_t* bits = kLdsGrid ? sbits : g.bits;
     const double* tc = kLdsRays ? scphi : cphi;
     const double* ts = kLdsRays ? ssphi : sphi;
     const float* sc = kLdsRays ? sscan : scan;
     const int lane = threadIdx.x & 63;
-    const int waves_total = gridDim.x * (kTB / 64);
-    for (int h = blockIdx.x * (kTB / 64) + (threadIdx.x >> 6); h < nh; h += waves_total) {
+    const int waves_total = gridDim.x * (kBT / 64);
+    for (int h = blockIdx.x * (kBT / 64) + (threadIdx.x >> 6); h < nh; h += waves_total) {
         const float x = hyp[3 * h], y = hyp[3 * h + 1];
         double st, ct;
         det_sincos((double)hyp[3 * h + 2], st, ct);
@@ -210,12 +238,70 @@ __global__ __launch_bounds__(kTB) void k_gl_score(Grid g, const float* __restric
             const double cp = tc[r], sp = ts[r];
             const float dx = (float)(ct * cp - st * sp);
             const float dy = (float)(st * cp + ct * sp);
-            const float vm = cast_ray(bits, g.W, g.H, g.res, g.ox, g.oy, x, y, dx, dy, rmax);
+            float vm;
+            if (kBytes)
+                vm = cast_ray(OccBytes{kLdsGrid ? smem : g.bytes}, g.W, g.H, g.res, g.ox, g.oy, x, y, dx, dy, rmax);
+            else
+                vm = cast_ray(OccBits{kLdsGrid ? reinterpret_cast<const uint32_t*>(smem) : g.bits}, g.W, g.H, g.res,
+                              g.ox, g.oy, x, y, dx, dy, rmax);
             acc += fabsf(sc[r] - vm);
         }
         acc = wave_sum_down(acc);
         if (lane == 0) caer[h] = acc;
     }
+}
+
+// CAER scoring in two passes for batches too small to fill the device one wave per
+// hypothesis (2k hypotheses are 2k waves; one lane per ray gives 11k).  Pass 1: one lane per
+// (hypothesis, ray) over the whole batch, consecutive lanes on consecutive rays of one
+// hypothesis, storing |scan - cast|; the grid sits in LDS as bytes (1024-thread workgroups,
+// two per CU) or bits.  Pass 2: one wave per hypothesis adds them in the order of k_gl_score
+// (lane l: rays l, l + 64, ...; then the same shuffle-down tree), so CAER has the same bits.
+template <int kBT, bool kBytes>
+__global__ __launch_bounds__(kBT) void k_gl_ray_diffs(Grid g, const float* __restrict__ hyp, int nh,
+                                                      const float* __restrict__ scan,
+                                                      const double* __restrict__ cphi,
+                                                      const double* __restrict__ sphi, int nrays, float rmax,
+                                                      float* __restrict__ diff) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    if (kBytes) {
+        const uint4* src = reinterpret_cast<const uint4*>(g.bytes);
+        uint4* dst = reinterpret_cast<uint4*>(smem);
+        for (int i = threadIdx.x; i < g.nbytes / 16; i += kBT) dst[i] = src[i];
+    } else {
+        uint32_t* sbits = reinterpret_cast<uint32_t*>(smem);
+        for (int i = threadIdx.x; i < g.nwords; i += kBT) sbits[i] = g.bits[i];
+    }
+    __syncthreads();
+    const size_t total = (size_t)nh * (size_t)nrays;
+    for (size_t t = (size_t)blockIdx.x * kBT + threadIdx.x; t < total; t += (size_t)gridDim.x * kBT) {
+        const int h = (int)(t / (size_t)nrays), r = (int)(t - (size_t)h * nrays);
+        const float x = hyp[3 * h], y = hyp[3 * h + 1];
+        double st, ct;
+        det_sincos((double)hyp[3 * h + 2], st, ct);
+        const double cp = cphi[r], sp = sphi[r];
+        const float dx = (float)(ct * cp - st * sp);
+        const float dy = (float)(st * cp + ct * sp);
+        float vm;
+        if (kBytes)
+            vm = cast_ray(OccBytes{smem}, g.W, g.H, g.res, g.ox, g.oy, x, y, dx, dy, rmax);
+        else
+            vm = cast_ray(OccBits{reinterpret_cast<const uint32_t*>(smem)}, g.W, g.H, g.res, g.ox, g.oy, x, y, dx, dy,
+                          rmax);
+        diff[t] = fabsf(scan[r] - vm);
+    }
+}
+
+__global__ __launch_bounds__(kTB) void k_gl_caer_sum(const float* __restrict__ diff, int nh, int nrays,
+                                                     float* __restrict__ caer) {
+    const int h = blockIdx.x * (kTB / 64) + (threadIdx.x >> 6);
+    if (h >= nh) return;
+    const int lane = threadIdx.x & 63;
+    const float* d = diff + (size_t)h * nrays;
+    float acc = 0.f;
+    for (int r = lane; r < nrays; r += 64) acc += d[r];
+    acc = wave_sum_down(acc);
+    if (lane == 0) caer[h] = acc;
 }
 
 // Virtual scans for poses[3n] -> out[n][nrays] (training data / tests).
@@ -243,7 +329,7 @@ __global__ __launch_bounds__(kTB) void k_gl_render(Grid g, const float* __restri
         const double cp = cphi[r], sp = sphi[r];
         const float dx = (float)(ct * cp - st * sp);
         const float dy = (float)(st * cp + ct * sp);
-        out[t] = cast_ray(bits, g.W, g.H, g.res, g.ox, g.oy, x, y, dx, dy, rmax);
+        out[t] = cast_ray(OccBits{bits}, g.W, g.H, g.res, g.ox, g.oy, x, y, dx, dy, rmax);
     }
 }
 
@@ -452,30 +538,70 @@ int ensure_rays(GlState* s, int nrays, float span) {
     return 0;
 }
 
+int ray_diffs_reserve(GlState* s, size_t n) {
+    if (n <= s->diff_cap) return 0;
+    (void)hipFree(s->d_diff);
+    s->d_diff = nullptr;
+    s->diff_cap = 0;
+    if (hipMalloc((void**)&s->d_diff, sizeof(float) * n) != hipSuccess) return -1;
+    s->diff_cap = n;
+    return 0;
+}
+
 int score_launch(GlState* s, int nrays, float rmax) {
-    const bool lds_grid = s->g.nwords <= kLdsBitsMaxWords;
     const bool lds_rays = nrays <= kLdsRaysMax;
-    const size_t shm = (lds_grid ? 4 * (size_t)((s->g.nwords + 3) & ~3) : 0) +
-                       (lds_rays ? (size_t)nrays * (8 + 8 + 4) : 0);
-    const int waves = (s->nh + 0);
-    int blocks = (waves + 3) / 4;
-    if (blocks > 4096) blocks = 4096;
-    if (blocks < 1) blocks = 1;
+    const size_t ray_lds = lds_rays ? (size_t)nrays * (8 + 8 + 4) : 0;
     Grid g = s->g;
     g.bits = s->d_bits;
+    g.bytes = s->d_bytes;
     prof_b(s, PFGLX_STAGE_SCORE);
-    if (lds_grid && lds_rays)
-        hipLaunchKernelGGL((k_gl_score<true, true>), dim3(blocks), dim3(kTB), shm, s->stream, g, s->d_hyp, s->nh,
-                           s->d_scan, s->d_cphi, s->d_sphi, nrays, rmax, s->d_caer);
-    else if (lds_grid)
-        hipLaunchKernelGGL((k_gl_score<true, false>), dim3(blocks), dim3(kTB), shm, s->stream, g, s->d_hyp, s->nh,
-                           s->d_scan, s->d_cphi, s->d_sphi, nrays, rmax, s->d_caer);
-    else if (lds_rays)
-        hipLaunchKernelGGL((k_gl_score<false, true>), dim3(blocks), dim3(kTB), shm, s->stream, g, s->d_hyp, s->nh,
-                           s->d_scan, s->d_cphi, s->d_sphi, nrays, rmax, s->d_caer);
-    else
-        hipLaunchKernelGGL((k_gl_score<false, false>), dim3(blocks), dim3(kTB), shm, s->stream, g, s->d_hyp, s->nh,
-                           s->d_scan, s->d_cphi, s->d_sphi, nrays, rmax, s->d_caer);
+    int mode = s->score_mode != 0 ? s->score_mode : (s->nh < kScoreBlockMaxHyp ? 2 : 1);
+    const size_t total = (size_t)s->nh * (size_t)nrays;
+    const bool diff_bytes = (size_t)g.nbytes <= kLdsScoreBytesMax;
+    if (mode == 2 && (total > kRayDiffsMax || (!diff_bytes && g.nwords > kLdsBitsMaxWords) ||
+                      ray_diffs_reserve(s, total) != 0))
+        mode = 1;  // two-pass route unavailable: the wave kernel
+    if (mode == 2) {
+        if (diff_bytes) {
+            const int blocks = (int)std::max<size_t>(1, std::min<size_t>((total + 1023) / 1024, 2048));
+            hipLaunchKernelGGL((k_gl_ray_diffs<1024, true>), dim3(blocks), dim3(1024), (size_t)g.nbytes, s->stream, g,
+                               s->d_hyp, s->nh, s->d_scan, s->d_cphi, s->d_sphi, nrays, rmax, s->d_diff);
+        } else {
+            const int blocks = (int)std::max<size_t>(1, std::min<size_t>((total + kTB - 1) / kTB, 8192));
+            hipLaunchKernelGGL((k_gl_ray_diffs<kTB, false>), dim3(blocks), dim3(kTB), 4 * (size_t)((g.nwords + 3) & ~3),
+                               s->stream, g, s->d_hyp, s->nh, s->d_scan, s->d_cphi, s->d_sphi, nrays, rmax, s->d_diff);
+        }
+        hipLaunchKernelGGL(k_gl_caer_sum, dim3((s->nh + 3) / 4), dim3(kTB), 0, s->stream, s->d_diff, s->nh, nrays,
+                           s->d_caer);
+    } else if ((size_t)g.nbytes + ray_lds <= kLdsScoreBytesMax && s->nh >= kScoreBytesMinHyp) {
+        // byte grid: 1024-thread blocks, two per CU (enough hypotheses to give every CU two)
+        const size_t shm = (size_t)g.nbytes + ray_lds;
+        int blocks = (s->nh + 15) / 16;
+        blocks = std::max(1, std::min(blocks, 1024));
+        if (lds_rays)
+            hipLaunchKernelGGL((k_gl_score<1024, true, true, true>), dim3(blocks), dim3(1024), shm, s->stream, g,
+                               s->d_hyp, s->nh, s->d_scan, s->d_cphi, s->d_sphi, nrays, rmax, s->d_caer);
+        else
+            hipLaunchKernelGGL((k_gl_score<1024, true, true, false>), dim3(blocks), dim3(1024), shm, s->stream, g,
+                               s->d_hyp, s->nh, s->d_scan, s->d_cphi, s->d_sphi, nrays, rmax, s->d_caer);
+    } else {
+        const bool lds_grid = s->g.nwords <= kLdsBitsMaxWords;
+        const size_t shm = (lds_grid ? 4 * (size_t)((s->g.nwords + 3) & ~3) : 0) + ray_lds;
+        int blocks = (s->nh + 3) / 4;
+        blocks = std::max(1, std::min(blocks, 4096));
+        if (lds_grid && lds_rays)
+            hipLaunchKernelGGL((k_gl_score<kTB, false, true, true>), dim3(blocks), dim3(kTB), shm, s->stream, g,
+                               s->d_hyp, s->nh, s->d_scan, s->d_cphi, s->d_sphi, nrays, rmax, s->d_caer);
+        else if (lds_grid)
+            hipLaunchKernelGGL((k_gl_score<kTB, false, true, false>), dim3(blocks), dim3(kTB), shm, s->stream, g,
+                               s->d_hyp, s->nh, s->d_scan, s->d_cphi, s->d_sphi, nrays, rmax, s->d_caer);
+        else if (lds_rays)
+            hipLaunchKernelGGL((k_gl_score<kTB, false, false, true>), dim3(blocks), dim3(kTB), shm, s->stream, g,
+                               s->d_hyp, s->nh, s->d_scan, s->d_cphi, s->d_sphi, nrays, rmax, s->d_caer);
+        else
+            hipLaunchKernelGGL((k_gl_score<kTB, false, false, false>), dim3(blocks), dim3(kTB), shm, s->stream, g,
+                               s->d_hyp, s->nh, s->d_scan, s->d_cphi, s->d_sphi, nrays, rmax, s->d_caer);
+    }
     prof_e(s);
     GL_LAUNCH_CHECK();
     return 0;
@@ -745,7 +871,9 @@ void* pfgl_create(int W, int H, float res, float ox, float oy, const unsigned ch
     const long long ncell = (long long)W * H;
     const long long Wp = W + 2, Hp = H + 2;  // framed grid (frame cells set)
     const int nwords = (int)((Wp * Hp + 31) / 32);
+    const int nbytes = (int)((Wp * Hp + 15) / 16 * 16);
     std::vector<uint32_t> bits((size_t)nwords, 0u);
+    std::vector<uint8_t> bytes((size_t)nbytes, 0u);
     std::vector<int32_t> freexy;
     freexy.reserve((size_t)ncell * 2);
     for (long long y = 0; y < Hp; ++y)
@@ -754,6 +882,7 @@ void* pfgl_create(int W, int H, float res, float ox, float oy, const unsigned ch
             if (frame || occ[(y - 1) * W + (x - 1)]) {
                 const long long i = y * Wp + x;
                 bits[(size_t)(i >> 5)] |= 1u << (i & 31);
+                bytes[(size_t)i] = 1;
             }
         }
     for (long long i = 0; i < ncell; ++i)
@@ -766,12 +895,14 @@ void* pfgl_create(int W, int H, float res, float ox, float oy, const unsigned ch
     s->g.W = W;
     s->g.H = H;
     s->g.nwords = nwords;
+    s->g.nbytes = nbytes;
     s->g.res = res;
     s->g.ox = ox;
     s->g.oy = oy;
     bool ok = hipSetDevice(0) == hipSuccess && hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) == hipSuccess;
     ok = ok && hipEventCreateWithFlags(&s->scan_ev, hipEventDisableTiming) == hipSuccess;
     ok = ok && hipMalloc((void**)&s->d_bits, sizeof(uint32_t) * (size_t)nwords) == hipSuccess;
+    ok = ok && hipMalloc((void**)&s->d_bytes, (size_t)nbytes) == hipSuccess;
     ok = ok && hipMalloc((void**)&s->d_free, sizeof(int32_t) * (freexy.size() + 2)) == hipSuccess;
     ok = ok && hipMalloc((void**)&s->d_hyp, sizeof(float) * 3 * (size_t)cap_hyp) == hipSuccess;
     ok = ok && hipMalloc((void**)&s->d_caer, sizeof(float) * (size_t)cap_hyp) == hipSuccess;
@@ -782,6 +913,7 @@ void* pfgl_create(int W, int H, float res, float ox, float oy, const unsigned ch
     ok = ok && hipMalloc((void**)&s->d_outcaer, sizeof(float) * (size_t)cap_hyp) == hipSuccess;
     ok = ok && hipMalloc((void**)&s->d_outidx, sizeof(int32_t) * (size_t)cap_hyp) == hipSuccess;
     ok = ok && hipMemcpy(s->d_bits, bits.data(), sizeof(uint32_t) * (size_t)nwords, hipMemcpyHostToDevice) == hipSuccess;
+    ok = ok && hipMemcpy(s->d_bytes, bytes.data(), (size_t)nbytes, hipMemcpyHostToDevice) == hipSuccess;
     if (ok && !freexy.empty())
         ok = hipMemcpy(s->d_free, freexy.data(), sizeof(int32_t) * freexy.size(), hipMemcpyHostToDevice) == hipSuccess;
     ok = ok && hipMemset(s->d_hist, 0, sizeof(unsigned) * 256) == hipSuccess;
@@ -797,9 +929,9 @@ int pfgl_destroy(void* h) {
     GlState* s = static_cast<GlState*>(h);
     if (!s) return 0;
     if (s->stream) (void)hipStreamSynchronize(s->stream);
-    void* ptrs[] = {s->d_bits, s->d_free,    s->d_hyp,      s->d_caer,   s->d_scan, s->d_cphi,
+    void* ptrs[] = {s->d_bits, s->d_bytes, s->d_free,    s->d_hyp,      s->d_caer,   s->d_scan, s->d_cphi,
                     s->d_sphi, s->d_tk,      s->d_hist,     s->d_win,    s->d_sortbuf, s->d_out,
-                    s->d_outcaer, s->d_outidx, s->d_poses, s->d_ranges};
+                    s->d_outcaer, s->d_outidx, s->d_poses, s->d_ranges, s->d_diff};
     for (void* p : ptrs) (void)hipFree(p);
     if (s->h_scan) (void)hipHostFree(s->h_scan);
     if (s->scan_ev) (void)hipEventDestroy(s->scan_ev);
@@ -1014,6 +1146,13 @@ int pfglx_set_count(void* h, int n) {
 void* pfglx_stream(void* h) {
     GlState* s = static_cast<GlState*>(h);
     return s ? (void*)s->stream : nullptr;
+}
+
+int pfglx_set_score_mode(void* h, int mode) {
+    GlState* s = static_cast<GlState*>(h);
+    if (!s || mode < 0 || mode > 2) return -1;
+    s->score_mode = mode;
+    return 0;
 }
 
 int pfglx_profile_enable(void* h, int enable) {

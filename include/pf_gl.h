@@ -37,6 +37,10 @@ int pfglx_render(void *h, const float *poses3n, int n, int nrays, float rmax, fl
 /* Same with device pointers, enqueued on `stream` (NULL = the handle's stream). */
 int pfglx_render_device(void *h, const float *d_poses3n, int n, int nrays, float rmax, float ang_span,
                         float *d_out, void *stream);
+/* Scoring kernel: 0 chooses by hypothesis count (default), 1 one wave per hypothesis (lanes
+ * stride the rays), 2 one lane per ray with the per-ray errors summed in a second pass.  All
+ * give the same CAER bits. */
+int pfglx_set_score_mode(void *h, int mode);
 int pfglx_profile_enable(void *h, int enable);
 int pfglx_profile_read(void *h, double *ms, int *count);
 int pfglx_sync(void *h);

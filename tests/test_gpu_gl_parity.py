@@ -48,11 +48,13 @@ def test_render_bitexact(gpu):
         assert np.array_equal(r, o), f"{np.sum(r != o)} mismatches for nrays={nrays}"
 
 
+@pytest.mark.parametrize("mode", [0, 1, 2])
 @pytest.mark.parametrize("nrays", [360, 64, 1000, 2000])
-def test_caer_bitexact_and_topk(gpu, nrays):
+def test_caer_bitexact_and_topk(gpu, nrays, mode):
     cells, res, org = obstacle_map(103)
     n = 20_000
     g = GlKernels(cells, res, org[0], org[1], n)
+    g.set_score_mode(mode)
     og = OracleGrid(cells, res, org)
     truth = (3.3, 6.1, 0.7)
     scan = og.map_scan64(*truth, nrays, TAU, 8.0).astype(np.float32)
@@ -71,6 +73,20 @@ def test_caer_bitexact_and_topk(gpu, nrays):
         assert np.array_equal(kc, ocaer[want])
     plain = g.topk(10)
     assert np.array_equal(plain, hyp[bottom_k(ocaer, 10)])
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+@pytest.mark.parametrize("n", [1, 5_000])
+def test_caer_bitexact_small_batches(gpu, n, mode):
+    """Small hypothesis counts through both scoring kernels (bit grid in the wave kernel)."""
+    cells, res, org = obstacle_map(104)
+    g = GlKernels(cells, res, org[0], org[1], max(n, 1))
+    g.set_score_mode(mode)
+    og = OracleGrid(cells, res, org)
+    scan = og.map_scan64(2.2, 7.4, -1.1, 360, TAU, 8.0).astype(np.float32)
+    g.generate(n, 11)
+    g.score(scan, 8.0, TAU)
+    assert np.array_equal(g.caer(), og.caer32(scan, g.hypotheses(), np.float32(TAU), 8.0))
 
 
 @pytest.mark.parametrize("n", [1, 700, 65_536, 65_537, 150_000])
