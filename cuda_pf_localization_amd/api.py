@@ -565,7 +565,7 @@ class LearnedParams:
     fallback_caer_per_ray: float = 0.05
     seed: int = 1
     refine_rounds: int = 3
-    refine_parents: int = 32
+    refine_parents: int = 16
     refine_children: int = 64
     refine_sigma_xy: float = 0.12
     refine_sigma_yaw: float = 0.06
