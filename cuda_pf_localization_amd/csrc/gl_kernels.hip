@@ -251,44 +251,48 @@ __global__ __launch_bounds__(kBT) void k_gl_score(Grid g, const float* __restric
     }
 }
 
-// CAER scoring in two passes for batches too small to fill the device one wave per
-// hypothesis (2k hypotheses are 2k waves; one lane per ray gives 11k).  Pass 1: one lane per
-// (hypothesis, ray) over the whole batch, consecutive lanes on consecutive rays of one
-// hypothesis, storing |scan - cast|; the grid sits in LDS as bytes (1024-thread workgroups,
-// two per CU) or bits.  Pass 2: one wave per hypothesis adds them in the order of k_gl_score
-// (lane l: rays l, l + 64, ...; then the same shuffle-down tree), so CAER has the same bits.
-template <int kBT, bool kBytes>
-__global__ __launch_bounds__(kBT) void k_gl_ray_diffs(Grid g, const float* __restrict__ hyp, int nh,
-                                                      const float* __restrict__ scan,
-                                                      const double* __restrict__ cphi,
-                                                      const double* __restrict__ sphi, int nrays, float rmax,
-                                                      float* __restrict__ diff) {
+// One lane per (pose, ray) over a whole batch, consecutive lanes on consecutive rays of one
+// pose, so a batch of a few thousand poses still fills every CU.  kDiff = false writes the
+// cast range (virtual scans: training data, tests); kDiff = true writes |scan - range|, the
+// first pass of the two-pass CAER for batches too small to fill the device one wave per
+// hypothesis (2k hypotheses are 2k waves; one lane per ray gives 11k), which k_gl_caer_sum
+// then adds in the order of k_gl_score (lane l: rays l, l + 64, ...; then the same
+// shuffle-down tree), so CAER keeps its bits.  Directions are computed exactly as in
+// k_gl_score (per-pose det_sincos, rotated f64 table entry rounded to f32).  The grid: kGrid 0
+// bytes staged in LDS (1024-thread workgroups, two per CU), 1 bits staged in LDS, 2 bits read
+// through L2.
+template <int kBT, int kGrid, bool kDiff>
+__global__ __launch_bounds__(kBT) void k_gl_rays(Grid g, const float* __restrict__ poses, int n,
+                                                 const float* __restrict__ scan, const double* __restrict__ cphi,
+                                                 const double* __restrict__ sphi, int nrays, float rmax,
+                                                 float* __restrict__ out) {
     extern __shared__ __align__(16) unsigned char smem[];
-    if (kBytes) {
+    if (kGrid == 0) {
         const uint4* src = reinterpret_cast<const uint4*>(g.bytes);
         uint4* dst = reinterpret_cast<uint4*>(smem);
         for (int i = threadIdx.x; i < g.nbytes / 16; i += kBT) dst[i] = src[i];
-    } else {
+        __syncthreads();
+    } else if (kGrid == 1) {
         uint32_t* sbits = reinterpret_cast<uint32_t*>(smem);
         for (int i = threadIdx.x; i < g.nwords; i += kBT) sbits[i] = g.bits[i];
+        __syncthreads();
     }
-    __syncthreads();
-    const size_t total = (size_t)nh * (size_t)nrays;
+    const size_t total = (size_t)n * (size_t)nrays;
     for (size_t t = (size_t)blockIdx.x * kBT + threadIdx.x; t < total; t += (size_t)gridDim.x * kBT) {
         const int h = (int)(t / (size_t)nrays), r = (int)(t - (size_t)h * nrays);
-        const float x = hyp[3 * h], y = hyp[3 * h + 1];
+        const float x = poses[3 * h], y = poses[3 * h + 1];
         double st, ct;
-        det_sincos((double)hyp[3 * h + 2], st, ct);
+        det_sincos((double)poses[3 * h + 2], st, ct);
         const double cp = cphi[r], sp = sphi[r];
         const float dx = (float)(ct * cp - st * sp);
         const float dy = (float)(st * cp + ct * sp);
         float vm;
-        if (kBytes)
+        if (kGrid == 0)
             vm = cast_ray(OccBytes{smem}, g.W, g.H, g.res, g.ox, g.oy, x, y, dx, dy, rmax);
         else
-            vm = cast_ray(OccBits{reinterpret_cast<const uint32_t*>(smem)}, g.W, g.H, g.res, g.ox, g.oy, x, y, dx, dy,
-                          rmax);
-        diff[t] = fabsf(scan[r] - vm);
+            vm = cast_ray(OccBits{kGrid == 1 ? reinterpret_cast<const uint32_t*>(smem) : g.bits}, g.W, g.H, g.res,
+                          g.ox, g.oy, x, y, dx, dy, rmax);
+        out[t] = kDiff ? fabsf(scan[r] - vm) : vm;
     }
 }
 
@@ -302,35 +306,6 @@ __global__ __launch_bounds__(kTB) void k_gl_caer_sum(const float* __restrict__ d
     for (int r = lane; r < nrays; r += 64) acc += d[r];
     acc = wave_sum_down(acc);
     if (lane == 0) caer[h] = acc;
-}
-
-// Virtual scans for poses[3n] -> out[n][nrays] (training data / tests).
-// Renders n poses (ranges[h * nrays + r]): one lane per ray over the whole batch, so a
-// batch of a few thousand scans still fills every CU; the bit grid is staged in LDS when it
-// fits.  Directions are computed exactly as in k_gl_score (per-pose det_sincos, rotated
-// f64 table entry rounded to f32).
-template <bool kLdsGrid>
-__global__ __launch_bounds__(kTB) void k_gl_render(Grid g, const float* __restrict__ poses, int n,
-                                                   const double* __restrict__ cphi, const double* __restrict__ sphi,
-                                                   int nrays, float rmax, float* __restrict__ out) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    uint32_t* sbits = reinterpret_cast<uint32_t*>(smem);
-    if (kLdsGrid) {
-        for (int i = threadIdx.x; i < g.nwords; i += kTB) sbits[i] = g.bits[i];
-        __syncthreads();
-    }
-    const uint32_t* bits = kLdsGrid ? sbits : g.bits;
-    const size_t total = (size_t)n * (size_t)nrays;
-    for (size_t t = (size_t)blockIdx.x * kTB + threadIdx.x; t < total; t += (size_t)gridDim.x * kTB) {
-        const int h = (int)(t / (size_t)nrays), r = (int)(t - (size_t)h * nrays);
-        const float x = poses[3 * h], y = poses[3 * h + 1];
-        double st, ct;
-        det_sincos((double)poses[3 * h + 2], st, ct);
-        const double cp = cphi[r], sp = sphi[r];
-        const float dx = (float)(ct * cp - st * sp);
-        const float dy = (float)(st * cp + ct * sp);
-        out[t] = cast_ray(OccBits{bits}, g.W, g.H, g.res, g.ox, g.oy, x, y, dx, dy, rmax);
-    }
 }
 
 // ---------------------------------------------------------------- bottom-k selection
@@ -538,6 +513,9 @@ int ensure_rays(GlState* s, int nrays, float span) {
     return 0;
 }
 
+int rays_launch(GlState* s, Grid g, const float* poses, int n, const float* scan, int nrays, float rmax, float* out,
+                hipStream_t st);
+
 int ray_diffs_reserve(GlState* s, size_t n) {
     if (n <= s->diff_cap) return 0;
     (void)hipFree(s->d_diff);
@@ -557,20 +535,10 @@ int score_launch(GlState* s, int nrays, float rmax) {
     prof_b(s, PFGLX_STAGE_SCORE);
     int mode = s->score_mode != 0 ? s->score_mode : (s->nh < kScoreBlockMaxHyp ? 2 : 1);
     const size_t total = (size_t)s->nh * (size_t)nrays;
-    const bool diff_bytes = (size_t)g.nbytes <= kLdsScoreBytesMax;
-    if (mode == 2 && (total > kRayDiffsMax || (!diff_bytes && g.nwords > kLdsBitsMaxWords) ||
-                      ray_diffs_reserve(s, total) != 0))
+    if (mode == 2 && (total > kRayDiffsMax || ray_diffs_reserve(s, total) != 0))
         mode = 1;  // two-pass route unavailable: the wave kernel
     if (mode == 2) {
-        if (diff_bytes) {
-            const int blocks = (int)std::max<size_t>(1, std::min<size_t>((total + 1023) / 1024, 2048));
-            hipLaunchKernelGGL((k_gl_ray_diffs<1024, true>), dim3(blocks), dim3(1024), (size_t)g.nbytes, s->stream, g,
-                               s->d_hyp, s->nh, s->d_scan, s->d_cphi, s->d_sphi, nrays, rmax, s->d_diff);
-        } else {
-            const int blocks = (int)std::max<size_t>(1, std::min<size_t>((total + kTB - 1) / kTB, 8192));
-            hipLaunchKernelGGL((k_gl_ray_diffs<kTB, false>), dim3(blocks), dim3(kTB), 4 * (size_t)((g.nwords + 3) & ~3),
-                               s->stream, g, s->d_hyp, s->nh, s->d_scan, s->d_cphi, s->d_sphi, nrays, rmax, s->d_diff);
-        }
+        if (rays_launch(s, g, s->d_hyp, s->nh, s->d_scan, nrays, rmax, s->d_diff, s->stream) != 0) return -1;
         hipLaunchKernelGGL(k_gl_caer_sum, dim3((s->nh + 3) / 4), dim3(kTB), 0, s->stream, s->d_diff, s->nh, nrays,
                            s->d_caer);
     } else if ((size_t)g.nbytes + ray_lds <= kLdsScoreBytesMax && s->nh >= kScoreBytesMinHyp) {
@@ -826,19 +794,47 @@ __global__ __launch_bounds__(kTB) void k_gl_refine(const float* __restrict__ par
     hyp[3 * h + 2] = t;
 }
 
-int render_launch(GlState* s, const Grid& g, const float* poses, int n, int nrays, float rmax, float* out,
-                  hipStream_t st, int /*unused*/) {
+// One-lane-per-ray pass over n poses (k_gl_rays): scan == nullptr renders ranges, otherwise
+// writes |scan - range|.  The byte grid goes to LDS when it fits, else the bits.
+int rays_launch(GlState* s, Grid g, const float* poses, int n, const float* scan, int nrays, float rmax, float* out,
+                hipStream_t st) {
+    g.bits = s->d_bits;
+    g.bytes = s->d_bytes;
     const size_t total = (size_t)n * (size_t)nrays;
-    size_t want = (total + kTB - 1) / kTB;
-    const int blocks = (int)(want < 16384 ? (want < 1 ? 1 : want) : 16384);
-    if (g.nwords <= kLdsBitsMaxWords)
-        hipLaunchKernelGGL(k_gl_render<true>, dim3(blocks), dim3(kTB), 4 * (size_t)g.nwords, st, g, poses, n,
-                           s->d_cphi, s->d_sphi, nrays, rmax, out);
-    else
-        hipLaunchKernelGGL(k_gl_render<false>, dim3(blocks), dim3(kTB), 0, st, g, poses, n, s->d_cphi, s->d_sphi,
-                           nrays, rmax, out);
+    const double* cp = s->d_cphi;
+    const double* sp = s->d_sphi;
+    if ((size_t)g.nbytes <= kLdsScoreBytesMax) {
+        const int blocks = (int)std::max<size_t>(1, std::min<size_t>((total + 1023) / 1024, 2048));
+        if (scan)
+            hipLaunchKernelGGL((k_gl_rays<1024, 0, true>), dim3(blocks), dim3(1024), (size_t)g.nbytes, st, g, poses, n,
+                               scan, cp, sp, nrays, rmax, out);
+        else
+            hipLaunchKernelGGL((k_gl_rays<1024, 0, false>), dim3(blocks), dim3(1024), (size_t)g.nbytes, st, g, poses,
+                               n, scan, cp, sp, nrays, rmax, out);
+    } else {
+        const int blocks = (int)std::max<size_t>(1, std::min<size_t>((total + kTB - 1) / kTB, 16384));
+        const bool lds = g.nwords <= kLdsBitsMaxWords;
+        const size_t shm = lds ? 4 * (size_t)((g.nwords + 3) & ~3) : 0;
+        if (lds && scan)
+            hipLaunchKernelGGL((k_gl_rays<kTB, 1, true>), dim3(blocks), dim3(kTB), shm, st, g, poses, n, scan, cp, sp,
+                               nrays, rmax, out);
+        else if (lds)
+            hipLaunchKernelGGL((k_gl_rays<kTB, 1, false>), dim3(blocks), dim3(kTB), shm, st, g, poses, n, scan, cp,
+                               sp, nrays, rmax, out);
+        else if (scan)
+            hipLaunchKernelGGL((k_gl_rays<kTB, 2, true>), dim3(blocks), dim3(kTB), 0, st, g, poses, n, scan, cp, sp,
+                               nrays, rmax, out);
+        else
+            hipLaunchKernelGGL((k_gl_rays<kTB, 2, false>), dim3(blocks), dim3(kTB), 0, st, g, poses, n, scan, cp, sp,
+                               nrays, rmax, out);
+    }
     GL_LAUNCH_CHECK();
     return 0;
+}
+
+int render_launch(GlState* s, const Grid& g, const float* poses, int n, int nrays, float rmax, float* out,
+                  hipStream_t st, int /*unused*/) {
+    return rays_launch(s, g, poses, n, nullptr, nrays, rmax, out, st);
 }
 
 int upload_scan(GlState* s, const float* scan, int nrays, float span) {
