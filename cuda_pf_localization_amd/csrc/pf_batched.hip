@@ -57,6 +57,7 @@ struct Batch {
     size_t params_bytes = 0;
     ParamBlock dp{}, hp{};
     float* d_ratio = nullptr;
+    size_t stage_bytes = 0;   // parameter blocks + observation capacity (one allocation per side)
     ObsRec* d_obs = nullptr;  // inside the d_params allocation, after the parameter blocks
     ObsRec* h_obs = nullptr;  // inside h_params
     hipEvent_t ev_done = nullptr;  // end of a cycle's copy-back (polled)
@@ -105,17 +106,27 @@ void free_batch(Batch* b) {
     delete b;
 }
 
-// Every device stage of a cycle (one graph): predict, weight, normalise (ESS + resample
-// predicate), resample (skipped on the device when the predicate is false), estimate sums with
-// the Markley mean, covariances.
+// The cycle's packed inputs (parameter blocks, then the joined observations) to the device:
+// the whole staging area, so that one captured copy serves every cycle.
+int stage_in(Batch* b) {
+    PB_CHECK(hipMemcpyAsync(b->d_params, b->h_params, b->stage_bytes, hipMemcpyHostToDevice, b->s.stream));
+    return 0;
+}
+
+// Every device stage of a cycle (one graph): inputs in, predict, weight, normalise (ESS +
+// resample predicate), resample (skipped on the device when the predicate is false), estimate
+// sums with the Markley mean, covariances, results out.
 int stages_all(Batch* b) {
     PfState* s = &b->s;
+    if (stage_in(b) != 0) return -1;
     if (launch_motion(s, ParamRef<MotionParams>::array(b->dp.motion), 0) != 0) return -1;
     if (launch_weight(s, ParamRef<WeightParams>::array(b->dp.weight)) != 0) return -1;
     if (launch_normalize(s, 0, b->d_ratio) != 0) return -1;
     if (launch_resample(s, ParamRef<InjectParams>::array(b->dp.inject), s->d_inj_blk, s->inj_nblk) != 0) return -1;
     if (launch_est_sums(s, b->dp.cov) != 0) return -1;
-    return launch_est_covs(s, ParamRef<CovParams>::array(b->dp.cov));
+    if (launch_est_covs(s, ParamRef<CovParams>::array(b->dp.cov)) != 0) return -1;
+    PB_CHECK(hipMemcpyAsync(b->h_fdev, s->fdev, sizeof(FilterDev) * (size_t)b->nf, hipMemcpyDeviceToHost, s->stream));
+    return 0;
 }
 
 int capture(Batch* b, int (*fn)(Batch*), hipGraphExec_t* out) {
@@ -154,7 +165,8 @@ int inject_count(const pfb_filter_cfg& c) {
     return k > c.n ? c.n : k;
 }
 
-// Host half of a cycle before the device stages: parameters + joined observations -> device.
+// Host half of a cycle before the device stages: parameters + joined observations packed into
+// the pinned staging area (stage_in copies it as the cycle's first device step).
 int upload_inputs(Batch* b, const pfb_cycle_in* in) {
     PfState* s = &b->s;
     int total = 0;
@@ -186,8 +198,6 @@ int upload_inputs(Batch* b, const pfb_cycle_in* in) {
         ip.k = inject_count(b->cfg[f]);
         ip.pad = 0;
     }
-    PB_CHECK(hipMemcpyAsync(b->d_params, b->h_params, b->params_bytes + sizeof(ObsRec) * (size_t)total,
-                            hipMemcpyHostToDevice, s->stream));
     return 0;
 }
 
@@ -225,9 +235,9 @@ void* pfb_create(int nf, const pfb_filter_cfg* cfg, int obs_cap) {
     ok = ok && set_inject_blocks(&b->s, kinj) == 0;
     // one staging allocation per side: parameter blocks, then the joined observations, so a
     // cycle's inputs go over in one copy
-    const size_t stage_bytes = b->params_bytes + sizeof(ObsRec) * (size_t)nf * obs_cap;
-    ok = ok && hipMalloc((void**)&b->d_params, stage_bytes) == hipSuccess;
-    ok = ok && hipHostMalloc((void**)&b->h_params, stage_bytes, hipHostMallocDefault) == hipSuccess;
+    b->stage_bytes = b->params_bytes + sizeof(ObsRec) * (size_t)nf * obs_cap;
+    ok = ok && hipMalloc((void**)&b->d_params, b->stage_bytes) == hipSuccess;
+    ok = ok && hipHostMalloc((void**)&b->h_params, b->stage_bytes, hipHostMallocDefault) == hipSuccess;
     ok = ok && hipMalloc((void**)&b->d_ratio, sizeof(float) * (size_t)nf) == hipSuccess;
     ok = ok && hipEventCreateWithFlags(&b->ev_done, hipEventDisableTiming) == hipSuccess;
     if (ok) {
@@ -291,7 +301,6 @@ int pfb_cycle(void* h, const pfb_cycle_in* in, pfb_estimate* out) {
     PfState* s = &b->s;
     if (upload_inputs(b, in) != 0) return -1;
     if (run_cycle(b) != 0) return -1;
-    PB_CHECK(hipMemcpyAsync(b->h_fdev, s->fdev, sizeof(FilterDev) * (size_t)b->nf, hipMemcpyDeviceToHost, s->stream));
     PB_CHECK(hipEventRecord(b->ev_done, s->stream));
     // poll rather than block: the cycle is short and a sleeping wait adds wake-up latency
     hipError_t q;
@@ -334,7 +343,7 @@ int pfb_autotune(void* h, const pfb_cycle_in* in) {
     s->prof_on = false;
     // parameters of a cycle (without advancing the filters' draw state afterwards)
     std::vector<uint64_t> gs = b->gseed, gn = b->gen;
-    if (upload_inputs(b, in) != 0) return -1;
+    if (upload_inputs(b, in) != 0 || stage_in(b) != 0) return -1;
     b->gseed = gs;
     b->gen = gn;
     hipEvent_t e0, e1;
