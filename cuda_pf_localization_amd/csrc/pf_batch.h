@@ -136,9 +136,8 @@ struct PfState {
     // reduction partials / scan state
     double *part_w = nullptr, *part_n = nullptr, *part_est = nullptr, *part_cov = nullptr;
     uint64_t* scan_st = nullptr;               // look-back status words, two per 1024-particle tile
-    unsigned long long* scan_ctl = nullptr;     // [0] status epoch, [1] tile tickets
-    int scan_resident = 0;                      // scan blocks the device holds at once
-    int scan_spin_limit = 1 << 16;              // reads per predecessor before summing directly
+    unsigned long long* scan_ctl = nullptr;     // status epoch of the look-back words
+    int scan_spin_limit = 64;                   // reads of a predecessor's status before summing it here
     FilterDev* fdev = nullptr;
     // observations (reference ABI path)
     ObsRec* d_obs = nullptr;

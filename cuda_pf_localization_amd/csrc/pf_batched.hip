@@ -58,6 +58,7 @@ struct Batch {
     ParamBlock dp{}, hp{};
     float* d_ratio = nullptr;
     size_t stage_bytes = 0;   // parameter blocks + observation capacity (one allocation per side)
+    size_t stage_used = 0;    // parameter blocks + this cycle's joined observations
     ObsRec* d_obs = nullptr;  // inside the d_params allocation, after the parameter blocks
     ObsRec* h_obs = nullptr;  // inside h_params
     hipEvent_t ev_done = nullptr;  // end of a cycle's copy-back (polled)
@@ -106,19 +107,18 @@ void free_batch(Batch* b) {
     delete b;
 }
 
-// The cycle's packed inputs (parameter blocks, then the joined observations) to the device:
-// the whole staging area, so that one captured copy serves every cycle.
+// The cycle's packed inputs (parameter blocks, then the joined observations of this cycle) to
+// the device: one copy of exactly the bytes in use, ahead of the cycle's graph.
 int stage_in(Batch* b) {
-    PB_CHECK(hipMemcpyAsync(b->d_params, b->h_params, b->stage_bytes, hipMemcpyHostToDevice, b->s.stream));
+    PB_CHECK(hipMemcpyAsync(b->d_params, b->h_params, b->stage_used, hipMemcpyHostToDevice, b->s.stream));
     return 0;
 }
 
-// Every device stage of a cycle (one graph): inputs in, predict, weight, normalise (ESS +
-// resample predicate), resample (skipped on the device when the predicate is false), estimate
-// sums with the Markley mean, covariances, results out.
+// Every device stage of a cycle (one graph): predict, weight, normalise (ESS + resample
+// predicate), resample (skipped on the device when the predicate is false), estimate sums with
+// the Markley mean, covariances, results out.
 int stages_all(Batch* b) {
     PfState* s = &b->s;
-    if (stage_in(b) != 0) return -1;
     if (launch_motion(s, ParamRef<MotionParams>::array(b->dp.motion), 0) != 0) return -1;
     if (launch_weight(s, ParamRef<WeightParams>::array(b->dp.weight)) != 0) return -1;
     if (launch_normalize(s, 0, b->d_ratio) != 0) return -1;
@@ -166,7 +166,7 @@ int inject_count(const pfb_filter_cfg& c) {
 }
 
 // Host half of a cycle before the device stages: parameters + joined observations packed into
-// the pinned staging area (stage_in copies it as the cycle's first device step).
+// the pinned staging area (stage_in copies it to the device).
 int upload_inputs(Batch* b, const pfb_cycle_in* in) {
     PfState* s = &b->s;
     int total = 0;
@@ -198,6 +198,7 @@ int upload_inputs(Batch* b, const pfb_cycle_in* in) {
         ip.k = inject_count(b->cfg[f]);
         ip.pad = 0;
     }
+    b->stage_used = b->params_bytes + sizeof(ObsRec) * (size_t)total;
     return 0;
 }
 
@@ -299,7 +300,7 @@ int pfb_cycle(void* h, const pfb_cycle_in* in, pfb_estimate* out) {
     Batch* b = static_cast<Batch*>(h);
     if (!b || !in) return -1;
     PfState* s = &b->s;
-    if (upload_inputs(b, in) != 0) return -1;
+    if (upload_inputs(b, in) != 0 || stage_in(b) != 0) return -1;
     if (run_cycle(b) != 0) return -1;
     PB_CHECK(hipEventRecord(b->ev_done, s->stream));
     // poll rather than block: the cycle is short and a sleeping wait adds wake-up latency

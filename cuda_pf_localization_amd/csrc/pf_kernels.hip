@@ -530,15 +530,11 @@ __device__ inline float4 tile_w(const float* __restrict__ w, int tile) {
 // resample, advances the epoch).  Tiles of filters that do not resample refresh their words
 // with kind 0, so every word is at most one launch old.
 //
-// A tile waits only on lower-numbered tiles.  When every tile of the launch fits on the device
-// at once the tile is the block index (workgroups are dispatched in index order); otherwise the
-// tile comes from a ticket counter taken when the block starts, so its predecessors' blocks are
-// already running, and the last ticket re-arms the counter.
-//
-// A wait is bounded (spin_limit reads per predecessor); a tile whose look-back does not complete
-// within it sums its predecessors' weights itself, which is slower but gives the same bits, so
-// the kernel ends whatever the scheduling.  The filter's last tile stores the total T and
-// advances the LCG (pf_kernels.cu:647-657: only when T > 0).
+// Waits are bounded: a predecessor whose status has not appeared within spin_limit reads has
+// its aggregate summed by the waiting workgroup itself (the same integer it would publish), so
+// no tile depends on another workgroup being scheduled and the result does not depend on the
+// order in which workgroups run.  The filter's last tile stores the total T and advances the
+// LCG (pf_kernels.cu:647-657: only when T > 0).
 constexpr uint32_t kLbAgg = 1, kLbPre = 2;
 constexpr int kLbPer = 4, kLbWin = kTB * kLbPer;  // predecessors per thread and per look-back window
 
@@ -576,21 +572,14 @@ __device__ inline uint64_t block_sum_u64(uint64_t v, uint64_t* s_w) {
 }
 
 __global__ __launch_bounds__(kTB) void k_scan(Layout L, FilterDev* F, const float* __restrict__ w,
-                                              uint64_t* __restrict__ cum, uint64_t* st, unsigned long long* ctl,
-                                              int ntile, int use_ticket, int spin_limit) {
+                                              uint64_t* __restrict__ cum, uint64_t* st, const unsigned long long* ctl,
+                                              int spin_limit) {
     __shared__ uint64_t s_w[4];
-    __shared__ int s_tile, s_failed, s_dmin;
+    __shared__ int s_dmin, s_nun;
     __shared__ uint64_t s_excl;
-    if (threadIdx.x == 0) {
-        int tk = (int)blockIdx.x;
-        if (use_ticket) {
-            tk = (int)atomicAdd(&ctl[1], 1ull);
-            if (tk == ntile - 1) atomicExch(&ctl[1], 0ull);
-        }
-        s_tile = tk;
-    }
-    __syncthreads();
-    const int tile = s_tile;
+    __shared__ int s_un_tile[kLbWin];     // predecessors of a window found unpublished
+    __shared__ uint64_t s_un_val[kLbWin];  // their aggregates, summed here
+    const int tile = blockIdx.x;
     const float4 wv = tile_w(w, tile);  // issued before the filter lookup
     const uint32_t epoch = (uint32_t)ctl[0] & 0x3fffffffu;
     const int f = L.blk_filter[tile * (kTile / kTB)];
@@ -610,7 +599,6 @@ __global__ __launch_bounds__(kTB) void k_scan(Layout L, FilterDev* F, const floa
     const uint64_t agg = block_scan_u64(q[3], incl, s_w);
     if (threadIdx.x == 0) {
         lb_publish(st, tile, (epoch << 2) | (tile == t0 ? kLbPre : kLbAgg), agg);
-        s_failed = 0;
         s_excl = 0;
     }
     __syncthreads();
@@ -639,17 +627,45 @@ __global__ __launch_bounds__(kTB) void k_scan(Layout L, FilterDev* F, const floa
                 if (all || it >= spin_limit) break;
                 __builtin_amdgcn_s_sleep(1);
             }
-            int bad = 0, dmin = INT_MAX;
+            int bad = 0;
 #pragma unroll
-            for (int j = 0; j < kLbPer; ++j) {
-                bad |= (in[j] && kind[j] == 0) ? 1 : 0;
-                if (in[j] && kind[j] == kLbPre) dmin = min(dmin, tid + kTB * j);
-            }
-            if (tid == 0) s_dmin = INT_MAX;
+            for (int j = 0; j < kLbPer; ++j) bad |= (in[j] && kind[j] == 0) ? 1 : 0;
+            if (tid == 0) s_nun = 0;
             if (__syncthreads_or(bad)) {
-                if (tid == 0) s_failed = 1;
-                break;
+                // predecessors still unpublished: the workgroup sums each of their tiles itself
+                // (the same integer aggregate they would publish), so nothing waits longer
+                int slot[kLbPer];
+#pragma unroll
+                for (int j = 0; j < kLbPer; ++j) {
+                    slot[j] = -1;
+                    if (in[j] && kind[j] == 0) {
+                        slot[j] = atomicAdd(&s_nun, 1);
+                        s_un_tile[slot[j]] = base - tid - kTB * j;
+                    }
+                }
+                __syncthreads();
+                const int nun = s_nun;
+                for (int u = 0; u < nun; ++u) {
+                    const int ut = s_un_tile[u];
+                    uint64_t qq[4], inc2;
+                    tile_q(tile_w(w, ut), ut, off, nf, qq);
+                    const uint64_t a = block_scan_u64(qq[0] + qq[1] + qq[2] + qq[3], inc2, s_w);
+                    if (tid == 0) s_un_val[u] = a;
+                }
+                __syncthreads();
+#pragma unroll
+                for (int j = 0; j < kLbPer; ++j)
+                    if (slot[j] >= 0) {
+                        v[j] = s_un_val[slot[j]];
+                        kind[j] = kLbAgg;
+                    }
             }
+            int dmin = INT_MAX;
+#pragma unroll
+            for (int j = 0; j < kLbPer; ++j)
+                if (in[j] && kind[j] == kLbPre) dmin = min(dmin, tid + kTB * j);
+            if (tid == 0) s_dmin = INT_MAX;
+            __syncthreads();
             if (dmin != INT_MAX) atomicMin(&s_dmin, dmin);
             __syncthreads();
             const int dstar = s_dmin;  // nearest predecessor holding an inclusive prefix
@@ -663,15 +679,7 @@ __global__ __launch_bounds__(kTB) void k_scan(Layout L, FilterDev* F, const floa
         if (tid == 0) s_excl = ex;
     }
     __syncthreads();
-    uint64_t excl = s_excl;
-    if (s_failed) {  // look-back did not complete in time: sum the predecessors directly
-        excl = 0;
-        for (int t = t0; t < tile; ++t) {
-            uint64_t qq[4], inc2;
-            tile_q(tile_w(w, t), t, off, nf, qq);
-            excl += block_scan_u64(qq[0] + qq[1] + qq[2] + qq[3], inc2, s_w);
-        }
-    }
+    const uint64_t excl = s_excl;
     if (threadIdx.x == 0) {
         if (tile != t0) lb_publish(st, tile, (epoch << 2) | kLbPre, excl + agg);
         if (tile == t0 + nt - 1) {
@@ -1087,17 +1095,8 @@ int state_alloc(PfState* s, const std::vector<int>& ns) {
     PF_CHECK(hipMalloc((void**)&s->part_cov, sizeof(double) * kCovN * (size_t)ntile));
     PF_CHECK(hipMalloc((void**)&s->scan_st, sizeof(uint64_t) * 2 * (size_t)ntile));
     PF_CHECK(hipMemsetAsync(s->scan_st, 0, sizeof(uint64_t) * 2 * (size_t)ntile, s->stream));
-    PF_CHECK(hipMalloc((void**)&s->scan_ctl, 2 * sizeof(unsigned long long)));
-    PF_CHECK(hipMemsetAsync(s->scan_ctl, 0, 2 * sizeof(unsigned long long), s->stream));
-    {
-        // scan tiles that can be resident at once (block index order suffices up to this many)
-        int per_cu = 0, dev = 0;
-        hipDeviceProp_t prop;
-        PF_CHECK(hipGetDevice(&dev));
-        PF_CHECK(hipGetDeviceProperties(&prop, dev));
-        PF_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_scan, kTB, 0));
-        s->scan_resident = per_cu * prop.multiProcessorCount;
-    }
+    PF_CHECK(hipMalloc((void**)&s->scan_ctl, sizeof(unsigned long long)));
+    PF_CHECK(hipMemsetAsync(s->scan_ctl, 0, sizeof(unsigned long long), s->stream));
     // layout tables
     std::vector<int> blk_filter(nb256);
     for (int f = 0; f < nf; ++f)
@@ -1229,7 +1228,7 @@ int launch_resample(PfState* s, ParamRef<InjectParams> inj, const int32_t* inj_t
     const int ntile = s->NP / kTile;
     prof_begin(s, PFX_STAGE_RESAMPLE_SCAN);
     hipLaunchKernelGGL(k_scan, dim3(ntile), dim3(kTB), 0, s->stream, s->L, s->fdev, s->w, s->cum, s->scan_st,
-                       s->scan_ctl, ntile, ntile > s->scan_resident ? 1 : 0, s->scan_spin_limit);
+                       s->scan_ctl, s->scan_spin_limit);
     prof_end(s);
     PF_LAUNCH_CHECK();
     prof_begin(s, PFX_STAGE_RESAMPLE_GATHER);
