@@ -165,9 +165,9 @@ int pfx_quat_mean(const double *M16, double *q4);
 /* Likelihood kernel launch shape: 1, 2 or 4 particles per lane (default 1).  All shapes give
  * bit-identical results. */
 int pfx_set_weight_variant(void *h, int variant);
-/* Resampling scan: reads of a predecessor tile's status before the tile sums that tile's
- * weights itself (default 1 << 16; 0 makes every waiting tile take the direct sum).  Both routes
- * give the same bits; small limits exist to exercise the direct route. */
+/* Resampling scan: reads of a predecessor tile's status before the waiting tile sums that
+ * predecessor's weights itself (default 64; 0 sums every predecessor not yet published at the
+ * first read).  Both routes give the same bits. */
 int pfx_set_scan_spin_limit(void *h, int limit);
 /* Blocks until every queued operation on the handle's stream has finished. */
 int pfx_sync(void *h);
