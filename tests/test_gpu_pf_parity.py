@@ -332,8 +332,8 @@ def test_resample_scan_direct_route_bitexact(gpu, n):
 
 
 def test_resample_bitexact_more_tiles_than_resident(gpu):
-    """4.2M particles: more scan tiles than the device holds at once, so tiles come from the
-    ticket counter; ancestors still bit-exact, over two launches."""
+    """4.2M particles: more scan tiles than the device holds at once (later tiles start while
+    earlier ones are still finishing); ancestors still bit-exact, over two launches."""
     n = 4_200_000
     pos, quat, w = weighted_state(n, 77)
     k, o = pair(n, 5)
