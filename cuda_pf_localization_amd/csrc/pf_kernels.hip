@@ -9,14 +9,15 @@
 //     1024-particle alignment, so one launch serves one filter or a batch of filters.
 //   * k_motion: reinit/predict with counter-based splitmix64 draws and an fma-only
 //     Box-Muller (no RNG state in memory), pose algebra in f64, f32 storage.
-//   * k_weight: one particle per lane, observation+landmark records read as wave-uniform
-//     scalar loads, cancellation-prone residuals in f64 (full-rate FP64 FMA on CDNA4),
-//     fused online-softmax block partials and a deterministic last-block finalise.
-//   * resampling: fixed-point (2^-56) weights, integer decoupled-lookback scan with
-//     dynamic tile tickets, exact 128-bit pointer comparisons, ping-pong gather (no copies),
+//   * k_weight: one particle per lane (or 2 / 4), observation+landmark records read as
+//     wave-uniform scalar loads, residuals in f64 (30 FP64 instructions per pair), online-
+//     softmax block partials reduced in a fixed order by a one-workgroup-per-filter finalize.
+//   * resampling: fixed-point (2^-56) weights, a single-pass integer scan with decoupled
+//     look-back (bounded waits; an unpublished predecessor is summed by the waiting
+//     workgroup), exact 128-bit pointer comparisons, ping-pong gather (no copies),
 //     deterministic injection (per-target pick lists summed in pick order).
-//   * estimate: wave64 butterfly reductions into f64 block partials with a deterministic
-//     last-block finalise per filter.
+//   * estimate: wave64 butterfly reductions into f64 block partials, a fixed-order
+//     one-workgroup-per-filter finalize, and (fused cycle) the Markley mean on the device.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
