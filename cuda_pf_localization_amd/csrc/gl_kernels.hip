@@ -4,9 +4,11 @@
 // crates/pf_core/src/global_loc.rs:262-271).  Pipeline per localize():
 //   generate  hypotheses uniform over the free cells (counter-based splitmix64 draws;
 //             free cells compacted in row-major order, like global_loc.rs:365-368)
-//   score     one wave64 per hypothesis, lane l casts rays l, l+64, ...; 2-D DDA over a
-//             bit-packed occupancy grid staged in LDS; CAER = sum |real - virtual| with a
-//             fixed lane order and wave tree (deterministic)
+//   score     2-D DDA over the occupancy grid framed by set cells, staged in LDS (one byte
+//             per cell, or bit-packed); large batches one wave64 per hypothesis with lane l
+//             casting rays l, l+64, ...; small batches one lane per ray and a second pass
+//             that sums in that same order; CAER = sum |real - virtual| with a fixed lane
+//             order and wave tree (deterministic, identical bits on both routes)
 //   topk      bottom-k by radix *select* over unique 64-bit keys (ordered CAER bits << 32 |
 //             hypothesis index), then one workgroup bitonic-sorts the k winners; ties keep
 //             index order as the CPU twin's stable sort does (global_loc.rs:395-397).

@@ -3,12 +3,11 @@
 //
 // A cycle (particle_filter.rs:110-124 for every filter at once):
 //   host   : per-filter odometry / weight / injection parameters and the observations joined
-//            with each filter's landmark table -> one pinned block, one H2D copy each
+//            with each filter's landmark table -> one pinned block, one H2D copy
 //   device : motion -> likelihood (+ softmax partials) -> normalise (+ ESS of the stored
-//            weights) -> resample predicate (ESS < ratio n) -> scan -> gather -> injection ->
-//            estimate sums            [graph A when enabled]
-//   host   : Markley mean of every filter (types.rs:128-144) -> covariance parameters
-//   device : estimate covariances     [graph B when enabled]
+//            weights and the resample predicate ESS < ratio n) -> scan -> gather ->
+//            injection -> estimate sums + Markley mean (types.rs:128-144) -> estimate
+//            covariances -> one D2H of every filter's results  [one graph when enabled]
 // The resample decision never leaves the device; filters that do not resample skip the
 // resampling kernels tile by tile.
 #include <hip/hip_runtime.h>
