@@ -135,6 +135,8 @@ KERNEL_SYMBOLS = [
     ("pfb_upload_landmarks", C.c_int, [vp, C.c_int, P(f32), C.c_int]),
     ("pfb_reinit", C.c_int, [vp, P(f32), P(f32), P(f32)]),
     ("pfb_cycle", C.c_int, [vp, P(PfbCycleIn), P(PfbEstimate)]),
+    ("pfb_stage_cycles", C.c_int, [vp, C.c_int, P(PfbCycleIn)]),
+    ("pfb_run_staged", C.c_int, [vp, P(PfbEstimate)]),
     ("pfb_use_graph", C.c_int, [vp, C.c_int]),
     ("pfb_autotune", C.c_int, [vp, P(PfbCycleIn)]),
     ("pfb_get_state", C.c_int, [vp, C.c_int, P(f32), P(f32), P(f32)]),

@@ -716,6 +716,24 @@ class BatchedFilters:
         _check(self.L.pfb_cycle(self.h, inputs, out), "pfb_cycle")
         return out
 
+    def stage_cycles(self, cycles):
+        """Uploads the inputs of several cycles (each a make_inputs() array) to the device."""
+        k = len(cycles)
+        arr = (_lib.PfbCycleIn * (k * self.nf))()
+        for c, inp in enumerate(cycles):
+            for f in range(self.nf):
+                arr[c * self.nf + f] = inp[f]
+        _check(self.L.pfb_stage_cycles(self.h, k, arr), "pfb_stage_cycles")
+        self._staged = k
+
+    def run_staged(self):
+        """Runs the staged cycles back to back; returns their estimates [cycle][filter]."""
+        k = getattr(self, "_staged", 0)
+        out = (_lib.PfbEstimate * (k * self.nf))()
+        _check(self.L.pfb_run_staged(self.h, out), "pfb_run_staged")
+        self._staged = 0
+        return [out[c * self.nf:(c + 1) * self.nf] for c in range(k)]
+
     def use_graph(self, on: bool = True):
         _check(self.L.pfb_use_graph(self.h, 1 if on else 0), "pfb_use_graph")
 

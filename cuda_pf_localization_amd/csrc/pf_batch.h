@@ -39,7 +39,10 @@ struct FilterDev {
     double sums[kEstN];
     double covs[kCovN];
     double qmean[4];  // device Markley mean of the last estimate (wxyz, w >= 0)
-    int qvalid, pad;  // 0: degenerate moment matrix (qmean = identity)
+    int qvalid;       // 0: degenerate moment matrix (qmean = identity)
+    // 1: the last resampling gather left the estimate-sum partials of the current poses and
+    // weights (one per 256-particle block); cleared by every pass that changes poses or weights
+    int est_src;
 };
 
 struct Layout {
@@ -50,7 +53,9 @@ struct Layout {
     const float* f_winv;    // (float)(1/n)
 };
 
-// Observation joined with its landmark, read as wave-uniform scalar loads (64 B).
+// Observation joined with its landmark, read as wave-uniform scalar loads (64 B).  For a
+// filter on the scaled range-bearing path (WeightParams.fast_rb) the range-bearing records
+// hold v / (2 u_gate) and 4 range instead of v and range.
 struct ObsRec {
     double lx, ly, lz;
     double vx, vy, vz;
@@ -73,6 +78,13 @@ struct WeightParams {
     double cart_gate2, brg_gate2;  // squared-residual gate thresholds (+inf: no gate)
     double rng_gate;               // |range residual| gate g sr (+inf: no gate)
     double u_gate;                            // 1 - cos of the bearing gate (+inf: none)
+    // Scaled range-bearing path (u_gate <= 0.005): residuals formed from 2 b, the bearing as
+    // us = u / u_gate so that the gate is the [0, 1] clamp, eb^2 = us (c0 + c1 us + c2 us^2 +
+    // c3 us^3), the range residual at 4x scale against rng_gate4 = 4 g sr.
+    int fast_rb, pad_fast;
+    double cg;                 // 1 / u_gate
+    double c0, c1, c2, c3;     // series coefficients in us
+    double rng_gate4;
     const ObsRec* obs;
     int nobs;  // joined records
     int nrb;   // of which range-bearing (they come first)
@@ -129,7 +141,7 @@ struct PfState {
     double* ll = nullptr;
     uint64_t* cum = nullptr;
     int32_t* anc = nullptr;
-    int32_t *inj_idx = nullptr, *inj_next = nullptr, *inj_head = nullptr;
+    int32_t *inj_next = nullptr, *inj_head = nullptr;  // injection pick lists (head -1: empty)
     double* inj_jit = nullptr;
     int32_t* d_inj_blk = nullptr;  // particle-layout blocks that can hold injection picks
     int inj_nblk = 0;
@@ -161,18 +173,24 @@ void prof_end(PfState* s);
 int launch_motion(PfState* s, ParamRef<MotionParams> A, int reinit);
 int launch_weight(PfState* s, ParamRef<WeightParams> A);
 int launch_normalize(PfState* s, int force_uniform, const float* d_ratio);
-int launch_resample(PfState* s, ParamRef<InjectParams> inj, const int32_t* inj_tab, int inj_nblk);
+// Scan, injection picks, gather (with the picks' jitter and the estimate-sum partials); with flip
+// the resampled set then becomes current (else the fused cycle's sums finalize does it).
+int launch_resample(PfState* s, ParamRef<InjectParams> inj, const int32_t* inj_tab, int inj_nblk, bool flip);
 // Block table of the injection kernels from each filter's largest pick count.
 int set_inject_blocks(PfState* s, const std::vector<int>& kmax);
-// Estimate sums; with d_cov non-null also the Markley mean of every filter's sums on the
-// device -> FilterDev.qmean and the covariance pass parameters d_cov[f].
-int launch_est_sums(PfState* s, CovParams* d_cov);
-int launch_est_covs(PfState* s, ParamRef<CovParams> A);
+// Estimate sums (the partials of a preceding resampling gather when it left them); with d_cov
+// non-null also the Markley mean of every filter's sums on the device -> FilterDev.qmean and
+// the covariance pass parameters d_cov[f]; with flip the resampled set becomes current.
+int launch_est_sums(PfState* s, CovParams* d_cov, bool flip);
+// Covariances; with results non-null every filter's scalars are also copied to results[f].
+int launch_est_covs(PfState* s, ParamRef<CovParams> A, FilterDev* results = nullptr);
 int launch_predicate(PfState* s, const float* d_ratio, int force);
 WeightParams make_weight_params(double sigma_r, double sigma_b, double gate);
 // Joins observations with their landmarks (unknown ids dropped); range-bearing records
-// first, then Cartesian; *nrb = number of range-bearing records.
-int join_obs(const struct pfc_obs* obs, int nobs, const float* lm, int lm_cap, ObsRec* out, int* nrb);
+// first, then Cartesian; *nrb = number of range-bearing records.  The range-bearing records
+// are stored scaled for the path p selects (p.fast_rb).
+int join_obs(const struct pfc_obs* obs, int nobs, const float* lm, int lm_cap, const WeightParams& p, ObsRec* out,
+             int* nrb);
 
 // Device address of a FilterDev member of filter f.
 template <class M>

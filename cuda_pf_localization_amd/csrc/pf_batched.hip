@@ -66,6 +66,14 @@ struct Batch {
     bool use_graph = false;
     bool captured = false;
     hipGraphExec_t gA = nullptr;  // the captured cycle
+    // staged cycles (inputs resident on the device before the run)
+    int st_cycles = 0;
+    bool st_pending = false;
+    char* d_staged = nullptr;  // st_cycles staging blocks of stage_bytes each
+    char* h_staged = nullptr;
+    FilterDev* d_results = nullptr;  // st_cycles x nf filter scalars after each cycle
+    FilterDev* h_results = nullptr;
+    hipGraphExec_t gS = nullptr;  // every staged cycle in one graph
 };
 
 ParamBlock carve(char* base, int nf) {
@@ -92,6 +100,12 @@ size_t params_size(int nf) {
 void free_batch(Batch* b) {
     if (b->s.stream) (void)hipStreamSynchronize(b->s.stream);
     if (b->gA) (void)hipGraphExecDestroy(b->gA);
+    if (b->gS) (void)hipGraphExecDestroy(b->gS);
+    b->gS = nullptr;
+    (void)hipFree(b->d_staged);
+    (void)hipHostFree(b->h_staged);
+    (void)hipFree(b->d_results);
+    (void)hipHostFree(b->h_results);
     state_free_particles(&b->s);
     (void)hipFree(b->d_params);  // parameter blocks and observations share one allocation
     (void)hipHostFree(b->h_params);
@@ -121,8 +135,9 @@ int stages_all(Batch* b) {
     if (launch_motion(s, ParamRef<MotionParams>::array(b->dp.motion), 0) != 0) return -1;
     if (launch_weight(s, ParamRef<WeightParams>::array(b->dp.weight)) != 0) return -1;
     if (launch_normalize(s, 0, b->d_ratio) != 0) return -1;
-    if (launch_resample(s, ParamRef<InjectParams>::array(b->dp.inject), s->d_inj_blk, s->inj_nblk) != 0) return -1;
-    if (launch_est_sums(s, b->dp.cov) != 0) return -1;
+    if (launch_resample(s, ParamRef<InjectParams>::array(b->dp.inject), s->d_inj_blk, s->inj_nblk, false) != 0)
+        return -1;
+    if (launch_est_sums(s, b->dp.cov, true) != 0) return -1;  // + the resampled set becomes current
     if (launch_est_covs(s, ParamRef<CovParams>::array(b->dp.cov)) != 0) return -1;
     PB_CHECK(hipMemcpyAsync(b->h_fdev, s->fdev, sizeof(FilterDev) * (size_t)b->nf, hipMemcpyDeviceToHost, s->stream));
     return 0;
@@ -164,16 +179,16 @@ int inject_count(const pfb_filter_cfg& c) {
     return k > c.n ? c.n : k;
 }
 
-// Host half of a cycle before the device stages: parameters + joined observations packed into
-// the pinned staging area (stage_in copies it to the device).
-int upload_inputs(Batch* b, const pfb_cycle_in* in) {
-    PfState* s = &b->s;
+// Host half of a cycle before the device stages: the parameter blocks hp (host) and the joined
+// observations h_obs (host), whose device copy will sit at d_obs; advances every filter's draw
+// generation.  Returns the bytes in use (parameter blocks + observations).
+size_t pack_inputs(Batch* b, const pfb_cycle_in* in, const ParamBlock& hp, ObsRec* h_obs, ObsRec* d_obs) {
     int total = 0;
     for (int f = 0; f < b->nf; ++f) {
         const pfb_cycle_in& c = in[f];
         b->gseed[f] += pfd::kGenBump;  // cpu.rs:198
         b->gen[f] += 1;                // cu:588
-        MotionParams& m = b->hp.motion[f];
+        MotionParams& m = hp.motion[f];
         memset(&m, 0, sizeof(m));
         for (int k = 0; k < 3; ++k) {
             m.t[k] = c.t[k];
@@ -184,21 +199,84 @@ int upload_inputs(Batch* b, const pfb_cycle_in* in) {
         m.gseed = b->gseed[f];
         const int nobs = c.nobs > b->obs_cap ? b->obs_cap : c.nobs;
         int mobs = 0, nrb = 0;
-        if (nobs > 0 && c.obs)
-            mobs = join_obs(c.obs, nobs, b->lm[f].data(), b->lm_cap[f], b->h_obs + total, &nrb);
         WeightParams wp = make_weight_params(b->cfg[f].sigma_r, b->cfg[f].sigma_b, b->cfg[f].gate);
-        wp.obs = b->d_obs + total;
+        if (nobs > 0 && c.obs)
+            mobs = join_obs(c.obs, nobs, b->lm[f].data(), b->lm_cap[f], wp, h_obs + total, &nrb);
+        wp.obs = d_obs + total;
         wp.nobs = mobs;
         wp.nrb = nrb;
-        b->hp.weight[f] = wp;
+        hp.weight[f] = wp;
         total += mobs;
-        InjectParams& ip = b->hp.inject[f];
+        InjectParams& ip = hp.inject[f];
         ip.gen = b->gen[f];
         ip.k = inject_count(b->cfg[f]);
         ip.pad = 0;
     }
-    b->stage_used = b->params_bytes + sizeof(ObsRec) * (size_t)total;
+    return b->params_bytes + sizeof(ObsRec) * (size_t)total;
+}
+
+int upload_inputs(Batch* b, const pfb_cycle_in* in) {
+    b->stage_used = pack_inputs(b, in, b->hp, b->h_obs, b->d_obs);
     return 0;
+}
+
+// Estimate of one filter from its device scalars (Estimate, types.rs:92-103).
+void fill_estimate(const FilterDev& fd, pfb_estimate& o) {
+    static const int idx[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+    for (int k = 0; k < 3; ++k) o.pos[k] = fd.sums[k];
+    for (int k = 0; k < 4; ++k) o.quat[k] = fd.qmean[k];
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) {
+            o.pos_cov[3 * r + c] = fd.covs[idx[r][c]];
+            o.att_cov[3 * r + c] = fd.covs[6 + idx[r][c]];
+        }
+    o.ess = fd.sums[13] > 0.0 ? 1.0 / fd.sums[13] : 0.0;
+    o.resampled = fd.do_resample;
+    o.valid = fd.qvalid;
+}
+
+// Device stages of one staged cycle: parameters read in place from its staging block, every
+// filter's scalars copied to results by the covariance finalize (no copy node).
+int stages_staged(Batch* b, const ParamBlock& dp, FilterDev* results) {
+    PfState* s = &b->s;
+    if (launch_motion(s, ParamRef<MotionParams>::array(dp.motion), 0) != 0) return -1;
+    if (launch_weight(s, ParamRef<WeightParams>::array(dp.weight)) != 0) return -1;
+    if (launch_normalize(s, 0, b->d_ratio) != 0) return -1;
+    if (launch_resample(s, ParamRef<InjectParams>::array(dp.inject), s->d_inj_blk, s->inj_nblk, false) != 0)
+        return -1;
+    if (launch_est_sums(s, dp.cov, true) != 0) return -1;
+    return launch_est_covs(s, ParamRef<CovParams>::array(dp.cov), results);
+}
+
+int capture_staged(Batch* b) {
+    hipGraph_t g = nullptr;
+    PB_CHECK(hipStreamBeginCapture(b->s.stream, hipStreamCaptureModeThreadLocal));
+    int rc = 0;
+    for (int k = 0; k < b->st_cycles && rc == 0; ++k)
+        rc = stages_staged(b, carve(b->d_staged + (size_t)k * b->stage_bytes, b->nf), b->d_results + (size_t)k * b->nf);
+    hipError_t e = hipStreamEndCapture(b->s.stream, &g);
+    if (rc != 0 || e != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        fprintf(stderr, "pf_hip staged graph capture failed\n");
+        return -1;
+    }
+    e = hipGraphInstantiate(&b->gS, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    PB_CHECK(e);
+    return 0;
+}
+
+void free_staged(Batch* b) {
+    if (b->gS) (void)hipGraphExecDestroy(b->gS);
+    b->gS = nullptr;
+    (void)hipFree(b->d_staged);
+    (void)hipHostFree(b->h_staged);
+    (void)hipFree(b->d_results);
+    (void)hipHostFree(b->h_results);
+    b->d_staged = b->h_staged = nullptr;
+    b->d_results = b->h_results = nullptr;
+    b->st_cycles = 0;
+    b->st_pending = false;
 }
 
 }  // namespace
@@ -307,22 +385,60 @@ int pfb_cycle(void* h, const pfb_cycle_in* in, pfb_estimate* out) {
     while ((q = hipEventQuery(b->ev_done)) == hipErrorNotReady) {
     }
     PB_CHECK(q);
-    if (out) {
-        static const int idx[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
-        for (int f = 0; f < b->nf; ++f) {
-            const FilterDev& fd = b->h_fdev[f];
-            for (int k = 0; k < 3; ++k) out[f].pos[k] = fd.sums[k];
-            for (int k = 0; k < 4; ++k) out[f].quat[k] = fd.qmean[k];
-            for (int r = 0; r < 3; ++r)
-                for (int c = 0; c < 3; ++c) {
-                    out[f].pos_cov[3 * r + c] = fd.covs[idx[r][c]];
-                    out[f].att_cov[3 * r + c] = fd.covs[6 + idx[r][c]];
-                }
-            out[f].ess = fd.sums[13] > 0.0 ? 1.0 / fd.sums[13] : 0.0;
-            out[f].resampled = fd.do_resample;
-            out[f].valid = fd.qvalid;
-        }
+    if (out)
+        for (int f = 0; f < b->nf; ++f) fill_estimate(b->h_fdev[f], out[f]);
+    return 0;
+}
+
+int pfb_stage_cycles(void* h, int ncycles, const pfb_cycle_in* in) {
+    Batch* b = static_cast<Batch*>(h);
+    if (!b || !in || ncycles <= 0) return -1;
+    PB_CHECK(hipStreamSynchronize(b->s.stream));
+    free_staged(b);
+    const size_t sb = b->stage_bytes, nfd = (size_t)ncycles * b->nf;
+    if (hipMalloc((void**)&b->d_staged, sb * ncycles) != hipSuccess ||
+        hipHostMalloc((void**)&b->h_staged, sb * ncycles, hipHostMallocDefault) != hipSuccess ||
+        hipMalloc((void**)&b->d_results, sizeof(FilterDev) * nfd) != hipSuccess ||
+        hipHostMalloc((void**)&b->h_results, sizeof(FilterDev) * nfd, hipHostMallocDefault) != hipSuccess) {
+        fprintf(stderr, "pf_hip pfb_stage_cycles: allocation of %d cycles failed\n", ncycles);
+        free_staged(b);
+        return -1;
     }
+    memset(b->h_staged, 0, sb * ncycles);
+    for (int k = 0; k < ncycles; ++k) {
+        char* hb = b->h_staged + (size_t)k * sb;
+        ObsRec* h_obs = reinterpret_cast<ObsRec*>(hb + b->params_bytes);
+        ObsRec* d_obs = reinterpret_cast<ObsRec*>(b->d_staged + (size_t)k * sb + b->params_bytes);
+        (void)pack_inputs(b, in + (size_t)k * b->nf, carve(hb, b->nf), h_obs, d_obs);
+    }
+    PB_CHECK(hipMemcpy(b->d_staged, b->h_staged, sb * ncycles, hipMemcpyHostToDevice));
+    b->st_cycles = ncycles;
+    if (b->use_graph && capture_staged(b) != 0) {
+        free_staged(b);
+        return -1;
+    }
+    b->st_pending = true;
+    return 0;
+}
+
+int pfb_run_staged(void* h, pfb_estimate* out) {
+    Batch* b = static_cast<Batch*>(h);
+    if (!b || !b->st_pending) return -1;
+    b->st_pending = false;
+    PfState* s = &b->s;
+    if (b->gS) {
+        PB_CHECK(hipGraphLaunch(b->gS, s->stream));
+    } else {
+        for (int k = 0; k < b->st_cycles; ++k)
+            if (stages_staged(b, carve(b->d_staged + (size_t)k * b->stage_bytes, b->nf),
+                              b->d_results + (size_t)k * b->nf) != 0)
+                return -1;
+    }
+    const size_t nfd = (size_t)b->st_cycles * b->nf;
+    PB_CHECK(hipMemcpyAsync(b->h_results, b->d_results, sizeof(FilterDev) * nfd, hipMemcpyDeviceToHost, s->stream));
+    PB_CHECK(hipStreamSynchronize(s->stream));
+    if (out)
+        for (size_t i = 0; i < nfd; ++i) fill_estimate(b->h_results[i], out[i]);
     return 0;
 }
 

@@ -93,6 +93,7 @@ __global__ __launch_bounds__(kTB) void k_motion(Layout L, PoseBufs B, FilterDev*
     const int off = L.f_off[f];
     const int li = blk * kTB + (int)threadIdx.x - off;
     if (li >= L.f_n[f]) return;
+    if (li == 0) F[f].est_src = 0;  // poses change: partials of an earlier gather are stale
     const MotionParams& p = A.get(f);
     const int gi = off + li;
     const PoseSet S = B.s[F[f].parity];
@@ -261,7 +262,33 @@ __device__ inline void body_offset(const BodyMap& b, const ObsRec& o, double& bx
     bz = __fma_rn(b.m20, o.lx, __fma_rn(b.m21, o.ly, __fma_rn(b.m22, o.lz, -b.mpz)));
 }
 
-template <bool kFastBearing>
+// Gated range-bearing pair on the scaled path (27 FP64 instructions with the body offset).
+// In: b2 = 2 b (the particle's body map doubled, exact), the record's vs = v / (2 u_gate) and
+// rng4 = 4 range.  X = |b2|^2 = 4 |b|^2; y0h = rsq(X) ~ 1 / (2 |b|); one Newton step
+// written as q = 3 - X y0h^2 gives 1/|b| = y0h q and |b| = X y0h q / 4, so
+//   ec4 = X y0h q - rng4 = 4 (|b| - range)            (gated as min(|ec4|, 4 g sr))
+//   us  = clamp((1 + (b.v) / |b|) / u_gate, 0, 1)      (the clamp bit of the FMA is the gate)
+// and eb^2 = u acos-series(u) = us (c0 + c1 us + c2 us^2 + c3 us^3) with u = u_gate us.  The range
+// sum comes out 16x, which the caller's final scale removes.
+__device__ inline void rb_scaled(double bx, double by, double bz, double vx, double vy, double vz, double rng4,
+                                 const WeightParams& p, double& accr, double& accb) {
+    const double X = __fma_rn(bx, bx, __fma_rn(by, by, __fma_rn(bz, bz, 4e-18)));
+    const double pvs = __fma_rn(bx, vx, __fma_rn(by, vy, bz * vz));
+    const double y0h = __builtin_amdgcn_rsq(X);
+    const double P = X * y0h;
+    const double q = __fma_rn(-P, y0h, 3.0);
+    const double ec = fmin(fabs(__fma_rn(P, q, -rng4)), p.rng_gate4);
+    accr = __fma_rn(ec, ec, accr);
+    const double y = y0h * q;
+    const double us = fmin(fmax(__fma_rn(pvs, y, p.cg), 0.0), 1.0);
+    double s = __fma_rn(p.c3, us, p.c2);
+    s = __fma_rn(s, us, p.c1);
+    s = __fma_rn(s, us, p.c0);
+    accb = __fma_rn(us, s, accb);
+}
+
+// Gated range-bearing pair for a bearing gate wider than u = 0.005 (or none): series up to
+// u = 0.125, acos beyond.
 __device__ inline void rb_from_b(double bx, double by, double bz, double vx, double vy, double vz, double range,
                                  const WeightParams& p, double& accr, double& accb) {
     const double r2 = __fma_rn(bx, bx, __fma_rn(by, by, __fma_rn(bz, bz, 1e-18)));
@@ -271,11 +298,6 @@ __device__ inline void rb_from_b(double bx, double by, double bz, double vx, dou
     const double y = __fma_rn(y0, __fma_rn(-hy, y0, 0.5), y0);
     const double ec = fmin(fabs(__fma_rn(r2, y, -range)), p.rng_gate);
     accr = __fma_rn(ec, ec, accr);
-    if (kFastBearing) {
-        const double u = fmin(fmax(__fma_rn(pv, y, 1.0), 0.0), p.u_gate);
-        accb = __fma_rn(u, acos_sq_p4(u), accb);
-        return;
-    }
     const double u = fmin(fmax(__fma_rn(pv, y, 1.0), 0.0), 2.0);
     double eb2;
     if (u <= 0.005) {
@@ -289,9 +311,10 @@ __device__ inline void rb_from_b(double bx, double by, double bz, double vx, dou
     accb += fmin(eb2, p.brg_gate2);
 }
 
-__device__ inline void cart_from_b(double bx, double by, double bz, double vx, double vy, double vz,
+// Cartesian pair; hs = 0.5 when b comes from the doubled body map (hs b is then exact).
+__device__ inline void cart_from_b(double bx, double by, double bz, double vx, double vy, double vz, double hs,
                                    const WeightParams& p, double& accr) {
-    const double ex = bx - vx, ey = by - vy, ez = bz - vz;
+    const double ex = __fma_rn(hs, bx, -vx), ey = __fma_rn(hs, by, -vy), ez = __fma_rn(hs, bz, -vz);
     const double r2 = __fma_rn(ex, ex, __fma_rn(ey, ey, ez * ez));
     accr += fmin(r2, p.cart_gate2);
 }
@@ -312,18 +335,28 @@ __global__ __launch_bounds__(kTB) void k_weight(Layout L, PoseBufs B, FilterDev*
     int li[kPPT];
     bool valid[kPPT];
     BodyMap bm[kPPT];
-    double accr[kPPT], accb[kPPT], acc[kPPT];
+    double accr[kPPT], accb[kPPT], accc[kPPT], acc[kPPT];
+    const bool fast = p.fast_rb != 0;
+    const double hs = fast ? 0.5 : 1.0;  // Cartesian records see b = hs * (mapped offset)
 #pragma unroll
     for (int e = 0; e < kPPT; ++e) {
         li[e] = blk * kSpan + e * kTB + (int)threadIdx.x - off;
         valid[e] = li[e] < nf;
         bm[e] = body_map(S, off + (valid[e] ? li[e] : 0));
+        if (fast) {  // doubled body map: every offset it forms is exactly 2 b
+            BodyMap& m = bm[e];
+            m.m00 *= 2.0; m.m01 *= 2.0; m.m02 *= 2.0;
+            m.m10 *= 2.0; m.m11 *= 2.0; m.m12 *= 2.0;
+            m.m20 *= 2.0; m.m21 *= 2.0; m.m22 *= 2.0;
+            m.mpx *= 2.0; m.mpy *= 2.0; m.mpz *= 2.0;
+        }
         accr[e] = 0.0;
         accb[e] = 0.0;
+        accc[e] = 0.0;
     }
     ConstObsPtr obs = (ConstObsPtr)p.obs;
     const int nobs = p.nobs, nrb = p.nrb;  // range-bearing records first, then Cartesian
-    if (p.u_gate <= 0.005) {
+    if (fast) {
 #pragma unroll 2
         for (int k = 0; k < nrb; ++k) {
             const ObsRec o = obs[k];
@@ -331,7 +364,7 @@ __global__ __launch_bounds__(kTB) void k_weight(Layout L, PoseBufs B, FilterDev*
             for (int e = 0; e < kPPT; ++e) {
                 double bx, by, bz;
                 body_offset(bm[e], o, bx, by, bz);
-                rb_from_b<true>(bx, by, bz, o.vx, o.vy, o.vz, o.range, p, accr[e], accb[e]);
+                rb_scaled(bx, by, bz, o.vx, o.vy, o.vz, o.range, p, accr[e], accb[e]);
             }
         }
     } else {
@@ -341,7 +374,7 @@ __global__ __launch_bounds__(kTB) void k_weight(Layout L, PoseBufs B, FilterDev*
             for (int e = 0; e < kPPT; ++e) {
                 double bx, by, bz;
                 body_offset(bm[e], o, bx, by, bz);
-                rb_from_b<false>(bx, by, bz, o.vx, o.vy, o.vz, o.range, p, accr[e], accb[e]);
+                rb_from_b(bx, by, bz, o.vx, o.vy, o.vz, o.range, p, accr[e], accb[e]);
             }
         }
     }
@@ -351,11 +384,13 @@ __global__ __launch_bounds__(kTB) void k_weight(Layout L, PoseBufs B, FilterDev*
         for (int e = 0; e < kPPT; ++e) {
             double bx, by, bz;
             body_offset(bm[e], o, bx, by, bz);
-            cart_from_b(bx, by, bz, o.vx, o.vy, o.vz, p, accr[e]);
+            cart_from_b(bx, by, bz, o.vx, o.vy, o.vz, hs, p, accc[e]);
         }
     }
+    const double sr = fast ? p.inv2rr * 0.0625 : p.inv2rr;  // the scaled range sum is 16x
 #pragma unroll
-    for (int e = 0; e < kPPT; ++e) acc[e] = -__fma_rn(accr[e], p.inv2rr, accb[e] * p.inv2bb);
+    for (int e = 0; e < kPPT; ++e)
+        acc[e] = -__fma_rn(accr[e], sr, __fma_rn(accc[e], p.inv2rr, accb[e] * p.inv2bb));
     double mloc = -INFINITY;
 #pragma unroll
     for (int e = 0; e < kPPT; ++e)
@@ -478,6 +513,7 @@ __global__ __launch_bounds__(kFinTB) void k_norm_fin(Layout L, FilterDev* F, con
     const double tot = block16_sum(v, sh);
     if (threadIdx.x == 0) {
         FilterDev& fd = F[f];
+        fd.est_src = 0;  // weights changed
         fd.sum_w2 = tot;
         fd.ess = tot > 0.0 ? 1.0 / tot : 0.0;
         if (ratio) {
@@ -734,19 +770,112 @@ __device__ inline int wave_first_at_or_above(const uint64_t* c, int lo, int hi, 
     return lo + __popcll(__ballot(below));
 }
 
+// Cross-lane folds of 64-bit values on CDNA4 (one 32-bit permutation per half).  Rows are 16
+// lanes.  fold32(a, b): v_permlane32_swap exchanges rows 2-3 of a with rows 0-1 of b, so the
+// sum leaves a(l) + a(l+32) in lanes 0-31 and b(l-32) + b(l) in lanes 32-63.  fold16(a, b):
+// v_permlane16_swap exchanges the odd rows of a with the even rows of b, leaving a summed over
+// row pairs (0,1), (2,3) in rows 0 and 2 and b likewise in rows 1 and 3.
+__device__ inline double join64(unsigned lo, unsigned hi) {
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | (uint64_t)lo);
+}
+__device__ inline double fold32(double a, double b) {
+    const uint64_t ua = __builtin_bit_cast(uint64_t, a), ub = __builtin_bit_cast(uint64_t, b);
+    const auto l = __builtin_amdgcn_permlane32_swap((unsigned)ua, (unsigned)ub, false, false);
+    const auto h = __builtin_amdgcn_permlane32_swap((unsigned)(ua >> 32), (unsigned)(ub >> 32), false, false);
+    return join64(l[0], h[0]) + join64(l[1], h[1]);
+}
+__device__ inline double fold16(double a, double b) {
+    const uint64_t ua = __builtin_bit_cast(uint64_t, a), ub = __builtin_bit_cast(uint64_t, b);
+    const auto l = __builtin_amdgcn_permlane16_swap((unsigned)ua, (unsigned)ub, false, false);
+    const auto h = __builtin_amdgcn_permlane16_swap((unsigned)(ua >> 32), (unsigned)(ub >> 32), false, false);
+    return join64(l[0], h[0]) + join64(l[1], h[1]);
+}
+// x of the lane n places further along its row (DPP row_ror).
+template <int N>
+__device__ inline double row_ror(double x) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, x);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)u, 0x120 + N, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(u >> 32), 0x120 + N, 0xf, 0xf, false);
+    return join64((unsigned)lo, (unsigned)hi);
+}
+
+// Block partial of NV values (4 waves, every lane active).  In each wave the values are folded
+// across the halves and then across row pairs (value i with value i + H, so no lane holds more
+// than H2 = ceil(NV / 4) of them), then summed within each row by DPP rotations; lane 16 r
+// then holds the wave sums of the values base_r .. base_r + H2 - 1.  A fixed order throughout,
+// and one LDS exchange across the waves.  Value k of block blk goes to part[k * stride + blk], so
+// that a finalize reading one value over consecutive blocks loads consecutive doubles.
+template <int NV>
+__device__ inline void block_partials(double (&acc)[NV], double* __restrict__ part, int blk, int stride) {
+    constexpr int H1 = (NV + 1) / 2, H2 = (H1 + 1) / 2;
+    __shared__ double shv[4][NV];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    double s1[H1], s2[H2];
+#pragma unroll
+    for (int i = 0; i < H1; ++i) s1[i] = fold32(acc[i], i + H1 < NV ? acc[i + H1] : 0.0);
+#pragma unroll
+    for (int i = 0; i < H2; ++i) s2[i] = fold16(s1[i], i + H2 < H1 ? s1[i + H2] : 0.0);
+#pragma unroll
+    for (int i = 0; i < H2; ++i) {
+        s2[i] += row_ror<8>(s2[i]);
+        s2[i] += row_ror<4>(s2[i]);
+        s2[i] += row_ror<2>(s2[i]);
+        s2[i] += row_ror<1>(s2[i]);
+    }
+    if ((lane & 15) == 0) {
+        const int r = lane >> 4;  // rows 0, 1: values [0, H1) split at H2; rows 2, 3: [H1, NV) likewise
+        const int base = r == 0 ? 0 : r == 1 ? H2 : r == 2 ? H1 : H1 + H2;
+        const int end = r == 0 ? H2 : r == 1 ? H1 : r == 2 ? (H1 + H2 < NV ? H1 + H2 : NV) : NV;
+#pragma unroll
+        for (int i = 0; i < H2; ++i)
+            if (base + i < end) shv[wv][base + i] = s2[i];
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < NV) {
+        const int k = threadIdx.x;
+        part[(size_t)k * stride + blk] = (shv[0][k] + shv[1][k]) + (shv[2][k] + shv[3][k]);
+    }
+}
+
+// One particle's terms of the estimate sums (cu:354-404 layout): [0..2] w p, [3..12] w q_r q_c
+// (r <= c), [13] w^2.
+__device__ inline void est_terms(double wi, double px, double py, double pz, const double (&q)[4],
+                                 double (&acc)[kEstN]) {
+    acc[0] = __fma_rn(wi, px, acc[0]);
+    acc[1] = __fma_rn(wi, py, acc[1]);
+    acc[2] = __fma_rn(wi, pz, acc[2]);
+    int k = 3;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const double wq = wi * q[r];
+#pragma unroll
+        for (int c = r; c < 4; ++c) {
+            acc[k] = __fma_rn(wq, q[c], acc[k]);
+            ++k;
+        }
+    }
+    acc[13] = __fma_rn(wi, wi, acc[13]);
+}
+
 constexpr int kGatherLds = 2048;  // cumulative weights staged per block (16 KB)
 
 // Merge-path gather: each block owns 256 consecutive outputs; two global searches (one wave
 // each) bound their ancestors to [jlo, jhi], that slice of the scan is staged in LDS and every
 // lane searches it locally.  Poses move from the current set to the other set (no copies);
-// weights -> 1/n; k_flip makes the other set current after the injection.
+// weights -> 1/n.  An output that injection picks target (k_inj_pick linked them into a list
+// at head[output]) adds their jitters in ascending pick order in f64 before its single f32
+// store, and resets its list head.  The block then leaves its estimate-sum partial of the
+// final poses and weights (est_src = 1), so the estimate needs no separate pass over them.
+// The other set becomes current afterwards (k_flip, or the sums finalize of the fused cycle).
 __global__ __launch_bounds__(kTB) void k_gather(Layout L, FilterDev* F, PoseBufs B, const uint64_t* __restrict__ cum,
-                                                float* __restrict__ w, int32_t* __restrict__ anc) {
+                                                float* __restrict__ w, int32_t* __restrict__ anc,
+                                                int32_t* __restrict__ head, const int32_t* __restrict__ next,
+                                                const double* __restrict__ jit, double* __restrict__ part_est) {
     __shared__ int s_lo, s_hi;
     __shared__ uint64_t s_cum[kGatherLds];
     const int blk = blockIdx.x;
     const int f = L.blk_filter[blk];
-    const FilterDev& fd = F[f];
+    FilterDev& fd = F[f];
     if (!fd.do_resample) return;
     const int off = L.f_off[f];
     const int nf = L.f_n[f];
@@ -761,6 +890,10 @@ __global__ __launch_bounds__(kTB) void k_gather(Layout L, FilterDev* F, PoseBufs
     const uint64_t A = (uint64_t)nf << 32;
     const uint64_t U = (uint64_t)fd.u32;
     const uint64_t* c = cum + off;
+    const float winv = L.f_winv[f];
+    double acc[kEstN];
+#pragma unroll
+    for (int k = 0; k < kEstN; ++k) acc[k] = 0.0;
     if (i0 < nf) {
         const int i1 = min(i0 + kTB - 1, nf - 1);
         if (threadIdx.x < 64) {
@@ -785,21 +918,50 @@ __global__ __launch_bounds__(kTB) void k_gather(Layout L, FilterDev* F, PoseBufs
             else
                 j = first_at_or_above(c, jlo, jhi, A, T, Bi);
             const PoseSet S = B.s[par], D = B.s[par ^ 1];
+            float v[7];
 #pragma unroll
-            for (int k = 0; k < 7; ++k) D.c[k][off + li] = S.c[k][off + j];
+            for (int k = 0; k < 7; ++k) v[k] = S.c[k][off + j];
+            const int hd = head[off + li];
+            if (hd >= 0) {
+                // ascending traversal of the (unordered) pick list: repeatedly take the smallest
+                // pick above the previous one (cu:333-352 adds them one after another)
+                double ax = (double)v[0], ay = (double)v[1], az = (double)v[2];
+                int prev = -1;
+                while (true) {
+                    int best = 0x7fffffff;
+                    for (int q = hd; q >= 0; q = next[off + q])
+                        if (q > prev && q < best) best = q;
+                    if (best == 0x7fffffff) break;
+                    ax += jit[3 * (off + best)];
+                    ay += jit[3 * (off + best) + 1];
+                    az += jit[3 * (off + best) + 2];
+                    prev = best;
+                }
+                v[0] = (float)ax;
+                v[1] = (float)ay;
+                v[2] = (float)az;
+                head[off + li] = -1;
+            }
+#pragma unroll
+            for (int k = 0; k < 7; ++k) D.c[k][off + li] = v[k];
             anc[off + li] = j;
-            w[off + li] = L.f_winv[f];
+            w[off + li] = winv;
+            const double q[4] = {(double)v[3], (double)v[4], (double)v[5], (double)v[6]};
+            est_terms((double)winv, (double)v[0], (double)v[1], (double)v[2], q, acc);
         }
     }
+    block_partials<kEstN>(acc, part_est, blk, (int)gridDim.x);
+    if (i0 == 0 && threadIdx.x == 0) fd.est_src = 1;
 }
 
-// Injection picks (cu:333-352 formula): target index and f64 jitter of pick j.  The three
-// injection kernels run over the particle-layout blocks that can hold picks: tab lists them
-// (blocks whose first local index is below the filter's largest pick count), or, when tab is
-// null, the grid is the first ceil(k / 256) blocks of a single filter.
+// Injection picks (cu:333-352 formula): pick j draws its target and f64 jitter and pushes itself
+// onto its target's list (head[] is -1 everywhere at rest: allocation sets it, k_gather resets
+// the heads it consumes).  Runs over the particle-layout blocks that can hold picks: tab lists
+// them (blocks whose first local index is below the filter's largest pick count), or, when tab
+// is null, the grid is the first ceil(k / 256) blocks of a single filter.
 __global__ __launch_bounds__(kTB) void k_inj_pick(Layout L, FilterDev* F, ParamRef<InjectParams> A, const int32_t* __restrict__ tab,
-                                                  int32_t* __restrict__ idx, double* __restrict__ jit,
-                                                  int32_t* __restrict__ head) {
+                                                  double* __restrict__ jit, int32_t* __restrict__ head,
+                                                  int32_t* __restrict__ next) {
     const int blk = tab ? tab[blockIdx.x] : (int)blockIdx.x;
     const int f = L.blk_filter[blk];
     const FilterDev& fd = F[f];
@@ -815,58 +977,10 @@ __global__ __launch_bounds__(kTB) void k_inj_pick(Layout L, FilterDev* F, ParamR
     const double jx = u01_co(sm_next(z)) - 0.5;
     const double jy = u01_co(sm_next(z)) - 0.5;
     const double jz = u01_co(sm_next(z)) - 0.5;
-    idx[off + j] = t;
     jit[3 * (off + j)] = jx;
     jit[3 * (off + j) + 1] = jy;
     jit[3 * (off + j) + 2] = jz;
-    head[off + t] = -1;
-}
-
-__global__ __launch_bounds__(kTB) void k_inj_link(Layout L, FilterDev* F, ParamRef<InjectParams> A, const int32_t* __restrict__ tab,
-                                                  const int32_t* __restrict__ idx, int32_t* __restrict__ head,
-                                                  int32_t* __restrict__ next) {
-    const int blk = tab ? tab[blockIdx.x] : (int)blockIdx.x;
-    const int f = L.blk_filter[blk];
-    const FilterDev& fd = F[f];
-    const InjectParams& p = A.get(f);
-    const int off = L.f_off[f];
-    const int j = blk * kTB + (int)threadIdx.x - off;
-    if (!fd.do_resample || fd.total == 0 || j >= p.k) return;
-    const int t = idx[off + j];
     next[off + j] = atomicExch(&head[off + t], j);
-}
-
-// The pick that ended up as list head applies every pick of its target in ascending pick order.
-__global__ __launch_bounds__(kTB) void k_inj_apply(Layout L, FilterDev* F, PoseBufs B, ParamRef<InjectParams> A, const int32_t* __restrict__ tab,
-                                                   const int32_t* __restrict__ idx, const double* __restrict__ jit,
-                                                   const int32_t* __restrict__ head,
-                                                   const int32_t* __restrict__ next) {
-    const int blk = tab ? tab[blockIdx.x] : (int)blockIdx.x;
-    const int f = L.blk_filter[blk];
-    const FilterDev& fd = F[f];
-    const InjectParams& p = A.get(f);
-    const int off = L.f_off[f];
-    const int j = blk * kTB + (int)threadIdx.x - off;
-    if (!fd.do_resample || fd.total == 0 || j >= p.k) return;
-    const int t = idx[off + j];
-    if (head[off + t] != j) return;
-    const PoseSet S = B.s[fd.parity ^ 1];  // the gathered set (k_flip runs afterwards)
-    double ax = (double)S.c[0][off + t], ay = (double)S.c[1][off + t], az = (double)S.c[2][off + t];
-    // ascending traversal of the (unordered) list: repeatedly take the smallest pick > prev
-    int prev = -1;
-    while (true) {
-        int best = 0x7fffffff;
-        for (int q = j; q >= 0; q = next[off + q])
-            if (q > prev && q < best) best = q;
-        if (best == 0x7fffffff) break;
-        ax += jit[3 * (off + best)];
-        ay += jit[3 * (off + best) + 1];
-        az += jit[3 * (off + best) + 2];
-        prev = best;
-    }
-    S.c[0][off + t] = (float)ax;
-    S.c[1][off + t] = (float)ay;
-    S.c[2][off + t] = (float)az;
 }
 
 // The gathered set becomes current for filters that resampled with a positive total; the
@@ -879,43 +993,57 @@ __global__ void k_flip(FilterDev* F, int nfilt, unsigned long long* scan_ctl) {
 
 // ---------------------------------------------------------------------------- estimate
 
-// Block partial of NV values: wave butterflies and one LDS exchange.
-template <int NV>
-__device__ inline void block_partials(double (&acc)[NV], double* __restrict__ part, int blk) {
-    __shared__ double shv[4][NV];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-#pragma unroll
-    for (int k = 0; k < NV; ++k) acc[k] = wave_sum(acc[k]);
-    if (lane == 0)
-#pragma unroll
-        for (int k = 0; k < NV; ++k) shv[wv][k] = acc[k];
-    __syncthreads();
-    if ((int)threadIdx.x < NV) {
-        const int k = threadIdx.x;
-        part[(size_t)NV * blk + k] = (shv[0][k] + shv[1][k]) + (shv[2][k] + shv[3][k]);
-    }
-}
-
 // Per filter (one block each): one wave per value sums the filter's block partials in a fixed
-// strided order, then a fixed butterfly.  which = 0 -> sums, 1 -> covariances.  With cov
-// non-null (sums only) thread 0 then takes the Markley mean of the finished sums and writes the
-// covariance pass parameters.
+// strided order (four independent loads in flight per lane, added in that same order), then a
+// fixed butterfly.  which = 0 -> sums (partials per 256-particle block when the resampling
+// gather left them, est_src = 1, else per 1024-particle tile), 1 -> covariances (per tile);
+// value k of block b at part[k * stride + b].  With cov non-null (sums only) thread 0 then
+// takes the Markley mean of the finished sums and writes the covariance pass parameters.  With
+// flip (sums of the fused cycle) the resampled set becomes current and the scan's status epoch
+// advances, as k_flip.  With out non-null (covariances of a staged cycle) the filter's scalars
+// are copied to out[f]: the other members from F[f] (final since earlier kernels), the new
+// values by the lanes that hold them.
 template <int NV>
 __global__ __launch_bounds__(kFinTB) void k_est_fin(Layout L, FilterDev* F, const double* __restrict__ part,
-                                                    int which, CovParams* cov) {
+                                                    int stride, int which, CovParams* cov, int flip,
+                                                    unsigned long long* scan_ctl, FilterDev* out) {
     static_assert(NV <= kFinTB / 64, "one wave per value");
     __shared__ double sv[NV];
     const int f = blockIdx.x;
-    const int b0 = L.f_off[f] / kTile, nb = L.f_padn[f] / kTile;
+    const int span = (which == 0 && F[f].est_src) ? kTB : kTile;
+    const int b0 = L.f_off[f] / span, nb = L.f_padn[f] / span;
     const int k = threadIdx.x >> 6, lane = threadIdx.x & 63;
     if (k < NV) {
+        const double* pk = part + (size_t)k * stride + b0;
         double v = 0.0;
-        for (int b = lane; b < nb; b += 64) v += part[(size_t)NV * (b0 + b) + k];
+        int b = lane;
+        for (; b + 192 < nb; b += 256) {
+            const double x0 = pk[b], x1 = pk[b + 64], x2 = pk[b + 128], x3 = pk[b + 192];
+            v += x0;
+            v += x1;
+            v += x2;
+            v += x3;
+        }
+        for (; b < nb; b += 64) v += pk[b];
         v = wave_sum(v);
         if (lane == 0) {
             (which == 0 ? F[f].sums : F[f].covs)[k] = v;
+            if (out) (which == 0 ? out[f].sums : out[f].covs)[k] = v;
             sv[k] = v;
         }
+    }
+    if (out) {
+        static_assert(sizeof(FilterDev) % sizeof(uint64_t) == 0, "FilterDev copied as 64-bit words");
+        constexpr int kWords = (int)(sizeof(FilterDev) / sizeof(uint64_t));
+        const int v0 = (int)((which == 0 ? offsetof(FilterDev, sums) : offsetof(FilterDev, covs)) / sizeof(uint64_t));
+        const int t = (int)threadIdx.x;
+        if (t < kWords && (t < v0 || t >= v0 + NV))
+            reinterpret_cast<uint64_t*>(out + f)[t] = reinterpret_cast<const uint64_t*>(F + f)[t];
+    }
+    if (flip && threadIdx.x == 0) {
+        FilterDev& fd = F[f];
+        if (fd.do_resample && fd.total > 0) fd.parity ^= 1;
+        if (f == 0) scan_ctl[0] = (scan_ctl[0] + 1ull) & 0x3fffffffull;
     }
     if (!cov) return;
     __syncthreads();
@@ -923,11 +1051,12 @@ __global__ __launch_bounds__(kFinTB) void k_est_fin(Layout L, FilterDev* F, cons
 }
 
 // est layout (cu:354-404): [0..2] sum w p, [3..12] sum w q_r q_c (r<=c), [13] sum w^2.
+// Filters whose partials the resampling gather already left (est_src = 1) are skipped.
 __global__ __launch_bounds__(kTB) void k_est_sums(Layout L, FilterDev* F, PoseBufs B, const float* __restrict__ w,
-                                                  double* __restrict__ part) {
-    __shared__ double sh[4];
+                                                  double* __restrict__ part, int stride) {
     const int blk = blockIdx.x;  // 1024-particle tile
     const int f = L.blk_filter[blk * (kTile / kTB)];
+    if (F[f].est_src) return;
     const int off = L.f_off[f];
     const int nf = L.f_n[f];
     const PoseSet S = B.s[F[f].parity];
@@ -945,34 +1074,18 @@ __global__ __launch_bounds__(kTB) void k_est_sums(Layout L, FilterDev* F, PoseBu
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
             if (li0 + e < nf) {
-                const double wi = (double)vv[28 + e];
-                const double px = vv[0 + e], py = vv[4 + e], pz = vv[8 + e];
                 const double q[4] = {(double)vv[12 + e], (double)vv[16 + e], (double)vv[20 + e], (double)vv[24 + e]};
-                acc[0] = __fma_rn(wi, px, acc[0]);
-                acc[1] = __fma_rn(wi, py, acc[1]);
-                acc[2] = __fma_rn(wi, pz, acc[2]);
-                int k = 3;
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const double wq = wi * q[r];
-#pragma unroll
-                    for (int c = r; c < 4; ++c) {
-                        acc[k] = __fma_rn(wq, q[c], acc[k]);
-                        ++k;
-                    }
-                }
-                acc[13] = __fma_rn(wi, wi, acc[13]);
+                est_terms((double)vv[28 + e], (double)vv[0 + e], (double)vv[4 + e], (double)vv[8 + e], q, acc);
             }
         }
     }
-    block_partials<kEstN>(acc, part, blk);
+    block_partials<kEstN>(acc, part, blk, stride);
 }
 
 // cov layout (cu:406-471): [0..5] position, [6..11] attitude tangent (axis-angle of qm^-1 q,
 // canonical sign, angle = 2 atan2(|v|, w)).
 __global__ __launch_bounds__(kTB) void k_est_covs(Layout L, FilterDev* F, PoseBufs B, const float* __restrict__ w,
-                                                  double* __restrict__ part, ParamRef<CovParams> A) {
-    __shared__ double sh[4];
+                                                  double* __restrict__ part, int stride, ParamRef<CovParams> A) {
     const int blk = blockIdx.x;
     const int f = L.blk_filter[blk * (kTile / kTB)];
     const int off = L.f_off[f];
@@ -1036,7 +1149,7 @@ __global__ __launch_bounds__(kTB) void k_est_covs(Layout L, FilterDev* F, PoseBu
             }
         }
     }
-    block_partials<kCovN>(acc, part, blk);
+    block_partials<kCovN>(acc, part, blk, stride);
 }
 
 // Sets the per-filter resample predicate: force (ABI call) or ESS < ratio * n (fused cycle).
@@ -1085,15 +1198,15 @@ int state_alloc(PfState* s, const std::vector<int>& ns) {
     PF_CHECK(hipMalloc((void**)&s->cum, sizeof(uint64_t) * (size_t)NP));
     PF_CHECK(hipMalloc((void**)&s->anc, sizeof(int32_t) * (size_t)NP));
     PF_CHECK(hipMemsetAsync(s->anc, 0, sizeof(int32_t) * (size_t)NP, s->stream));
-    PF_CHECK(hipMalloc((void**)&s->inj_idx, sizeof(int32_t) * (size_t)NP));
     PF_CHECK(hipMalloc((void**)&s->inj_next, sizeof(int32_t) * (size_t)NP));
     PF_CHECK(hipMalloc((void**)&s->inj_head, sizeof(int32_t) * (size_t)NP));
+    PF_CHECK(hipMemsetAsync(s->inj_head, 0xff, sizeof(int32_t) * (size_t)NP, s->stream));  // empty lists (-1)
     PF_CHECK(hipMalloc((void**)&s->inj_jit, sizeof(double) * 3 * (size_t)NP));
     const int nb256 = NP / kTB, ntile = NP / kTile;
     PF_CHECK(hipMalloc((void**)&s->part_w, sizeof(double) * 3 * (size_t)nb256));
     PF_CHECK(hipMalloc((void**)&s->part_n, sizeof(double) * (size_t)nb256));
-    PF_CHECK(hipMalloc((void**)&s->part_est, sizeof(double) * kEstN * (size_t)ntile));
-    PF_CHECK(hipMalloc((void**)&s->part_cov, sizeof(double) * kCovN * (size_t)ntile));
+    PF_CHECK(hipMalloc((void**)&s->part_est, sizeof(double) * kEstN * (size_t)nb256));  // per 256 (gather) or 1024
+    PF_CHECK(hipMalloc((void**)&s->part_cov, sizeof(double) * kCovN * (size_t)nb256));
     PF_CHECK(hipMalloc((void**)&s->scan_st, sizeof(uint64_t) * 2 * (size_t)ntile));
     PF_CHECK(hipMemsetAsync(s->scan_st, 0, sizeof(uint64_t) * 2 * (size_t)ntile, s->stream));
     PF_CHECK(hipMalloc((void**)&s->scan_ctl, sizeof(unsigned long long)));
@@ -1139,7 +1252,7 @@ void state_free_particles(PfState* s) {
             s->pose.s[b].c[c] = nullptr;
         }
     void** ptrs[] = {(void**)&s->w,        (void**)&s->ll,          (void**)&s->cum,       (void**)&s->anc,
-                     (void**)&s->inj_idx,  (void**)&s->inj_next,    (void**)&s->inj_head,  (void**)&s->inj_jit,
+                     (void**)&s->inj_next,    (void**)&s->inj_head,  (void**)&s->inj_jit,
                      (void**)&s->part_w,   (void**)&s->part_n,  (void**)&s->part_est,    (void**)&s->part_cov,  (void**)&s->scan_st,
                      (void**)&s->scan_ctl, (void**)&s->d_blk_filter, (void**)&s->d_fn,     (void**)&s->d_foff,
                      (void**)&s->d_fpadn,  (void**)&s->d_fwinv,     (void**)&s->fdev,      (void**)&s->d_inj_blk};
@@ -1225,60 +1338,59 @@ int launch_normalize(PfState* s, int force_uniform, const float* d_ratio) {
     return 0;
 }
 
-int launch_resample(PfState* s, ParamRef<InjectParams> inj, const int32_t* inj_tab, int inj_nblk) {
+int launch_resample(PfState* s, ParamRef<InjectParams> inj, const int32_t* inj_tab, int inj_nblk, bool flip) {
     const int ntile = s->NP / kTile;
     prof_begin(s, PFX_STAGE_RESAMPLE_SCAN);
     hipLaunchKernelGGL(k_scan, dim3(ntile), dim3(kTB), 0, s->stream, s->L, s->fdev, s->w, s->cum, s->scan_st,
                        s->scan_ctl, s->scan_spin_limit);
     prof_end(s);
     PF_LAUNCH_CHECK();
-    prof_begin(s, PFX_STAGE_RESAMPLE_GATHER);
-    hipLaunchKernelGGL(k_gather, dim3(s->NP / kTB), dim3(kTB), 0, s->stream, s->L, s->fdev, s->pose, s->cum, s->w,
-                       s->anc);
-    prof_end(s);
-    PF_LAUNCH_CHECK();
     if (inj_nblk > 0) {
         prof_begin(s, PFX_STAGE_INJECT);
-        const dim3 g(inj_nblk), b(kTB);
-        hipLaunchKernelGGL(k_inj_pick, g, b, 0, s->stream, s->L, s->fdev, inj, inj_tab, s->inj_idx, s->inj_jit,
-                           s->inj_head);
-        hipLaunchKernelGGL(k_inj_link, g, b, 0, s->stream, s->L, s->fdev, inj, inj_tab, s->inj_idx, s->inj_head,
-                           s->inj_next);
-        hipLaunchKernelGGL(k_inj_apply, g, b, 0, s->stream, s->L, s->fdev, s->pose, inj, inj_tab, s->inj_idx,
+        hipLaunchKernelGGL(k_inj_pick, dim3(inj_nblk), dim3(kTB), 0, s->stream, s->L, s->fdev, inj, inj_tab,
                            s->inj_jit, s->inj_head, s->inj_next);
         prof_end(s);
         PF_LAUNCH_CHECK();
     }
-    prof_begin(s, PFX_STAGE_FINALIZE);
-    hipLaunchKernelGGL(k_flip, dim3((s->nf + 63) / 64), dim3(64), 0, s->stream, s->fdev, s->nf, s->scan_ctl);
+    prof_begin(s, PFX_STAGE_RESAMPLE_GATHER);
+    hipLaunchKernelGGL(k_gather, dim3(s->NP / kTB), dim3(kTB), 0, s->stream, s->L, s->fdev, s->pose, s->cum, s->w,
+                       s->anc, s->inj_head, s->inj_next, s->inj_jit, s->part_est);
     prof_end(s);
     PF_LAUNCH_CHECK();
+    if (flip) {
+        prof_begin(s, PFX_STAGE_FINALIZE);
+        hipLaunchKernelGGL(k_flip, dim3((s->nf + 63) / 64), dim3(64), 0, s->stream, s->fdev, s->nf, s->scan_ctl);
+        prof_end(s);
+        PF_LAUNCH_CHECK();
+    }
     return 0;
 }
 
-int launch_est_sums(PfState* s, CovParams* d_cov) {
+int launch_est_sums(PfState* s, CovParams* d_cov, bool flip) {
     prof_begin(s, PFX_STAGE_EST_SUMS);
     hipLaunchKernelGGL(k_est_sums, dim3(s->NP / kTile), dim3(kTB), 0, s->stream, s->L, s->fdev, s->pose, s->w,
-                       s->part_est);
+                       s->part_est, s->NP / kTB);
     prof_end(s);
     PF_LAUNCH_CHECK();
     prof_begin(s, PFX_STAGE_FINALIZE);
-    hipLaunchKernelGGL(k_est_fin<kEstN>, dim3(s->nf), dim3(kFinTB), 0, s->stream, s->L, s->fdev, s->part_est, 0,
-                       d_cov);
+    hipLaunchKernelGGL(k_est_fin<kEstN>, dim3(s->nf), dim3(kFinTB), 0, s->stream, s->L, s->fdev, s->part_est,
+                       s->NP / kTB, 0,
+                       d_cov, flip ? 1 : 0, s->scan_ctl, nullptr);
     prof_end(s);
     PF_LAUNCH_CHECK();
     return 0;
 }
 
-int launch_est_covs(PfState* s, ParamRef<CovParams> A) {
+int launch_est_covs(PfState* s, ParamRef<CovParams> A, FilterDev* results) {
     prof_begin(s, PFX_STAGE_EST_COVS);
     hipLaunchKernelGGL(k_est_covs, dim3(s->NP / kTile), dim3(kTB), 0, s->stream, s->L, s->fdev, s->pose, s->w,
-                       s->part_cov, A);
+                       s->part_cov, s->NP / kTB, A);
     prof_end(s);
     PF_LAUNCH_CHECK();
     prof_begin(s, PFX_STAGE_FINALIZE);
-    hipLaunchKernelGGL(k_est_fin<kCovN>, dim3(s->nf), dim3(kFinTB), 0, s->stream, s->L, s->fdev, s->part_cov, 1,
-                       nullptr);
+    hipLaunchKernelGGL(k_est_fin<kCovN>, dim3(s->nf), dim3(kFinTB), 0, s->stream, s->L, s->fdev, s->part_cov,
+                       s->NP / kTB, 1,
+                       nullptr, 0, s->scan_ctl, results);
     prof_end(s);
     PF_LAUNCH_CHECK();
     return 0;
@@ -1323,11 +1435,30 @@ WeightParams make_weight_params(double sigma_r, double sigma_b, double gate) {
         }
         p.u_gate = lo;
     }
+    // Scaled range-bearing path when the bearing gate bounds u below the 4-term series' range:
+    // the series coefficients in us = u / u_gate (u acos-series(u) = us (c0 + c1 us + c2 us^2 +
+    // c3 us^3)), so that us = 1 yields the gated value.
+    p.fast_rb = (p.u_gate > 0.0 && p.u_gate <= 0.005) ? 1 : 0;
+    p.pad_fast = 0;
+    if (p.fast_rb) {
+        const double ug = p.u_gate;
+        p.cg = 1.0 / ug;
+        p.c0 = 2.0 * ug;
+        p.c1 = (8.0 / 24.0) * ug * ug;
+        p.c2 = (16.0 / 180.0) * ug * ug * ug;
+        p.c3 = (32.0 / 1120.0) * ug * ug * ug * ug;
+        p.rng_gate4 = 4.0 * p.rng_gate;
+    }
     return p;
 }
 
-// Joins observations with the landmark table (skips ids outside [0, lm_cap), cu:242).
-int join_obs(const pfc_obs* obs, int nobs, const float* lm, int lm_cap, ObsRec* out, int* nrb) {
+// Joins observations with the landmark table (skips ids outside [0, lm_cap), cu:242).  On the
+// scaled range-bearing path (p.fast_rb) the range-bearing records carry v / (2 u_gate) and
+// 4 range (see rb_scaled).
+int join_obs(const pfc_obs* obs, int nobs, const float* lm, int lm_cap, const WeightParams& p, ObsRec* out,
+             int* nrb) {
+    const double vscale = p.fast_rb ? 0.5 / p.u_gate : 1.0;
+    const double rscale = p.fast_rb ? 4.0 : 1.0;
     // range-bearing records first (in input order), then Cartesian ones (in input order)
     int m = 0, rb = 0;
     for (int pass = 0; pass < 2; ++pass) {
@@ -1339,10 +1470,11 @@ int join_obs(const pfc_obs* obs, int nobs, const float* lm, int lm_cap, ObsRec* 
             r.lx = lm[3 * o.id];
             r.ly = lm[3 * o.id + 1];
             r.lz = lm[3 * o.id + 2];
-            r.vx = o.dx;
-            r.vy = o.dy;
-            r.vz = o.dz;
-            r.range = o.range;
+            const double vs = pass == 0 ? vscale : 1.0;
+            r.vx = (double)o.dx * vs;
+            r.vy = (double)o.dy * vs;
+            r.vz = (double)o.dz * vs;
+            r.range = (double)o.range * (pass == 0 ? rscale : 1.0);
             r.mode = o.mode == 0 ? 0 : 1;
             r.pad = 0;
         }
@@ -1471,11 +1603,11 @@ int pfc_weight(void* h, const void* obs, int nobs, float sigma_r, float sigma_b,
     // the staging buffer may still feed the previous call's copy
     PF_CHECK(hipEventSynchronize(s->obs_ev));
     int nrb = 0;
-    const int m = join_obs(static_cast<const pfc_obs*>(obs), nobs, s->lm.data(), s->lm_cap, s->h_obs, &nrb);
+    WeightParams p = make_weight_params(sigma_r, sigma_b, gate);
+    const int m = join_obs(static_cast<const pfc_obs*>(obs), nobs, s->lm.data(), s->lm_cap, p, s->h_obs, &nrb);
     if (m == 0) return launch_normalize(s, 1, nullptr);  // every id unknown: ll = 0 for all -> uniform
     PF_CHECK(hipMemcpyAsync(s->d_obs, s->h_obs, sizeof(ObsRec) * (size_t)m, hipMemcpyHostToDevice, s->stream));
     PF_CHECK(hipEventRecord(s->obs_ev, s->stream));
-    WeightParams p = make_weight_params(sigma_r, sigma_b, gate);
     p.obs = s->d_obs;
     p.nobs = m;
     p.nrb = nrb;
@@ -1495,13 +1627,13 @@ int pfc_resample(void* h, float random_inject_ratio) {
         if (k > s->n[0]) k = s->n[0];
     }
     ip.k = k;
-    return launch_resample(s, ParamRef<InjectParams>::value(ip), nullptr, (k + kTB - 1) / kTB);
+    return launch_resample(s, ParamRef<InjectParams>::value(ip), nullptr, (k + kTB - 1) / kTB, true);
 }
 
 int pfc_est_sums(void* h, double* out) {
     PfState* s = static_cast<PfState*>(h);
     if (!s || s->nf != 1 || !out) return -1;
-    if (launch_est_sums(s, nullptr) != 0) return -1;
+    if (launch_est_sums(s, nullptr, false) != 0) return -1;
     PF_CHECK(hipMemcpyAsync(out, fdev_field<double>(s, 0, offsetof(FilterDev, sums)), sizeof(double) * kEstN,
                             hipMemcpyDeviceToHost, s->stream));
     PF_CHECK(hipStreamSynchronize(s->stream));
@@ -1587,6 +1719,8 @@ int pfx_set_state(void* h, const float* pos, const float* quat, const float* w) 
         for (int i = 0; i < n; ++i) tmp[i] = u;
         PF_CHECK(hipMemcpy(s->w, tmp.data(), sizeof(float) * (size_t)n, hipMemcpyHostToDevice));
     }
+    const int zero = 0;  // the estimate partials of the last gather no longer describe the state
+    PF_CHECK(hipMemcpy(fdev_field<int>(s, 0, offsetof(FilterDev, est_src)), &zero, sizeof(int), hipMemcpyHostToDevice));
     return 0;
 }
 

@@ -133,3 +133,45 @@ def test_filters_without_observations_coast(gpu):
         out = bf.cycle(inp)
     assert abs(out[0].pos[0] - 21.0) < 0.5 and abs(out[1].pos[0] - 20.0) < 0.5
     assert out[0].ess > 0.99 * sizes[0]
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_staged_run_equals_cycles(gpu, graph):
+    """Device-resident inputs (pfb_stage_cycles / pfb_run_staged) reproduce pfb_cycle exactly."""
+    sizes = [3000, 1024, 9000]
+    lms = [20, 8, 64]
+    cycles = 6
+    scs = scenarios(sizes, lms, cycles + 2, 400)
+    a = make_batch(sizes, scs, 0.02)
+    b = make_batch(sizes, scs, 0.02)
+    a.use_graph(graph)
+    b.use_graph(graph)
+    for bf in (a, b):  # live cycle first: staging continues from the current state
+        inp, keep = batch_inputs(bf, scs, 0)
+        bf.cycle(inp)
+    ref = []
+    for k in range(1, cycles + 1):
+        inp, keep = batch_inputs(a, scs, k)
+        ref.append(a.cycle(inp))
+    staged = [batch_inputs(b, scs, k) for k in range(1, cycles + 1)]
+    b.stage_cycles([inp for inp, keep in staged])
+    got = b.run_staged()
+    assert len(got) == cycles
+    n_rs = 0
+    for k in range(cycles):
+        for f in range(len(sizes)):
+            r, g = ref[k][f], got[k][f]
+            for name in ("pos", "quat", "pos_cov", "att_cov"):
+                assert np.array_equal(np.array(getattr(r, name)[:]), np.array(getattr(g, name)[:])), (k, f, name)
+            assert r.ess == g.ess and r.resampled == g.resampled and r.valid == g.valid
+            n_rs += r.resampled
+    assert n_rs > 0
+    with pytest.raises(RuntimeError):
+        b.run_staged()  # a staging runs once
+    inp, keep = batch_inputs(a, scs, cycles + 1)
+    oa = a.cycle(inp)
+    inp, keep = batch_inputs(b, scs, cycles + 1)
+    ob = b.cycle(inp)
+    for f in range(len(sizes)):
+        assert np.array_equal(np.array(oa[f].pos[:]), np.array(ob[f].pos[:]))
+        assert np.array_equal(a.get_state(f)[0], b.get_state(f)[0])
