@@ -295,8 +295,9 @@ def run_pf_fused(n, sc, steps, warmup, steps_host):
     weight, normalise + ESS predicate, resample, estimate sums, Markley mean, covariances) on the
     device.  Two timings:
       * resident (the headline): the inputs of `steps` cycles (odometry + 256 observations each)
-        are staged in HBM first; the cycles then run back to back from one captured hipGraph and
-        every cycle's estimate is read back at the end (pfb_stage_cycles / pfb_run_staged);
+        are staged in HBM first; the cycles are then enqueued back to back (each cycle's kernels
+        read its parameters in place) and every cycle's estimate is read back at the end
+        (pfb_stage_cycles / pfb_run_staged);
       * host inputs: `steps_host` pfb_cycle calls, each with one H2D copy of the cycle's inputs,
         the cycle graph and one D2H of the estimate that the host waits for (PCIe-inclusive).
     Both continue the same filter; the staged cycles follow the host-input ones."""
@@ -421,7 +422,7 @@ def main():
         "config": {"workload": "config 1: 1M-particle PF cycle, 256 landmarks / RB observations, 1% outliers, "
                                "inject 0.01, gate 3", "particles": n, "landmarks": 256, "observations": 256,
                    "driver": "fused cycle, inputs resident in HBM (pfb_stage_cycles + pfb_run_staged: "
-                             "one filter, every cycle in one hipGraph, estimates read back at the end)"},
+                             "one filter, cycles enqueued back to back, estimates read back at the end)"},
         "particles_per_s": round(n / (ms_fused * 1e-3), 1),
         "ms_per_cycle_host_inputs": round(ms_host, 4),
         "ms_per_cycle_abi_driver": round(ms_fast, 4),

@@ -73,7 +73,6 @@ struct Batch {
     char* h_staged = nullptr;
     FilterDev* d_results = nullptr;  // st_cycles x nf filter scalars after each cycle
     FilterDev* h_results = nullptr;
-    hipGraphExec_t gS = nullptr;  // every staged cycle in one graph
 };
 
 ParamBlock carve(char* base, int nf) {
@@ -100,8 +99,6 @@ size_t params_size(int nf) {
 void free_batch(Batch* b) {
     if (b->s.stream) (void)hipStreamSynchronize(b->s.stream);
     if (b->gA) (void)hipGraphExecDestroy(b->gA);
-    if (b->gS) (void)hipGraphExecDestroy(b->gS);
-    b->gS = nullptr;
     (void)hipFree(b->d_staged);
     (void)hipHostFree(b->h_staged);
     (void)hipFree(b->d_results);
@@ -248,27 +245,7 @@ int stages_staged(Batch* b, const ParamBlock& dp, FilterDev* results) {
     return launch_est_covs(s, ParamRef<CovParams>::array(dp.cov), results);
 }
 
-int capture_staged(Batch* b) {
-    hipGraph_t g = nullptr;
-    PB_CHECK(hipStreamBeginCapture(b->s.stream, hipStreamCaptureModeThreadLocal));
-    int rc = 0;
-    for (int k = 0; k < b->st_cycles && rc == 0; ++k)
-        rc = stages_staged(b, carve(b->d_staged + (size_t)k * b->stage_bytes, b->nf), b->d_results + (size_t)k * b->nf);
-    hipError_t e = hipStreamEndCapture(b->s.stream, &g);
-    if (rc != 0 || e != hipSuccess) {
-        if (g) (void)hipGraphDestroy(g);
-        fprintf(stderr, "pf_hip staged graph capture failed\n");
-        return -1;
-    }
-    e = hipGraphInstantiate(&b->gS, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    PB_CHECK(e);
-    return 0;
-}
-
 void free_staged(Batch* b) {
-    if (b->gS) (void)hipGraphExecDestroy(b->gS);
-    b->gS = nullptr;
     (void)hipFree(b->d_staged);
     (void)hipHostFree(b->h_staged);
     (void)hipFree(b->d_results);
@@ -413,10 +390,6 @@ int pfb_stage_cycles(void* h, int ncycles, const pfb_cycle_in* in) {
     }
     PB_CHECK(hipMemcpy(b->d_staged, b->h_staged, sb * ncycles, hipMemcpyHostToDevice));
     b->st_cycles = ncycles;
-    if (b->use_graph && capture_staged(b) != 0) {
-        free_staged(b);
-        return -1;
-    }
     b->st_pending = true;
     return 0;
 }
@@ -426,14 +399,13 @@ int pfb_run_staged(void* h, pfb_estimate* out) {
     if (!b || !b->st_pending) return -1;
     b->st_pending = false;
     PfState* s = &b->s;
-    if (b->gS) {
-        PB_CHECK(hipGraphLaunch(b->gS, s->stream));
-    } else {
-        for (int k = 0; k < b->st_cycles; ++k)
-            if (stages_staged(b, carve(b->d_staged + (size_t)k * b->stage_bytes, b->nf),
-                              b->d_results + (size_t)k * b->nf) != 0)
-                return -1;
-    }
+    // Stream launches, one cycle after another with no synchronisation: the host enqueues a
+    // cycle far faster than the device runs it, so the device never waits.  (One graph holding
+    // every cycle stalled for milliseconds between groups of cycles when replayed.)
+    for (int k = 0; k < b->st_cycles; ++k)
+        if (stages_staged(b, carve(b->d_staged + (size_t)k * b->stage_bytes, b->nf),
+                          b->d_results + (size_t)k * b->nf) != 0)
+            return -1;
     const size_t nfd = (size_t)b->st_cycles * b->nf;
     PB_CHECK(hipMemcpyAsync(b->h_results, b->d_results, sizeof(FilterDev) * nfd, hipMemcpyDeviceToHost, s->stream));
     PB_CHECK(hipStreamSynchronize(s->stream));

@@ -55,8 +55,8 @@ int pfb_reinit(void *h, const float *pose7, const float *pos_std, const float *r
 int pfb_cycle(void *h, const pfb_cycle_in *in, pfb_estimate *out);
 /* Device-resident run of ncycles cycles.  pfb_stage_cycles packs in[k * nf + f] (cycle k,
  * filter f) exactly as pfb_cycle would, advancing the filters' draw state, and uploads every
- * cycle's inputs to device memory (with graphs enabled, all cycles are captured in one graph);
- * pfb_run_staged then runs them back to back with no host round trip between cycles and
+ * cycle's inputs to device memory; pfb_run_staged then enqueues them back to back (kernels
+ * read each cycle's parameters in place) with no host round trip between cycles and
  * returns out[k * nf + f], identical to what ncycles pfb_cycle calls return.  A staging is run
  * once. */
 int pfb_stage_cycles(void *h, int ncycles, const pfb_cycle_in *in);
