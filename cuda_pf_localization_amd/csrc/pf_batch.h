@@ -172,16 +172,21 @@ void prof_begin(PfState* s, int stage);
 void prof_end(PfState* s);
 int launch_motion(PfState* s, ParamRef<MotionParams> A, int reinit);
 int launch_weight(PfState* s, ParamRef<WeightParams> A);
-int launch_normalize(PfState* s, int force_uniform, const float* d_ratio);
-// Scan, injection picks, gather (with the picks' jitter and the estimate-sum partials); with flip
-// the resampled set then becomes current (else the fused cycle's sums finalize does it).
-int launch_resample(PfState* s, ParamRef<InjectParams> inj, const int32_t* inj_tab, int inj_nblk, bool flip);
+// Normalised weights, ESS and (with d_ratio) the resample predicate; with scan (fused cycle) the
+// normalisation is done inside the resampling scan of every filter, whose LCG then advances here.
+int launch_normalize(PfState* s, int force_uniform, const float* d_ratio, bool scan = false);
+// Scan (unless launch_normalize already scanned), injection picks, gather (with the picks' jitter
+// and the estimate-sum partials of every filter); with flip the resampled set then becomes
+// current (else the fused cycle's sums finalize does it).
+int launch_resample(PfState* s, ParamRef<InjectParams> inj, const int32_t* inj_tab, int inj_nblk, bool flip,
+                    bool scan = true);
 // Block table of the injection kernels from each filter's largest pick count.
 int set_inject_blocks(PfState* s, const std::vector<int>& kmax);
 // Estimate sums (the partials of a preceding resampling gather when it left them); with d_cov
 // non-null also the Markley mean of every filter's sums on the device -> FilterDev.qmean and
 // the covariance pass parameters d_cov[f]; with flip the resampled set becomes current.
-int launch_est_sums(PfState* s, CovParams* d_cov, bool flip);
+// pass = false: the gather of the same cycle left the partials of every filter.
+int launch_est_sums(PfState* s, CovParams* d_cov, bool flip, bool pass = true);
 // Covariances; with results non-null every filter's scalars are also copied to results[f].
 int launch_est_covs(PfState* s, ParamRef<CovParams> A, FilterDev* results = nullptr);
 int launch_predicate(PfState* s, const float* d_ratio, int force);

@@ -131,10 +131,12 @@ int stages_all(Batch* b) {
     PfState* s = &b->s;
     if (launch_motion(s, ParamRef<MotionParams>::array(b->dp.motion), 0) != 0) return -1;
     if (launch_weight(s, ParamRef<WeightParams>::array(b->dp.weight)) != 0) return -1;
-    if (launch_normalize(s, 0, b->d_ratio) != 0) return -1;
-    if (launch_resample(s, ParamRef<InjectParams>::array(b->dp.inject), s->d_inj_blk, s->inj_nblk, false) != 0)
+    if (launch_normalize(s, 0, b->d_ratio, true) != 0) return -1;  // + the scan of every filter
+    if (launch_resample(s, ParamRef<InjectParams>::array(b->dp.inject), s->d_inj_blk, s->inj_nblk, false, false) !=
+        0)
         return -1;
-    if (launch_est_sums(s, b->dp.cov, true) != 0) return -1;  // + the resampled set becomes current
+    // the gather left every filter's sum partials; + the resampled set becomes current
+    if (launch_est_sums(s, b->dp.cov, true, false) != 0) return -1;
     if (launch_est_covs(s, ParamRef<CovParams>::array(b->dp.cov)) != 0) return -1;
     PB_CHECK(hipMemcpyAsync(b->h_fdev, s->fdev, sizeof(FilterDev) * (size_t)b->nf, hipMemcpyDeviceToHost, s->stream));
     return 0;
@@ -238,10 +240,10 @@ int stages_staged(Batch* b, const ParamBlock& dp, FilterDev* results) {
     PfState* s = &b->s;
     if (launch_motion(s, ParamRef<MotionParams>::array(dp.motion), 0) != 0) return -1;
     if (launch_weight(s, ParamRef<WeightParams>::array(dp.weight)) != 0) return -1;
-    if (launch_normalize(s, 0, b->d_ratio) != 0) return -1;
-    if (launch_resample(s, ParamRef<InjectParams>::array(dp.inject), s->d_inj_blk, s->inj_nblk, false) != 0)
+    if (launch_normalize(s, 0, b->d_ratio, true) != 0) return -1;
+    if (launch_resample(s, ParamRef<InjectParams>::array(dp.inject), s->d_inj_blk, s->inj_nblk, false, false) != 0)
         return -1;
-    if (launch_est_sums(s, dp.cov, true) != 0) return -1;
+    if (launch_est_sums(s, dp.cov, true, false) != 0) return -1;
     return launch_est_covs(s, ParamRef<CovParams>::array(dp.cov), results);
 }
 

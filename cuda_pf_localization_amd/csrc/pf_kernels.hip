@@ -476,38 +476,56 @@ __global__ __launch_bounds__(kFinTB) void k_weight_fin(Layout L, FilterDev* F, c
     }
 }
 
-// w = exp(ll - max) / sum (or 1/n when the weight pass was degenerate / had no observations),
-// plus block partials of sum w^2 over the stored f32 weights.
+// Stored weights of the 4 consecutive particles of this lane in 1024-particle tile `tile`:
+// w = exp(ll - max) / sum, or 1/n when uni (the weight pass was degenerate, had no
+// observations, or uniform weights are forced); 0 past the filter's end.  w2 receives the
+// lane's sum of squares of the stored f32 weights, in particle order.
+__device__ inline float4 tile_norm(const double* __restrict__ ll, int tile, const Layout& L, int f,
+                                   const FilterDev& fd, bool uni, double& w2) {
+    const int base = tile * kTile + (int)threadIdx.x * 4;
+    const double2 a = *reinterpret_cast<const double2*>(ll + base);
+    const double2 b = *reinterpret_cast<const double2*>(ll + base + 2);
+    const double v[4] = {a.x, a.y, b.x, b.y};
+    const int li0 = base - L.f_off[f], nf = L.f_n[f];
+    const float winv = L.f_winv[f];
+    float o[4];
+    w2 = 0.0;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        o[e] = 0.f;
+        if (li0 + e < nf) {
+            o[e] = uni ? winv : (float)(exp(v[e] - fd.max_ll) / fd.sum_e);
+            w2 += (double)o[e] * (double)o[e];
+        }
+    }
+    return make_float4(o[0], o[1], o[2], o[3]);
+}
+
+// Normalised weights of every 1024-particle tile plus its partial of sum w^2 (the same tile
+// function and block tree as the fused normalise-and-scan, so both give the same bits).
 __global__ __launch_bounds__(kTB) void k_normalize(Layout L, FilterDev* F, const double* __restrict__ ll,
                                                    float* __restrict__ w, double* __restrict__ part, int force_uniform) {
     __shared__ double sh[4];
-    const int blk = blockIdx.x;
-    const int gi = blk * kTB + (int)threadIdx.x;
-    const double llv = ll[gi];  // issued before the filter lookup (gi < NP always)
-    const int f = L.blk_filter[blk];
-    const int off = L.f_off[f];
-    const int nf = L.f_n[f];
-    const int li = gi - off;
+    const int tile = blockIdx.x;
+    const int f = L.blk_filter[tile * (kTile / kTB)];
     const FilterDev& fd = F[f];
-    const bool uni = force_uniform || fd.uniform;
-    double w2 = 0.0;
-    if (li < nf) {
-        const float wf = uni ? L.f_winv[f] : (float)(exp(llv - fd.max_ll) / fd.sum_e);
-        w[off + li] = wf;
-        w2 = (double)wf * (double)wf;
-    }
+    double w2;
+    const float4 wv = tile_norm(ll, tile, L, f, fd, force_uniform || fd.uniform, w2);
+    *reinterpret_cast<float4*>(w + tile * kTile + (int)threadIdx.x * 4) = wv;
     const double bs = block_sum(w2, sh);
-    if (threadIdx.x == 0) part[blk] = bs;
+    if (threadIdx.x == 0) part[tile] = bs;
 }
 
 // Per filter: ESS = 1 / sum w^2 of the stored weights -- the quantity the reference driver
 // tests (particle_filter.rs:115-117 via pfc_est_sums out[13]) -- and, when ratio is given,
-// the resample predicate ESS < ratio * n.
+// the resample predicate ESS < ratio * n.  With advance_lcg (after the fused normalise-and-
+// scan, which scanned every filter) a filter that resamples with a positive total advances its
+// LCG here (pf_kernels.cu:647-657), as the scan's last tile does otherwise.
 __global__ __launch_bounds__(kFinTB) void k_norm_fin(Layout L, FilterDev* F, const double* __restrict__ part,
-                                                     const float* __restrict__ ratio) {
+                                                     const float* __restrict__ ratio, int advance_lcg) {
     __shared__ double sh[kFinTB / 64];
     const int f = blockIdx.x;
-    const int b0 = L.f_off[f] / kTB, nb = L.f_padn[f] / kTB;
+    const int b0 = L.f_off[f] / kTile, nb = L.f_padn[f] / kTile;
     double v = 0.0;
     for (int b = (int)threadIdx.x; b < nb; b += kFinTB) v += part[b0 + b];
     const double tot = block16_sum(v, sh);
@@ -519,6 +537,11 @@ __global__ __launch_bounds__(kFinTB) void k_norm_fin(Layout L, FilterDev* F, con
         if (ratio) {
             const double r = (double)ratio[f];
             fd.do_resample = (r > 0.0 && fd.ess < r * (double)L.f_n[f]) ? 1 : 0;
+        }
+        if (advance_lcg && fd.do_resample && fd.total > 0) {
+            const uint64_t lcg = fd.lcg * 6364136223846793005ull + 1442695040888963407ull;  // cu:653
+            fd.lcg = lcg;
+            fd.u32 = (uint32_t)(lcg >> 32);
         }
     }
 }
@@ -608,22 +631,38 @@ __device__ inline uint64_t block_sum_u64(uint64_t v, uint64_t* s_w) {
     return s_w[0] + s_w[1] + s_w[2] + s_w[3];
 }
 
-__global__ __launch_bounds__(kTB) void k_scan(Layout L, FilterDev* F, const float* __restrict__ w,
+// kFromLL (fused cycle): the tile first normalises its log-likelihoods (tile_norm, the
+// weights and the sum-w^2 partial stored as k_normalize stores them) and every filter is scanned
+// whatever its resample predicate, which k_norm_fin forms afterwards together with the LCG
+// advance; an unpublished predecessor is then summed from its log-likelihoods.
+template <bool kFromLL>
+__global__ __launch_bounds__(kTB) void k_scan(Layout L, FilterDev* F, const double* __restrict__ ll,
+                                              float* __restrict__ w, double* __restrict__ part_n,
                                               uint64_t* __restrict__ cum, uint64_t* st, const unsigned long long* ctl,
                                               int spin_limit) {
     __shared__ uint64_t s_w[4];
+    __shared__ double s_n[4];
     __shared__ int s_dmin, s_nun;
     __shared__ uint64_t s_excl;
     __shared__ int s_un_tile[kLbWin];     // predecessors of a window found unpublished
     __shared__ uint64_t s_un_val[kLbWin];  // their aggregates, summed here
     const int tile = blockIdx.x;
-    const float4 wv = tile_w(w, tile);  // issued before the filter lookup
     const uint32_t epoch = (uint32_t)ctl[0] & 0x3fffffffu;
     const int f = L.blk_filter[tile * (kTile / kTB)];
     FilterDev& fd = F[f];
-    if (!fd.do_resample) {
-        if (threadIdx.x == 0) lb_publish(st, tile, epoch << 2, 0);
-        return;
+    float4 wv;
+    if (kFromLL) {
+        double w2;
+        wv = tile_norm(ll, tile, L, f, fd, fd.uniform != 0, w2);
+        *reinterpret_cast<float4*>(w + tile * kTile + (int)threadIdx.x * 4) = wv;
+        const double bs = block_sum(w2, s_n);
+        if (threadIdx.x == 0) part_n[tile] = bs;
+    } else {
+        wv = tile_w(w, tile);
+        if (!fd.do_resample) {
+            if (threadIdx.x == 0) lb_publish(st, tile, epoch << 2, 0);
+            return;
+        }
     }
     const int off = L.f_off[f], nf = L.f_n[f];
     const int t0 = off / kTile, nt = L.f_padn[f] / kTile;
@@ -685,7 +724,14 @@ __global__ __launch_bounds__(kTB) void k_scan(Layout L, FilterDev* F, const floa
                 for (int u = 0; u < nun; ++u) {
                     const int ut = s_un_tile[u];
                     uint64_t qq[4], inc2;
-                    tile_q(tile_w(w, ut), ut, off, nf, qq);
+                    float4 uw;
+                    if (kFromLL) {
+                        double w2u;
+                        uw = tile_norm(ll, ut, L, f, fd, fd.uniform != 0, w2u);
+                    } else {
+                        uw = tile_w(w, ut);
+                    }
+                    tile_q(uw, ut, off, nf, qq);
                     const uint64_t a = block_scan_u64(qq[0] + qq[1] + qq[2] + qq[3], inc2, s_w);
                     if (tid == 0) s_un_val[u] = a;
                 }
@@ -722,7 +768,7 @@ __global__ __launch_bounds__(kTB) void k_scan(Layout L, FilterDev* F, const floa
         if (tile == t0 + nt - 1) {
             const uint64_t T = excl + agg;
             fd.total = T;
-            if (T > 0) {
+            if (!kFromLL && T > 0) {
                 const uint64_t lcg = fd.lcg * 6364136223846793005ull + 1442695040888963407ull;  // cu:653
                 fd.lcg = lcg;
                 fd.u32 = (uint32_t)(lcg >> 32);
@@ -864,9 +910,11 @@ constexpr int kGatherLds = 2048;  // cumulative weights staged per block (16 KB)
 // lane searches it locally.  Poses move from the current set to the other set (no copies);
 // weights -> 1/n.  An output that injection picks target (k_inj_pick linked them into a list
 // at head[output]) adds their jitters in ascending pick order in f64 before its single f32
-// store, and resets its list head.  The block then leaves its estimate-sum partial of the
-// final poses and weights (est_src = 1), so the estimate needs no separate pass over them.
-// The other set becomes current afterwards (k_flip, or the sums finalize of the fused cycle).
+// store, and resets its list head.  Every block then leaves the estimate-sum partial of its
+// particles' final poses and weights (est_src = 1), also for a filter that does not resample
+// (its current poses and weights) or whose total is zero (weights set to 1/n), so the estimate
+// needs no separate pass.  The other set becomes current afterwards (k_flip, or the sums
+// finalize of the fused cycle).
 __global__ __launch_bounds__(kTB) void k_gather(Layout L, FilterDev* F, PoseBufs B, const uint64_t* __restrict__ cum,
                                                 float* __restrict__ w, int32_t* __restrict__ anc,
                                                 int32_t* __restrict__ head, const int32_t* __restrict__ next,
@@ -876,24 +924,39 @@ __global__ __launch_bounds__(kTB) void k_gather(Layout L, FilterDev* F, PoseBufs
     const int blk = blockIdx.x;
     const int f = L.blk_filter[blk];
     FilterDev& fd = F[f];
-    if (!fd.do_resample) return;
     const int off = L.f_off[f];
     const int nf = L.f_n[f];
     const int i0 = blk * kTB - off;
     const int li = i0 + (int)threadIdx.x;
     const uint64_t T = fd.total;
     const int par = fd.parity;
-    if (T == 0) {
-        if (li < nf) w[off + li] = L.f_winv[f];  // cu:647-650: degenerate total -> uniform weights only
+    const float winv = L.f_winv[f];
+    double acc[kEstN];
+#pragma unroll
+    for (int k = 0; k < kEstN; ++k) acc[k] = 0.0;
+    if (!fd.do_resample || T == 0) {
+        // no poses move; cu:647-650: a degenerate total only makes the weights uniform
+        if (li < nf) {
+            const PoseSet S = B.s[par];
+            float wi;
+            if (fd.do_resample) {
+                wi = winv;
+                w[off + li] = wi;
+            } else {
+                wi = w[off + li];
+            }
+            const double q[4] = {(double)S.c[3][off + li], (double)S.c[4][off + li], (double)S.c[5][off + li],
+                                 (double)S.c[6][off + li]};
+            est_terms((double)wi, (double)S.c[0][off + li], (double)S.c[1][off + li], (double)S.c[2][off + li], q,
+                      acc);
+        }
+        block_partials<kEstN>(acc, part_est, blk, (int)gridDim.x);
+        if (i0 == 0 && threadIdx.x == 0) fd.est_src = 1;
         return;
     }
     const uint64_t A = (uint64_t)nf << 32;
     const uint64_t U = (uint64_t)fd.u32;
     const uint64_t* c = cum + off;
-    const float winv = L.f_winv[f];
-    double acc[kEstN];
-#pragma unroll
-    for (int k = 0; k < kEstN; ++k) acc[k] = 0.0;
     if (i0 < nf) {
         const int i1 = min(i0 + kTB - 1, nf - 1);
         if (threadIdx.x < 64) {
@@ -995,9 +1058,8 @@ __global__ void k_flip(FilterDev* F, int nfilt, unsigned long long* scan_ctl) {
 
 // Per filter (one block each): one wave per value sums the filter's block partials in a fixed
 // strided order (four independent loads in flight per lane, added in that same order), then a
-// fixed butterfly.  which = 0 -> sums (partials per 256-particle block when the resampling
-// gather left them, est_src = 1, else per 1024-particle tile), 1 -> covariances (per tile);
-// value k of block b at part[k * stride + b].  With cov non-null (sums only) thread 0 then
+// fixed butterfly.  which = 0 -> sums (partials per 256-particle block), 1 -> covariances (per
+// 1024-particle tile); value k of block b at part[k * stride + b].  With cov non-null (sums only) thread 0 then
 // takes the Markley mean of the finished sums and writes the covariance pass parameters.  With
 // flip (sums of the fused cycle) the resampled set becomes current and the scan's status epoch
 // advances, as k_flip.  With out non-null (covariances of a staged cycle) the filter's scalars
@@ -1010,7 +1072,7 @@ __global__ __launch_bounds__(kFinTB) void k_est_fin(Layout L, FilterDev* F, cons
     static_assert(NV <= kFinTB / 64, "one wave per value");
     __shared__ double sv[NV];
     const int f = blockIdx.x;
-    const int span = (which == 0 && F[f].est_src) ? kTB : kTile;
+    const int span = which == 0 ? kTB : kTile;
     const int b0 = L.f_off[f] / span, nb = L.f_padn[f] / span;
     const int k = threadIdx.x >> 6, lane = threadIdx.x & 63;
     if (k < NV) {
@@ -1052,32 +1114,24 @@ __global__ __launch_bounds__(kFinTB) void k_est_fin(Layout L, FilterDev* F, cons
 
 // est layout (cu:354-404): [0..2] sum w p, [3..12] sum w q_r q_c (r<=c), [13] sum w^2.
 // Filters whose partials the resampling gather already left (est_src = 1) are skipped.
+// One particle per lane in 256-particle blocks: the terms and block tree of k_gather's partials,
+// so that either pass gives the same bits for the same poses and weights.
 __global__ __launch_bounds__(kTB) void k_est_sums(Layout L, FilterDev* F, PoseBufs B, const float* __restrict__ w,
                                                   double* __restrict__ part, int stride) {
-    const int blk = blockIdx.x;  // 1024-particle tile
-    const int f = L.blk_filter[blk * (kTile / kTB)];
+    const int blk = blockIdx.x;
+    const int f = L.blk_filter[blk];
     if (F[f].est_src) return;
     const int off = L.f_off[f];
     const int nf = L.f_n[f];
+    const int li = blk * kTB + (int)threadIdx.x - off;
     const PoseSet S = B.s[F[f].parity];
     double acc[kEstN];
 #pragma unroll
     for (int k = 0; k < kEstN; ++k) acc[k] = 0.0;
-    const int base = blk * kTile + (int)threadIdx.x * 4;
-    const int li0 = base - off;
-    if (li0 < nf) {
-        float4 v[8];
-        v[7] = *reinterpret_cast<const float4*>(w + base);
-#pragma unroll
-        for (int c = 0; c < 7; ++c) v[c] = *reinterpret_cast<const float4*>(S.c[c] + base);
-        const float* vv = reinterpret_cast<const float*>(v);
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            if (li0 + e < nf) {
-                const double q[4] = {(double)vv[12 + e], (double)vv[16 + e], (double)vv[20 + e], (double)vv[24 + e]};
-                est_terms((double)vv[28 + e], (double)vv[0 + e], (double)vv[4 + e], (double)vv[8 + e], q, acc);
-            }
-        }
+    if (li < nf) {
+        const int gi = off + li;
+        const double q[4] = {(double)S.c[3][gi], (double)S.c[4][gi], (double)S.c[5][gi], (double)S.c[6][gi]};
+        est_terms((double)w[gi], (double)S.c[0][gi], (double)S.c[1][gi], (double)S.c[2][gi], q, acc);
     }
     block_partials<kEstN>(acc, part, blk, stride);
 }
@@ -1325,26 +1379,37 @@ int launch_weight(PfState* s, ParamRef<WeightParams> A) {
     return 0;
 }
 
-int launch_normalize(PfState* s, int force_uniform, const float* d_ratio) {
-    prof_begin(s, PFX_STAGE_NORMALIZE);
-    hipLaunchKernelGGL(k_normalize, dim3(s->NP / kTB), dim3(kTB), 0, s->stream, s->L, s->fdev, s->ll, s->w,
-                       s->part_n, force_uniform);
+int launch_normalize(PfState* s, int force_uniform, const float* d_ratio, bool scan) {
+    const int ntile = s->NP / kTile;
+    if (scan) {
+        prof_begin(s, PFX_STAGE_NORMALIZE);
+        hipLaunchKernelGGL(k_scan<true>, dim3(ntile), dim3(kTB), 0, s->stream, s->L, s->fdev, s->ll, s->w, s->part_n,
+                           s->cum, s->scan_st, s->scan_ctl, s->scan_spin_limit);
+    } else {
+        prof_begin(s, PFX_STAGE_NORMALIZE);
+        hipLaunchKernelGGL(k_normalize, dim3(ntile), dim3(kTB), 0, s->stream, s->L, s->fdev, s->ll, s->w, s->part_n,
+                           force_uniform);
+    }
     prof_end(s);
     PF_LAUNCH_CHECK();
     prof_begin(s, PFX_STAGE_FINALIZE);
-    hipLaunchKernelGGL(k_norm_fin, dim3(s->nf), dim3(kFinTB), 0, s->stream, s->L, s->fdev, s->part_n, d_ratio);
+    hipLaunchKernelGGL(k_norm_fin, dim3(s->nf), dim3(kFinTB), 0, s->stream, s->L, s->fdev, s->part_n, d_ratio,
+                       scan ? 1 : 0);
     prof_end(s);
     PF_LAUNCH_CHECK();
     return 0;
 }
 
-int launch_resample(PfState* s, ParamRef<InjectParams> inj, const int32_t* inj_tab, int inj_nblk, bool flip) {
+int launch_resample(PfState* s, ParamRef<InjectParams> inj, const int32_t* inj_tab, int inj_nblk, bool flip,
+                    bool scan) {
     const int ntile = s->NP / kTile;
-    prof_begin(s, PFX_STAGE_RESAMPLE_SCAN);
-    hipLaunchKernelGGL(k_scan, dim3(ntile), dim3(kTB), 0, s->stream, s->L, s->fdev, s->w, s->cum, s->scan_st,
-                       s->scan_ctl, s->scan_spin_limit);
-    prof_end(s);
-    PF_LAUNCH_CHECK();
+    if (scan) {
+        prof_begin(s, PFX_STAGE_RESAMPLE_SCAN);
+        hipLaunchKernelGGL(k_scan<false>, dim3(ntile), dim3(kTB), 0, s->stream, s->L, s->fdev, s->ll, s->w, s->part_n,
+                           s->cum, s->scan_st, s->scan_ctl, s->scan_spin_limit);
+        prof_end(s);
+        PF_LAUNCH_CHECK();
+    }
     if (inj_nblk > 0) {
         prof_begin(s, PFX_STAGE_INJECT);
         hipLaunchKernelGGL(k_inj_pick, dim3(inj_nblk), dim3(kTB), 0, s->stream, s->L, s->fdev, inj, inj_tab,
@@ -1366,16 +1431,17 @@ int launch_resample(PfState* s, ParamRef<InjectParams> inj, const int32_t* inj_t
     return 0;
 }
 
-int launch_est_sums(PfState* s, CovParams* d_cov, bool flip) {
-    prof_begin(s, PFX_STAGE_EST_SUMS);
-    hipLaunchKernelGGL(k_est_sums, dim3(s->NP / kTile), dim3(kTB), 0, s->stream, s->L, s->fdev, s->pose, s->w,
-                       s->part_est, s->NP / kTB);
-    prof_end(s);
-    PF_LAUNCH_CHECK();
+int launch_est_sums(PfState* s, CovParams* d_cov, bool flip, bool pass) {
+    if (pass) {
+        prof_begin(s, PFX_STAGE_EST_SUMS);
+        hipLaunchKernelGGL(k_est_sums, dim3(s->NP / kTB), dim3(kTB), 0, s->stream, s->L, s->fdev, s->pose, s->w,
+                           s->part_est, s->NP / kTB);
+        prof_end(s);
+        PF_LAUNCH_CHECK();
+    }
     prof_begin(s, PFX_STAGE_FINALIZE);
     hipLaunchKernelGGL(k_est_fin<kEstN>, dim3(s->nf), dim3(kFinTB), 0, s->stream, s->L, s->fdev, s->part_est,
-                       s->NP / kTB, 0,
-                       d_cov, flip ? 1 : 0, s->scan_ctl, nullptr);
+                       s->NP / kTB, 0, d_cov, flip ? 1 : 0, s->scan_ctl, nullptr);
     prof_end(s);
     PF_LAUNCH_CHECK();
     return 0;
@@ -1389,8 +1455,7 @@ int launch_est_covs(PfState* s, ParamRef<CovParams> A, FilterDev* results) {
     PF_LAUNCH_CHECK();
     prof_begin(s, PFX_STAGE_FINALIZE);
     hipLaunchKernelGGL(k_est_fin<kCovN>, dim3(s->nf), dim3(kFinTB), 0, s->stream, s->L, s->fdev, s->part_cov,
-                       s->NP / kTB, 1,
-                       nullptr, 0, s->scan_ctl, results);
+                       s->NP / kTB, 1, nullptr, 0, s->scan_ctl, results);
     prof_end(s);
     PF_LAUNCH_CHECK();
     return 0;
