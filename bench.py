@@ -324,10 +324,11 @@ def run_pf_fused(n, sc, steps, warmup, steps_host):
     t0 = time.perf_counter()
     outs = bf.run_staged()
     ms = (time.perf_counter() - t0) * 1000.0 / steps
+    ms_dev = bf.staged_device_ms() / steps
     err = sum(math.dist(outs[i][0].pos[:], sc.steps[k0 + i].truth_pos) for i in range(steps)) / steps
     n_rs = sum(int(outs[i][0].resampled) for i in range(steps))
     bf.close()
-    return ms, ms_host, err, n_rs
+    return ms, ms_dev, ms_host, err, n_rs
 
 
 def cpu_baseline(sc, n, budget_s=12.0):
@@ -392,7 +393,7 @@ def main():
     n = args.particles
     steps_host = min(args.steps, 100)
     sc = config1_scenario(cycles=max(args.steps + args.warmup + steps_host, 50), seed=7)
-    ms_fused, ms_host, err_fused, n_rs_fused = run_pf_fused(n, sc, args.steps, args.warmup, steps_host)
+    ms_fused, ms_fused_dev, ms_host, err_fused, n_rs_fused = run_pf_fused(n, sc, args.steps, args.warmup, steps_host)
     ms_fast, err, n_rs, stages = run_pf(n, sc, args.steps, args.warmup, fast=True, profile=False)
     ms_exact, _, _, _ = run_pf(n, sc, min(args.steps, 50), args.warmup, fast=False, profile=False)
     _, _, _, stages = run_pf(n, sc, min(args.steps, 50), args.warmup, fast=True, profile=True)
@@ -424,6 +425,7 @@ def main():
                    "driver": "fused cycle, inputs resident in HBM (pfb_stage_cycles + pfb_run_staged: "
                              "one filter, cycles enqueued back to back, estimates read back at the end)"},
         "particles_per_s": round(n / (ms_fused * 1e-3), 1),
+        "ms_per_cycle_device": round(ms_fused_dev, 4),
         "ms_per_cycle_host_inputs": round(ms_host, 4),
         "ms_per_cycle_abi_driver": round(ms_fast, 4),
         "ms_per_cycle_exact_driver": round(ms_exact, 4),

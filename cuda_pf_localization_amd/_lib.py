@@ -137,6 +137,7 @@ KERNEL_SYMBOLS = [
     ("pfb_cycle", C.c_int, [vp, P(PfbCycleIn), P(PfbEstimate)]),
     ("pfb_stage_cycles", C.c_int, [vp, C.c_int, P(PfbCycleIn)]),
     ("pfb_run_staged", C.c_int, [vp, P(PfbEstimate)]),
+    ("pfb_staged_device_ms", C.c_int, [vp, P(f32)]),
     ("pfb_use_graph", C.c_int, [vp, C.c_int]),
     ("pfb_autotune", C.c_int, [vp, P(PfbCycleIn)]),
     ("pfb_get_state", C.c_int, [vp, C.c_int, P(f32), P(f32), P(f32)]),

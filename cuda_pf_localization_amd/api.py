@@ -734,6 +734,12 @@ class BatchedFilters:
         self._staged = 0
         return [out[c * self.nf:(c + 1) * self.nf] for c in range(k)]
 
+    def staged_device_ms(self) -> float:
+        """Device time of the last run_staged (first to last kernel, HIP events)."""
+        ms = C.c_float(0.0)
+        _check(self.L.pfb_staged_device_ms(self.h, C.byref(ms)), "pfb_staged_device_ms")
+        return float(ms.value)
+
     def use_graph(self, on: bool = True):
         _check(self.L.pfb_use_graph(self.h, 1 if on else 0), "pfb_use_graph")
 

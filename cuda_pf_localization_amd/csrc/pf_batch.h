@@ -173,8 +173,10 @@ void prof_end(PfState* s);
 int launch_motion(PfState* s, ParamRef<MotionParams> A, int reinit);
 int launch_weight(PfState* s, ParamRef<WeightParams> A);
 // Normalised weights, ESS and (with d_ratio) the resample predicate; with scan (fused cycle) the
-// normalisation is done inside the resampling scan of every filter, whose LCG then advances here.
-int launch_normalize(PfState* s, int force_uniform, const float* d_ratio, bool scan = false);
+// normalisation is done inside the resampling scan of every filter, which also draws the
+// injection picks inj of a possible resample, and whose LCG then advances here.
+int launch_normalize(PfState* s, int force_uniform, const float* d_ratio, bool scan = false,
+                     ParamRef<InjectParams> inj = ParamRef<InjectParams>::value({}));
 // Scan (unless launch_normalize already scanned), injection picks, gather (with the picks' jitter
 // and the estimate-sum partials of every filter); with flip the resampled set then becomes
 // current (else the fused cycle's sums finalize does it).

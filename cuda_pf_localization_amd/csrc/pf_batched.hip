@@ -73,6 +73,8 @@ struct Batch {
     char* h_staged = nullptr;
     FilterDev* d_results = nullptr;  // st_cycles x nf filter scalars after each cycle
     FilterDev* h_results = nullptr;
+    hipEvent_t st_ev[2] = {nullptr, nullptr};  // device time of the last staged run
+    float st_ms = 0.f;
 };
 
 ParamBlock carve(char* base, int nf) {
@@ -107,6 +109,8 @@ void free_batch(Batch* b) {
     (void)hipFree(b->d_params);  // parameter blocks and observations share one allocation
     (void)hipHostFree(b->h_params);
     if (b->ev_done) (void)hipEventDestroy(b->ev_done);
+    for (hipEvent_t e : b->st_ev)
+        if (e) (void)hipEventDestroy(e);
     (void)hipFree(b->d_ratio);
     (void)hipHostFree(b->h_fdev);
     for (auto& p : b->s.prof_pending) {
@@ -131,7 +135,8 @@ int stages_all(Batch* b) {
     PfState* s = &b->s;
     if (launch_motion(s, ParamRef<MotionParams>::array(b->dp.motion), 0) != 0) return -1;
     if (launch_weight(s, ParamRef<WeightParams>::array(b->dp.weight)) != 0) return -1;
-    if (launch_normalize(s, 0, b->d_ratio, true) != 0) return -1;  // + the scan of every filter
+    // + the scan of every filter and its injection picks
+    if (launch_normalize(s, 0, b->d_ratio, true, ParamRef<InjectParams>::array(b->dp.inject)) != 0) return -1;
     if (launch_resample(s, ParamRef<InjectParams>::array(b->dp.inject), s->d_inj_blk, s->inj_nblk, false, false) !=
         0)
         return -1;
@@ -240,7 +245,7 @@ int stages_staged(Batch* b, const ParamBlock& dp, FilterDev* results) {
     PfState* s = &b->s;
     if (launch_motion(s, ParamRef<MotionParams>::array(dp.motion), 0) != 0) return -1;
     if (launch_weight(s, ParamRef<WeightParams>::array(dp.weight)) != 0) return -1;
-    if (launch_normalize(s, 0, b->d_ratio, true) != 0) return -1;
+    if (launch_normalize(s, 0, b->d_ratio, true, ParamRef<InjectParams>::array(dp.inject)) != 0) return -1;
     if (launch_resample(s, ParamRef<InjectParams>::array(dp.inject), s->d_inj_blk, s->inj_nblk, false, false) != 0)
         return -1;
     if (launch_est_sums(s, dp.cov, true, false) != 0) return -1;
@@ -404,15 +409,27 @@ int pfb_run_staged(void* h, pfb_estimate* out) {
     // Stream launches, one cycle after another with no synchronisation: the host enqueues a
     // cycle far faster than the device runs it, so the device never waits.  (One graph holding
     // every cycle stalled for milliseconds between groups of cycles when replayed.)
+    for (hipEvent_t& e : b->st_ev)
+        if (!e) PB_CHECK(hipEventCreate(&e));
+    PB_CHECK(hipEventRecord(b->st_ev[0], s->stream));
     for (int k = 0; k < b->st_cycles; ++k)
         if (stages_staged(b, carve(b->d_staged + (size_t)k * b->stage_bytes, b->nf),
                           b->d_results + (size_t)k * b->nf) != 0)
             return -1;
+    PB_CHECK(hipEventRecord(b->st_ev[1], s->stream));
     const size_t nfd = (size_t)b->st_cycles * b->nf;
     PB_CHECK(hipMemcpyAsync(b->h_results, b->d_results, sizeof(FilterDev) * nfd, hipMemcpyDeviceToHost, s->stream));
     PB_CHECK(hipStreamSynchronize(s->stream));
+    PB_CHECK(hipEventElapsedTime(&b->st_ms, b->st_ev[0], b->st_ev[1]));
     if (out)
         for (size_t i = 0; i < nfd; ++i) fill_estimate(b->h_results[i], out[i]);
+    return 0;
+}
+
+int pfb_staged_device_ms(void* h, float* ms) {
+    Batch* b = static_cast<Batch*>(h);
+    if (!b || !ms) return -1;
+    *ms = b->st_ms;
     return 0;
 }
 

@@ -264,11 +264,59 @@ __device__ inline uint64_t qweight(float wf) {
     return (uint64_t)(w * 0x1.0p56);
 }
 
-// Dominant unit eigenvector of (M + M^T)/2 (4x4, row-major) by cyclic Jacobi rotations, sign
-// chosen so that w >= 0: the Markley quaternion mean (types.rs:128-144).  Host and device
-// run this same function; it uses only IEEE-exact operations (+ - * / sqrt), so both give
-// the same bits under -ffp-contract=off.  Returns -1 for a zero eigenvector.
-__host__ __device__ inline int quat_mean4(const double* M, double* q) {
+// Jacobi rotation (c, s) that zeroes a_pq of a symmetric matrix: tan(2 phi) = 2 a_pq / (a_qq -
+// a_pp), the smaller root; the identity when a_pq = 0.
+__host__ __device__ inline void jacobi_cs(double app, double aqq, double apq, double& c, double& s) {
+    if (apq == 0.0) {
+        c = 1.0;
+        s = 0.0;
+        return;
+    }
+    const double tau = (aqq - app) / (2.0 * apq);
+    const double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+    c = 1.0 / sqrt(1.0 + t * t);
+    s = t * c;
+}
+
+// A <- J^T A J and V <- V J for the rotation J of plane (p, r).
+template <int P, int R>
+__host__ __device__ inline void jacobi_apply(double (&a)[4][4], double (&v)[4][4], double c, double s) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const double kp = a[k][P], kr = a[k][R];
+        a[k][P] = c * kp - s * kr;
+        a[k][R] = s * kp + c * kr;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const double pk = a[P][k], rk = a[R][k];
+        a[P][k] = c * pk - s * rk;
+        a[R][k] = s * pk + c * rk;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const double kp = v[k][P], kr = v[k][R];
+        v[k][P] = c * kp - s * kr;
+        v[k][R] = s * kp + c * kr;
+    }
+}
+
+// One round of two rotations in disjoint planes (p1, r1) and (p2, r2): both angles are taken from
+// the same matrix (neither rotation changes the entries the other one's angle depends on), so
+// the two chains of divisions and square roots are independent.
+template <int P1, int R1, int P2, int R2>
+__host__ __device__ inline void jacobi_round(double (&a)[4][4], double (&v)[4][4]) {
+    double c1, s1, c2, s2;
+    jacobi_cs(a[P1][P1], a[R1][R1], a[P1][R1], c1, s1);
+    jacobi_cs(a[P2][P2], a[R2][R2], a[P2][R2], c2, s2);
+    jacobi_apply<P1, R1>(a, v, c1, s1);
+    jacobi_apply<P2, R2>(a, v, c2, s2);
+}
+
+// Dominant unit eigenvector of (M + M^T)/2 (4x4, row-major) by Jacobi sweeps in the parallel
+// ordering (three rounds of two disjoint rotations cover the six planes), sign chosen so that
+// w >= 0.  Returns -1 for a zero eigenvector.
+__host__ __device__ inline __attribute__((noinline)) int quat_mean4_jacobi(const double* M, double* q) {
     // Every loop below is unrolled so that all indices are compile-time constants and the
     // 4x4 arrays stay in registers on the device (one thread runs this per filter).
     double a[4][4], v[4][4];
@@ -288,35 +336,9 @@ __host__ __device__ inline int quat_mean4(const double* M, double* q) {
             for (int c = r + 1; c < 4; ++c) off += a[r][c] * a[r][c];
         }
         if (off <= 1e-32 * diag || off == 0.0) break;
-#pragma unroll
-        for (int p = 0; p < 3; ++p)
-#pragma unroll
-            for (int r = p + 1; r < 4; ++r) {
-                const double apr = a[p][r];
-                if (apr == 0.0) continue;
-                // rotation angle zeroing a[p][r]: tan(2 phi) = 2 a_pr / (a_rr - a_pp)
-                const double tau = (a[r][r] - a[p][p]) / (2.0 * apr);
-                const double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
-                const double c = 1.0 / sqrt(1.0 + t * t), s = t * c;
-#pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    const double kp = a[k][p], kr = a[k][r];
-                    a[k][p] = c * kp - s * kr;
-                    a[k][r] = s * kp + c * kr;
-                }
-#pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    const double pk = a[p][k], rk = a[r][k];
-                    a[p][k] = c * pk - s * rk;
-                    a[r][k] = s * pk + c * rk;
-                }
-#pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    const double kp = v[k][p], kr = v[k][r];
-                    v[k][p] = c * kp - s * kr;
-                    v[k][r] = s * kp + c * kr;
-                }
-            }
+        jacobi_round<0, 1, 2, 3>(a, v);
+        jacobi_round<0, 2, 1, 3>(a, v);
+        jacobi_round<0, 3, 1, 2>(a, v);
     }
     // column of the largest diagonal entry (first one on ties), picked by selects
     double amax = a[0][0];
@@ -327,6 +349,72 @@ __host__ __device__ inline int quat_mean4(const double* M, double* q) {
         amax = take ? a[i][i] : amax;
 #pragma unroll
         for (int r = 0; r < 4; ++r) vb[r] = take ? v[r][i] : vb[r];
+    }
+    double nrm = 0.0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) nrm += vb[r] * vb[r];
+    nrm = sqrt(nrm);
+    if (!(nrm > 0.0)) return -1;
+    const double sg = vb[0] < 0.0 ? -1.0 : 1.0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) q[r] = sg * vb[r] / nrm;
+    return 0;
+}
+
+// The Markley quaternion mean (types.rs:128-144): the dominant unit eigenvector of the moment
+// matrix S = (M + M^T)/2 (4x4, row-major, positive semi-definite), sign chosen so that w >= 0.
+// S is scaled to unit trace and squared repeatedly, P <- P^2 / tr(P^2): the eigenvalue ratios
+// square at every step, so P tends to the projector v v^T, and it has converged when tr(P^2) = 1
+// to rounding (a unit-trace PSD matrix with tr(P^2) = 1 has one non-zero eigenvalue); v is then
+// the column of the largest diagonal entry, normalised.  Only multiplications, additions, one
+// division per step and one square root: a short dependence chain for the single lane that runs
+// it on the device.  A spectrum whose two largest eigenvalues lie within ~1e-6 of each other does
+// not converge in 24 squarings and is left to the Jacobi sweeps.  Host and device run this same
+// function; it uses only IEEE-exact operations (+ - * / sqrt), so both give the same bits under
+// -ffp-contract=off.  Returns -1 for a zero eigenvector.
+__host__ __device__ __forceinline__ int quat_mean4(const double* M, double* q) {
+    double p[4][4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) p[r][c] = 0.5 * (M[4 * r + c] + M[4 * c + r]);
+    const double t = (p[0][0] + p[1][1]) + (p[2][2] + p[3][3]);
+    if (!(t > 0.0) || !(t < INFINITY)) return quat_mean4_jacobi(M, q);
+    const double it = 1.0 / t;
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) p[r][c] *= it;
+    bool done = false;
+    for (int k = 0; k < 24 && !done; ++k) {
+        double s2[4][4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+            for (int c = r; c < 4; ++c) {
+                const double x = (p[r][0] * p[0][c] + p[r][1] * p[1][c]) + (p[r][2] * p[2][c] + p[r][3] * p[3][c]);
+                s2[r][c] = x;
+                s2[c][r] = x;
+            }
+        const double t2 = (s2[0][0] + s2[1][1]) + (s2[2][2] + s2[3][3]);
+        if (!(t2 > 0.0)) return quat_mean4_jacobi(M, q);
+        done = t2 >= 1.0 - 1e-15;
+        const double i2 = 1.0 / t2;
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) p[r][c] = s2[r][c] * i2;
+    }
+    if (!done) return quat_mean4_jacobi(M, q);
+    // column of the largest diagonal entry (first one on ties), picked by selects
+    double amax = p[0][0];
+    double vb[4] = {p[0][0], p[1][0], p[2][0], p[3][0]};
+#pragma unroll
+    for (int i = 1; i < 4; ++i) {
+        const bool take = p[i][i] > amax;
+        amax = take ? p[i][i] : amax;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) vb[r] = take ? p[r][i] : vb[r];
     }
     double nrm = 0.0;
 #pragma unroll

@@ -422,7 +422,7 @@ __global__ __launch_bounds__(kTB) void k_weight(Layout L, PoseBufs B, FilterDev*
 // function, same IEEE operation sequence), and the covariance pass parameters with the mean
 // handed over at f32 precision (cuda.rs:291-294).  A degenerate moment matrix yields the
 // identity and valid = 0.
-__device__ inline void markley_mean(FilterDev& fd, const double* sums, CovParams& cov) {
+__device__ __forceinline__ void markley_mean(FilterDev& fd, const double* sums, CovParams& cov) {
     double M[16];
     int e = 3;
 #pragma unroll
@@ -452,17 +452,38 @@ __global__ __launch_bounds__(kFinTB) void k_weight_fin(Layout L, FilterDev* F, c
     __shared__ double sh[kFinTB / 64];
     const int f = blockIdx.x;
     const int b0 = L.f_off[f] / span, nb = L.f_padn[f] / span;
+    const int t = (int)threadIdx.x;
+    // lane t takes blocks t, t + 1024, ... in that order; four loads in flight per pass
     double m = -INFINITY;
-    for (int b = (int)threadIdx.x; b < nb; b += kFinTB) m = fmax(m, part[3 * (b0 + b)]);
+    for (int c = 0; c < nb; c += 4 * kFinTB) {
+        double x[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int b = c + t + u * kFinTB;
+            x[u] = b < nb ? part[3 * (b0 + b)] : -INFINITY;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) m = fmax(m, x[u]);
+    }
     const double M = block16_max(m, sh);
     double s = 0.0, s2 = 0.0;
-    for (int b = (int)threadIdx.x; b < nb; b += kFinTB) {
-        const double mbb = part[3 * (b0 + b)];
-        if (mbb > -INFINITY) {
-            const double sc = exp(mbb - M);
-            s += part[3 * (b0 + b) + 1] * sc;
-            s2 += part[3 * (b0 + b) + 2] * (sc * sc);
+    for (int c = 0; c < nb; c += 4 * kFinTB) {
+        double mb[4], e1[4], e2[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int b = c + t + u * kFinTB;
+            const bool in = b < nb;
+            mb[u] = in ? part[3 * (b0 + b)] : -INFINITY;
+            e1[u] = in ? part[3 * (b0 + b) + 1] : 0.0;
+            e2[u] = in ? part[3 * (b0 + b) + 2] : 0.0;
         }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (mb[u] > -INFINITY) {
+                const double sc = exp(mb[u] - M);
+                s += e1[u] * sc;
+                s2 += e2[u] * (sc * sc);
+            }
     }
     const double Ssum = block16_sum(s, sh);
     const double S2sum = block16_sum(s2, sh);
@@ -577,6 +598,24 @@ __device__ inline float4 tile_w(const float* __restrict__ w, int tile) {
     return *reinterpret_cast<const float4*>(w + tile * kTile + (int)threadIdx.x * 4);
 }
 
+// Injection pick j of a filter (cu:333-352 formula) drawn from the LCG state lcg of this resample:
+// its target and f64 jitter; the pick pushes itself onto its target's list (head[] is -1
+// everywhere at rest: allocation sets it, k_gather resets every head of its filter).
+__device__ inline void inj_pick(int off, int nf, uint64_t lcg, uint64_t gen, int j, double* __restrict__ jit,
+                                int32_t* __restrict__ head, int32_t* __restrict__ next) {
+    uint64_t z = lcg ^ (gen * kPhi) ^ (((uint64_t)j + 1) * kC1);
+    if (z == 0) z = 0x243F6A8885A308D3ull;
+    int t = (int)(u01_co(sm_next(z)) * (double)nf);
+    if (t >= nf) t = nf - 1;
+    const double jx = u01_co(sm_next(z)) - 0.5;
+    const double jy = u01_co(sm_next(z)) - 0.5;
+    const double jz = u01_co(sm_next(z)) - 0.5;
+    jit[3 * (off + j)] = jx;
+    jit[3 * (off + j) + 1] = jy;
+    jit[3 * (off + j) + 2] = jz;
+    next[off + j] = atomicExch(&head[off + t], j);
+}
+
 // Single-pass scan of the fixed-point weights with decoupled look-back (integer sums: any
 // association gives the same bits).
 //
@@ -639,7 +678,8 @@ template <bool kFromLL>
 __global__ __launch_bounds__(kTB) void k_scan(Layout L, FilterDev* F, const double* __restrict__ ll,
                                               float* __restrict__ w, double* __restrict__ part_n,
                                               uint64_t* __restrict__ cum, uint64_t* st, const unsigned long long* ctl,
-                                              int spin_limit) {
+                                              int spin_limit, ParamRef<InjectParams> inj, double* __restrict__ jit,
+                                              int32_t* __restrict__ head, int32_t* __restrict__ next) {
     __shared__ uint64_t s_w[4];
     __shared__ double s_n[4];
     __shared__ int s_dmin, s_nun;
@@ -666,6 +706,17 @@ __global__ __launch_bounds__(kTB) void k_scan(Layout L, FilterDev* F, const doub
     }
     const int off = L.f_off[f], nf = L.f_n[f];
     const int t0 = off / kTile, nt = L.f_padn[f] / kTile;
+    if (kFromLL) {
+        // this tile's share of the injection picks, drawn before the resample predicate is known
+        // with the LCG value its advance will give (k_norm_fin advances fd.lcg only afterwards)
+        const InjectParams ip = inj.get(f);
+        if (ip.k > 0) {
+            const int per = (ip.k + nt - 1) / nt;
+            const int j0 = (tile - t0) * per, j1 = min(ip.k, j0 + per);
+            const uint64_t lcg1 = fd.lcg * 6364136223846793005ull + 1442695040888963407ull;  // cu:653
+            for (int j = j0 + (int)threadIdx.x; j < j1; j += kTB) inj_pick(off, nf, lcg1, ip.gen, j, jit, head, next);
+        }
+    }
     uint64_t q[4];
     tile_q(wv, tile, off, nf, q);
     q[1] += q[0];
@@ -845,17 +896,18 @@ __device__ inline double row_ror(double x) {
     return join64((unsigned)lo, (unsigned)hi);
 }
 
-// Block partial of NV values (4 waves, every lane active).  In each wave the values are folded
-// across the halves and then across row pairs (value i with value i + H, so no lane holds more
-// than H2 = ceil(NV / 4) of them), then summed within each row by DPP rotations; lane 16 r
-// then holds the wave sums of the values base_r .. base_r + H2 - 1.  A fixed order throughout,
-// and one LDS exchange across the waves.  Value k of block blk goes to part[k * stride + blk], so
-// that a finalize reading one value over consecutive blocks loads consecutive doubles.
+// Wave sums of NV values per lane (every lane active): the values are folded across the halves
+// and then across row pairs (value i with value i + H, so no lane holds more than
+// H2 = ceil(NV / 4) of them), then summed within each row by DPP rotations.  wave_fold_store then
+// has lane 16 r store the sums of the values base_r .. base_r + H2 - 1 of its row into row[].
 template <int NV>
-__device__ inline void block_partials(double (&acc)[NV], double* __restrict__ part, int blk, int stride) {
-    constexpr int H1 = (NV + 1) / 2, H2 = (H1 + 1) / 2;
-    __shared__ double shv[4][NV];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+struct Fold {
+    static constexpr int H1 = (NV + 1) / 2, H2 = (H1 + 1) / 2;
+};
+template <int NV>
+__device__ inline void wave_fold_store(const double (&acc)[NV], double* __restrict__ row) {
+    constexpr int H1 = Fold<NV>::H1, H2 = Fold<NV>::H2;
+    const int lane = threadIdx.x & 63;
     double s1[H1], s2[H2];
 #pragma unroll
     for (int i = 0; i < H1; ++i) s1[i] = fold32(acc[i], i + H1 < NV ? acc[i + H1] : 0.0);
@@ -874,8 +926,17 @@ __device__ inline void block_partials(double (&acc)[NV], double* __restrict__ pa
         const int end = r == 0 ? H2 : r == 1 ? H1 : r == 2 ? (H1 + H2 < NV ? H1 + H2 : NV) : NV;
 #pragma unroll
         for (int i = 0; i < H2; ++i)
-            if (base + i < end) shv[wv][base + i] = s2[i];
+            if (base + i < end) row[base + i] = s2[i];
     }
+}
+
+// Block partial of NV values (4 waves, every lane active): wave folds, then the four waves in a
+// fixed order.  Value k of block blk goes to part[k * stride + blk], so that a finalize reading
+// one value over consecutive blocks loads consecutive doubles.
+template <int NV>
+__device__ inline void block_partials(double (&acc)[NV], double* __restrict__ part, int blk, int stride) {
+    __shared__ double shv[4][NV];
+    wave_fold_store<NV>(acc, shv[threadIdx.x >> 6]);
     __syncthreads();
     if ((int)threadIdx.x < NV) {
         const int k = threadIdx.x;
@@ -935,8 +996,10 @@ __global__ __launch_bounds__(kTB) void k_gather(Layout L, FilterDev* F, PoseBufs
 #pragma unroll
     for (int k = 0; k < kEstN; ++k) acc[k] = 0.0;
     if (!fd.do_resample || T == 0) {
-        // no poses move; cu:647-650: a degenerate total only makes the weights uniform
+        // no poses move; cu:647-650: a degenerate total only makes the weights uniform.  Picks
+        // the fused scan drew in advance are dropped.
         if (li < nf) {
+            if (head[off + li] >= 0) head[off + li] = -1;
             const PoseSet S = B.s[par];
             float wi;
             if (fd.do_resample) {
@@ -1017,11 +1080,10 @@ __global__ __launch_bounds__(kTB) void k_gather(Layout L, FilterDev* F, PoseBufs
     if (i0 == 0 && threadIdx.x == 0) fd.est_src = 1;
 }
 
-// Injection picks (cu:333-352 formula): pick j draws its target and f64 jitter and pushes itself
-// onto its target's list (head[] is -1 everywhere at rest: allocation sets it, k_gather resets
-// the heads it consumes).  Runs over the particle-layout blocks that can hold picks: tab lists
-// them (blocks whose first local index is below the filter's largest pick count), or, when tab
-// is null, the grid is the first ceil(k / 256) blocks of a single filter.
+// Injection picks of a resample that the scan did not draw (reference ABI path), after the scan
+// advanced the LCG.  Runs over the particle-layout blocks that can hold picks: tab lists them
+// (blocks whose first local index is below the filter's largest pick count), or, when tab is
+// null, the grid is the first ceil(k / 256) blocks of a single filter.
 __global__ __launch_bounds__(kTB) void k_inj_pick(Layout L, FilterDev* F, ParamRef<InjectParams> A, const int32_t* __restrict__ tab,
                                                   double* __restrict__ jit, int32_t* __restrict__ head,
                                                   int32_t* __restrict__ next) {
@@ -1032,18 +1094,7 @@ __global__ __launch_bounds__(kTB) void k_inj_pick(Layout L, FilterDev* F, ParamR
     const int off = L.f_off[f];
     const int j = blk * kTB + (int)threadIdx.x - off;
     if (!fd.do_resample || fd.total == 0 || j >= p.k) return;
-    const int nf = L.f_n[f];
-    uint64_t z = fd.lcg ^ (p.gen * kPhi) ^ (((uint64_t)j + 1) * kC1);
-    if (z == 0) z = 0x243F6A8885A308D3ull;
-    int t = (int)(u01_co(sm_next(z)) * (double)nf);
-    if (t >= nf) t = nf - 1;
-    const double jx = u01_co(sm_next(z)) - 0.5;
-    const double jy = u01_co(sm_next(z)) - 0.5;
-    const double jz = u01_co(sm_next(z)) - 0.5;
-    jit[3 * (off + j)] = jx;
-    jit[3 * (off + j) + 1] = jy;
-    jit[3 * (off + j) + 2] = jz;
-    next[off + j] = atomicExch(&head[off + t], j);
+    inj_pick(off, L.f_n[f], fd.lcg, p.gen, j, jit, head, next);
 }
 
 // The gathered set becomes current for filters that resampled with a positive total; the
@@ -1056,49 +1107,58 @@ __global__ void k_flip(FilterDev* F, int nfilt, unsigned long long* scan_ctl) {
 
 // ---------------------------------------------------------------------------- estimate
 
-// Per filter (one block each): one wave per value sums the filter's block partials in a fixed
-// strided order (four independent loads in flight per lane, added in that same order), then a
-// fixed butterfly.  which = 0 -> sums (partials per 256-particle block), 1 -> covariances (per
-// 1024-particle tile); value k of block b at part[k * stride + b].  With cov non-null (sums only) thread 0 then
-// takes the Markley mean of the finished sums and writes the covariance pass parameters.  With
-// flip (sums of the fused cycle) the resampled set becomes current and the scan's status epoch
-// advances, as k_flip.  With out non-null (covariances of a staged cycle) the filter's scalars
-// are copied to out[f]: the other members from F[f] (final since earlier kernels), the new
-// values by the lanes that hold them.
+// Per filter (one 512-thread block each, so that the register budget holds the loads in flight
+// and the Markley mean without spills): lane t adds, for every value, the filter's block partials
+// t, t + 512, ... in that order (value k of block b at part[k * stride + b]; four blocks' worth of
+// loads in flight), then the 8 waves are folded and summed in a fixed order.  which = 0 -> sums
+// (partials per 256-particle block), 1 -> covariances (per 1024-particle tile).  With cov
+// non-null (sums only) thread 0 then takes the Markley mean of the finished sums and writes the
+// covariance pass parameters.  With flip (sums of the fused cycle) the resampled set becomes
+// current and the scan's status epoch advances, as k_flip.  With out non-null (covariances of a
+// staged cycle) the filter's scalars are copied to out[f]: the other members from F[f] (final
+// since earlier kernels), the new values by the lanes that hold them.
+constexpr int kEstFinTB = 512;
 template <int NV>
-__global__ __launch_bounds__(kFinTB) void k_est_fin(Layout L, FilterDev* F, const double* __restrict__ part,
-                                                    int stride, int which, CovParams* cov, int flip,
-                                                    unsigned long long* scan_ctl, FilterDev* out) {
-    static_assert(NV <= kFinTB / 64, "one wave per value");
+__global__ __launch_bounds__(kEstFinTB) void k_est_fin(Layout L, FilterDev* F, const double* __restrict__ part,
+                                                       int stride, int which, CovParams* cov, int flip,
+                                                       unsigned long long* scan_ctl, FilterDev* out) {
+    __shared__ double shv[kEstFinTB / 64][NV];
     __shared__ double sv[NV];
     const int f = blockIdx.x;
     const int span = which == 0 ? kTB : kTile;
     const int b0 = L.f_off[f] / span, nb = L.f_padn[f] / span;
-    const int k = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    if (k < NV) {
-        const double* pk = part + (size_t)k * stride + b0;
-        double v = 0.0;
-        int b = lane;
-        for (; b + 192 < nb; b += 256) {
-            const double x0 = pk[b], x1 = pk[b + 64], x2 = pk[b + 128], x3 = pk[b + 192];
-            v += x0;
-            v += x1;
-            v += x2;
-            v += x3;
+    const int t = (int)threadIdx.x;
+    double acc[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) acc[k] = 0.0;
+    const double* pf = part + b0;
+    for (int c = 0; c < nb; c += 4 * kEstFinTB) {
+        double x[4][NV];  // a missing block reads as 0, which leaves the sums unchanged
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int b = c + t + u * kEstFinTB;
+#pragma unroll
+            for (int k = 0; k < NV; ++k) x[u][k] = b < nb ? pf[(size_t)k * stride + b] : 0.0;
         }
-        for (; b < nb; b += 64) v += pk[b];
-        v = wave_sum(v);
-        if (lane == 0) {
-            (which == 0 ? F[f].sums : F[f].covs)[k] = v;
-            if (out) (which == 0 ? out[f].sums : out[f].covs)[k] = v;
-            sv[k] = v;
-        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int k = 0; k < NV; ++k) acc[k] += x[u][k];
+    }
+    wave_fold_store<NV>(acc, shv[t >> 6]);
+    __syncthreads();
+    if (t < NV) {
+        double v = shv[0][t];
+#pragma unroll
+        for (int wv = 1; wv < kEstFinTB / 64; ++wv) v += shv[wv][t];
+        (which == 0 ? F[f].sums : F[f].covs)[t] = v;
+        if (out) (which == 0 ? out[f].sums : out[f].covs)[t] = v;
+        sv[t] = v;
     }
     if (out) {
         static_assert(sizeof(FilterDev) % sizeof(uint64_t) == 0, "FilterDev copied as 64-bit words");
         constexpr int kWords = (int)(sizeof(FilterDev) / sizeof(uint64_t));
         const int v0 = (int)((which == 0 ? offsetof(FilterDev, sums) : offsetof(FilterDev, covs)) / sizeof(uint64_t));
-        const int t = (int)threadIdx.x;
         if (t < kWords && (t < v0 || t >= v0 + NV))
             reinterpret_cast<uint64_t*>(out + f)[t] = reinterpret_cast<const uint64_t*>(F + f)[t];
     }
@@ -1185,8 +1245,27 @@ __global__ __launch_bounds__(kTB) void k_est_covs(Layout L, FilterDev* F, PoseBu
                 const double hv2 = d.x * d.x + d.y * d.y + d.z * d.z;
                 double aav[3] = {0.0, 0.0, 0.0};
                 if (hv2 > 1e-24) {
-                    const double hv = sqrt(hv2);
-                    const double ratio = 2.0 * atan2(hv, d.w) / hv;
+                    // angle / |v| = 2 atan2(|v|, w) / |v|; for a half-angle tangent z = |v| / w <= 1/8
+                    // that is (2 / w) atan(z) / z = (2 / w) sum_k (-1)^k z^2k / (2k + 1), nine terms
+                    // (the first omitted one is below 3e-18), with no square root or arctangent
+                    double ratio;
+                    if (d.w > 0.0 && hv2 <= 0.015625 * (d.w * d.w)) {
+                        const double rw = 1.0 / d.w;
+                        const double z2 = hv2 * (rw * rw);
+                        double sr = 1.0 / 17.0;
+                        sr = __fma_rn(sr, -z2, 1.0 / 15.0);
+                        sr = __fma_rn(sr, -z2, 1.0 / 13.0);
+                        sr = __fma_rn(sr, -z2, 1.0 / 11.0);
+                        sr = __fma_rn(sr, -z2, 1.0 / 9.0);
+                        sr = __fma_rn(sr, -z2, 1.0 / 7.0);
+                        sr = __fma_rn(sr, -z2, 1.0 / 5.0);
+                        sr = __fma_rn(sr, -z2, 1.0 / 3.0);
+                        sr = __fma_rn(sr, -z2, 1.0);
+                        ratio = 2.0 * rw * sr;
+                    } else {
+                        const double hv = sqrt(hv2);
+                        ratio = 2.0 * atan2(hv, d.w) / hv;
+                    }
                     aav[0] = d.x * ratio;
                     aav[1] = d.y * ratio;
                     aav[2] = d.z * ratio;
@@ -1379,12 +1458,13 @@ int launch_weight(PfState* s, ParamRef<WeightParams> A) {
     return 0;
 }
 
-int launch_normalize(PfState* s, int force_uniform, const float* d_ratio, bool scan) {
+int launch_normalize(PfState* s, int force_uniform, const float* d_ratio, bool scan, ParamRef<InjectParams> inj) {
     const int ntile = s->NP / kTile;
     if (scan) {
         prof_begin(s, PFX_STAGE_NORMALIZE);
         hipLaunchKernelGGL(k_scan<true>, dim3(ntile), dim3(kTB), 0, s->stream, s->L, s->fdev, s->ll, s->w, s->part_n,
-                           s->cum, s->scan_st, s->scan_ctl, s->scan_spin_limit);
+                           s->cum, s->scan_st, s->scan_ctl, s->scan_spin_limit, inj, s->inj_jit, s->inj_head,
+                           s->inj_next);
     } else {
         prof_begin(s, PFX_STAGE_NORMALIZE);
         hipLaunchKernelGGL(k_normalize, dim3(ntile), dim3(kTB), 0, s->stream, s->L, s->fdev, s->ll, s->w, s->part_n,
@@ -1406,11 +1486,12 @@ int launch_resample(PfState* s, ParamRef<InjectParams> inj, const int32_t* inj_t
     if (scan) {
         prof_begin(s, PFX_STAGE_RESAMPLE_SCAN);
         hipLaunchKernelGGL(k_scan<false>, dim3(ntile), dim3(kTB), 0, s->stream, s->L, s->fdev, s->ll, s->w, s->part_n,
-                           s->cum, s->scan_st, s->scan_ctl, s->scan_spin_limit);
+                           s->cum, s->scan_st, s->scan_ctl, s->scan_spin_limit, ParamRef<InjectParams>::value({}),
+                           s->inj_jit, s->inj_head, s->inj_next);
         prof_end(s);
         PF_LAUNCH_CHECK();
     }
-    if (inj_nblk > 0) {
+    if (scan && inj_nblk > 0) {  // (the fused normalise-and-scan drew them otherwise)
         prof_begin(s, PFX_STAGE_INJECT);
         hipLaunchKernelGGL(k_inj_pick, dim3(inj_nblk), dim3(kTB), 0, s->stream, s->L, s->fdev, inj, inj_tab,
                            s->inj_jit, s->inj_head, s->inj_next);
@@ -1440,7 +1521,7 @@ int launch_est_sums(PfState* s, CovParams* d_cov, bool flip, bool pass) {
         PF_LAUNCH_CHECK();
     }
     prof_begin(s, PFX_STAGE_FINALIZE);
-    hipLaunchKernelGGL(k_est_fin<kEstN>, dim3(s->nf), dim3(kFinTB), 0, s->stream, s->L, s->fdev, s->part_est,
+    hipLaunchKernelGGL(k_est_fin<kEstN>, dim3(s->nf), dim3(kEstFinTB), 0, s->stream, s->L, s->fdev, s->part_est,
                        s->NP / kTB, 0, d_cov, flip ? 1 : 0, s->scan_ctl, nullptr);
     prof_end(s);
     PF_LAUNCH_CHECK();
@@ -1454,7 +1535,7 @@ int launch_est_covs(PfState* s, ParamRef<CovParams> A, FilterDev* results) {
     prof_end(s);
     PF_LAUNCH_CHECK();
     prof_begin(s, PFX_STAGE_FINALIZE);
-    hipLaunchKernelGGL(k_est_fin<kCovN>, dim3(s->nf), dim3(kFinTB), 0, s->stream, s->L, s->fdev, s->part_cov,
+    hipLaunchKernelGGL(k_est_fin<kCovN>, dim3(s->nf), dim3(kEstFinTB), 0, s->stream, s->L, s->fdev, s->part_cov,
                        s->NP / kTB, 1, nullptr, 0, s->scan_ctl, results);
     prof_end(s);
     PF_LAUNCH_CHECK();

@@ -61,6 +61,9 @@ int pfb_cycle(void *h, const pfb_cycle_in *in, pfb_estimate *out);
  * once. */
 int pfb_stage_cycles(void *h, int ncycles, const pfb_cycle_in *in);
 int pfb_run_staged(void *h, pfb_estimate *out);
+/* Device time (HIP events on the library stream) from the first to the last kernel of the last
+ * pfb_run_staged, in ms. */
+int pfb_staged_device_ms(void *h, float *ms);
 /* 1: replay the device stages from captured hipGraphs (captured on the next cycle). */
 int pfb_use_graph(void *h, int enable);
 /* Times the tunable launch shapes on the current state and keeps the fastest;
