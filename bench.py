@@ -37,9 +37,19 @@ FP64_VALU_PEAK_TFLOPS = 78.6
 # 1 - dot 2, acos^2 series 11, gated accumulate 2 -> 54 FLOP, plus one f32 rsqrt and two
 # conversions that are not counted.
 WEIGHT_FLOP_PER_PAIR = 54.0
-# Algorithmic HBM bytes per particle (SoA f32 pose 28 B, weight 4 B, log-likelihood 8 B).
+# Algorithmic HBM bytes per particle (SoA f32 pose 28 B, weight 4 B, log-likelihood 8 B,
+# fixed-point cumulative weight 8 B).  ABI driver stages:
 BYTES = {"predict": 56, "weight": 36, "normalize": 12, "resample_scan": 12, "resample_gather": 72,
          "est_sums": 32, "est_covs": 32}
+# Fused cycle stages: normalize = normalisation + scan (read ll, write w and cum), resample_gather
+# = gather + injection + estimate-sum partials (read cum window, pose, list head; write pose, w,
+# ancestor), finalize = the one-workgroup-per-filter reductions (no particle traffic).
+FUSED_BYTES = {"predict": 56, "weight": 36, "normalize": 20, "resample_gather": 76, "est_covs": 32,
+               "finalize": 0}
+# Cycle with resampling (predict 56, weight 32, softmax 8, estimate 64, resample 64) and the
+# likelihood's share of it.
+CYCLE_BYTES = 224
+BYTES_WEIGHT_ALG = 32
 
 
 PROFILE_PF = os.path.join(ROOT, "profiles", "pf_cycle_1M_particles_256_obs.txt")
@@ -327,8 +337,21 @@ def run_pf_fused(n, sc, steps, warmup, steps_host):
     ms_dev = bf.staged_device_ms() / steps
     err = sum(math.dist(outs[i][0].pos[:], sc.steps[k0 + i].truth_pos) for i in range(steps)) / steps
     n_rs = sum(int(outs[i][0].resampled) for i in range(steps))
+    # the same staged cycles again with HIP events around every stage (on the library stream):
+    # the per-stage breakdown of the fused cycle and the likelihood kernel's average duration
+    k1 = k0 + steps
+    n_prof = min(steps, len(inputs) - k1)
+    bf.stage_cycles([inputs[k1 + i][0] for i in range(n_prof)])
+    bf.profile(True)
+    bf.profile_read()
+    t0 = time.perf_counter()
+    bf.run_staged()
+    ms_prof = (time.perf_counter() - t0) * 1000.0 / n_prof
+    prof = bf.profile_read()
+    bf.profile(False)
     bf.close()
-    return ms, ms_dev, ms_host, err, n_rs
+    stages = {k: {"ms_total": v[0], "launches": v[1], "ms_avg": v[0] / v[1] if v[1] else 0.0} for k, v in prof.items()}
+    return ms, ms_dev, ms_host, err, n_rs, stages, ms_prof, n_prof
 
 
 def cpu_baseline(sc, n, budget_s=12.0):
@@ -392,21 +415,34 @@ def main():
     from cuda_pf_localization_amd.sim import config1_scenario
     n = args.particles
     steps_host = min(args.steps, 100)
-    sc = config1_scenario(cycles=max(args.steps + args.warmup + steps_host, 50), seed=7)
-    ms_fused, ms_fused_dev, ms_host, err_fused, n_rs_fused = run_pf_fused(n, sc, args.steps, args.warmup, steps_host)
+    steps_prof = min(args.steps, 50)
+    sc = config1_scenario(cycles=max(args.steps + args.warmup + steps_host + steps_prof, 50), seed=7)
+    (ms_fused, ms_fused_dev, ms_host, err_fused, n_rs_fused, fstages, ms_fprof,
+     n_fprof) = run_pf_fused(n, sc, args.steps, args.warmup, steps_host)
     ms_fast, err, n_rs, stages = run_pf(n, sc, args.steps, args.warmup, fast=True, profile=False)
     ms_exact, _, _, _ = run_pf(n, sc, min(args.steps, 50), args.warmup, fast=False, profile=False)
     _, _, _, stages = run_pf(n, sc, min(args.steps, 50), args.warmup, fast=True, profile=True)
-    w = stages["weight"]
+    # the likelihood kernel's duration from the profiled staged (fused, inputs-resident) run
+    w = fstages["weight"]
     pairs = n * 256
     achieved_tflops = WEIGHT_FLOP_PER_PAIR * pairs / (w["ms_avg"] * 1e-3) / 1e12 if w["ms_avg"] else None
-    stage_bw = {}
-    for s, b in BYTES.items():
-        st = stages.get(s)
-        if st and st["launches"]:
-            gbs = b * n / (st["ms_avg"] * 1e-3) / 1e9
-            stage_bw[s] = {"ms_avg": round(st["ms_avg"], 5), "GB/s": round(gbs, 1),
-                           "frac_hbm": round(gbs / HBM_PEAK_GBS, 3)}
+
+    def stage_table(st_ms, nbytes):
+        out = {}
+        for s_, b_ in nbytes.items():
+            st = st_ms.get(s_)
+            if st and st["launches"]:
+                gbs = b_ * n / (st["ms_avg"] * 1e-3) / 1e9 if b_ else None
+                out[s_] = {"ms_avg": round(st["ms_avg"], 5), "GB/s": round(gbs, 1) if gbs else None,
+                           "frac_hbm": round(gbs / HBM_PEAK_GBS, 3) if gbs else None}
+        return out
+    stage_bw = stage_table(fstages, FUSED_BYTES)
+    stage_bw_abi = stage_table(stages, BYTES)
+    # cycle roofline: every stage at the HBM peak on its algorithmic bytes, the likelihood at the
+    # FP64 peak on its FLOP (the larger of its two bounds)
+    t_roof_us = ((CYCLE_BYTES - BYTES_WEIGHT_ALG) * n / (HBM_PEAK_GBS * 1e9)
+                 + max(BYTES_WEIGHT_ALG * n / (HBM_PEAK_GBS * 1e9),
+                       WEIGHT_FLOP_PER_PAIR * pairs / (FP64_VALU_PEAK_TFLOPS * 1e12))) * 1e6
     line = {
         "metric": "PF ms/cycle @1M particles; GL time-to-0.5m-fix, learned vs plain dispersal",
         "value": round(ms_fused, 4),
@@ -433,7 +469,15 @@ def main():
         "mean_track_err_m_abi_driver": round(err, 4),
         "resample_cycles": int(n_rs_fused),
         "stages": stage_bw,
-        "roofline": {"kernel": "k_weight", "bound": "valu", "achieved": round(achieved_tflops, 2) if achieved_tflops
+        "stages_profiled_ms_per_cycle": round(ms_fprof, 4),
+        "stages_profiled_cycles": n_fprof,
+        "stages_abi_driver": stage_bw_abi,
+        "cycle_roofline": {"t_roof_us": round(t_roof_us, 1), "t_measured_us": round(ms_fused * 1e3, 1),
+                           "frac": round(t_roof_us / (ms_fused * 1e3), 3),
+                           "basis": f"{CYCLE_BYTES} B/particle at 8 TB/s, likelihood at the FP64 peak"},
+        "roofline": {"kernel": "k_weight", "bound": "mfma", "pipe": "FP64 VALU (the MI355X FP64 vector and matrix "
+                     "peaks are both 78.6 TFLOP/s)",
+                     "achieved": round(achieved_tflops, 2) if achieved_tflops
                      else None, "peak": FP64_VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved_tflops / FP64_VALU_PEAK_TFLOPS, 3) if achieved_tflops else None,
                      "traffic": profiled_traffic("k_weight"), "traffic_algorithmic": 36 * n,

@@ -41,7 +41,7 @@ struct FilterDev {
     double qmean[4];  // device Markley mean of the last estimate (wxyz, w >= 0)
     int qvalid;       // 0: degenerate moment matrix (qmean = identity)
     // 1: the last resampling gather left the estimate-sum partials of the current poses and
-    // weights (one per 256-particle block); cleared by every pass that changes poses or weights
+    // weights (one per 1024-particle tile); cleared by every pass that changes poses or weights
     int est_src;
 };
 

@@ -75,6 +75,7 @@ struct Batch {
     FilterDev* h_results = nullptr;
     hipEvent_t st_ev[2] = {nullptr, nullptr};  // device time of the last staged run
     float st_ms = 0.f;
+    bool prof_req = false;  // per-stage events requested (recorded in eager cycles and staged runs)
 };
 
 ParamBlock carve(char* base, int nf) {
@@ -406,6 +407,8 @@ int pfb_run_staged(void* h, pfb_estimate* out) {
     if (!b || !b->st_pending) return -1;
     b->st_pending = false;
     PfState* s = &b->s;
+    const bool prof_saved = s->prof_on;
+    s->prof_on = b->prof_req;
     // Stream launches, one cycle after another with no synchronisation: the host enqueues a
     // cycle far faster than the device runs it, so the device never waits.  (One graph holding
     // every cycle stalled for milliseconds between groups of cycles when replayed.)
@@ -417,6 +420,7 @@ int pfb_run_staged(void* h, pfb_estimate* out) {
                           b->d_results + (size_t)k * b->nf) != 0)
             return -1;
     PB_CHECK(hipEventRecord(b->st_ev[1], s->stream));
+    s->prof_on = prof_saved;
     const size_t nfd = (size_t)b->st_cycles * b->nf;
     PB_CHECK(hipMemcpyAsync(b->h_results, b->d_results, sizeof(FilterDev) * nfd, hipMemcpyDeviceToHost, s->stream));
     PB_CHECK(hipStreamSynchronize(s->stream));
@@ -438,7 +442,7 @@ int pfb_use_graph(void* h, int enable) {
     if (!b) return -1;
     b->use_graph = enable != 0;
     if (!b->use_graph) drop_graphs(b);
-    if (b->use_graph) b->s.prof_on = false;  // events are not recorded inside captured graphs
+    b->s.prof_on = b->prof_req && !b->use_graph;  // events are not recorded inside captured graphs
     return 0;
 }
 
@@ -514,7 +518,8 @@ int pfb_get_ancestors(void* h, int f, int32_t* anc) {
 int pfb_profile_enable(void* h, int enable) {
     Batch* b = static_cast<Batch*>(h);
     if (!b) return -1;
-    b->s.prof_on = enable != 0 && !b->use_graph;
+    b->prof_req = enable != 0;
+    b->s.prof_on = b->prof_req && !b->use_graph;
     return 0;
 }
 

@@ -249,12 +249,6 @@ __device__ inline float wave_sum_down(float v) {
     return v;
 }
 
-// Exact unsigned 64x64 -> 128 product compare: a*b < c*d.
-__device__ inline bool mul128_lt(uint64_t a, uint64_t b, uint64_t c, uint64_t d) {
-    const uint64_t h1 = __umul64hi(a, b), l1 = a * b;
-    const uint64_t h2 = __umul64hi(c, d), l2 = c * d;
-    return (h1 < h2) || (h1 == h2 && l1 < l2);
-}
 
 // Fixed-point systematic-resampling weight floor(w * 2^56), w clamped to [0, 64).
 __device__ inline uint64_t qweight(float wf) {

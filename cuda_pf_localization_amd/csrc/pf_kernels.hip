@@ -499,8 +499,10 @@ __global__ __launch_bounds__(kFinTB) void k_weight_fin(Layout L, FilterDev* F, c
 
 // Stored weights of the 4 consecutive particles of this lane in 1024-particle tile `tile`:
 // w = exp(ll - max) / sum, or 1/n when uni (the weight pass was degenerate, had no
-// observations, or uniform weights are forced); 0 past the filter's end.  w2 receives the
-// lane's sum of squares of the stored f32 weights, in particle order.
+// observations, or uniform weights are forced); 0 past the filter's end.  Below ll - max = -104
+// the f32 weight is 0 without evaluating exp: exp(-104) < 2^-150 and sum >= 1 (the maximum's own
+// term), so the quotient rounds to 0.  w2 receives the lane's sum of squares of the stored f32
+// weights, in particle order.
 __device__ inline float4 tile_norm(const double* __restrict__ ll, int tile, const Layout& L, int f,
                                    const FilterDev& fd, bool uni, double& w2) {
     const int base = tile * kTile + (int)threadIdx.x * 4;
@@ -515,7 +517,8 @@ __device__ inline float4 tile_norm(const double* __restrict__ ll, int tile, cons
     for (int e = 0; e < 4; ++e) {
         o[e] = 0.f;
         if (li0 + e < nf) {
-            o[e] = uni ? winv : (float)(exp(v[e] - fd.max_ll) / fd.sum_e);
+            const double x = v[e] - fd.max_ll;
+            o[e] = uni ? winv : (x < -104.0 ? 0.f : (float)(exp(x) / fd.sum_e));
             w2 += (double)o[e] * (double)o[e];
         }
     }
@@ -835,36 +838,55 @@ __global__ __launch_bounds__(kTB) void k_scan(Layout L, FilterDev* F, const doub
 }
 
 // Systematic pick: ancestor(i) = first j with cum_j * n 2^32 >= T (U32 + i 2^32) -- the
-// pointer t_i = (U + i) T / n of cpu.rs:276-285 compared exactly in 128 bits.
-__device__ inline int first_at_or_above(const uint64_t* c, int lo, int hi, uint64_t A, uint64_t T, uint64_t Bi) {
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (mul128_lt(c[mid], A, T, Bi))
-            lo = mid + 1;
-        else
-            hi = mid;
+// pointer t_i = (U + i) T / n of cpu.rs:276-285 compared exactly.  With K_i = ceil(T (U32 +
+// i 2^32) / 2^32) (computed once per output) the test cum_j n 2^32 < T (U32 + i 2^32) is
+// cum_j n < K_i: one 64 x 32-bit product per probe (at most 96 bits: cum < 2^62, n < 2^31), or
+// none when the window's cum_j n are staged.
+struct U128 {
+    uint64_t hi, lo;
+};
+__device__ inline bool lt128(const U128& a, const U128& b) { return a.hi < b.hi || (a.hi == b.hi && a.lo < b.lo); }
+__device__ inline U128 mul_n(uint64_t c, uint32_t n) {
+    const uint64_t a = (uint64_t)(uint32_t)c * n;  // low half of c times n
+    const uint64_t b = (c >> 32) * n;              // high half of c times n
+    U128 r;
+    r.lo = a + (b << 32);
+    r.hi = (b >> 32) + (r.lo < a ? 1u : 0u);
+    return r;
+}
+__device__ inline U128 pointer_key(uint64_t T, uint64_t Bi) {
+    const uint64_t plo = T * Bi, phi = __umul64hi(T, Bi);
+    U128 k;
+    k.lo = (phi << 32) | (plo >> 32);
+    k.hi = phi >> 32;
+    if (plo & 0xffffffffull) {
+        k.lo += 1;
+        if (k.lo == 0) k.hi += 1;
     }
-    return lo;
+    return k;
 }
 
-// The same search by one wave (result wave-uniform): a 64-ary search in which lane l samples
-// lo + l * step; the samples below the target form a prefix of the lanes, whose length bounds
-// the answer to one step.  About log64(hi - lo) rounds of loads instead of log2(hi - lo)
-// dependent ones.
-__device__ inline int wave_first_at_or_above(const uint64_t* c, int lo, int hi, uint64_t A, uint64_t T, uint64_t Bi) {
+// Merge-path split of the merged sequence of outputs (keys K_i, i < n) and particles (keys
+// cum_j n, j < n), particle j ahead of output i when cum_j n < K_i: the number of outputs among
+// its first d items.  By one wave (result wave-uniform), 64-ary: lane l tests output
+// i = lo + l * step against particle d - 1 - i; the outputs that still precede it form a prefix.
+__device__ inline int wave_merge_split(const uint64_t* c, uint32_t n, uint64_t T, uint64_t U, int d) {
     const int lane = threadIdx.x & 63;
+    int lo = max(0, d - (int)n), hi = min(d, (int)n);  // answer in [lo, hi]
+    // ahead(i): output i comes before particle d - 1 - i, i.e. K_i <= cum_{d-1-i} n (the split
+    // lies above i); false from some i on
     while (hi - lo > 64) {
         const int step = (hi - lo + 63) >> 6;
-        const int j = lo + lane * step;
-        const bool below = j < hi && mul128_lt(c[j], A, T, Bi);
-        const int k = __popcll(__ballot(below));
+        const int i = lo + lane * step;
+        const bool ahead = i < hi && !lt128(mul_n(c[d - 1 - i], n), pointer_key(T, U + ((uint64_t)i << 32)));
+        const int k = __popcll(__ballot(ahead));
         const int nlo = k > 0 ? lo + (k - 1) * step + 1 : lo;
         hi = min(lo + k * step, hi);
         lo = nlo;
     }
-    const int j = lo + lane;
-    const bool below = j < hi && mul128_lt(c[j], A, T, Bi);
-    return lo + __popcll(__ballot(below));
+    const int i = lo + lane;
+    const bool ahead = i < hi && !lt128(mul_n(c[d - 1 - i], n), pointer_key(T, U + ((uint64_t)i << 32)));
+    return lo + __popcll(__ballot(ahead));
 }
 
 // Cross-lane folds of 64-bit values on CDNA4 (one 32-bit permutation per half).  Rows are 16
@@ -964,120 +986,171 @@ __device__ inline void est_terms(double wi, double px, double py, double pz, con
     acc[13] = __fma_rn(wi, wi, acc[13]);
 }
 
-constexpr int kGatherLds = 2048;  // cumulative weights staged per block (16 KB)
+constexpr int kGatherBlk = kTile;              // a block per 1024-particle tile of the filter
+constexpr int kMergeItems = 2 * kGatherBlk;    // merge-path items (outputs + particles) per block
+constexpr int kMergeOut = kMergeItems / kTB;   // outputs per lane at most
 
-// Merge-path gather: each block owns 256 consecutive outputs; two global searches (one wave
-// each) bound their ancestors to [jlo, jhi], that slice of the scan is staged in LDS and every
-// lane searches it locally.  Poses move from the current set to the other set (no copies);
-// weights -> 1/n.  An output that injection picks target (k_inj_pick linked them into a list
-// at head[output]) adds their jitters in ascending pick order in f64 before its single f32
-// store, and resets its list head.  Every block then leaves the estimate-sum partial of its
-// particles' final poses and weights (est_src = 1), also for a filter that does not resample
-// (its current poses and weights) or whose total is zero (weights set to 1/n), so the estimate
-// needs no separate pass.  The other set becomes current afterwards (k_flip, or the sums
-// finalize of the fused cycle).
+// Jitter of the injection picks linked at head hd (an output's list), added in ascending pick
+// order (cu:333-352 adds them one after another) in f64 to the gathered position v[0..2], which
+// is then rounded once to f32.
+__device__ inline void apply_picks(int off, int hd, const int32_t* __restrict__ next, const double* __restrict__ jit,
+                                   float (&v)[7]) {
+    // ascending traversal of the (unordered) list: repeatedly take the smallest pick above the
+    // previous one
+    double ax = (double)v[0], ay = (double)v[1], az = (double)v[2];
+    int prev = -1;
+    while (true) {
+        int best = 0x7fffffff;
+        for (int q = hd; q >= 0; q = next[off + q])
+            if (q > prev && q < best) best = q;
+        if (best == 0x7fffffff) break;
+        ax += jit[3 * (off + best)];
+        ay += jit[3 * (off + best) + 1];
+        az += jit[3 * (off + best) + 2];
+        prev = best;
+    }
+    v[0] = (float)ax;
+    v[1] = (float)ay;
+    v[2] = (float)az;
+}
+
+// Merge-path gather.  The outputs (pointer keys K_i) and the particles (cum_j n) form one merged
+// sequence of 2n items, particle j ahead of output i when cum_j n < K_i, so that ancestor(i) is
+// the number of particles ahead of output i; block b of a filter takes items [2048 b, 2048 b +
+// 2048) of it (two one-wave searches find where its outputs and particles start and end), stages
+// its particles' cum_j n in LDS and finds each of its outputs' ancestors there.  Every block has
+// 2048 items whatever the weights (with a few hundred heavy particles the ancestors of 1024
+// consecutive outputs can span thousands of particles), and the grid is one block per tile.
+// Poses move from the current set to the other set (no copies); weights -> 1/n.  An output that
+// injection picks target (linked into a list at head[output]) adds their jitters (apply_picks).
+// Every block then leaves the estimate-sum partial of its outputs' final poses and weights
+// (est_src = 1); for a filter that does not resample (its current poses and weights) or whose
+// total is zero (weights set to 1/n) the block takes the tile's particles instead, so the
+// estimate needs no separate pass.  The other set becomes current afterwards (k_flip, or the
+// sums finalize of the fused cycle).
 __global__ __launch_bounds__(kTB) void k_gather(Layout L, FilterDev* F, PoseBufs B, const uint64_t* __restrict__ cum,
                                                 float* __restrict__ w, int32_t* __restrict__ anc,
                                                 int32_t* __restrict__ head, const int32_t* __restrict__ next,
-                                                const double* __restrict__ jit, double* __restrict__ part_est) {
+                                                const double* __restrict__ jit, double* __restrict__ part_est,
+                                                int stride) {
     __shared__ int s_lo, s_hi;
-    __shared__ uint64_t s_cum[kGatherLds];
+    __shared__ U128 s_cn[kMergeItems];
     const int blk = blockIdx.x;
-    const int f = L.blk_filter[blk];
+    const int f = L.blk_filter[blk * (kGatherBlk / kTB)];
     FilterDev& fd = F[f];
     const int off = L.f_off[f];
     const int nf = L.f_n[f];
-    const int i0 = blk * kTB - off;
-    const int li = i0 + (int)threadIdx.x;
+    const int i0 = blk * kGatherBlk - off;
+    const int t = (int)threadIdx.x;
     const uint64_t T = fd.total;
     const int par = fd.parity;
     const float winv = L.f_winv[f];
+    const PoseSet S = B.s[par];
     double acc[kEstN];
 #pragma unroll
     for (int k = 0; k < kEstN; ++k) acc[k] = 0.0;
     if (!fd.do_resample || T == 0) {
         // no poses move; cu:647-650: a degenerate total only makes the weights uniform.  Picks
         // the fused scan drew in advance are dropped.
-        if (li < nf) {
-            if (head[off + li] >= 0) head[off + li] = -1;
-            const PoseSet S = B.s[par];
-            float wi;
-            if (fd.do_resample) {
-                wi = winv;
-                w[off + li] = wi;
-            } else {
-                wi = w[off + li];
+#pragma unroll
+        for (int e = 0; e < kTile / kTB; ++e) {
+            const int li = i0 + t + e * kTB;
+            if (li < nf) {
+                const int gi = off + li;
+                if (head[gi] >= 0) head[gi] = -1;
+                float wi;
+                if (fd.do_resample) {
+                    wi = winv;
+                    w[gi] = wi;
+                } else {
+                    wi = w[gi];
+                }
+                const double q[4] = {(double)S.c[3][gi], (double)S.c[4][gi], (double)S.c[5][gi], (double)S.c[6][gi]};
+                est_terms((double)wi, (double)S.c[0][gi], (double)S.c[1][gi], (double)S.c[2][gi], q, acc);
             }
-            const double q[4] = {(double)S.c[3][off + li], (double)S.c[4][off + li], (double)S.c[5][off + li],
-                                 (double)S.c[6][off + li]};
-            est_terms((double)wi, (double)S.c[0][off + li], (double)S.c[1][off + li], (double)S.c[2][off + li], q,
-                      acc);
         }
-        block_partials<kEstN>(acc, part_est, blk, (int)gridDim.x);
-        if (i0 == 0 && threadIdx.x == 0) fd.est_src = 1;
+        block_partials<kEstN>(acc, part_est, blk, stride);
+        if (i0 == 0 && t == 0) fd.est_src = 1;
         return;
     }
-    const uint64_t A = (uint64_t)nf << 32;
+    const uint32_t n32 = (uint32_t)nf;
     const uint64_t U = (uint64_t)fd.u32;
     const uint64_t* c = cum + off;
-    if (i0 < nf) {
-        const int i1 = min(i0 + kTB - 1, nf - 1);
-        if (threadIdx.x < 64) {
-            const int r = wave_first_at_or_above(c, 0, nf - 1, A, T, U + ((uint64_t)i0 << 32));
-            if (threadIdx.x == 0) s_lo = r;
-        } else if (threadIdx.x < 128) {
-            const int r = wave_first_at_or_above(c, 0, nf - 1, A, T, U + ((uint64_t)i1 << 32));
-            if (threadIdx.x == 64) s_hi = r;
+    const int d0 = min((blk - off / kTile) * kMergeItems, 2 * nf), d1 = min(d0 + kMergeItems, 2 * nf);
+    if (d0 < d1) {
+        if (t < 64) {
+            const int r = wave_merge_split(c, n32, T, U, d0);
+            if (t == 0) s_lo = r;
+        } else if (t < 128) {
+            const int r = wave_merge_split(c, n32, T, U, d1);
+            if (t == 64) s_hi = r;
         }
         __syncthreads();
-        const int jlo = s_lo, jhi = s_hi;
-        const int span = jhi - jlo + 1;
-        const bool staged = span <= kGatherLds;
-        if (staged)
-            for (int t = (int)threadIdx.x; t < span; t += kTB) s_cum[t] = c[jlo + t];
+        const int ia = s_lo, ib = s_hi;          // outputs [ia, ib)
+        const int ja = d0 - ia, jb = d1 - ib;    // particles [ja, jb), at most kMergeItems of them
+        for (int u = t; u < jb - ja; u += kTB) s_cn[u] = mul_n(c[ja + u], n32);
         __syncthreads();
-        if (li < nf) {
-            const uint64_t Bi = U + ((uint64_t)li << 32);
-            int j;
-            if (staged)
-                j = jlo + first_at_or_above(s_cum, 0, span - 1, A, T, Bi);
-            else
-                j = first_at_or_above(c, jlo, jhi, A, T, Bi);
-            const PoseSet S = B.s[par], D = B.s[par ^ 1];
-            float v[7];
+        const PoseSet D = B.s[par ^ 1];
+        // this lane's outputs ia + t + 256 e (e < ne, block-uniform): searches in lock step
+        // (independent LDS chains, as many halvings as the block's particle count needs), then
+        // every pose load and list head in flight together, then the stores
+        const int np = jb - ja, ne = (ib - ia + kTB - 1) / kTB;
+        const int nsteps = 32 - __clz(np);  // halvings that close a range of np + 1 candidates
+        int jj[kMergeOut], hd[kMergeOut];
+        bool ok[kMergeOut];
+        {
+            U128 K[kMergeOut];
+            int lo[kMergeOut], hi[kMergeOut];
 #pragma unroll
-            for (int k = 0; k < 7; ++k) v[k] = S.c[k][off + j];
-            const int hd = head[off + li];
-            if (hd >= 0) {
-                // ascending traversal of the (unordered) pick list: repeatedly take the smallest
-                // pick above the previous one (cu:333-352 adds them one after another)
-                double ax = (double)v[0], ay = (double)v[1], az = (double)v[2];
-                int prev = -1;
-                while (true) {
-                    int best = 0x7fffffff;
-                    for (int q = hd; q >= 0; q = next[off + q])
-                        if (q > prev && q < best) best = q;
-                    if (best == 0x7fffffff) break;
-                    ax += jit[3 * (off + best)];
-                    ay += jit[3 * (off + best) + 1];
-                    az += jit[3 * (off + best) + 2];
-                    prev = best;
-                }
-                v[0] = (float)ax;
-                v[1] = (float)ay;
-                v[2] = (float)az;
+            for (int e = 0; e < kMergeOut; ++e) {
+                const int li = ia + t + e * kTB;
+                ok[e] = li < ib;
+                lo[e] = 0;
+                hi[e] = ok[e] ? np : 0;  // first staged cn >= K, or np (particle jb) when none
+                if (e < ne) K[e] = pointer_key(T, U + ((uint64_t)(ok[e] ? li : 0) << 32));
+            }
+            for (int step = 0; step < nsteps; ++step) {
+#pragma unroll
+                for (int e = 0; e < kMergeOut; ++e)
+                    if (e < ne && lo[e] < hi[e]) {
+                        const int mid = (lo[e] + hi[e]) >> 1;
+                        if (lt128(s_cn[mid], K[e]))
+                            lo[e] = mid + 1;
+                        else
+                            hi[e] = mid;
+                    }
+            }
+#pragma unroll
+            for (int e = 0; e < kMergeOut; ++e) jj[e] = ja + lo[e];
+        }
+        float v[kMergeOut][7];
+#pragma unroll
+        for (int e = 0; e < kMergeOut; ++e) {
+            if (e >= ne) break;
+            const int li = ia + t + e * kTB;
+            hd[e] = ok[e] ? head[off + li] : -1;
+#pragma unroll
+            for (int k = 0; k < 7; ++k) v[e][k] = ok[e] ? S.c[k][off + jj[e]] : 0.f;
+        }
+#pragma unroll
+        for (int e = 0; e < kMergeOut; ++e) {
+            if (e >= ne) break;
+            if (!ok[e]) continue;
+            const int li = ia + t + e * kTB;
+            if (hd[e] >= 0) {
+                apply_picks(off, hd[e], next, jit, v[e]);
                 head[off + li] = -1;
             }
 #pragma unroll
-            for (int k = 0; k < 7; ++k) D.c[k][off + li] = v[k];
-            anc[off + li] = j;
+            for (int k = 0; k < 7; ++k) D.c[k][off + li] = v[e][k];
+            anc[off + li] = jj[e];
             w[off + li] = winv;
-            const double q[4] = {(double)v[3], (double)v[4], (double)v[5], (double)v[6]};
-            est_terms((double)winv, (double)v[0], (double)v[1], (double)v[2], q, acc);
+            const double q[4] = {(double)v[e][3], (double)v[e][4], (double)v[e][5], (double)v[e][6]};
+            est_terms((double)winv, (double)v[e][0], (double)v[e][1], (double)v[e][2], q, acc);
         }
     }
-    block_partials<kEstN>(acc, part_est, blk, (int)gridDim.x);
-    if (i0 == 0 && threadIdx.x == 0) fd.est_src = 1;
+    block_partials<kEstN>(acc, part_est, blk, stride);
+    if (i0 == 0 && t == 0) fd.est_src = 1;
 }
 
 // Injection picks of a resample that the scan did not draw (reference ABI path), after the scan
@@ -1108,10 +1181,10 @@ __global__ void k_flip(FilterDev* F, int nfilt, unsigned long long* scan_ctl) {
 // ---------------------------------------------------------------------------- estimate
 
 // Per filter (one 512-thread block each, so that the register budget holds the loads in flight
-// and the Markley mean without spills): lane t adds, for every value, the filter's block partials
-// t, t + 512, ... in that order (value k of block b at part[k * stride + b]; four blocks' worth of
-// loads in flight), then the 8 waves are folded and summed in a fixed order.  which = 0 -> sums
-// (partials per 256-particle block), 1 -> covariances (per 1024-particle tile).  With cov
+// and the Markley mean without spills): lane t adds, for every value, the filter's 1024-particle
+// tile partials t, t + 512, ... in that order (value k of tile b at part[k * stride + b]; four
+// tiles' worth of loads in flight), then the 8 waves are folded and summed in a fixed order.
+// which = 0 -> sums, 1 -> covariances.  With cov
 // non-null (sums only) thread 0 then takes the Markley mean of the finished sums and writes the
 // covariance pass parameters.  With flip (sums of the fused cycle) the resampled set becomes
 // current and the scan's status epoch advances, as k_flip.  With out non-null (covariances of a
@@ -1125,8 +1198,7 @@ __global__ __launch_bounds__(kEstFinTB) void k_est_fin(Layout L, FilterDev* F, c
     __shared__ double shv[kEstFinTB / 64][NV];
     __shared__ double sv[NV];
     const int f = blockIdx.x;
-    const int span = which == 0 ? kTB : kTile;
-    const int b0 = L.f_off[f] / span, nb = L.f_padn[f] / span;
+    const int b0 = L.f_off[f] / kTile, nb = L.f_padn[f] / kTile;
     const int t = (int)threadIdx.x;
     double acc[NV];
 #pragma unroll
@@ -1174,24 +1246,29 @@ __global__ __launch_bounds__(kEstFinTB) void k_est_fin(Layout L, FilterDev* F, c
 
 // est layout (cu:354-404): [0..2] sum w p, [3..12] sum w q_r q_c (r<=c), [13] sum w^2.
 // Filters whose partials the resampling gather already left (est_src = 1) are skipped.
-// One particle per lane in 256-particle blocks: the terms and block tree of k_gather's partials,
-// so that either pass gives the same bits for the same poses and weights.
+// 1024-particle tiles, lane t taking particles t, t + 256, t + 512, t + 768 of the tile: the
+// terms, order and block tree of k_gather's partials, so that either pass gives the same bits
+// for the same poses and weights.
 __global__ __launch_bounds__(kTB) void k_est_sums(Layout L, FilterDev* F, PoseBufs B, const float* __restrict__ w,
                                                   double* __restrict__ part, int stride) {
     const int blk = blockIdx.x;
-    const int f = L.blk_filter[blk];
+    const int f = L.blk_filter[blk * (kTile / kTB)];
     if (F[f].est_src) return;
     const int off = L.f_off[f];
     const int nf = L.f_n[f];
-    const int li = blk * kTB + (int)threadIdx.x - off;
+    const int i0 = blk * kTile - off;
     const PoseSet S = B.s[F[f].parity];
     double acc[kEstN];
 #pragma unroll
     for (int k = 0; k < kEstN; ++k) acc[k] = 0.0;
-    if (li < nf) {
-        const int gi = off + li;
-        const double q[4] = {(double)S.c[3][gi], (double)S.c[4][gi], (double)S.c[5][gi], (double)S.c[6][gi]};
-        est_terms((double)w[gi], (double)S.c[0][gi], (double)S.c[1][gi], (double)S.c[2][gi], q, acc);
+#pragma unroll
+    for (int e = 0; e < kTile / kTB; ++e) {
+        const int li = i0 + (int)threadIdx.x + e * kTB;
+        if (li < nf) {
+            const int gi = off + li;
+            const double q[4] = {(double)S.c[3][gi], (double)S.c[4][gi], (double)S.c[5][gi], (double)S.c[6][gi]};
+            est_terms((double)w[gi], (double)S.c[0][gi], (double)S.c[1][gi], (double)S.c[2][gi], q, acc);
+        }
     }
     block_partials<kEstN>(acc, part, blk, stride);
 }
@@ -1338,8 +1415,8 @@ int state_alloc(PfState* s, const std::vector<int>& ns) {
     const int nb256 = NP / kTB, ntile = NP / kTile;
     PF_CHECK(hipMalloc((void**)&s->part_w, sizeof(double) * 3 * (size_t)nb256));
     PF_CHECK(hipMalloc((void**)&s->part_n, sizeof(double) * (size_t)nb256));
-    PF_CHECK(hipMalloc((void**)&s->part_est, sizeof(double) * kEstN * (size_t)nb256));  // per 256 (gather) or 1024
-    PF_CHECK(hipMalloc((void**)&s->part_cov, sizeof(double) * kCovN * (size_t)nb256));
+    PF_CHECK(hipMalloc((void**)&s->part_est, sizeof(double) * kEstN * (size_t)ntile));
+    PF_CHECK(hipMalloc((void**)&s->part_cov, sizeof(double) * kCovN * (size_t)ntile));
     PF_CHECK(hipMalloc((void**)&s->scan_st, sizeof(uint64_t) * 2 * (size_t)ntile));
     PF_CHECK(hipMemsetAsync(s->scan_st, 0, sizeof(uint64_t) * 2 * (size_t)ntile, s->stream));
     PF_CHECK(hipMalloc((void**)&s->scan_ctl, sizeof(unsigned long long)));
@@ -1499,8 +1576,8 @@ int launch_resample(PfState* s, ParamRef<InjectParams> inj, const int32_t* inj_t
         PF_LAUNCH_CHECK();
     }
     prof_begin(s, PFX_STAGE_RESAMPLE_GATHER);
-    hipLaunchKernelGGL(k_gather, dim3(s->NP / kTB), dim3(kTB), 0, s->stream, s->L, s->fdev, s->pose, s->cum, s->w,
-                       s->anc, s->inj_head, s->inj_next, s->inj_jit, s->part_est);
+    hipLaunchKernelGGL(k_gather, dim3(s->NP / kGatherBlk), dim3(kTB), 0, s->stream, s->L, s->fdev, s->pose, s->cum,
+                       s->w, s->anc, s->inj_head, s->inj_next, s->inj_jit, s->part_est, s->NP / kTile);
     prof_end(s);
     PF_LAUNCH_CHECK();
     if (flip) {
@@ -1515,14 +1592,14 @@ int launch_resample(PfState* s, ParamRef<InjectParams> inj, const int32_t* inj_t
 int launch_est_sums(PfState* s, CovParams* d_cov, bool flip, bool pass) {
     if (pass) {
         prof_begin(s, PFX_STAGE_EST_SUMS);
-        hipLaunchKernelGGL(k_est_sums, dim3(s->NP / kTB), dim3(kTB), 0, s->stream, s->L, s->fdev, s->pose, s->w,
-                           s->part_est, s->NP / kTB);
+        hipLaunchKernelGGL(k_est_sums, dim3(s->NP / kTile), dim3(kTB), 0, s->stream, s->L, s->fdev, s->pose, s->w,
+                           s->part_est, s->NP / kTile);
         prof_end(s);
         PF_LAUNCH_CHECK();
     }
     prof_begin(s, PFX_STAGE_FINALIZE);
     hipLaunchKernelGGL(k_est_fin<kEstN>, dim3(s->nf), dim3(kEstFinTB), 0, s->stream, s->L, s->fdev, s->part_est,
-                       s->NP / kTB, 0, d_cov, flip ? 1 : 0, s->scan_ctl, nullptr);
+                       s->NP / kTile, 0, d_cov, flip ? 1 : 0, s->scan_ctl, nullptr);
     prof_end(s);
     PF_LAUNCH_CHECK();
     return 0;
@@ -1531,12 +1608,12 @@ int launch_est_sums(PfState* s, CovParams* d_cov, bool flip, bool pass) {
 int launch_est_covs(PfState* s, ParamRef<CovParams> A, FilterDev* results) {
     prof_begin(s, PFX_STAGE_EST_COVS);
     hipLaunchKernelGGL(k_est_covs, dim3(s->NP / kTile), dim3(kTB), 0, s->stream, s->L, s->fdev, s->pose, s->w,
-                       s->part_cov, s->NP / kTB, A);
+                       s->part_cov, s->NP / kTile, A);
     prof_end(s);
     PF_LAUNCH_CHECK();
     prof_begin(s, PFX_STAGE_FINALIZE);
     hipLaunchKernelGGL(k_est_fin<kCovN>, dim3(s->nf), dim3(kEstFinTB), 0, s->stream, s->L, s->fdev, s->part_cov,
-                       s->NP / kTB, 1, nullptr, 0, s->scan_ctl, results);
+                       s->NP / kTile, 1, nullptr, 0, s->scan_ctl, results);
     prof_end(s);
     PF_LAUNCH_CHECK();
     return 0;

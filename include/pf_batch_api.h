@@ -72,6 +72,8 @@ int pfb_autotune(void *h, const pfb_cycle_in *in);
 /* Particles of filter f (pos 3n, quat 4n, weights n; any may be NULL). */
 int pfb_get_state(void *h, int f, float *pos3n, float *quat4n, float *w);
 int pfb_get_ancestors(void *h, int f, int32_t *anc);
+/* Per-stage HIP events (PFX_STAGE_*) around the kernels of eager cycles and of staged runs
+ * (not inside captured graphs); pfb_profile_read returns and resets the accumulated times. */
 int pfb_profile_enable(void *h, int enable);
 int pfb_profile_read(void *h, double *ms, int *count);
 
