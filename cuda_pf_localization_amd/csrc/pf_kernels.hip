@@ -709,17 +709,6 @@ __global__ __launch_bounds__(kTB) void k_scan(Layout L, FilterDev* F, const doub
     }
     const int off = L.f_off[f], nf = L.f_n[f];
     const int t0 = off / kTile, nt = L.f_padn[f] / kTile;
-    if (kFromLL) {
-        // this tile's share of the injection picks, drawn before the resample predicate is known
-        // with the LCG value its advance will give (k_norm_fin advances fd.lcg only afterwards)
-        const InjectParams ip = inj.get(f);
-        if (ip.k > 0) {
-            const int per = (ip.k + nt - 1) / nt;
-            const int j0 = (tile - t0) * per, j1 = min(ip.k, j0 + per);
-            const uint64_t lcg1 = fd.lcg * 6364136223846793005ull + 1442695040888963407ull;  // cu:653
-            for (int j = j0 + (int)threadIdx.x; j < j1; j += kTB) inj_pick(off, nf, lcg1, ip.gen, j, jit, head, next);
-        }
-    }
     uint64_t q[4];
     tile_q(wv, tile, off, nf, q);
     q[1] += q[0];
@@ -835,6 +824,18 @@ __global__ __launch_bounds__(kTB) void k_scan(Layout L, FilterDev* F, const doub
     cp[1] = pre + q[1];
     cp[2] = pre + q[2];
     cp[3] = pre + q[3];
+    if (kFromLL) {
+        // this tile's share of the injection picks, drawn before the resample predicate is known
+        // with the LCG value its advance will give (k_norm_fin advances fd.lcg only afterwards);
+        // after the tile's own prefix is published, off the successors' look-back path
+        const InjectParams ip = inj.get(f);
+        if (ip.k > 0) {
+            const int per = (ip.k + nt - 1) / nt;
+            const int j0 = (tile - t0) * per, j1 = min(ip.k, j0 + per);
+            const uint64_t lcg1 = fd.lcg * 6364136223846793005ull + 1442695040888963407ull;  // cu:653
+            for (int j = j0 + (int)threadIdx.x; j < j1; j += kTB) inj_pick(off, nf, lcg1, ip.gen, j, jit, head, next);
+        }
+    }
 }
 
 // Systematic pick: ancestor(i) = first j with cum_j * n 2^32 >= T (U32 + i 2^32) -- the
@@ -988,7 +989,6 @@ __device__ inline void est_terms(double wi, double px, double py, double pz, con
 
 constexpr int kGatherBlk = kTile;              // a block per 1024-particle tile of the filter
 constexpr int kMergeItems = 2 * kGatherBlk;    // merge-path items (outputs + particles) per block
-constexpr int kMergeOut = kMergeItems / kTB;   // outputs per lane at most
 
 // Jitter of the injection picks linked at head hd (an output's list), added in ascending pick
 // order (cu:333-352 adds them one after another) in f64 to the gathered position v[0..2], which
@@ -1091,62 +1091,112 @@ __global__ __launch_bounds__(kTB) void k_gather(Layout L, FilterDev* F, PoseBufs
         for (int u = t; u < jb - ja; u += kTB) s_cn[u] = mul_n(c[ja + u], n32);
         __syncthreads();
         const PoseSet D = B.s[par ^ 1];
-        // this lane's outputs ia + t + 256 e (e < ne, block-uniform): searches in lock step
-        // (independent LDS chains, as many halvings as the block's particle count needs), then
-        // every pose load and list head in flight together, then the stores
-        const int np = jb - ja, ne = (ib - ia + kTB - 1) / kTB;
-        const int nsteps = 32 - __clz(np);  // halvings that close a range of np + 1 candidates
-        int jj[kMergeOut], hd[kMergeOut];
-        bool ok[kMergeOut];
-        {
-            U128 K[kMergeOut];
-            int lo[kMergeOut], hi[kMergeOut];
+        const int np = jb - ja;
+        // Lane t takes groups of four consecutive outputs a0 + 4 g (g = t, t + 256, ...), a0 the
+        // multiple of 4 at or below ia, so that a full group stores 16 bytes per stream.  The
+        // group's first output is found by a binary search over the staged particles; every
+        // further output starts from the previous ancestor (ancestors do not decrease) and is
+        // usually the same particle, whose pose is then reused.  K_i = K_0 + i T (K_0 =
+        // ceil(T U / 2^32)), so no 64 x 64-bit product per output.
+        const U128 TU = mul_n(T, (uint32_t)U);
+        U128 K0;
+        K0.lo = (TU.hi << 32) | (TU.lo >> 32);
+        K0.hi = TU.hi >> 32;
+        if (TU.lo & 0xffffffffull) {
+            K0.lo += 1;
+            if (K0.lo == 0) K0.hi += 1;
+        }
+        const int a0 = ia & ~3;
+        const int ngroups = (ib - a0 + 3) >> 2;
+        for (int g = t; g < ngroups; g += kTB) {
+            const int o0 = a0 + 4 * g;
+            int jr[4];
+            bool ok[4];
+            float v[4][7];
+            int lo = 0;
 #pragma unroll
-            for (int e = 0; e < kMergeOut; ++e) {
-                const int li = ia + t + e * kTB;
-                ok[e] = li < ib;
-                lo[e] = 0;
-                hi[e] = ok[e] ? np : 0;  // first staged cn >= K, or np (particle jb) when none
-                if (e < ne) K[e] = pointer_key(T, U + ((uint64_t)(ok[e] ? li : 0) << 32));
-            }
-            for (int step = 0; step < nsteps; ++step) {
-#pragma unroll
-                for (int e = 0; e < kMergeOut; ++e)
-                    if (e < ne && lo[e] < hi[e]) {
-                        const int mid = (lo[e] + hi[e]) >> 1;
-                        if (lt128(s_cn[mid], K[e]))
-                            lo[e] = mid + 1;
+            for (int r = 0; r < 4; ++r) {
+                const int o = o0 + r;
+                ok[r] = o >= ia && o < ib;
+                jr[r] = -1;
+                if (!ok[r]) continue;
+                const U128 io = mul_n(T, (uint32_t)o);
+                U128 K;
+                K.lo = K0.lo + io.lo;
+                K.hi = K0.hi + io.hi + (K.lo < K0.lo ? 1u : 0u);
+                // first staged cn >= K in [lo, np] (np: the ancestor is particle jb)
+                if (lo < np && lt128(s_cn[lo], K)) {
+                    int hi = np;
+                    ++lo;
+                    while (lo < hi) {
+                        const int mid = (lo + hi) >> 1;
+                        if (lt128(s_cn[mid], K))
+                            lo = mid + 1;
                         else
-                            hi[e] = mid;
+                            hi = mid;
                     }
+                }
+                jr[r] = ja + lo;
+            }
+            // poses of the distinct ancestors, list heads
+            int hd[4];
+            const bool full = ok[0] && ok[3];
+            if (full) {
+                const int4 h4 = *reinterpret_cast<const int4*>(head + off + o0);
+                hd[0] = h4.x;
+                hd[1] = h4.y;
+                hd[2] = h4.z;
+                hd[3] = h4.w;
+            } else {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) hd[r] = ok[r] ? head[off + o0 + r] : -1;
             }
 #pragma unroll
-            for (int e = 0; e < kMergeOut; ++e) jj[e] = ja + lo[e];
-        }
-        float v[kMergeOut][7];
+            for (int r = 0; r < 4; ++r) {
+                if (!ok[r]) continue;
+                if (r > 0 && jr[r] == jr[r - 1]) {
 #pragma unroll
-        for (int e = 0; e < kMergeOut; ++e) {
-            if (e >= ne) break;
-            const int li = ia + t + e * kTB;
-            hd[e] = ok[e] ? head[off + li] : -1;
+                    for (int k = 0; k < 7; ++k) v[r][k] = v[r - 1][k];
+                } else {
 #pragma unroll
-            for (int k = 0; k < 7; ++k) v[e][k] = ok[e] ? S.c[k][off + jj[e]] : 0.f;
-        }
+                    for (int k = 0; k < 7; ++k) v[r][k] = S.c[k][off + jr[r]];
+                }
+            }
+            // injection, stores, estimate terms (in output order)
 #pragma unroll
-        for (int e = 0; e < kMergeOut; ++e) {
-            if (e >= ne) break;
-            if (!ok[e]) continue;
-            const int li = ia + t + e * kTB;
-            if (hd[e] >= 0) {
-                apply_picks(off, hd[e], next, jit, v[e]);
-                head[off + li] = -1;
+            for (int r = 0; r < 4; ++r)
+                if (ok[r] && hd[r] >= 0) {
+                    float pv[7];
+#pragma unroll
+                    for (int k = 0; k < 7; ++k) pv[k] = v[r][k];
+                    apply_picks(off, hd[r], next, jit, pv);
+                    v[r][0] = pv[0];
+                    v[r][1] = pv[1];
+                    v[r][2] = pv[2];
+                    head[off + o0 + r] = -1;
+                }
+            if (full) {
+#pragma unroll
+                for (int k = 0; k < 7; ++k)
+                    *reinterpret_cast<float4*>(D.c[k] + off + o0) = make_float4(v[0][k], v[1][k], v[2][k], v[3][k]);
+                *reinterpret_cast<int4*>(anc + off + o0) = make_int4(jr[0], jr[1], jr[2], jr[3]);
+                *reinterpret_cast<float4*>(w + off + o0) = make_float4(winv, winv, winv, winv);
+            } else {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    if (!ok[r]) continue;
+#pragma unroll
+                    for (int k = 0; k < 7; ++k) D.c[k][off + o0 + r] = v[r][k];
+                    anc[off + o0 + r] = jr[r];
+                    w[off + o0 + r] = winv;
+                }
             }
 #pragma unroll
-            for (int k = 0; k < 7; ++k) D.c[k][off + li] = v[e][k];
-            anc[off + li] = jj[e];
-            w[off + li] = winv;
-            const double q[4] = {(double)v[e][3], (double)v[e][4], (double)v[e][5], (double)v[e][6]};
-            est_terms((double)winv, (double)v[e][0], (double)v[e][1], (double)v[e][2], q, acc);
+            for (int r = 0; r < 4; ++r) {
+                if (!ok[r]) continue;
+                const double q[4] = {(double)v[r][3], (double)v[r][4], (double)v[r][5], (double)v[r][6]};
+                est_terms((double)winv, (double)v[r][0], (double)v[r][1], (double)v[r][2], q, acc);
+            }
         }
     }
     block_partials<kEstN>(acc, part_est, blk, stride);
