@@ -995,9 +995,18 @@ constexpr int kMergeItems = 2 * kGatherBlk;    // merge-path items (outputs + pa
 // is then rounded once to f32.
 __device__ inline void apply_picks(int off, int hd, const int32_t* __restrict__ next, const double* __restrict__ jit,
                                    float (&v)[7]) {
+    double ax = (double)v[0], ay = (double)v[1], az = (double)v[2];
+    // a single pick (almost every list): its link and its jitter read together, one round trip
+    const int nx = next[off + hd];
+    const double jx = jit[3 * (off + hd)], jy = jit[3 * (off + hd) + 1], jz = jit[3 * (off + hd) + 2];
+    if (nx < 0) {
+        v[0] = (float)(ax + jx);
+        v[1] = (float)(ay + jy);
+        v[2] = (float)(az + jz);
+        return;
+    }
     // ascending traversal of the (unordered) list: repeatedly take the smallest pick above the
     // previous one
-    double ax = (double)v[0], ay = (double)v[1], az = (double)v[2];
     int prev = -1;
     while (true) {
         int best = 0x7fffffff;
