@@ -1389,13 +1389,13 @@ __global__ __launch_bounds__(kTB) void k_est_covs(Layout L, FilterDev* F, PoseBu
                         const double rw = 1.0 / d.w;
                         const double z2 = hv2 * (rw * rw);
                         double sr = 1.0 / 17.0;
-                        sr = __fma_rn(sr, -z2, 1.0 / 15.0);
-                        sr = __fma_rn(sr, -z2, 1.0 / 13.0);
-                        sr = __fma_rn(sr, -z2, 1.0 / 11.0);
-                        sr = __fma_rn(sr, -z2, 1.0 / 9.0);
-                        sr = __fma_rn(sr, -z2, 1.0 / 7.0);
-                        sr = __fma_rn(sr, -z2, 1.0 / 5.0);
-                        sr = __fma_rn(sr, -z2, 1.0 / 3.0);
+                        sr = __fma_rn(sr, -z2, sconst(1.0 / 15.0));
+                        sr = __fma_rn(sr, -z2, sconst(1.0 / 13.0));
+                        sr = __fma_rn(sr, -z2, sconst(1.0 / 11.0));
+                        sr = __fma_rn(sr, -z2, sconst(1.0 / 9.0));
+                        sr = __fma_rn(sr, -z2, sconst(1.0 / 7.0));
+                        sr = __fma_rn(sr, -z2, sconst(1.0 / 5.0));
+                        sr = __fma_rn(sr, -z2, sconst(1.0 / 3.0));
                         sr = __fma_rn(sr, -z2, 1.0);
                         ratio = 2.0 * rw * sr;
                     } else {
