@@ -41,11 +41,12 @@ WEIGHT_FLOP_PER_PAIR = 54.0
 # fixed-point cumulative weight 8 B).  ABI driver stages:
 BYTES = {"predict": 56, "weight": 36, "normalize": 12, "resample_scan": 12, "resample_gather": 72,
          "est_sums": 32, "est_covs": 32}
-# Fused cycle stages: normalize = normalisation + scan (read ll, write w and cum), resample_gather
+# Fused cycle stages: normalize = normalisation + tile aggregates + injection picks (read ll,
+# write w), resample_scan = the scan from the tile aggregates (read w, write cum), resample_gather
 # = gather + injection + estimate-sum partials (read cum window, pose, list head; write pose, w,
 # ancestor), finalize = the one-workgroup-per-filter reductions (no particle traffic).
-FUSED_BYTES = {"predict": 56, "weight": 36, "normalize": 20, "resample_gather": 76, "est_covs": 32,
-               "finalize": 0}
+FUSED_BYTES = {"predict": 56, "weight": 36, "normalize": 12, "resample_scan": 12, "resample_gather": 76,
+               "est_covs": 32, "finalize": 0}
 # Cycle with resampling (predict 56, weight 32, softmax 8, estimate 64, resample 64) and the
 # likelihood's share of it.
 CYCLE_BYTES = 224

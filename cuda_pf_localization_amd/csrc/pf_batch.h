@@ -147,6 +147,7 @@ struct PfState {
     int inj_nblk = 0;
     // reduction partials / scan state
     double *part_w = nullptr, *part_n = nullptr, *part_est = nullptr, *part_cov = nullptr;
+    uint64_t* part_q = nullptr;                // fixed-point weight aggregate of every 1024-particle tile
     uint64_t* scan_st = nullptr;               // look-back status words, two per 1024-particle tile
     unsigned long long* scan_ctl = nullptr;     // status epoch of the look-back words
     int scan_spin_limit = 64;                   // reads of a predecessor's status before summing it here
@@ -172,9 +173,10 @@ void prof_begin(PfState* s, int stage);
 void prof_end(PfState* s);
 int launch_motion(PfState* s, ParamRef<MotionParams> A, int reinit);
 int launch_weight(PfState* s, ParamRef<WeightParams> A);
-// Normalised weights, ESS and (with d_ratio) the resample predicate; with scan (fused cycle) the
-// normalisation is done inside the resampling scan of every filter, which also draws the
-// injection picks inj of a possible resample, and whose LCG then advances here.
+// Normalised weights, ESS and (with d_ratio) the resample predicate; with scan (fused cycle,
+// d_ratio required) the normalisation pass also leaves every tile's fixed-point aggregate and
+// draws the injection picks inj of a possible resample, and the resampling scan of every filter
+// follows, its last tile forming ESS, the predicate and the LCG advance.
 int launch_normalize(PfState* s, int force_uniform, const float* d_ratio, bool scan = false,
                      ParamRef<InjectParams> inj = ParamRef<InjectParams>::value({}));
 // Scan (unless launch_normalize already scanned), injection picks, gather (with the picks' jitter

@@ -526,7 +526,7 @@ __device__ inline float4 tile_norm(const double* __restrict__ ll, int tile, cons
 }
 
 // Normalised weights of every 1024-particle tile plus its partial of sum w^2 (the same tile
-// function and block tree as the fused normalise-and-scan, so both give the same bits).
+// function and block tree as the fused cycle's k_norm_agg, so both give the same bits).
 __global__ __launch_bounds__(kTB) void k_normalize(Layout L, FilterDev* F, const double* __restrict__ ll,
                                                    float* __restrict__ w, double* __restrict__ part, int force_uniform) {
     __shared__ double sh[4];
@@ -542,11 +542,10 @@ __global__ __launch_bounds__(kTB) void k_normalize(Layout L, FilterDev* F, const
 
 // Per filter: ESS = 1 / sum w^2 of the stored weights -- the quantity the reference driver
 // tests (particle_filter.rs:115-117 via pfc_est_sums out[13]) -- and, when ratio is given,
-// the resample predicate ESS < ratio * n.  With advance_lcg (after the fused normalise-and-
-// scan, which scanned every filter) a filter that resamples with a positive total advances its
-// LCG here (pf_kernels.cu:647-657), as the scan's last tile does otherwise.
+// the resample predicate ESS < ratio * n.  (The fused cycle forms them in k_scan_agg, in the
+// same order.)
 __global__ __launch_bounds__(kFinTB) void k_norm_fin(Layout L, FilterDev* F, const double* __restrict__ part,
-                                                     const float* __restrict__ ratio, int advance_lcg) {
+                                                     const float* __restrict__ ratio) {
     __shared__ double sh[kFinTB / 64];
     const int f = blockIdx.x;
     const int b0 = L.f_off[f] / kTile, nb = L.f_padn[f] / kTile;
@@ -561,11 +560,6 @@ __global__ __launch_bounds__(kFinTB) void k_norm_fin(Layout L, FilterDev* F, con
         if (ratio) {
             const double r = (double)ratio[f];
             fd.do_resample = (r > 0.0 && fd.ess < r * (double)L.f_n[f]) ? 1 : 0;
-        }
-        if (advance_lcg && fd.do_resample && fd.total > 0) {
-            const uint64_t lcg = fd.lcg * 6364136223846793005ull + 1442695040888963407ull;  // cu:653
-            fd.lcg = lcg;
-            fd.u32 = (uint32_t)(lcg >> 32);
         }
     }
 }
@@ -673,18 +667,10 @@ __device__ inline uint64_t block_sum_u64(uint64_t v, uint64_t* s_w) {
     return s_w[0] + s_w[1] + s_w[2] + s_w[3];
 }
 
-// kFromLL (fused cycle): the tile first normalises its log-likelihoods (tile_norm, the
-// weights and the sum-w^2 partial stored as k_normalize stores them) and every filter is scanned
-// whatever its resample predicate, which k_norm_fin forms afterwards together with the LCG
-// advance; an unpublished predecessor is then summed from its log-likelihoods.
-template <bool kFromLL>
-__global__ __launch_bounds__(kTB) void k_scan(Layout L, FilterDev* F, const double* __restrict__ ll,
-                                              float* __restrict__ w, double* __restrict__ part_n,
+__global__ __launch_bounds__(kTB) void k_scan(Layout L, FilterDev* F, const float* __restrict__ w,
                                               uint64_t* __restrict__ cum, uint64_t* st, const unsigned long long* ctl,
-                                              int spin_limit, ParamRef<InjectParams> inj, double* __restrict__ jit,
-                                              int32_t* __restrict__ head, int32_t* __restrict__ next) {
+                                              int spin_limit) {
     __shared__ uint64_t s_w[4];
-    __shared__ double s_n[4];
     __shared__ int s_dmin, s_nun;
     __shared__ uint64_t s_excl;
     __shared__ int s_un_tile[kLbWin];     // predecessors of a window found unpublished
@@ -693,19 +679,10 @@ __global__ __launch_bounds__(kTB) void k_scan(Layout L, FilterDev* F, const doub
     const uint32_t epoch = (uint32_t)ctl[0] & 0x3fffffffu;
     const int f = L.blk_filter[tile * (kTile / kTB)];
     FilterDev& fd = F[f];
-    float4 wv;
-    if (kFromLL) {
-        double w2;
-        wv = tile_norm(ll, tile, L, f, fd, fd.uniform != 0, w2);
-        *reinterpret_cast<float4*>(w + tile * kTile + (int)threadIdx.x * 4) = wv;
-        const double bs = block_sum(w2, s_n);
-        if (threadIdx.x == 0) part_n[tile] = bs;
-    } else {
-        wv = tile_w(w, tile);
-        if (!fd.do_resample) {
-            if (threadIdx.x == 0) lb_publish(st, tile, epoch << 2, 0);
-            return;
-        }
+    const float4 wv = tile_w(w, tile);
+    if (!fd.do_resample) {
+        if (threadIdx.x == 0) lb_publish(st, tile, epoch << 2, 0);
+        return;
     }
     const int off = L.f_off[f], nf = L.f_n[f];
     const int t0 = off / kTile, nt = L.f_padn[f] / kTile;
@@ -767,14 +744,7 @@ __global__ __launch_bounds__(kTB) void k_scan(Layout L, FilterDev* F, const doub
                 for (int u = 0; u < nun; ++u) {
                     const int ut = s_un_tile[u];
                     uint64_t qq[4], inc2;
-                    float4 uw;
-                    if (kFromLL) {
-                        double w2u;
-                        uw = tile_norm(ll, ut, L, f, fd, fd.uniform != 0, w2u);
-                    } else {
-                        uw = tile_w(w, ut);
-                    }
-                    tile_q(uw, ut, off, nf, qq);
+                    tile_q(tile_w(w, ut), ut, off, nf, qq);
                     const uint64_t a = block_scan_u64(qq[0] + qq[1] + qq[2] + qq[3], inc2, s_w);
                     if (tid == 0) s_un_val[u] = a;
                 }
@@ -811,7 +781,7 @@ __global__ __launch_bounds__(kTB) void k_scan(Layout L, FilterDev* F, const doub
         if (tile == t0 + nt - 1) {
             const uint64_t T = excl + agg;
             fd.total = T;
-            if (!kFromLL && T > 0) {
+            if (T > 0) {
                 const uint64_t lcg = fd.lcg * 6364136223846793005ull + 1442695040888963407ull;  // cu:653
                 fd.lcg = lcg;
                 fd.u32 = (uint32_t)(lcg >> 32);
@@ -824,16 +794,107 @@ __global__ __launch_bounds__(kTB) void k_scan(Layout L, FilterDev* F, const doub
     cp[1] = pre + q[1];
     cp[2] = pre + q[2];
     cp[3] = pre + q[3];
-    if (kFromLL) {
-        // this tile's share of the injection picks, drawn before the resample predicate is known
-        // with the LCG value its advance will give (k_norm_fin advances fd.lcg only afterwards);
-        // after the tile's own prefix is published, off the successors' look-back path
-        const InjectParams ip = inj.get(f);
-        if (ip.k > 0) {
-            const int per = (ip.k + nt - 1) / nt;
-            const int j0 = (tile - t0) * per, j1 = min(ip.k, j0 + per);
-            const uint64_t lcg1 = fd.lcg * 6364136223846793005ull + 1442695040888963407ull;  // cu:653
-            for (int j = j0 + (int)threadIdx.x; j < j1; j += kTB) inj_pick(off, nf, lcg1, ip.gen, j, jit, head, next);
+}
+
+// Fused cycle, first pass of the resampling scan: the normalised weights of every 1024-particle
+// tile and its sum-w^2 partial (tile_norm and the block tree of k_normalize, so both give the same
+// bits), the tile's fixed-point aggregate sum q (the value a look-back tile publishes), and the
+// tile's share of the filter's injection picks: drawn before the resample predicate is known,
+// with the LCG value its advance will give (k_scan_agg advances fd.lcg only afterwards).
+__global__ __launch_bounds__(kTB) void k_norm_agg(Layout L, FilterDev* F, const double* __restrict__ ll,
+                                                  float* __restrict__ w, double* __restrict__ part_n,
+                                                  uint64_t* __restrict__ part_q, ParamRef<InjectParams> inj,
+                                                  double* __restrict__ jit, int32_t* __restrict__ head,
+                                                  int32_t* __restrict__ next) {
+    __shared__ double sh[4];
+    __shared__ uint64_t s_w[4];
+    const int tile = blockIdx.x;
+    const int f = L.blk_filter[tile * (kTile / kTB)];
+    const FilterDev& fd = F[f];
+    const int off = L.f_off[f], nf = L.f_n[f];
+    double w2;
+    const float4 wv = tile_norm(ll, tile, L, f, fd, fd.uniform != 0, w2);
+    *reinterpret_cast<float4*>(w + tile * kTile + (int)threadIdx.x * 4) = wv;
+    uint64_t q[4];
+    tile_q(wv, tile, off, nf, q);
+    const double bs = block_sum(w2, sh);
+    const uint64_t agg = block_sum_u64(q[0] + q[1] + q[2] + q[3], s_w);
+    if (threadIdx.x == 0) {
+        part_n[tile] = bs;
+        part_q[tile] = agg;
+    }
+    const InjectParams ip = inj.get(f);
+    if (ip.k > 0) {
+        const int t0 = off / kTile, nt = L.f_padn[f] / kTile;
+        const int per = (ip.k + nt - 1) / nt;
+        const int j0 = (tile - t0) * per, j1 = min(ip.k, j0 + per);
+        const uint64_t lcg1 = fd.lcg * 6364136223846793005ull + 1442695040888963407ull;  // cu:653
+        for (int j = j0 + (int)threadIdx.x; j < j1; j += kTB) inj_pick(off, nf, lcg1, ip.gen, j, jit, head, next);
+    }
+}
+
+// Fused cycle, second pass of the resampling scan (reduce, then scan): the first pass
+// (k_norm_agg) left every tile's fixed-point aggregate, so a tile's exclusive
+// prefix is the sum of its filter's earlier aggregates, read once after that kernel ended (no
+// look-back, nothing to wait for); the tile then scans its own weights into cum.  Every filter
+// is scanned whatever its predicate.  The filter's last tile stores the total and, as k_norm_fin
+// does, forms ESS = 1 / sum w^2 from the tiles' w^2 partials, the resample predicate and the LCG
+// advance (pf_kernels.cu:647-657) -- the sum in k_norm_fin's order (1024 lanes, lane v summing
+// tiles v, v + 1024, ..., its 16 waves folded by wave_sum and added in sequence), which this
+// block's four waves reproduce as virtual waves w, w + 4, w + 8, w + 12, so both paths give the
+// same bits.
+__global__ __launch_bounds__(kTB) void k_scan_agg(Layout L, FilterDev* F, const float* __restrict__ w,
+                                                  const double* __restrict__ part_n,
+                                                  const uint64_t* __restrict__ part_q, uint64_t* __restrict__ cum,
+                                                  const float* __restrict__ ratio) {
+    __shared__ uint64_t s_w[4];
+    __shared__ double s_vw[kFinTB / 64];
+    const int tile = blockIdx.x;
+    const int f = L.blk_filter[tile * (kTile / kTB)];
+    const int off = L.f_off[f], nf = L.f_n[f];
+    const int t0 = off / kTile, nt = L.f_padn[f] / kTile;
+    uint64_t ex = 0;
+    for (int b = t0 + (int)threadIdx.x; b < tile; b += kTB) ex += part_q[b];
+    const uint64_t excl = block_sum_u64(ex, s_w);
+    uint64_t q[4];
+    tile_q(tile_w(w, tile), tile, off, nf, q);
+    q[1] += q[0];
+    q[2] += q[1];
+    q[3] += q[2];
+    uint64_t incl;
+    const uint64_t agg = block_scan_u64(q[3], incl, s_w);
+    const uint64_t pre = excl + incl - q[3];
+    uint64_t* cp = cum + tile * kTile + (int)threadIdx.x * 4;
+    cp[0] = pre + q[0];
+    cp[1] = pre + q[1];
+    cp[2] = pre + q[2];
+    cp[3] = pre + q[3];
+    if (tile != t0 + nt - 1) return;
+#pragma unroll
+    for (int j = 0; j < kFinTB / kTB; ++j) {
+        const int vl = (int)threadIdx.x + kTB * j;
+        double v = 0.0;
+        for (int b = vl; b < nt; b += kFinTB) v += part_n[t0 + b];
+        v = wave_sum(v);
+        if ((threadIdx.x & 63) == 0) s_vw[vl >> 6] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double tot = s_vw[0];
+#pragma unroll
+        for (int k = 1; k < kFinTB / 64; ++k) tot += s_vw[k];
+        FilterDev& fd = F[f];
+        const uint64_t T = excl + agg;
+        fd.total = T;
+        fd.est_src = 0;  // weights changed
+        fd.sum_w2 = tot;
+        fd.ess = tot > 0.0 ? 1.0 / tot : 0.0;
+        const double r = (double)ratio[f];
+        fd.do_resample = (r > 0.0 && fd.ess < r * (double)nf) ? 1 : 0;
+        if (fd.do_resample && T > 0) {
+            const uint64_t lcg = fd.lcg * 6364136223846793005ull + 1442695040888963407ull;  // cu:653
+            fd.lcg = lcg;
+            fd.u32 = (uint32_t)(lcg >> 32);
         }
     }
 }
@@ -1474,6 +1535,7 @@ int state_alloc(PfState* s, const std::vector<int>& ns) {
     const int nb256 = NP / kTB, ntile = NP / kTile;
     PF_CHECK(hipMalloc((void**)&s->part_w, sizeof(double) * 3 * (size_t)nb256));
     PF_CHECK(hipMalloc((void**)&s->part_n, sizeof(double) * (size_t)nb256));
+    PF_CHECK(hipMalloc((void**)&s->part_q, sizeof(uint64_t) * (size_t)ntile));
     PF_CHECK(hipMalloc((void**)&s->part_est, sizeof(double) * kEstN * (size_t)ntile));
     PF_CHECK(hipMalloc((void**)&s->part_cov, sizeof(double) * kCovN * (size_t)ntile));
     PF_CHECK(hipMalloc((void**)&s->scan_st, sizeof(uint64_t) * 2 * (size_t)ntile));
@@ -1522,7 +1584,7 @@ void state_free_particles(PfState* s) {
         }
     void** ptrs[] = {(void**)&s->w,        (void**)&s->ll,          (void**)&s->cum,       (void**)&s->anc,
                      (void**)&s->inj_next,    (void**)&s->inj_head,  (void**)&s->inj_jit,
-                     (void**)&s->part_w,   (void**)&s->part_n,  (void**)&s->part_est,    (void**)&s->part_cov,  (void**)&s->scan_st,
+                     (void**)&s->part_w,   (void**)&s->part_n,  (void**)&s->part_q,  (void**)&s->part_est,    (void**)&s->part_cov,  (void**)&s->scan_st,
                      (void**)&s->scan_ctl, (void**)&s->d_blk_filter, (void**)&s->d_fn,     (void**)&s->d_foff,
                      (void**)&s->d_fpadn,  (void**)&s->d_fwinv,     (void**)&s->fdev,      (void**)&s->d_inj_blk};
     for (void** p : ptrs) {
@@ -1597,20 +1659,26 @@ int launch_weight(PfState* s, ParamRef<WeightParams> A) {
 int launch_normalize(PfState* s, int force_uniform, const float* d_ratio, bool scan, ParamRef<InjectParams> inj) {
     const int ntile = s->NP / kTile;
     if (scan) {
+        if (!d_ratio) return -1;  // the fused scan forms the resample predicate
         prof_begin(s, PFX_STAGE_NORMALIZE);
-        hipLaunchKernelGGL(k_scan<true>, dim3(ntile), dim3(kTB), 0, s->stream, s->L, s->fdev, s->ll, s->w, s->part_n,
-                           s->cum, s->scan_st, s->scan_ctl, s->scan_spin_limit, inj, s->inj_jit, s->inj_head,
-                           s->inj_next);
-    } else {
-        prof_begin(s, PFX_STAGE_NORMALIZE);
-        hipLaunchKernelGGL(k_normalize, dim3(ntile), dim3(kTB), 0, s->stream, s->L, s->fdev, s->ll, s->w, s->part_n,
-                           force_uniform);
+        hipLaunchKernelGGL(k_norm_agg, dim3(ntile), dim3(kTB), 0, s->stream, s->L, s->fdev, s->ll, s->w, s->part_n,
+                           s->part_q, inj, s->inj_jit, s->inj_head, s->inj_next);
+        prof_end(s);
+        PF_LAUNCH_CHECK();
+        prof_begin(s, PFX_STAGE_RESAMPLE_SCAN);
+        hipLaunchKernelGGL(k_scan_agg, dim3(ntile), dim3(kTB), 0, s->stream, s->L, s->fdev, s->w, s->part_n,
+                           s->part_q, s->cum, d_ratio);
+        prof_end(s);
+        PF_LAUNCH_CHECK();
+        return 0;
     }
+    prof_begin(s, PFX_STAGE_NORMALIZE);
+    hipLaunchKernelGGL(k_normalize, dim3(ntile), dim3(kTB), 0, s->stream, s->L, s->fdev, s->ll, s->w, s->part_n,
+                       force_uniform);
     prof_end(s);
     PF_LAUNCH_CHECK();
     prof_begin(s, PFX_STAGE_FINALIZE);
-    hipLaunchKernelGGL(k_norm_fin, dim3(s->nf), dim3(kFinTB), 0, s->stream, s->L, s->fdev, s->part_n, d_ratio,
-                       scan ? 1 : 0);
+    hipLaunchKernelGGL(k_norm_fin, dim3(s->nf), dim3(kFinTB), 0, s->stream, s->L, s->fdev, s->part_n, d_ratio);
     prof_end(s);
     PF_LAUNCH_CHECK();
     return 0;
@@ -1621,9 +1689,8 @@ int launch_resample(PfState* s, ParamRef<InjectParams> inj, const int32_t* inj_t
     const int ntile = s->NP / kTile;
     if (scan) {
         prof_begin(s, PFX_STAGE_RESAMPLE_SCAN);
-        hipLaunchKernelGGL(k_scan<false>, dim3(ntile), dim3(kTB), 0, s->stream, s->L, s->fdev, s->ll, s->w, s->part_n,
-                           s->cum, s->scan_st, s->scan_ctl, s->scan_spin_limit, ParamRef<InjectParams>::value({}),
-                           s->inj_jit, s->inj_head, s->inj_next);
+        hipLaunchKernelGGL(k_scan, dim3(ntile), dim3(kTB), 0, s->stream, s->L, s->fdev, s->w, s->cum, s->scan_st,
+                           s->scan_ctl, s->scan_spin_limit);
         prof_end(s);
         PF_LAUNCH_CHECK();
     }
