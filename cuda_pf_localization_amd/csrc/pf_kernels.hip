@@ -446,27 +446,17 @@ __device__ __forceinline__ void markley_mean(FilterDev& fd, const double* sums, 
 
 // Per filter: M = max m_b, S = sum s_b e^(m_b - M), S2 = sum s2_b e^(2(m_b - M)) over the
 // filter's likelihood blocks, in a fixed order (one 1024-thread block per filter, so that each
-// lane has few partials to load; the kernel boundary orders it after every partial).
+// lane has few partials to load; the kernel boundary orders it after every partial).  One pass
+// over the partials: lane t folds blocks t, t + 1024, ... (four in flight) into its own running
+// maximum m_t and sums relative to it; then M = max m_t and S = sum_t s_t e^(m_t - M), S2 =
+// sum_t s2_t e^(2(m_t - M)) in the fixed 16-wave order of block16_sum.
 __global__ __launch_bounds__(kFinTB) void k_weight_fin(Layout L, FilterDev* F, const double* __restrict__ part,
                                                        int span) {
-    __shared__ double sh[kFinTB / 64];
+    __shared__ double sh[kFinTB / 64], sh1[kFinTB / 64], sh2[kFinTB / 64];
     const int f = blockIdx.x;
     const int b0 = L.f_off[f] / span, nb = L.f_padn[f] / span;
     const int t = (int)threadIdx.x;
-    // lane t takes blocks t, t + 1024, ... in that order; four loads in flight per pass
-    double m = -INFINITY;
-    for (int c = 0; c < nb; c += 4 * kFinTB) {
-        double x[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int b = c + t + u * kFinTB;
-            x[u] = b < nb ? part[3 * (b0 + b)] : -INFINITY;
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u) m = fmax(m, x[u]);
-    }
-    const double M = block16_max(m, sh);
-    double s = 0.0, s2 = 0.0;
+    double m = -INFINITY, s = 0.0, s2 = 0.0;
     for (int c = 0; c < nb; c += 4 * kFinTB) {
         double mb[4], e1[4], e2[4];
 #pragma unroll
@@ -477,16 +467,45 @@ __global__ __launch_bounds__(kFinTB) void k_weight_fin(Layout L, FilterDev* F, c
             e1[u] = in ? part[3 * (b0 + b) + 1] : 0.0;
             e2[u] = in ? part[3 * (b0 + b) + 2] : 0.0;
         }
+        double mn = m;
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
-            if (mb[u] > -INFINITY) {
-                const double sc = exp(mb[u] - M);
-                s += e1[u] * sc;
-                s2 += e2[u] * (sc * sc);
+        for (int u = 0; u < 4; ++u) mn = fmax(mn, mb[u]);
+        if (mn > -INFINITY) {
+            if (m > -INFINITY && m != mn) {  // (exp(0) = 1 leaves the sums as they are)
+                const double r = exp(m - mn);
+                s *= r;
+                s2 *= r * r;
             }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (mb[u] > -INFINITY) {
+                    const double sc = exp(mb[u] - mn);
+                    s += e1[u] * sc;
+                    s2 += e2[u] * (sc * sc);
+                }
+            m = mn;
+        }
     }
-    const double Ssum = block16_sum(s, sh);
-    const double S2sum = block16_sum(s2, sh);
+    const double M = block16_max(m, sh);
+    double sl = 0.0, s2l = 0.0;
+    if (m > -INFINITY) {
+        const double sc = exp(m - M);
+        sl = s * sc;
+        s2l = s2 * (sc * sc);
+    }
+    sl = wave_sum(sl);
+    s2l = wave_sum(s2l);
+    if ((t & 63) == 0) {  // (sh may still be read by block16_max's last loop)
+        sh1[t >> 6] = sl;
+        sh2[t >> 6] = s2l;
+    }
+    __syncthreads();
+    double Ssum = sh1[0], S2sum = sh2[0];
+#pragma unroll
+    for (int k = 1; k < kFinTB / 64; ++k) {
+        Ssum += sh1[k];
+        S2sum += sh2[k];
+    }
     if (threadIdx.x == 0) {
         FilterDev& fd = F[f];
         const bool ok = (Ssum > 0.0) && isfinite(Ssum) && isfinite(M);
