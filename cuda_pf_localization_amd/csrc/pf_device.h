@@ -35,6 +35,8 @@ __host__ __device__ inline uint64_t sm_next(uint64_t& x) {
 // A 64-bit constant held in a scalar register pair: the empty asm hides its value, so a
 // Horner step fma(p, x, c) becomes one v_fma_f64 with c as its scalar operand instead of two
 // v_mov_b32 that rebuild c in vector registers before a v_fmac_f64 (same operands, same bits).
+// Used only in kernels with scalar registers to spare: in the shared draw and sincos helpers it
+// pushed the CBGL kernels to 103 scalar registers and made them 20 % slower.
 __device__ __forceinline__ double sconst(double c) {
     asm("" : "+s"(c));
     return c;
@@ -64,16 +66,16 @@ __device__ inline double det_log(double x) {
     const double s = f * y;
     const double s2 = s * s;
     double p = 1.0 / 23.0;
-    p = __fma_rn(p, s2, sconst(1.0 / 21.0));
-    p = __fma_rn(p, s2, sconst(1.0 / 19.0));
-    p = __fma_rn(p, s2, sconst(1.0 / 17.0));
-    p = __fma_rn(p, s2, sconst(1.0 / 15.0));
-    p = __fma_rn(p, s2, sconst(1.0 / 13.0));
-    p = __fma_rn(p, s2, sconst(1.0 / 11.0));
-    p = __fma_rn(p, s2, sconst(1.0 / 9.0));
-    p = __fma_rn(p, s2, sconst(1.0 / 7.0));
-    p = __fma_rn(p, s2, sconst(1.0 / 5.0));
-    p = __fma_rn(p, s2, sconst(1.0 / 3.0));
+    p = __fma_rn(p, s2, 1.0 / 21.0);
+    p = __fma_rn(p, s2, 1.0 / 19.0);
+    p = __fma_rn(p, s2, 1.0 / 17.0);
+    p = __fma_rn(p, s2, 1.0 / 15.0);
+    p = __fma_rn(p, s2, 1.0 / 13.0);
+    p = __fma_rn(p, s2, 1.0 / 11.0);
+    p = __fma_rn(p, s2, 1.0 / 9.0);
+    p = __fma_rn(p, s2, 1.0 / 7.0);
+    p = __fma_rn(p, s2, 1.0 / 5.0);
+    p = __fma_rn(p, s2, 1.0 / 3.0);
     const double lnm = 2.0 * __fma_rn(s * s2, p, s);
     const double de = (double)e;
     const double r = __fma_rn(de, 1.9082149292705877e-10, lnm);
@@ -104,22 +106,22 @@ __device__ inline void det_sincos(double x, double& so, double& co) {
     r = __fma_rn(-k, 6.123233995736766e-17, r);
     const double r2 = r * r;
     double ps = 1.0 / 355687428096000.0;
-    ps = __fma_rn(ps, r2, sconst(-1.0 / 1307674368000.0));
-    ps = __fma_rn(ps, r2, sconst(1.0 / 6227020800.0));
-    ps = __fma_rn(ps, r2, sconst(-1.0 / 39916800.0));
-    ps = __fma_rn(ps, r2, sconst(1.0 / 362880.0));
-    ps = __fma_rn(ps, r2, sconst(-1.0 / 5040.0));
-    ps = __fma_rn(ps, r2, sconst(1.0 / 120.0));
-    ps = __fma_rn(ps, r2, sconst(-1.0 / 6.0));
+    ps = __fma_rn(ps, r2, -1.0 / 1307674368000.0);
+    ps = __fma_rn(ps, r2, 1.0 / 6227020800.0);
+    ps = __fma_rn(ps, r2, -1.0 / 39916800.0);
+    ps = __fma_rn(ps, r2, 1.0 / 362880.0);
+    ps = __fma_rn(ps, r2, -1.0 / 5040.0);
+    ps = __fma_rn(ps, r2, 1.0 / 120.0);
+    ps = __fma_rn(ps, r2, -1.0 / 6.0);
     const double sn = __fma_rn(r * r2, ps, r);
     double pc = 1.0 / 6402373705728000.0;
-    pc = __fma_rn(pc, r2, sconst(-1.0 / 20922789888000.0));
-    pc = __fma_rn(pc, r2, sconst(1.0 / 87178291200.0));
-    pc = __fma_rn(pc, r2, sconst(-1.0 / 479001600.0));
-    pc = __fma_rn(pc, r2, sconst(1.0 / 3628800.0));
-    pc = __fma_rn(pc, r2, sconst(-1.0 / 40320.0));
-    pc = __fma_rn(pc, r2, sconst(1.0 / 720.0));
-    pc = __fma_rn(pc, r2, sconst(-1.0 / 24.0));
+    pc = __fma_rn(pc, r2, -1.0 / 20922789888000.0);
+    pc = __fma_rn(pc, r2, 1.0 / 87178291200.0);
+    pc = __fma_rn(pc, r2, -1.0 / 479001600.0);
+    pc = __fma_rn(pc, r2, 1.0 / 3628800.0);
+    pc = __fma_rn(pc, r2, -1.0 / 40320.0);
+    pc = __fma_rn(pc, r2, 1.0 / 720.0);
+    pc = __fma_rn(pc, r2, -1.0 / 24.0);
     pc = __fma_rn(pc, r2, 0.5);
     const double cs = __fma_rn(-r2, pc, 1.0);
     const long q = (long)k & 3;
@@ -146,26 +148,26 @@ __device__ inline void det_cos_sinc(double h2, double& co, double& sinc) {
         ++m;
     }
     double c = 1.0 / 2432902008176640000.0;
-    c = __fma_rn(c, -h2, sconst(1.0 / 6402373705728000.0));
-    c = __fma_rn(c, -h2, sconst(1.0 / 20922789888000.0));
-    c = __fma_rn(c, -h2, sconst(1.0 / 87178291200.0));
-    c = __fma_rn(c, -h2, sconst(1.0 / 479001600.0));
-    c = __fma_rn(c, -h2, sconst(1.0 / 3628800.0));
-    c = __fma_rn(c, -h2, sconst(1.0 / 40320.0));
-    c = __fma_rn(c, -h2, sconst(1.0 / 720.0));
-    c = __fma_rn(c, -h2, sconst(1.0 / 24.0));
+    c = __fma_rn(c, -h2, 1.0 / 6402373705728000.0);
+    c = __fma_rn(c, -h2, 1.0 / 20922789888000.0);
+    c = __fma_rn(c, -h2, 1.0 / 87178291200.0);
+    c = __fma_rn(c, -h2, 1.0 / 479001600.0);
+    c = __fma_rn(c, -h2, 1.0 / 3628800.0);
+    c = __fma_rn(c, -h2, 1.0 / 40320.0);
+    c = __fma_rn(c, -h2, 1.0 / 720.0);
+    c = __fma_rn(c, -h2, 1.0 / 24.0);
     c = __fma_rn(c, -h2, 0.5);
     c = __fma_rn(c, -h2, 1.0);
     double s = 1.0 / 51090942171709440000.0;
-    s = __fma_rn(s, -h2, sconst(1.0 / 121645100408832000.0));
-    s = __fma_rn(s, -h2, sconst(1.0 / 355687428096000.0));
-    s = __fma_rn(s, -h2, sconst(1.0 / 1307674368000.0));
-    s = __fma_rn(s, -h2, sconst(1.0 / 6227020800.0));
-    s = __fma_rn(s, -h2, sconst(1.0 / 39916800.0));
-    s = __fma_rn(s, -h2, sconst(1.0 / 362880.0));
-    s = __fma_rn(s, -h2, sconst(1.0 / 5040.0));
-    s = __fma_rn(s, -h2, sconst(1.0 / 120.0));
-    s = __fma_rn(s, -h2, sconst(1.0 / 6.0));
+    s = __fma_rn(s, -h2, 1.0 / 121645100408832000.0);
+    s = __fma_rn(s, -h2, 1.0 / 355687428096000.0);
+    s = __fma_rn(s, -h2, 1.0 / 1307674368000.0);
+    s = __fma_rn(s, -h2, 1.0 / 6227020800.0);
+    s = __fma_rn(s, -h2, 1.0 / 39916800.0);
+    s = __fma_rn(s, -h2, 1.0 / 362880.0);
+    s = __fma_rn(s, -h2, 1.0 / 5040.0);
+    s = __fma_rn(s, -h2, 1.0 / 120.0);
+    s = __fma_rn(s, -h2, 1.0 / 6.0);
     s = __fma_rn(s, -h2, 1.0);
     for (int i = 0; i < m; ++i) {
         s = s * c;
