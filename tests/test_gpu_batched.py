@@ -47,9 +47,17 @@ def make_batch(sizes, scs, inject=0.0):
 
 @pytest.mark.parametrize("inject", [0.0, 0.05])
 def test_batched_equals_single_filters(gpu, inject):
-    sizes = [1000, 5000, 20_000, 777, 4096]
-    lms = [12, 40, 200, 4, 97]
-    cycles = 12
+    check_batch_vs_single([1000, 5000, 20_000, 777, 4096], [12, 40, 200, 4, 97], 12, inject)
+
+
+def test_large_filter_equals_single(gpu):
+    """A 4.2M-particle filter (4102 tiles) beside a small one: the fused scan's exclusive
+    prefixes span many loads per lane and its ESS sum runs several tiles per virtual lane, and
+    still matches the reference-ABI path (look-back scan, separate ESS finalizer) bit for bit."""
+    check_batch_vs_single([4_200_000, 3000], [64, 12], 4, 0.01)
+
+
+def check_batch_vs_single(sizes, lms, cycles, inject):
     scs = scenarios(sizes, lms, cycles, 100)
     bf = make_batch(sizes, scs, inject)
     singles = []
