@@ -633,7 +633,9 @@ __device__ inline void inj_pick(int off, int nf, uint64_t lcg, uint64_t gen, int
 }
 
 // Single-pass scan of the fixed-point weights with decoupled look-back (integer sums: any
-// association gives the same bits).
+// association gives the same bits): the reference-ABI path's pfc_resample, which cannot rely on
+// tile aggregates left by pfc_weight (the state may be set in between).  The fused cycle scans
+// in two passes instead (k_norm_agg, k_scan_agg below).
 //
 // Tiles are 1024 particles, one block each.  Each tile publishes its aggregate, looks back over
 // its filter's predecessors until it meets an inclusive prefix, and publishes its own inclusive
