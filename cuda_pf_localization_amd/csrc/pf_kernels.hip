@@ -446,52 +446,54 @@ __device__ __forceinline__ void markley_mean(FilterDev& fd, const double* sums, 
 
 // Per filter: M = max m_b, S = sum s_b e^(m_b - M), S2 = sum s2_b e^(2(m_b - M)) over the
 // filter's likelihood blocks, in a fixed order (one 1024-thread block per filter, so that each
-// lane has few partials to load; the kernel boundary orders it after every partial).  One pass
-// over the partials: lane t folds blocks t, t + 1024, ... (four in flight) into its own running
-// maximum m_t and sums relative to it; then M = max m_t and S = sum_t s_t e^(m_t - M), S2 =
-// sum_t s2_t e^(2(m_t - M)) in the fixed 16-wave order of block16_sum.
+// lane has few partials to load; the kernel boundary orders it after every partial).  Lane t
+// takes blocks t, t + 1024, ... in that order.  Up to four blocks per lane (filters up to 4M
+// particles at one particle per lane) the partials stay in registers from a single load; larger
+// filters read the maxima once more.  A term whose e^(m_b - M) underflows (m_b - M <= -746, the
+// exponential is exactly 0) is skipped without evaluating it: with peaked likelihoods that is
+// most blocks.  S and S2 are summed per lane, then over the 16 waves in order (block16_sum's
+// tree, two values at once).
 __global__ __launch_bounds__(kFinTB) void k_weight_fin(Layout L, FilterDev* F, const double* __restrict__ part,
                                                        int span) {
     __shared__ double sh[kFinTB / 64], sh1[kFinTB / 64], sh2[kFinTB / 64];
     const int f = blockIdx.x;
     const int b0 = L.f_off[f] / span, nb = L.f_padn[f] / span;
     const int t = (int)threadIdx.x;
-    double m = -INFINITY, s = 0.0, s2 = 0.0;
+    const bool resident = nb <= 4 * kFinTB;
+    double mb[4], e1[4], e2[4];
+    double m = -INFINITY;
     for (int c = 0; c < nb; c += 4 * kFinTB) {
-        double mb[4], e1[4], e2[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const int b = c + t + u * kFinTB;
             const bool in = b < nb;
             mb[u] = in ? part[3 * (b0 + b)] : -INFINITY;
-            e1[u] = in ? part[3 * (b0 + b) + 1] : 0.0;
-            e2[u] = in ? part[3 * (b0 + b) + 2] : 0.0;
+            e1[u] = in && resident ? part[3 * (b0 + b) + 1] : 0.0;
+            e2[u] = in && resident ? part[3 * (b0 + b) + 2] : 0.0;
         }
-        double mn = m;
 #pragma unroll
-        for (int u = 0; u < 4; ++u) mn = fmax(mn, mb[u]);
-        if (mn > -INFINITY) {
-            if (m > -INFINITY && m != mn) {  // (exp(0) = 1 leaves the sums as they are)
-                const double r = exp(m - mn);
-                s *= r;
-                s2 *= r * r;
-            }
-#pragma unroll
-            for (int u = 0; u < 4; ++u)
-                if (mb[u] > -INFINITY) {
-                    const double sc = exp(mb[u] - mn);
-                    s += e1[u] * sc;
-                    s2 += e2[u] * (sc * sc);
-                }
-            m = mn;
-        }
+        for (int u = 0; u < 4; ++u) m = fmax(m, mb[u]);
     }
     const double M = block16_max(m, sh);
     double sl = 0.0, s2l = 0.0;
-    if (m > -INFINITY) {
-        const double sc = exp(m - M);
-        sl = s * sc;
-        s2l = s2 * (sc * sc);
+    for (int c = 0; c < nb; c += 4 * kFinTB) {
+        if (!resident) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int b = c + t + u * kFinTB;
+                const bool in = b < nb;
+                mb[u] = in ? part[3 * (b0 + b)] : -INFINITY;
+                e1[u] = in ? part[3 * (b0 + b) + 1] : 0.0;
+                e2[u] = in ? part[3 * (b0 + b) + 2] : 0.0;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (mb[u] - M > -746.0) {
+                const double sc = exp(mb[u] - M);
+                sl += e1[u] * sc;
+                s2l += e2[u] * (sc * sc);
+            }
     }
     sl = wave_sum(sl);
     s2l = wave_sum(s2l);
