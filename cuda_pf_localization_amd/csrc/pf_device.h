@@ -320,7 +320,7 @@ __host__ __device__ inline void jacobi_round(double (&a)[4][4], double (&v)[4][4
 // Dominant unit eigenvector of (M + M^T)/2 (4x4, row-major) by Jacobi sweeps in the parallel
 // ordering (three rounds of two disjoint rotations cover the six planes), sign chosen so that
 // w >= 0.  Returns -1 for a zero eigenvector.
-__host__ __device__ inline __attribute__((noinline)) int quat_mean4_jacobi(const double* M, double* q) {
+__host__ __device__ inline int quat_mean4_jacobi(const double* M, double* q) {
     // Every loop below is unrolled so that all indices are compile-time constants and the
     // 4x4 arrays stay in registers on the device (one thread runs this per filter).
     double a[4][4], v[4][4];

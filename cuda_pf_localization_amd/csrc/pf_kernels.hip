@@ -1382,9 +1382,11 @@ __global__ __launch_bounds__(kEstFinTB) void k_est_fin(Layout L, FilterDev* F, c
         if (fd.do_resample && fd.total > 0) fd.parity ^= 1;
         if (f == 0) scan_ctl[0] = (scan_ctl[0] + 1ull) & 0x3fffffffull;
     }
-    if (!cov) return;
-    __syncthreads();
-    if (threadIdx.x == 0) markley_mean(F[f], sv, cov[f]);
+    if constexpr (NV == kEstN) {  // (the covariance instantiation carries no Markley code)
+        if (!cov) return;
+        __syncthreads();
+        if (threadIdx.x == 0) markley_mean(F[f], sv, cov[f]);
+    }
 }
 
 // est layout (cu:354-404): [0..2] sum w p, [3..12] sum w q_r q_c (r<=c), [13] sum w^2.
