@@ -876,11 +876,31 @@ __global__ __launch_bounds__(kTB) void k_scan_agg(Layout L, FilterDev* F, const 
     const int f = L.blk_filter[tile * (kTile / kTB)];
     const int off = L.f_off[f], nf = L.f_n[f];
     const int t0 = off / kTile, nt = L.f_padn[f] / kTile;
+    const bool last = tile == t0 + nt - 1;
+    // the tile's weights, and for the last tile of a filter of up to 1024 tiles its virtual
+    // lanes' single w^2 partials (0.0 + x = x: the same bits as the loop below), are loaded
+    // together with the aggregates, so the tile waits for one memory round trip
+    const float4 wv = tile_w(w, tile);
+    double pn[kFinTB / kTB];
+#pragma unroll
+    for (int j = 0; j < kFinTB / kTB; ++j) {
+        const int vl = (int)threadIdx.x + kTB * j;
+        pn[j] = last && nt <= kFinTB && vl < nt ? part_n[t0 + vl] : 0.0;
+    }
     uint64_t ex = 0;
-    for (int b = t0 + (int)threadIdx.x; b < tile; b += kTB) ex += part_q[b];
+    for (int c = t0; c < tile; c += 4 * kTB) {  // (four aggregates per lane in flight)
+        uint64_t a[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int b = c + (int)threadIdx.x + u * kTB;
+            a[u] = b < tile ? part_q[b] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) ex += a[u];
+    }
     const uint64_t excl = block_sum_u64(ex, s_w);
     uint64_t q[4];
-    tile_q(tile_w(w, tile), tile, off, nf, q);
+    tile_q(wv, tile, off, nf, q);
     q[1] += q[0];
     q[2] += q[1];
     q[3] += q[2];
@@ -892,12 +912,15 @@ __global__ __launch_bounds__(kTB) void k_scan_agg(Layout L, FilterDev* F, const 
     cp[1] = pre + q[1];
     cp[2] = pre + q[2];
     cp[3] = pre + q[3];
-    if (tile != t0 + nt - 1) return;
+    if (!last) return;
 #pragma unroll
     for (int j = 0; j < kFinTB / kTB; ++j) {
         const int vl = (int)threadIdx.x + kTB * j;
-        double v = 0.0;
-        for (int b = vl; b < nt; b += kFinTB) v += part_n[t0 + b];
+        double v = pn[j];
+        if (nt > kFinTB) {
+            v = 0.0;
+            for (int b = vl; b < nt; b += kFinTB) v += part_n[t0 + b];
+        }
         v = wave_sum(v);
         if ((threadIdx.x & 63) == 0) s_vw[vl >> 6] = v;
     }
