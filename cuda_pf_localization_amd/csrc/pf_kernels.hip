@@ -619,8 +619,11 @@ __device__ inline float4 tile_w(const float* __restrict__ w, int tile) {
 // Injection pick j of a filter (cu:333-352 formula) drawn from the LCG state lcg of this resample:
 // its target and f64 jitter; the pick pushes itself onto its target's list (head[] is -1
 // everywhere at rest: allocation sets it, k_gather resets every head of its filter).
-__device__ inline void inj_pick(int off, int nf, uint64_t lcg, uint64_t gen, int j, double* __restrict__ jit,
-                                int32_t* __restrict__ head, int32_t* __restrict__ next) {
+// inj_push returns the previous head, for next[off + j], so that a caller can store it later and
+// let the exchange's round trip overlap other work (the lists' order does not matter: the
+// gather applies a list's picks in ascending pick order).
+__device__ inline int32_t inj_push(int off, int nf, uint64_t lcg, uint64_t gen, int j, double* __restrict__ jit,
+                                   int32_t* __restrict__ head) {
     uint64_t z = lcg ^ (gen * kPhi) ^ (((uint64_t)j + 1) * kC1);
     if (z == 0) z = 0x243F6A8885A308D3ull;
     int t = (int)(u01_co(sm_next(z)) * (double)nf);
@@ -631,7 +634,11 @@ __device__ inline void inj_pick(int off, int nf, uint64_t lcg, uint64_t gen, int
     jit[3 * (off + j)] = jx;
     jit[3 * (off + j) + 1] = jy;
     jit[3 * (off + j) + 2] = jz;
-    next[off + j] = atomicExch(&head[off + t], j);
+    return atomicExch(&head[off + t], j);
+}
+__device__ inline void inj_pick(int off, int nf, uint64_t lcg, uint64_t gen, int j, double* __restrict__ jit,
+                                int32_t* __restrict__ head, int32_t* __restrict__ next) {
+    next[off + j] = inj_push(off, nf, lcg, gen, j, jit, head);
 }
 
 // Single-pass scan of the fixed-point weights with decoupled look-back (integer sums: any
@@ -835,6 +842,23 @@ __global__ __launch_bounds__(kTB) void k_norm_agg(Layout L, FilterDev* F, const 
     const int f = L.blk_filter[tile * (kTile / kTB)];
     const FilterDev& fd = F[f];
     const int off = L.f_off[f], nf = L.f_n[f];
+    // the first pick of each lane is pushed before the tile's own work, its link stored at the
+    // end (the exchange's round trip overlaps the log-likelihood loads); further ones after
+    const InjectParams ip = inj.get(f);
+    int j0 = 0, j1 = 0, pj = -1;
+    int32_t plink = -1;
+    uint64_t lcg1 = 0;
+    if (ip.k > 0) {
+        const int t0 = off / kTile, nt = L.f_padn[f] / kTile;
+        const int per = (ip.k + nt - 1) / nt;
+        j0 = (tile - t0) * per;
+        j1 = min(ip.k, j0 + per);
+        lcg1 = fd.lcg * 6364136223846793005ull + 1442695040888963407ull;  // cu:653
+        if (j0 + (int)threadIdx.x < j1) {
+            pj = j0 + (int)threadIdx.x;
+            plink = inj_push(off, nf, lcg1, ip.gen, pj, jit, head);
+        }
+    }
     double w2;
     const float4 wv = tile_norm(ll, tile, L, f, fd, fd.uniform != 0, w2);
     *reinterpret_cast<float4*>(w + tile * kTile + (int)threadIdx.x * 4) = wv;
@@ -846,14 +870,8 @@ __global__ __launch_bounds__(kTB) void k_norm_agg(Layout L, FilterDev* F, const 
         part_n[tile] = bs;
         part_q[tile] = agg;
     }
-    const InjectParams ip = inj.get(f);
-    if (ip.k > 0) {
-        const int t0 = off / kTile, nt = L.f_padn[f] / kTile;
-        const int per = (ip.k + nt - 1) / nt;
-        const int j0 = (tile - t0) * per, j1 = min(ip.k, j0 + per);
-        const uint64_t lcg1 = fd.lcg * 6364136223846793005ull + 1442695040888963407ull;  // cu:653
-        for (int j = j0 + (int)threadIdx.x; j < j1; j += kTB) inj_pick(off, nf, lcg1, ip.gen, j, jit, head, next);
-    }
+    if (pj >= 0) next[off + pj] = plink;
+    for (int j = j0 + (int)threadIdx.x + kTB; j < j1; j += kTB) inj_pick(off, nf, lcg1, ip.gen, j, jit, head, next);
 }
 
 // Fused cycle, second pass of the resampling scan (reduce, then scan): the first pass
