@@ -342,7 +342,8 @@ using namespace pfh;
 extern "C" {
 
 void* pfh_filter_new(const pfh_config* cfg) {
-    if (!cfg || cfg->particle_count <= 0) return nullptr;
+    // The device filter indexes particles with int (pfc_init takes int n, cu:534).
+    if (!cfg || cfg->particle_count <= 0 || cfg->particle_count > INT_MAX) return nullptr;
     ParticleFilter* f = new (std::nothrow) ParticleFilter(*cfg);
     if (!f) return nullptr;
     if (!f->create()) {
@@ -594,12 +595,21 @@ int pfh_map_from_pgm_yaml(const uint8_t* pgm, size_t pgm_len, const char* yaml_c
         }
     }
     idx += 1;
-    const size_t w = (size_t)dims[0], h = (size_t)dims[1], need = w * h;
-    if (pgm_len < idx + need) {
-        set_err(err, err_len,
-                "pgm: need " + std::to_string(need) + " px, have " + std::to_string(pgm_len > idx ? pgm_len - idx : 0));
+    // Width and height are bounded before their product is formed, so nothing can wrap: both in
+    // [1, INT_MAX] (the grid is handed on as int W, H) and w * h no larger than the body.
+    if (dims[0] < 1 || dims[0] > INT_MAX || dims[1] < 1 || dims[1] > INT_MAX) {
+        set_err(err, err_len, "pgm: width and height must lie in [1, INT_MAX]");
         return -1;
     }
+    const size_t w = (size_t)dims[0], h = (size_t)dims[1];
+    const size_t have = pgm_len > idx ? pgm_len - idx : 0;
+    if (w > have / h) {
+        set_err(err, err_len,
+                "pgm: header says " + std::to_string(w) + " x " + std::to_string(h) + " px, body has " +
+                    std::to_string(have));
+        return -1;
+    }
+    const size_t need = w * h;
     *W = (int)w;
     *H = (int)h;
     if (!cells) return 0;

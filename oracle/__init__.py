@@ -78,16 +78,23 @@ def build() -> str:
     return LIB_PATH
 
 
+def load(path: str) -> C.CDLL:
+    """Loads an oracle library built from pf_oracle.c + mlp_oracle.c and binds its signatures
+    (the mutation tests load deliberately broken builds of the same sources this way)."""
+    L = C.CDLL(path)
+    for name, res, args in _SIGS:
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    return L
+
+
 def lib() -> C.CDLL:
     global _L
     if _L is None:
         if not os.path.exists(LIB_PATH):
             build()
-        _L = C.CDLL(LIB_PATH)
-        for name, res, args in _SIGS:
-            fn = getattr(_L, name)
-            fn.restype = res
-            fn.argtypes = args
+        _L = load(LIB_PATH)
     return _L
 
 
@@ -118,8 +125,8 @@ def num_threads() -> int:
 class OracleFilter:
     """cpu.rs restated in C (f64); f32_storage rounds poses/weights to f32 after every stage."""
 
-    def __init__(self, n: int, seed: int, f32_storage: bool = True):
-        self.L = lib()
+    def __init__(self, n: int, seed: int, f32_storage: bool = True, library: C.CDLL | None = None):
+        self.L = library if library is not None else lib()
         self.n = n
         self.h = self.L.or_create(n, seed, 1 if f32_storage else 0)
 
@@ -196,9 +203,9 @@ class OracleFilter:
                 "att_cov": e[16:25].reshape(3, 3), "ess": float(e[25])}
 
 
-def particle_normals(gseed: int, i: int) -> np.ndarray:
+def particle_normals(gseed: int, i: int, library: C.CDLL | None = None) -> np.ndarray:
     o = np.zeros(6)
-    lib().or_particle_normals(gseed, i, _d(o))
+    (library if library is not None else lib()).or_particle_normals(gseed, i, _d(o))
     return o
 
 

@@ -67,8 +67,15 @@ def test_pgm_yaml_errors():
     yaml = "resolution: 0.1\norigin: [0, 0, 0]\n"
     with pytest.raises(ValueError, match="expected P5"):
         load_map_pgm_yaml(b"P2\n3 2\n255\n" + bytes(6), yaml)
-    with pytest.raises(ValueError, match="need 6 px"):
-        load_map_pgm_yaml(pgm_bytes(px)[:-2], yaml)
+    with pytest.raises(ValueError, match="3 x 2 px, body has 4"):
+        load_map_pgm_yaml(pgm_bytes(px)[:-2], yaml)  # truncated body
+    # dimensions whose product wraps a 64-bit size (2^32 x 2^32) or exceeds int are rejected
+    # before any pixel is read; zero dimensions likewise
+    for dims in (b"4294967296 4294967296", b"2147483648 1", b"1 2147483648", b"0 5", b"5 0"):
+        with pytest.raises(ValueError, match=r"\[1, INT_MAX\]"):
+            load_map_pgm_yaml(b"P5\n" + dims + b"\n255\n" + bytes(64), yaml)
+    with pytest.raises(ValueError, match="body has 64"):
+        load_map_pgm_yaml(b"P5\n2147483647 2147483647\n255\n" + bytes(64), yaml)
     with pytest.raises(ValueError, match="no resolution"):
         load_map_pgm_yaml(pgm_bytes(px), "origin: [0, 0, 0]\n")
     with pytest.raises(ValueError, match="no origin"):
