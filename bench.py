@@ -177,13 +177,8 @@ def run_batched(nf=64, cycles=100, warmup=2, seed=11):
     tile) against the same 64 filters run one after another through the plain path."""
     import ctypes as C
     from cuda_pf_localization_amd import OBS_DTYPE, BatchedFilters, ParticleFilter, PfConfig, _lib
-    from cuda_pf_localization_amd.sim import helix_arrays
-    rng = np.random.default_rng(seed)
-    ns = np.exp(rng.uniform(math.log(1000), math.log(500_000), nf)).astype(int)
-    ls = rng.integers(4, 257, nf)
-    phases = rng.uniform(0.0, 2 * math.pi, nf)
-    hs = [helix_arrays(int(ls[f]), (20.0, 20.0, 5.0), cycles, seed * 1000 + f, phase=float(phases[f]))
-          for f in range(nf)]
+    from cuda_pf_localization_amd.sim import config3_filters
+    ns, ls, hs = config3_filters(nf, cycles, seed)
     obs32, obs64 = [], []
     for h in hs:
         L = len(h.landmarks)

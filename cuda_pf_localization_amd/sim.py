@@ -419,3 +419,16 @@ def helix_arrays(n_landmarks: int, box, cycles: int, seed: int, phase: float = 0
         truth[k - 1] = npos
     p0, q0 = pose_at(0)
     return HelixArrays(lm, tuple(p0), q0, ot, oq, ov, orr, truth, odo_pos, odo_rot)
+
+
+def config3_filters(nf: int = 64, cycles: int = 100, seed: int = 11):
+    """The batched-filters workload: nf independent filters, particle counts log-uniform in
+    [1k, 500k], landmark counts uniform in {4..256} in per-filter 20 x 20 x 5 m boxes, each on
+    its own helix with a random phase (config-0 noise).  Returns (ns, ls, [HelixArrays])."""
+    rng = np.random.default_rng(seed)
+    ns = np.exp(rng.uniform(math.log(1000), math.log(500_000), nf)).astype(int)
+    ls = rng.integers(4, 257, nf)
+    phases = rng.uniform(0.0, 2 * math.pi, nf)
+    hs = [helix_arrays(int(ls[f]), (20.0, 20.0, 5.0), cycles, seed * 1000 + f, phase=float(phases[f]))
+          for f in range(nf)]
+    return ns, ls, hs

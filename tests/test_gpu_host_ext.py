@@ -30,6 +30,33 @@ def test_gl_to_pf_handoff(gpu):
     assert abs(d) < 2e-3 and abs(q[1]) < 2e-3 and abs(q[2]) < 2e-3
 
 
+def test_handoff_equals_oracle_reinit(gpu):
+    """The hand-off is reinitialize() about (x, y, z) with the yaw as a rotation about z
+    (README:84-86): bit-exact against the oracle's reinit (cpu.rs:180-195) with the same pose
+    after the host layer's f32 casts (cuda.rs:187-205)."""
+    from oracle import OracleFilter
+    n = 100_000
+    xyt = (4.2, 6.7, -2.3)
+    ps, rs = (0.05, 0.05, 0.02), (0.01, 0.01, 0.02)
+    pf = ParticleFilter(PfConfig(particle_count=n, seed=3))
+    pf.reinit_from_gl(xyt, z=1.25, pos_std=ps, rot_std=rs)
+    pos, quat, w = pf.snapshot()
+    f32 = lambda v: np.asarray(v, np.float32).astype(np.float64)
+    pose = f32([xyt[0], xyt[1], 1.25, math.cos(0.5 * xyt[2]), 0.0, 0.0, math.sin(0.5 * xyt[2])])
+    o = OracleFilter(n, 3, True)
+    o.reinit(pose, f32(ps), f32(rs))
+    opos, oquat, ow, _ = o.state()
+    assert np.array_equal(pos, opos) and np.array_equal(quat, oquat)
+    assert np.all(w == np.float32(1.0 / n))
+
+
+def test_particle_count_beyond_int_is_rejected(gpu):
+    """The device filter indexes particles with int (pfc_init, cu:534): a larger count is an
+    error, not a silently truncated filter."""
+    with pytest.raises(RuntimeError):
+        ParticleFilter(PfConfig(particle_count=2 ** 32 + 5, seed=1))
+
+
 def _steps(sc, k0, k1):
     out = []
     for st in sc.steps[k0:k1]:

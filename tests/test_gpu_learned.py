@@ -150,9 +150,30 @@ def test_refine_children(gpu):
     assert gk.topk_ex(1)[1][0] <= caer[0]
 
 
+def _localize_runs(gl, net, truths, scans, lp, seed0):
+    """Localises every scan with the learned proposal (LearnedParams lp); returns (successes,
+    fallbacks, ms per fix)."""
+    ok = fb = 0
+    t = 0.0
+    for i, (tr, s) in enumerate(zip(truths, scans)):
+        lp.seed = seed0 + i
+        a = time.perf_counter()
+        top, info = gl.localize_learned(net, s, lp)
+        t += time.perf_counter() - a
+        ok += math.hypot(top[0, 0] - tr[0], top[0, 1] - tr[1]) < 0.5
+        fb += info["used_fallback"]
+    return ok, fb, t * 1000.0 / len(scans)
+
+
 def test_learned_localization_ab(gpu):
-    """Config-4 style A/B on one map: trained proposal (8k mixture + 2k uniform, uniform
-    fallback) against 200k uniform hypotheses, same scans, same success criterion."""
+    """Config-4 A/B on one map, same scans and success criterion (position within 0.5 m):
+      learned     trained proposal, 8k mixture + 2k uniform, refinement, uniform fallback;
+      no network  the same pipeline with the mixture replaced by uniform hypotheses (10k
+                  uniform + refinement + fallback): what the trained network itself adds;
+      no refine   config 4's plain 8k + 2k with the fallback (no refinement rounds);
+      uniform     200k uniform hypotheses (the reference's GL path, gl.rs:305-327).
+    The learned proposal must fix >= 95 % of the poses at least 5x faster than 200k uniform
+    dispersal, and need fewer fallbacks than the same pipeline without the network."""
     cells, res, org = obstacle_map(100)
     gl = GlobalLoc(cells, res, org, GlParams(hypotheses=200_000, rmax=8.0, topk=64, seed=3))
     net = LearnedProposal(*FULL, max_batch=4096, seed=1)
@@ -164,26 +185,25 @@ def test_learned_localization_ab(gpu):
     rng = np.random.default_rng(100)
     scans = [(og.map_scan64(*t, 360, 2 * math.pi, 8.0) + rng.normal(size=360) * 0.02).astype(np.float32)
              for t in truths]
-    lp = LearnedParams(seed=4)
-    gl.localize_learned(net, scans[0], lp)  # warm-up
+    gl.localize_learned(net, scans[0], LearnedParams(seed=4))  # warm-up
     gl.localize(scans[0])
-    ok_l = ok_u = fb = 0
-    tl = tu = 0.0
+    ok_l, fb_l, ms_l = _localize_runs(gl, net, truths, scans, LearnedParams(), 1000)
+    ok_n, fb_n, ms_n = _localize_runs(gl, net, truths, scans, LearnedParams(n_mix=0, n_uniform=10_000), 1000)
+    ok_r, fb_r, ms_r = _localize_runs(gl, net, truths, scans, LearnedParams(refine_rounds=0), 1000)
+    ok_u = 0
+    tu = 0.0
     for i, (t, s) in enumerate(zip(truths, scans)):
-        lp.seed = 1000 + i
-        a = time.perf_counter()
-        top, info = gl.localize_learned(net, s, lp)
-        tl += time.perf_counter() - a
-        ok_l += math.hypot(top[0, 0] - t[0], top[0, 1] - t[1]) < 0.5
-        fb += info["used_fallback"]
-        print(f"pose {i}: err {math.hypot(top[0, 0] - t[0], top[0, 1] - t[1]):.3f} m, learned best "
-              f"{info['proposal_best_caer_per_ray']:.4f} m/ray, fallback {info['used_fallback']}")
         gl.set_seed(2000 + i)
         a = time.perf_counter()
         top = gl.localize(s)
         tu += time.perf_counter() - a
         ok_u += math.hypot(top[0, 0] - t[0], top[0, 1] - t[1]) < 0.5
-    print(f"train {train_s:.2f}s loss {first:.3f}->{last:.3f}; learned {ok_l}/20 in {tl * 50:.3f} ms/fix "
-          f"({fb} fallbacks); uniform {ok_u}/20 in {tu * 50:.3f} ms/fix")
+    ms_u = tu * 1000.0 / len(scans)
+    print(f"train {train_s:.2f}s loss {first:.3f}->{last:.3f}")
+    for name, ok, fb, ms in (("learned", ok_l, fb_l, ms_l), ("no network", ok_n, fb_n, ms_n),
+                             ("no refine", ok_r, fb_r, ms_r), ("uniform 200k", ok_u, 0, ms_u)):
+        print(f"{name:>12}: {ok}/20 fixes, {fb} fallbacks, {ms:.3f} ms/fix ({ms_u / ms:.2f}x uniform)")
     assert last < first
-    assert ok_l >= 18
+    assert ok_l >= 19, "learned proposal below 95 % success"
+    assert ms_u >= 5.0 * ms_l, f"learned {ms_l:.3f} ms/fix is not 5x faster than uniform {ms_u:.3f}"
+    assert fb_l < fb_n, "the network does not reduce the fallbacks of the same pipeline"
