@@ -142,6 +142,7 @@ KERNEL_SYMBOLS = [
     ("pfb_autotune", C.c_int, [vp, P(PfbCycleIn)]),
     ("pfb_get_state", C.c_int, [vp, C.c_int, P(f32), P(f32), P(f32)]),
     ("pfb_get_ancestors", C.c_int, [vp, C.c_int, P(i32)]),
+    ("pfb_get_cum", C.c_int, [vp, C.c_int, P(u64)]),
     ("pfb_profile_enable", C.c_int, [vp, C.c_int]),
     ("pfb_profile_read", C.c_int, [vp, P(f64), P(C.c_int)]),
     # learned proposal

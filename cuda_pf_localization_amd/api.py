@@ -762,6 +762,12 @@ class BatchedFilters:
         _check(self.L.pfb_get_ancestors(self.h, f, _ip(a)), "pfb_get_ancestors")
         return a
 
+    def cum(self, f: int):
+        """Fixed-point inclusive prefix of filter f's weights from the last resampling scan."""
+        a = np.zeros(self.ns[f], np.uint64)
+        _check(self.L.pfb_get_cum(self.h, f, a.ctypes.data_as(C.POINTER(C.c_uint64))), "pfb_get_cum")
+        return a
+
     def profile(self, enable: bool = True):
         _check(self.L.pfb_profile_enable(self.h, 1 if enable else 0), "pfb_profile_enable")
 

@@ -515,6 +515,14 @@ int pfb_get_ancestors(void* h, int f, int32_t* anc) {
     return 0;
 }
 
+int pfb_get_cum(void* h, int f, uint64_t* cum) {
+    Batch* b = static_cast<Batch*>(h);
+    if (!b || f < 0 || f >= b->nf || !cum) return -1;
+    PB_CHECK(hipStreamSynchronize(b->s.stream));
+    PB_CHECK(hipMemcpy(cum, b->s.cum + b->s.off[f], sizeof(uint64_t) * (size_t)b->s.n[f], hipMemcpyDeviceToHost));
+    return 0;
+}
+
 int pfb_profile_enable(void* h, int enable) {
     Batch* b = static_cast<Batch*>(h);
     if (!b) return -1;

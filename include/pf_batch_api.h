@@ -72,6 +72,10 @@ int pfb_autotune(void *h, const pfb_cycle_in *in);
 /* Particles of filter f (pos 3n, quat 4n, weights n; any may be NULL). */
 int pfb_get_state(void *h, int f, float *pos3n, float *quat4n, float *w);
 int pfb_get_ancestors(void *h, int f, int32_t *anc);
+/* Test hook: filter f's fixed-point inclusive prefix cum_j = sum_{i<=j} floor(w_i 2^56) from the
+ * last cycle's resampling scan (every filter is scanned, whether or not it resampled); the
+ * weights the resampling used are recovered exactly as (cum_j - cum_{j-1}) 2^-56. */
+int pfb_get_cum(void *h, int f, uint64_t *cum);
 /* Per-stage HIP events (PFX_STAGE_*) around the kernels of eager cycles and of staged runs
  * (not inside captured graphs); pfb_profile_read returns and resets the accumulated times. */
 int pfb_profile_enable(void *h, int enable);

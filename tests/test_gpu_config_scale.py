@@ -13,7 +13,10 @@ when ESS < N/2 (particle_filter.rs:111-124, cpu.rs:220-305).  Three drivers are 
   * the fused cycle (pfb_cycle, hipGraph), against the same oracle driver.
 Config 3: the 64-filter batch (N log-uniform in [1k, 500k], L uniform in [4, 256]) in one
 launch per stage; the smallest, the largest, the fewest-landmark filter and a filter given 4
-landmarks each run beside their own oracle.
+landmarks each run beside their own oracle.  Each cycle the weights the GPU's resampling scan
+used (recovered exactly from its fixed-point prefix) are checked against the oracle's within
+1e-5 relative and then given to the oracle, so ancestors are compared on identical weights
+(weights within 1e-5 can still move an ancestor pointer across a particle boundary).
 """
 import math
 
@@ -180,7 +183,12 @@ def test_config1_host_layer_vs_oracle_driver(gpu, config1, fast):
             assert np.array_equal(anc, o.ancestors()), f"cycle {c}: ancestors differ"
             n_rs += 1
         compare_estimate(eg.position, eg.attitude, eg.pos_cov, eg.att_cov, eg.ess, eo, f"cycle {c}")
-    pos, quat, _ = pf.snapshot()
+    # the device state itself (the host snapshot renormalises each quaternion in f64 like
+    # cuda.rs:343-356, so its values are not the stored f32 ones)
+    pos = np.zeros((N1, 3), np.float32)
+    quat = np.zeros((N1, 4), np.float32)
+    assert L.pfx_get_state(kh, pos.ctypes.data_as(_lib.P(_lib.f32)), quat.ctypes.data_as(_lib.P(_lib.f32)),
+                           None) == 0
     opos, oquat, _, _ = o.state()
     assert_poses_equal(pos, quat, opos, oquat, "final state")
     assert n_rs >= 1
@@ -268,20 +276,33 @@ def test_config3_64_filters_vs_oracle(gpu):
             o.predict(f32(h.odom_t[kc]), f32(h.odom_q[kc]), f32([h.odo_pos] * 3), f32([h.odo_rot] * 3))
             o.weight(obs_of(h, kc), h.landmarks.astype(np.float32), float(np.float32(0.05)),
                      float(np.float32(0.02)), 3.0)
+            opos, oquat, ow, _ = o.state()
             eo = o.estimate()
             rs = eo["ess"] < 0.5 * ns[f]
-            if rs:
-                o.resample(np.float32(0.0))
-                eo = o.estimate()
             e = out[f]
             what = f"filter {f} (N {ns[f]}, L {ls[f]}) cycle {kc}"
             assert bool(e.resampled) == rs, what
-            if rs:
-                assert np.array_equal(bf.ancestors(f), o.ancestors()), f"{what}: ancestors differ"
+            # the weights the GPU's resampling scan used, recovered exactly from its fixed-point
+            # prefix (q = floor(w 2^56) of an f32 weight has at most 24 significant bits)
+            q = np.diff(bf.cum(f), prepend=np.uint64(0))
+            w_used = q.astype(np.float64) * 2.0 ** -56
+            big = ow >= 2.0 ** -32
+            assert np.all(np.abs(w_used[big] - ow[big]) <= 1e-5 * ow[big]), f"{what}: scan weights"
+            assert np.all(np.abs(w_used[~big] - ow[~big]) <= 2.0 ** -56 + 1e-5 * ow[~big]), f"{what}: scan weights"
             pos, quat, w = bf.get_state(f)
-            opos, oquat, ow, _ = o.state()
-            assert_poses_equal(pos, quat, opos, oquat, what)
-            assert_weights_close(w, ow, what)
+            if rs:
+                # identical weights and offset: the oracle resamples the GPU's weights
+                o.set_state(opos, oquat, w_used)
+                o.resample(np.float32(0.0))
+                eo = o.estimate()
+                assert np.array_equal(bf.ancestors(f), o.ancestors()), f"{what}: ancestors differ"
+                opos, oquat, ow, _ = o.state()
+                assert_poses_equal(pos, quat, opos, oquat, what)
+                assert np.all(w == np.float32(1.0 / ns[f]))
+            else:
+                assert_poses_equal(pos, quat, opos, oquat, what)
+                assert_weights_close(w, ow, what)
+                o.set_state(opos, oquat, w.astype(np.float64))  # the next cycle starts from identical weights
             compare_estimate(np.array(e.pos[:]), np.array(e.quat[:]), np.array(e.pos_cov[:]).reshape(3, 3),
                              np.array(e.att_cov[:]).reshape(3, 3), e.ess, eo, what)
     bf.close()

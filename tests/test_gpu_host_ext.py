@@ -40,13 +40,18 @@ def test_handoff_equals_oracle_reinit(gpu):
     ps, rs = (0.05, 0.05, 0.02), (0.01, 0.01, 0.02)
     pf = ParticleFilter(PfConfig(particle_count=n, seed=3))
     pf.reinit_from_gl(xyt, z=1.25, pos_std=ps, rot_std=rs)
-    pos, quat, w = pf.snapshot()
+    from cuda_pf_localization_amd import _lib
+    pos = np.zeros((n, 3), np.float32)  # the stored f32 state (snapshot renormalises in f64)
+    quat = np.zeros((n, 4), np.float32)
+    w = np.zeros(n, np.float32)
+    assert _lib.kernels().pfx_get_state(pf.kernels_handle(), *(a.ctypes.data_as(_lib.P(_lib.f32))
+                                                               for a in (pos, quat, w))) == 0
     f32 = lambda v: np.asarray(v, np.float32).astype(np.float64)
     pose = f32([xyt[0], xyt[1], 1.25, math.cos(0.5 * xyt[2]), 0.0, 0.0, math.sin(0.5 * xyt[2])])
     o = OracleFilter(n, 3, True)
     o.reinit(pose, f32(ps), f32(rs))
     opos, oquat, ow, _ = o.state()
-    assert np.array_equal(pos, opos) and np.array_equal(quat, oquat)
+    assert np.array_equal(pos.astype(np.float64), opos) and np.array_equal(quat.astype(np.float64), oquat)
     assert np.all(w == np.float32(1.0 / n))
 
 

@@ -152,9 +152,10 @@ def test_refine_children(gpu):
 
 def _localize_runs(gl, net, truths, scans, lp, seed0):
     """Localises every scan with the learned proposal (LearnedParams lp); returns (successes,
-    fallbacks, ms per fix)."""
+    fallbacks, ms per fix, median best CAER per ray of the proposal before any fallback)."""
     ok = fb = 0
     t = 0.0
+    best = []
     for i, (tr, s) in enumerate(zip(truths, scans)):
         lp.seed = seed0 + i
         a = time.perf_counter()
@@ -162,7 +163,8 @@ def _localize_runs(gl, net, truths, scans, lp, seed0):
         t += time.perf_counter() - a
         ok += math.hypot(top[0, 0] - tr[0], top[0, 1] - tr[1]) < 0.5
         fb += info["used_fallback"]
-    return ok, fb, t * 1000.0 / len(scans)
+        best.append(info["proposal_best_caer_per_ray"])
+    return ok, fb, t * 1000.0 / len(scans), float(np.median(best))
 
 
 def test_learned_localization_ab(gpu):
@@ -172,8 +174,11 @@ def test_learned_localization_ab(gpu):
                   uniform + refinement + fallback): what the trained network itself adds;
       no refine   config 4's plain 8k + 2k with the fallback (no refinement rounds);
       uniform     200k uniform hypotheses (the reference's GL path, gl.rs:305-327).
+    Raw proposals (no refinement, no fallback): the trained 8k mixture + 2k uniform against 10k
+    uniform hypotheses, the best by CAER taken as the fix.
     The learned proposal must fix >= 95 % of the poses at least 5x faster than 200k uniform
-    dispersal, and need fewer fallbacks than the same pipeline without the network."""
+    dispersal; its raw proposal must fix more poses than 10k uniform hypotheses do, with a
+    closer best hypothesis."""
     cells, res, org = obstacle_map(100)
     gl = GlobalLoc(cells, res, org, GlParams(hypotheses=200_000, rmax=8.0, topk=64, seed=3))
     net = LearnedProposal(*FULL, max_batch=4096, seed=1)
@@ -187,9 +192,14 @@ def test_learned_localization_ab(gpu):
              for t in truths]
     gl.localize_learned(net, scans[0], LearnedParams(seed=4))  # warm-up
     gl.localize(scans[0])
-    ok_l, fb_l, ms_l = _localize_runs(gl, net, truths, scans, LearnedParams(), 1000)
-    ok_n, fb_n, ms_n = _localize_runs(gl, net, truths, scans, LearnedParams(n_mix=0, n_uniform=10_000), 1000)
-    ok_r, fb_r, ms_r = _localize_runs(gl, net, truths, scans, LearnedParams(refine_rounds=0), 1000)
+    ok_l, fb_l, ms_l, _ = _localize_runs(gl, net, truths, scans, LearnedParams(), 1000)
+    ok_n, fb_n, ms_n, _ = _localize_runs(gl, net, truths, scans, LearnedParams(n_mix=0, n_uniform=10_000), 1000)
+    ok_r, fb_r, ms_r, _ = _localize_runs(gl, net, truths, scans, LearnedParams(refine_rounds=0), 1000)
+    # the raw proposals alone (no refinement, no fallback): 8k mixture + 2k uniform against
+    # 10k uniform, ranked by CAER; the network is what separates them
+    never = dict(refine_rounds=0, fallback_caer_per_ray=1e30)
+    raw_l = _localize_runs(gl, net, truths, scans, LearnedParams(**never), 1000)
+    raw_n = _localize_runs(gl, net, truths, scans, LearnedParams(n_mix=0, n_uniform=10_000, **never), 1000)
     ok_u = 0
     tu = 0.0
     for i, (t, s) in enumerate(zip(truths, scans)):
@@ -203,7 +213,12 @@ def test_learned_localization_ab(gpu):
     for name, ok, fb, ms in (("learned", ok_l, fb_l, ms_l), ("no network", ok_n, fb_n, ms_n),
                              ("no refine", ok_r, fb_r, ms_r), ("uniform 200k", ok_u, 0, ms_u)):
         print(f"{name:>12}: {ok}/20 fixes, {fb} fallbacks, {ms:.3f} ms/fix ({ms_u / ms:.2f}x uniform)")
+    for name, (ok, _, ms, best) in (("raw learned", raw_l), ("raw 10k uniform", raw_n)):
+        print(f"{name:>15}: {ok}/20 fixes from the best raw hypothesis, median best CAER {best:.4f} m/ray, "
+              f"{ms:.3f} ms")
     assert last < first
     assert ok_l >= 19, "learned proposal below 95 % success"
     assert ms_u >= 5.0 * ms_l, f"learned {ms_l:.3f} ms/fix is not 5x faster than uniform {ms_u:.3f}"
-    assert fb_l < fb_n, "the network does not reduce the fallbacks of the same pipeline"
+    assert raw_l[0] >= 19 and raw_l[0] > raw_n[0], "the trained mixture does not beat 10k uniform hypotheses"
+    assert raw_l[3] < raw_n[3], "the trained mixture's best hypothesis is no closer than 10k uniform ones"
+    assert fb_l <= fb_n
