@@ -143,6 +143,7 @@ KERNEL_SYMBOLS = [
     ("pfb_get_state", C.c_int, [vp, C.c_int, P(f32), P(f32), P(f32)]),
     ("pfb_get_ancestors", C.c_int, [vp, C.c_int, P(i32)]),
     ("pfb_get_cum", C.c_int, [vp, C.c_int, P(u64)]),
+    ("pfx_est_sums_noalloc", C.c_int, [vp, vp, vp, vp, vp, vp, vp, vp, C.c_int, vp, vp]),
     ("pfb_profile_enable", C.c_int, [vp, C.c_int]),
     ("pfb_profile_read", C.c_int, [vp, P(f64), P(C.c_int)]),
     # learned proposal

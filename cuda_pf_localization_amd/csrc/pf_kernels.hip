@@ -25,6 +25,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -2260,3 +2261,112 @@ int pfx_sync(void* h) {
 }
 
 }  // extern "C"
+
+// ============================================================ estimate sums without allocation
+//
+// pfx_est_sums_noalloc runs the estimate-sum kernels of pfc_est_sums (k_est_sums, then
+// k_est_fin<14>) over particle streams the caller owns, with every buffer the kernels need
+// beyond the caller's inputs and output -- the one-filter layout tables, the filter's scalar
+// block and the per-tile f64 partials -- in the file-scope array `scratch` below.  It calls no
+// allocator, so it can run where hipMalloc cannot: under HIP graph capture, in serving loops.
+// Each GPU holds its own copy of `scratch` (about 2 MB) until the process ends; its device
+// address is looked up once per device with hipGetSymbolAddress and only that address is
+// handed to kernels and copies.  Every call on a GPU shares that GPU's copy: calls that
+// overlap on one GPU (two streams, two threads) race.
+namespace pfk {
+
+constexpr int kNoAllocMaxN = 1 << 24;  // particles per call
+constexpr size_t kNaAlign = 256;
+constexpr size_t naup(size_t b) { return (b + kNaAlign - 1) / kNaAlign * kNaAlign; }
+constexpr size_t kNaFilterOff = 0;                                     // FilterDev
+constexpr size_t kNaScalarOff = naup(sizeof(FilterDev));               // f_n, f_off, f_padn (int), f_winv (float)
+constexpr size_t kNaBlkOff = kNaScalarOff + kNaAlign;                  // blk_filter: one int per 256 particles
+constexpr size_t kNaPartOff = kNaBlkOff + naup(sizeof(int) * (kNoAllocMaxN / kTB));  // kEstN x tiles doubles
+constexpr size_t kNoAllocBytes = kNaPartOff + naup(sizeof(double) * kEstN * (kNoAllocMaxN / kTile));
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+// One-filter layout for n particles at offset 0 (every block belongs to filter 0) and the filter
+// fields k_est_sums reads (current pose set 0, no partials left by a gather).  Written on every
+// call: the array keeps the previous call's contents.
+__global__ void k_noalloc_setup(unsigned char* base, int n, int padn, int nblk) {
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    int* blk = reinterpret_cast<int*>(base + kNaBlkOff);
+    if (i < nblk) blk[i] = 0;
+    if (i == 0) {
+        int* sc = reinterpret_cast<int*>(base + kNaScalarOff);
+        sc[0] = n;
+        sc[1] = 0;
+        sc[2] = padn;
+        reinterpret_cast<float*>(sc)[3] = (float)(1.0 / (double)n);
+        FilterDev* F = reinterpret_cast<FilterDev*>(base + kNaFilterOff);
+        F->parity = 0;
+        F->est_src = 0;
+    }
+}
+
+}  // namespace pfk
+
+namespace {
+constexpr int kNaMaxDevices = 64;
+void* g_na_scratch[kNaMaxDevices] = {};
+std::mutex g_na_mutex;
+
+// Device address of `scratch` on the current device, looked up on the first call there.
+unsigned char* noalloc_scratch() {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kNaMaxDevices) {
+        fprintf(stderr, "pf_hip pfx_est_sums_noalloc: no current device\n");
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(g_na_mutex);
+    if (!g_na_scratch[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(pfk::scratch));
+        if (e != hipSuccess || !p) {
+            fprintf(stderr, "pf_hip pfx_est_sums_noalloc: hipGetSymbolAddress(scratch): %s\n", hipGetErrorString(e));
+            return nullptr;
+        }
+        g_na_scratch[dev] = p;
+    }
+    return static_cast<unsigned char*>(g_na_scratch[dev]);
+}
+}  // namespace
+
+extern "C" int pfx_est_sums_noalloc(const float* px, const float* py, const float* pz, const float* qw,
+                                    const float* qx, const float* qy, const float* qz, const float* w, int n,
+                                    double* out14, void* stream) {
+    using namespace pfk;
+    if (!px || !py || !pz || !qw || !qx || !qy || !qz || !w || !out14) return -1;
+    if (n <= 0 || n > kNoAllocMaxN) {
+        fprintf(stderr, "pf_hip pfx_est_sums_noalloc: n = %d outside [1, %d]\n", n, kNoAllocMaxN);
+        return -1;
+    }
+    unsigned char* base = noalloc_scratch();
+    if (!base) return -1;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const int padn = (n + kTile - 1) / kTile * kTile;
+    const int nblk = padn / kTB, ntile = padn / kTile;
+    hipLaunchKernelGGL(k_noalloc_setup, dim3((nblk + kTB - 1) / kTB), dim3(kTB), 0, st, base, n, padn, nblk);
+    PF_LAUNCH_CHECK();
+    int* sc = reinterpret_cast<int*>(base + kNaScalarOff);
+    Layout L{};
+    L.blk_filter = reinterpret_cast<const int*>(base + kNaBlkOff);
+    L.f_n = sc;
+    L.f_off = sc + 1;
+    L.f_padn = sc + 2;
+    L.f_winv = reinterpret_cast<const float*>(sc + 3);
+    FilterDev* F = reinterpret_cast<FilterDev*>(base + kNaFilterOff);
+    double* part = reinterpret_cast<double*>(base + kNaPartOff);
+    PoseBufs B{};
+    const float* c[7] = {px, py, pz, qw, qx, qy, qz};
+    for (int k = 0; k < 7; ++k) B.s[0].c[k] = B.s[1].c[k] = const_cast<float*>(c[k]);
+    hipLaunchKernelGGL(k_est_sums, dim3(ntile), dim3(kTB), 0, st, L, F, B, w, part, ntile);
+    PF_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_est_fin<kEstN>, dim3(1), dim3(kEstFinTB), 0, st, L, F, part, ntile, 0,
+                       static_cast<CovParams*>(nullptr), 0, static_cast<unsigned long long*>(nullptr),
+                       static_cast<FilterDev*>(nullptr));
+    PF_LAUNCH_CHECK();
+    PF_CHECK(hipMemcpyAsync(out14, F->sums, sizeof(double) * kEstN, hipMemcpyDefault, st));
+    return 0;
+}

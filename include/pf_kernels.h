@@ -172,6 +172,20 @@ int pfx_set_scan_spin_limit(void *h, int limit);
 /* Blocks until every queued operation on the handle's stream has finished. */
 int pfx_sync(void *h);
 
+/* Estimate sums without allocation.  No counterpart in the reference: for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Computes pfc_est_sums' out[14] (sum w p, the 10 upper-triangle moments
+ * sum w q_r q_c, sum w^2; same kernels, same bits) for n particles in caller-owned device
+ * streams px..qz, w (f32, n each) into out14 (device memory, 14 doubles), enqueued on `stream`
+ * (a hipStream_t; NULL = the null stream) and not waited for.  The kernels' scratch (layout
+ * tables, the filter's scalars, per-1024-particle f64 partials) is a fixed device array of
+ * about 2 MB per GPU held until the process ends; no hipMalloc/hipFree is called.  Calls that
+ * overlap on one GPU share that array and race.  Returns -1 (message on stderr) for a NULL
+ * pointer or n outside [1, PFX_NOALLOC_MAX_PARTICLES]. */
+#define PFX_NOALLOC_MAX_PARTICLES 16777216
+int pfx_est_sums_noalloc(const float *px, const float *py, const float *pz, const float *qw, const float *qx,
+                         const float *qy, const float *qz, const float *w, int n, double *out14, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
