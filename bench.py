@@ -409,6 +409,40 @@ def _oracle_gl(n_hyp, n_poses, seed=3):
     return t * 1000.0 / n_poses, ok
 
 
+def _oracle_pf_bench_rows(sizes=(1_000, 10_000, 100_000, 500_000), steps_n=100):
+    """The reference pf-bench's CPU rows (crates/pf_bench/src/main.rs:21-93) on the oracle: the
+    circle scenario (100 cycles, seed 777, Cartesian observations), seed 123, range noise 0.05,
+    no gate, no injection, ESS ratio 0.5, prior (1, 1, 1) m / (0.5, 0.5, 0.5) rad, one warm-up
+    update, then the mean wall time per update over the 100 steps and the mean position error."""
+    from oracle import OBS_DTYPE, OracleFilter
+    from cuda_pf_localization_amd.sim import circle_scenario
+    sc = circle_scenario(steps_n, 777)
+    lm = np.array([sc.landmarks[i] for i in range(len(sc.landmarks))], np.float32)
+    steps = [(st, np.array([(i, m, *v, r) for (i, m, v, r) in st.obs], OBS_DTYPE)) for st in sc.steps]
+    rows = {}
+    for n in sizes:
+        o = OracleFilter(n, 123, False)
+        o.reinit(np.array([*sc.prior_pos, *sc.prior_q]), np.ones(3), np.full(3, 0.5))
+
+        def update(st, obs):
+            o.predict(st.odom_t, st.odom_q, st.trans_noise, st.rot_noise)
+            o.weight(obs, lm, 0.05, 0.02, 0.0)
+            e = o.estimate()
+            if e["ess"] < 0.5 * n:
+                o.resample(np.float32(0.0))
+                e = o.estimate()
+            return e
+
+        update(*steps[0])
+        err = 0.0
+        t0 = time.perf_counter()
+        for st, obs in steps:
+            err += math.dist(update(st, obs)["position"], st.truth_pos)
+        rows[str(n)] = {"ms_per_cycle": round((time.perf_counter() - t0) * 1000.0 / steps_n, 3),
+                        "mean_track_err_m": round(err / steps_n, 4)}
+    return rows
+
+
 def cpu_baseline(sc, n, budget_s=12.0):
     """The oracle (C restatement of cpu.rs and of the global_loc.rs CPU twin, f64, OpenMP in
     place of rayon, the same parallel/sequential split) on this host.  `value` is the config-1
@@ -429,6 +463,7 @@ def cpu_baseline(sc, n, budget_s=12.0):
         set_threads(threads)
     ms0, c0 = _oracle_pf(sc0, 10_000, 42, 0.0, 60.0, 100)
     ms_gl, ok_gl = _oracle_gl(200_000, 3)
+    pf_bench_rows = _oracle_pf_bench_rows()
     return {"value": ms1, "unit": "ms/cycle", "cores": threads, "kind": "port",
             "sample": f"{c1} cycles of the 1M-particle, 256-observation workload (oracle: C restatement of "
                       f"cpu.rs, f64, OpenMP over particles like rayon)",
@@ -441,6 +476,7 @@ def cpu_baseline(sc, n, budget_s=12.0):
                 "config0_10k_1_thread": {"ms_per_cycle": round(ms0_1t, 3), "cycles": c0_1t, "threads": 1},
                 "gl_200k_hypotheses_all_threads": {"ms_per_localize": round(ms_gl, 1), "poses": 3,
                                                    "success": f"{ok_gl}/3", "threads": threads},
+                "pf_bench_circle_cpu_rows": pf_bench_rows,
             }}
 
 
@@ -561,6 +597,11 @@ def main():
     }
     if not args.no_gl:
         line["gl"] = run_gl(200_000, 20)
+    from cuda_pf_localization_amd import pf_bench
+    line["pf_bench_circle_hip_rows"] = {}
+    for nn in (1_000, 10_000, 100_000, 500_000):
+        ms_c, err_c = pf_bench.run_pf(nn)
+        line["pf_bench_circle_hip_rows"][str(nn)] = {"ms_per_cycle": round(ms_c, 3), "mean_track_err_m": round(err_c, 4)}
     if not args.no_batched:
         line["batched"] = run_batched()
     if not args.no_learned:

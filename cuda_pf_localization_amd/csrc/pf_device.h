@@ -4,7 +4,8 @@
 // particle index): the particle key is the splitmix64 finaliser of
 // gseed*phi + i*C1 (crates/pf_core/src/cpu.rs:151-161), its stream is the splitmix64
 // sequence of src/cuda/pf_kernels.cu:323-328, and normals come from a Box-Muller
-// transform whose log, sqrt and sincos are built from IEEE fma/mul/add only.  Every
+// transform whose log, sqrt and sincos are built from IEEE fma/mul/add and integer
+// operations only.  Every
 // operation here is explicit (the translation units are compiled with
 // -ffp-contract=off), so a CPU evaluation of the same sequence is bit-identical.
 #pragma once
@@ -43,62 +44,6 @@ __device__ __forceinline__ double sconst(double c) {
 }
 
 __device__ inline double u01_co(uint64_t z) { return (double)(z >> 11) * 0x1.0p-53; }
-__device__ inline double u01_oc(uint64_t z) { return (double)((z >> 11) + 1) * 0x1.0p-53; }
-
-// Natural log of a positive normal double: exponent split, m in [1/sqrt2, sqrt2),
-// ln m = 2 atanh(s), s = (m-1)/(m+1) with a Newton reciprocal.
-__device__ inline double det_log(double x) {
-    uint64_t b = __double_as_longlong(x);
-    int e = (int)((b >> 52) & 0x7ff) - 1023;
-    double m = __longlong_as_double((long long)((b & 0x000FFFFFFFFFFFFFull) | 0x3FF0000000000000ull));
-    if (m > 1.4142135623730951) {
-        m = m * 0.5;
-        e += 1;
-    }
-    const double f = m - 1.0;
-    const double d = m + 1.0;
-    double y = __fma_rn(-0.25, d, 1.0);
-#pragma unroll
-    for (int it = 0; it < 5; ++it) {
-        const double r = __fma_rn(-d, y, 1.0);
-        y = __fma_rn(y, r, y);
-    }
-    const double s = f * y;
-    const double s2 = s * s;
-    double p = 1.0 / 23.0;
-    p = __fma_rn(p, s2, 1.0 / 21.0);
-    p = __fma_rn(p, s2, 1.0 / 19.0);
-    p = __fma_rn(p, s2, 1.0 / 17.0);
-    p = __fma_rn(p, s2, 1.0 / 15.0);
-    p = __fma_rn(p, s2, 1.0 / 13.0);
-    p = __fma_rn(p, s2, 1.0 / 11.0);
-    p = __fma_rn(p, s2, 1.0 / 9.0);
-    p = __fma_rn(p, s2, 1.0 / 7.0);
-    p = __fma_rn(p, s2, 1.0 / 5.0);
-    p = __fma_rn(p, s2, 1.0 / 3.0);
-    const double lnm = 2.0 * __fma_rn(s * s2, p, s);
-    const double de = (double)e;
-    const double r = __fma_rn(de, 1.9082149292705877e-10, lnm);
-    return __fma_rn(de, 0.6931471803691238, r);
-}
-
-// Square root by an integer-seeded reciprocal-sqrt Newton iteration.
-__device__ inline double det_sqrt(double x) {
-    if (!(x > 0.0)) return 0.0;
-    uint64_t b = 0x5FE6EB50C7B537A9ull - (__double_as_longlong(x) >> 1);
-    double y = __longlong_as_double((long long)b);
-    const double h = 0.5 * x;
-#pragma unroll
-    for (int it = 0; it < 5; ++it) {
-        const double t = h * y;
-        const double r = __fma_rn(-t, y, 0.5);
-        y = __fma_rn(y, r, y);
-    }
-    const double s = x * y;
-    const double r = __fma_rn(-s, s, x);
-    return __fma_rn(r, 0.5 * y, s);
-}
-
 // sin and cos of x radians: pi/2 Cody-Waite reduction, Taylor on |r| <= pi/4.
 __device__ inline void det_sincos(double x, double& so, double& co) {
     const double k = floor(x * 0.63661977236758138 + 0.5);
@@ -177,20 +122,93 @@ __device__ inline void det_cos_sinc(double h2, double& co, double& sinc) {
     sinc = s;
 }
 
-// Six standard normals of particle i (three Box-Muller pairs).
+// Motion-noise draws.  Normals only need f32-level accuracy (the poses they perturb are stored
+// as f32), so the log, square root and sine/cosine of the Box-Muller transform are short
+// fma/mul/add sequences (~1e-9 relative) evaluated from the integer words directly.
+
+// -2 ln((a + 1) 2^-32): k = a + 1 = 2^e m, m in [sqrt(1/2), sqrt(2)), ln m = 2 atanh(s) with
+// s = (m - 1)/(m + 1); 1/(m + 1) from a linear minimax seed (relative error 0.015) and two
+// Newton steps; the atanh series to s^9.
+__device__ inline double mot_neg2ln(uint32_t a) {
+    const double k = (double)a + 1.0;
+    const uint64_t b = __double_as_longlong(k);
+    int e = (int)(b >> 52) - 1023;
+    double m = __longlong_as_double((long long)((b & 0x000FFFFFFFFFFFFFull) | 0x3FF0000000000000ull));
+    if (m > 1.4142135623730951) {
+        m = m * 0.5;
+        e += 1;
+    }
+    const double f = m - 1.0;
+    const double d = m + 1.0;
+    double y = __fma_rn(d, -0.23901599922648442, 0.9850615000483455);
+    y = __fma_rn(y, __fma_rn(-d, y, 1.0), y);
+    y = __fma_rn(y, __fma_rn(-d, y, 1.0), y);
+    const double s = f * y;
+    const double s2 = s * s;
+    double p = __fma_rn(s2, 1.0 / 9.0, 1.0 / 7.0);
+    p = __fma_rn(p, s2, 1.0 / 5.0);
+    p = __fma_rn(p, s2, 1.0 / 3.0);
+    const double h = __fma_rn(s * s2, p, s);  // ln(m) / 2
+    return __fma_rn((double)(32 - e), 1.3862943611198906, -4.0 * h);
+}
+
+// sqrt(x), x in [0, 45]: three Newton steps on the reciprocal square root from the integer seed
+// (~1e-11 relative); x = 0 gives 0 (the seed stays finite).
+__device__ inline double mot_sqrt(double x) {
+    double y = __longlong_as_double((long long)(0x5FE6EB50C7B537A9ull - (__double_as_longlong(x) >> 1)));
+    const double h = 0.5 * x;
+#pragma unroll
+    for (int it = 0; it < 3; ++it) {
+        const double t = h * y;
+        const double r = __fma_rn(-t, y, 0.5);
+        y = __fma_rn(y, r, y);
+    }
+    return x * y;
+}
+
+// sin and cos of the angle 2 pi b / 2^32: octant o = b >> 29, the 29-bit remainder reflected in
+// odd octants, x = remainder * pi / 2^31 in [0, pi/4]; Taylor sin to x^9, cos to x^10 (~2e-9);
+// the octant picks, swaps and negates.
+__device__ inline void mot_sincos_turn(uint32_t b, double& so, double& co) {
+    const uint32_t o = b >> 29;
+    const uint32_t fr = b & 0x1FFFFFFFu;
+    const uint32_t odd = o & 1u;
+    const uint32_t fx = odd ? 0x20000000u - fr : fr;
+    const double x = (double)fx * 1.4629180792671596e-09;
+    const double x2 = x * x;
+    double ps = __fma_rn(x2, 1.0 / 362880.0, -1.0 / 5040.0);
+    ps = __fma_rn(ps, x2, 1.0 / 120.0);
+    ps = __fma_rn(ps, x2, -1.0 / 6.0);
+    const double sx = __fma_rn(x * x2, ps, x);
+    double pc = __fma_rn(x2, -1.0 / 3628800.0, 1.0 / 40320.0);
+    pc = __fma_rn(pc, x2, -1.0 / 720.0);
+    pc = __fma_rn(pc, x2, 1.0 / 24.0);
+    pc = __fma_rn(pc, x2, -0.5);
+    const double cx = __fma_rn(x2, pc, 1.0);
+    const uint32_t q = o >> 1;
+    const uint32_t swap = odd ^ (q & 1u);
+    const double sv = swap ? cx : sx;
+    const double cv = swap ? sx : cx;
+    so = (q >> 1) ? -sv : sv;
+    co = ((q ^ (q >> 1)) & 1u) ? -cv : cv;
+}
+
+// Six standard normals of particle i: the particle key and the next two outputs of its
+// splitmix64 stream give six 32-bit words; Box-Muller pair p takes its radius from the high
+// half of value p, its angle from the low half.
 __device__ inline void particle_normals(uint64_t gseed, uint64_t i, double n6[6]) {
     uint64_t x = particle_key(gseed, i);
+    uint64_t z[3];
+    z[0] = x;
+    z[1] = sm_next(x);
+    z[2] = sm_next(x);
 #pragma unroll
     for (int p = 0; p < 3; ++p) {
-        const uint64_t za = sm_next(x);
-        const uint64_t zb = sm_next(x);
-        const double u1 = u01_oc(za);
-        const double u2 = u01_co(zb);
-        const double r = det_sqrt(-2.0 * det_log(u1));
-        double s, c;
-        det_sincos(u2 * 6.283185307179586, s, c);
-        n6[2 * p] = r * c;
-        n6[2 * p + 1] = r * s;
+        const double r = mot_sqrt(mot_neg2ln((uint32_t)(z[p] >> 32)));
+        double sn, cs;
+        mot_sincos_turn((uint32_t)z[p], sn, cs);
+        n6[2 * p] = r * cs;
+        n6[2 * p + 1] = r * sn;
     }
 }
 
@@ -198,35 +216,35 @@ struct dquat {
     double w, x, y, z;
 };
 
-// Hamilton product a (x) b.
+// Hamilton product a (x) b, each component one product and three fmas.
 __device__ inline dquat qmul(const dquat& a, const dquat& b) {
     dquat r;
-    r.w = a.w * b.w - a.x * b.x - a.y * b.y - a.z * b.z;
-    r.x = a.w * b.x + a.x * b.w + a.y * b.z - a.z * b.y;
-    r.y = a.w * b.y - a.x * b.z + a.y * b.w + a.z * b.x;
-    r.z = a.w * b.z + a.x * b.y - a.y * b.x + a.z * b.w;
+    r.w = __fma_rn(a.w, b.w, -__fma_rn(a.x, b.x, __fma_rn(a.y, b.y, a.z * b.z)));
+    r.x = __fma_rn(a.w, b.x, __fma_rn(a.x, b.w, __fma_rn(a.y, b.z, -(a.z * b.y))));
+    r.y = __fma_rn(a.w, b.y, __fma_rn(-a.x, b.z, __fma_rn(a.y, b.w, a.z * b.x)));
+    r.z = __fma_rn(a.w, b.z, __fma_rn(a.x, b.y, __fma_rn(-a.y, b.x, a.z * b.w)));
     return r;
 }
 
-// v' = v + w t + qv x t with t = 2 qv x v (the rotation formula of the reference:
-// pf_kernels.cu:119-131, nalgebra UnitQuaternion::transform_vector).
+// v' = v + 2 (w c + u x c) with c = u x v, u the vector part: the rotation formula of the
+// reference (pf_kernels.cu:119-131, nalgebra UnitQuaternion::transform_vector, t = 2 c), in fmas.
 __device__ inline void qrot(const dquat& q, double vx, double vy, double vz, double& ox, double& oy,
                             double& oz) {
-    const double tx = 2.0 * (q.y * vz - q.z * vy);
-    const double ty = 2.0 * (q.z * vx - q.x * vz);
-    const double tz = 2.0 * (q.x * vy - q.y * vx);
-    const double cx = q.y * tz - q.z * ty;
-    const double cy = q.z * tx - q.x * tz;
-    const double cz = q.x * ty - q.y * tx;
-    ox = vx + q.w * tx + cx;
-    oy = vy + q.w * ty + cy;
-    oz = vz + q.w * tz + cz;
+    const double cx = __fma_rn(q.y, vz, -(q.z * vy));
+    const double cy = __fma_rn(q.z, vx, -(q.x * vz));
+    const double cz = __fma_rn(q.x, vy, -(q.y * vx));
+    const double dx = __fma_rn(q.y, cz, -(q.z * cy));
+    const double dy = __fma_rn(q.z, cx, -(q.x * cz));
+    const double dz = __fma_rn(q.x, cy, -(q.y * cx));
+    ox = __fma_rn(2.0, __fma_rn(q.w, cx, dx), vx);
+    oy = __fma_rn(2.0, __fma_rn(q.w, cy, dy), vy);
+    oz = __fma_rn(2.0, __fma_rn(q.w, cz, dz), vz);
 }
 
 // Axis-angle noise quaternion exp(a/2) = (cos h, a sinc(h)/2), |a| = 2h (cpu.rs:30-42).
 __device__ inline dquat noise_dq(double sx, double sy, double sz, const double* n3) {
     const double ax = sx * n3[0], ay = sy * n3[1], az = sz * n3[2];
-    const double ang2 = ax * ax + ay * ay + az * az;
+    const double ang2 = __fma_rn(ax, ax, __fma_rn(ay, ay, az * az));
     dquat d{1.0, 0.0, 0.0, 0.0};
     if (ang2 > 1e-24) {
         double c, s;

@@ -56,25 +56,54 @@ using namespace pfd;
 
 namespace pfk {
 
+// Read-only tables and parameter blocks indexed by a wave-uniform value are read through the
+// constant address space, so the loads become scalar (s_load) reads through the scalar cache,
+// off the vector memory path and out of the vector registers.
+#if defined(__HIP_DEVICE_COMPILE__)
+template <class T>
+using ConstPtr = const T __attribute__((address_space(4)))*;
+#else
+template <class T>
+using ConstPtr = const T*;  // host pass of the single-source compile: never executed
+#endif
+
+// Parameter block of filter f (wave-uniform): from the per-filter device array or the launch
+// argument.
+template <class T>
+__device__ inline T uniform_params(const ParamRef<T>& A, int f) {
+    if (A.dev) return ((ConstPtr<T>)A.dev)[f];
+    return A.val;
+}
+
 // ============================================================================ kernels
 
-// Predict (cpu.rs:197-218) of local particle li stored at gi: the pose is read from S, moved,
-// rounded to f32 and written back; v receives the stored values (x y z w qx qy qz).
-__device__ inline void predict_pose(const MotionParams& p, const PoseSet& S, int gi, int li, float (&v)[7]) {
+// Reinit (cpu.rs:180-195) or predict (cpu.rs:197-218) of local particle li stored at gi: the
+// new pose, rounded to f32, in v (x y z w qx qy qz).  Nothing is stored here.
+__device__ inline void motion_pose(const MotionParams& p, const PoseSet& S, int gi, int li, int reinit,
+                                   float (&v)[7]) {
     double nn[6];
     particle_normals(p.gseed, (uint64_t)li, nn);
-    const dquat q{(double)S.c[3][gi], (double)S.c[4][gi], (double)S.c[5][gi], (double)S.c[6][gi]};
-    const double bx = p.t[0] + p.s_t[0] * nn[0];
-    const double by = p.t[1] + p.s_t[1] * nn[1];
-    const double bz = p.t[2] + p.s_t[2] * nn[2];
-    double wx, wy, wz;
-    qrot(q, bx, by, bz, wx, wy, wz);
-    const double x = (double)S.c[0][gi] + wx;
-    const double y = (double)S.c[1][gi] + wy;
-    const double z = (double)S.c[2][gi] + wz;
-    const dquat qo{p.q[0], p.q[1], p.q[2], p.q[3]};
-    const dquat dq = noise_dq(p.s_r[0], p.s_r[1], p.s_r[2], nn + 3);
-    const dquat qn = qmul(qmul(q, qo), dq);
+    dquat q, qn;
+    double x, y, z;
+    if (reinit) {
+        x = __fma_rn(p.s_t[0], nn[0], p.t[0]);
+        y = __fma_rn(p.s_t[1], nn[1], p.t[1]);
+        z = __fma_rn(p.s_t[2], nn[2], p.t[2]);
+        q = dquat{p.q[0], p.q[1], p.q[2], p.q[3]};
+        qn = qmul(q, noise_dq(p.s_r[0], p.s_r[1], p.s_r[2], nn + 3));
+    } else {
+        q = dquat{(double)S.c[3][gi], (double)S.c[4][gi], (double)S.c[5][gi], (double)S.c[6][gi]};
+        const double bx = __fma_rn(p.s_t[0], nn[0], p.t[0]);
+        const double by = __fma_rn(p.s_t[1], nn[1], p.t[1]);
+        const double bz = __fma_rn(p.s_t[2], nn[2], p.t[2]);
+        double wx, wy, wz;
+        qrot(q, bx, by, bz, wx, wy, wz);
+        x = (double)S.c[0][gi] + wx;
+        y = (double)S.c[1][gi] + wy;
+        z = (double)S.c[2][gi] + wz;
+        const dquat qo{p.q[0], p.q[1], p.q[2], p.q[3]};
+        qn = qmul(qmul(q, qo), noise_dq(p.s_r[0], p.s_r[1], p.s_r[2], nn + 3));
+    }
     v[0] = (float)x;
     v[1] = (float)y;
     v[2] = (float)z;
@@ -82,43 +111,27 @@ __device__ inline void predict_pose(const MotionParams& p, const PoseSet& S, int
     v[4] = (float)qn.x;
     v[5] = (float)qn.y;
     v[6] = (float)qn.z;
-#pragma unroll
-    for (int c = 0; c < 7; ++c) S.c[c][gi] = v[c];
 }
 
-// Reinit (cpu.rs:180-195) and predict (cpu.rs:197-218), one particle per lane.
+// Reinit (reinit = 1) and predict, one particle per lane; the filter, its layout entries and its
+// parameter block are wave-uniform scalar loads.
+template <int reinit>
 __global__ __launch_bounds__(kTB) void k_motion(Layout L, PoseBufs B, FilterDev* F, float* __restrict__ w,
-                                                ParamRef<MotionParams> A, int reinit) {
+                                                ParamRef<MotionParams> A) {
     const int blk = blockIdx.x;
-    const int f = L.blk_filter[blk];
-    const int off = L.f_off[f];
+    const int f = ((ConstPtr<int>)L.blk_filter)[blk];
+    const int off = ((ConstPtr<int>)L.f_off)[f];
     const int li = blk * kTB + (int)threadIdx.x - off;
-    if (li >= L.f_n[f]) return;
-    if (li == 0) F[f].est_src = 0;  // poses change: partials of an earlier gather are stale
-    const MotionParams& p = A.get(f);
+    if (li >= ((ConstPtr<int>)L.f_n)[f]) return;
     const int gi = off + li;
-    const PoseSet S = B.s[F[f].parity];
-    if (reinit) {
-        double nn[6];
-        particle_normals(p.gseed, (uint64_t)li, nn);
-        const double x = p.t[0] + p.s_t[0] * nn[0];
-        const double y = p.t[1] + p.s_t[1] * nn[1];
-        const double z = p.t[2] + p.s_t[2] * nn[2];
-        const dquat q0{p.q[0], p.q[1], p.q[2], p.q[3]};
-        const dquat dq = noise_dq(p.s_r[0], p.s_r[1], p.s_r[2], nn + 3);
-        const dquat q = qmul(q0, dq);
-        S.c[0][gi] = (float)x;
-        S.c[1][gi] = (float)y;
-        S.c[2][gi] = (float)z;
-        S.c[3][gi] = (float)q.w;
-        S.c[4][gi] = (float)q.x;
-        S.c[5][gi] = (float)q.y;
-        S.c[6][gi] = (float)q.z;
-        w[gi] = p.winv;
-    } else {
-        float v[7];
-        predict_pose(p, S, gi, li, v);
-    }
+    const MotionParams p = uniform_params(A, f);
+    const PoseSet S = B.s[((ConstPtr<FilterDev>)F)[f].parity];
+    float v[7];
+    motion_pose(p, S, gi, li, reinit, v);
+    if (li == 0) F[f].est_src = 0;  // poses change: partials of an earlier gather are stale
+#pragma unroll
+    for (int c = 0; c < 7; ++c) S.c[c][gi] = v[c];
+    if (reinit) w[gi] = p.winv;
 }
 
 // Block-wide f64 reductions (4 waves): deterministic tree order.
@@ -1695,7 +1708,8 @@ void prof_end(PfState* s) {
 
 int launch_motion(PfState* s, ParamRef<MotionParams> A, int reinit) {
     prof_begin(s, reinit ? PFX_STAGE_REINIT : PFX_STAGE_PREDICT);
-    hipLaunchKernelGGL(k_motion, dim3(s->NP / kTB), dim3(kTB), 0, s->stream, s->L, s->pose, s->fdev, s->w, A, reinit);
+    hipLaunchKernelGGL(reinit ? k_motion<1> : k_motion<0>, dim3(s->NP / kTB), dim3(kTB), 0, s->stream, s->L, s->pose,
+                       s->fdev, s->w, A);
     prof_end(s);
     PF_LAUNCH_CHECK();
     return 0;

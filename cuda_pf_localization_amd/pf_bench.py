@@ -11,9 +11,10 @@ position error.  GL table (--gl, bench:96-176): room_grid, truth (6, 7, 0.8), a 
 scan of the map, 5k / 50k / 200k hypotheses with GlParams defaults, 20 localisations with
 seeds 100 + i after one warm-up.
 
-The reference also prints rows for its rayon CPU backend; that backend exists here only as the
-test oracle, which the product does not load, so only the "hip" rows are printed (bench.py
-times the oracle beside the GPU).
+The reference also prints rows for its rayon CPU backend.  That backend exists here only as the
+test oracle (a C restatement of cpu.rs), which the product never loads, so this module prints the
+"hip" rows; the repository's bench.py prints the CPU rows of the same table from the oracle
+(`cpu_baseline.legs.pf_bench_circle_cpu_rows`) beside these (`pf_bench_circle_hip_rows`).
 """
 from __future__ import annotations
 

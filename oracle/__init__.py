@@ -26,11 +26,12 @@ OBS_DTYPE = np.dtype([("id", "<i4"), ("mode", "<i4"), ("dx", "<f4"), ("dy", "<f4
                       ("range", "<f4")])
 
 _SIGS = [
-    ("or_det_log", f64, [f64]),
-    ("or_det_sqrt", f64, [f64]),
     ("or_det_sincos", None, [f64, P(f64), P(f64)]),
     ("or_det_cos_sinc", None, [f64, P(f64), P(f64)]),
     ("or_particle_normals", None, [u64, u64, P(f64)]),
+    ("or_mot_neg2ln", f64, [C.c_uint32]),
+    ("or_mot_sqrt", f64, [f64]),
+    ("or_mot_sincos_turn", None, [C.c_uint32, P(f64), P(f64)]),
     ("or_create", vp, [C.c_int, u64, C.c_int]),
     ("or_destroy", None, [vp]),
     ("or_set_rng", None, [vp, u64, u64, u64]),
@@ -207,14 +208,6 @@ def particle_normals(gseed: int, i: int, library: C.CDLL | None = None) -> np.nd
     o = np.zeros(6)
     (library if library is not None else lib()).or_particle_normals(gseed, i, _d(o))
     return o
-
-
-def det_log(x: float) -> float:
-    return lib().or_det_log(x)
-
-
-def det_sqrt(x: float) -> float:
-    return lib().or_det_sqrt(x)
 
 
 def det_sincos(x: float):

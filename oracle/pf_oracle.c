@@ -18,9 +18,11 @@
  * Both this oracle and the GPU library instead consume one specified stream:
  *   key(i)  = cpu.rs:153-159 (splitmix64 finaliser of gseed*phi + i*C1), gseed bumped as
  *             cpu.rs:136 before every reinit/predict;
- *   z_k     = splitmix64 output of key + (k+1)*phi, k = 0..5 (cu:323-328 mixer);
- *   normals = Box-Muller on pairs (z0,z1),(z2,z3),(z4,z5) with the fma-only log, sqrt and
- *             sincos below, so oracle and GPU produce bit-identical draws.
+ *   words   = key, then the splitmix64 outputs of key + phi, key + 2 phi (cu:323-328 mixer),
+ *             each split into its high and low 32 bits;
+ *   normals = Box-Muller on the three (hi, lo) pairs: radius sqrt(-2 ln((hi + 1) 2^-32)),
+ *             angle 2 pi lo 2^-32, with the short fma-only log, sqrt and sincos below
+ *             (~1e-9 relative), so oracle and GPU produce bit-identical draws.
  * With f32 storage on (the GPU's storage precision), poses and weights are rounded to
  * f32 at the end of every stage, so per-stage parity can be checked against the GPU on
  * identical inputs.
@@ -70,64 +72,6 @@ static inline uint64_t sm64_next(uint64_t *x) {
 }
 
 static inline double u01_closed_open(uint64_t z) { return (double)(z >> 11) * 0x1.0p-53; }
-static inline double u01_open_closed(uint64_t z) { return (double)((z >> 11) + 1) * 0x1.0p-53; }
-
-/* fma-only natural log for positive normal x (atanh series, Newton reciprocal). */
-double or_det_log(double x) {
-    uint64_t b;
-    memcpy(&b, &x, 8);
-    int e = (int)((b >> 52) & 0x7ff) - 1023;
-    b = (b & 0x000FFFFFFFFFFFFFull) | 0x3FF0000000000000ull;
-    double m;
-    memcpy(&m, &b, 8);
-    if (m > 1.4142135623730951) {
-        m = m * 0.5;
-        e += 1;
-    }
-    double f = m - 1.0;
-    double d = m + 1.0;
-    double y = fma(-0.25, d, 1.0);
-    for (int it = 0; it < 5; ++it) {
-        double r = fma(-d, y, 1.0);
-        y = fma(y, r, y);
-    }
-    double s = f * y;
-    double s2 = s * s;
-    double p = 1.0 / 23.0;
-    p = fma(p, s2, 1.0 / 21.0);
-    p = fma(p, s2, 1.0 / 19.0);
-    p = fma(p, s2, 1.0 / 17.0);
-    p = fma(p, s2, 1.0 / 15.0);
-    p = fma(p, s2, 1.0 / 13.0);
-    p = fma(p, s2, 1.0 / 11.0);
-    p = fma(p, s2, 1.0 / 9.0);
-    p = fma(p, s2, 1.0 / 7.0);
-    p = fma(p, s2, 1.0 / 5.0);
-    p = fma(p, s2, 1.0 / 3.0);
-    double lnm = 2.0 * fma(s * s2, p, s);
-    double de = (double)e;
-    double r = fma(de, 1.9082149292705877e-10, lnm);
-    return fma(de, 0.6931471803691238, r);
-}
-
-/* fma-only square root (rsqrt seed + Newton, one final correction). */
-double or_det_sqrt(double x) {
-    if (!(x > 0.0)) return 0.0;
-    uint64_t b;
-    memcpy(&b, &x, 8);
-    b = 0x5FE6EB50C7B537A9ull - (b >> 1);
-    double y;
-    memcpy(&y, &b, 8);
-    double h = 0.5 * x;
-    for (int it = 0; it < 5; ++it) {
-        double t = h * y;
-        double r = fma(-t, y, 0.5);
-        y = fma(y, r, y);
-    }
-    double s = x * y;
-    double r = fma(-s, s, x);
-    return fma(r, 0.5 * y, s);
-}
 
 /* fma-only sin/cos of x radians (Cody-Waite pi/2 reduction + Taylor on |r| <= pi/4). */
 void or_det_sincos(double x, double *so, double *co) {
@@ -200,17 +144,94 @@ void or_det_cos_sinc(double h2, double *co, double *sinc) {
     *sinc = s;
 }
 
-/* The 6 standard normals of particle i at generation seed gseed. */
+/* -2 ln((a + 1) 2^-32) for a 32-bit word a, ~1e-9 relative: k = a + 1 = 2^e m with m in
+ * [sqrt(1/2), sqrt(2)), ln m = 2 atanh(s), s = (m - 1)/(m + 1); the reciprocal from a linear
+ * minimax seed on [1 + sqrt(1/2), 1 + sqrt(2)] (relative error 0.015) and two Newton steps, the
+ * atanh series to s^9. */
+double or_mot_neg2ln(uint32_t a) {
+    double k = (double)a + 1.0;
+    uint64_t b;
+    memcpy(&b, &k, 8);
+    int e = (int)(b >> 52) - 1023;
+    b = (b & 0x000FFFFFFFFFFFFFull) | 0x3FF0000000000000ull;
+    double m;
+    memcpy(&m, &b, 8);
+    if (m > 1.4142135623730951) {
+        m = m * 0.5;
+        e += 1;
+    }
+    double f = m - 1.0;
+    double d = m + 1.0;
+    double y = fma(d, -0.23901599922648442, 0.9850615000483455);
+    y = fma(y, fma(-d, y, 1.0), y);
+    y = fma(y, fma(-d, y, 1.0), y);
+    double s = f * y;
+    double s2 = s * s;
+    double p = fma(s2, 1.0 / 9.0, 1.0 / 7.0);
+    p = fma(p, s2, 1.0 / 5.0);
+    p = fma(p, s2, 1.0 / 3.0);
+    double h = fma(s * s2, p, s); /* ln(m) / 2 */
+    return fma((double)(32 - e), 1.3862943611198906, -4.0 * h);
+}
+
+/* sqrt(x) for x in [0, 45] by three Newton steps on the reciprocal square root from the integer
+ * seed (~1e-11 relative); x = 0 gives 0 (the seed stays finite). */
+double or_mot_sqrt(double x) {
+    uint64_t b;
+    memcpy(&b, &x, 8);
+    b = 0x5FE6EB50C7B537A9ull - (b >> 1);
+    double y;
+    memcpy(&y, &b, 8);
+    double h = 0.5 * x;
+    for (int it = 0; it < 3; ++it) {
+        double t = h * y;
+        double r = fma(-t, y, 0.5);
+        y = fma(y, r, y);
+    }
+    return x * y;
+}
+
+/* sin and cos of the turn fraction b 2^-32 (angle 2 pi b / 2^32) from the word's bits: octant
+ * o = b >> 29, the 29-bit remainder reflected in odd octants, x in [0, pi/4] by one product;
+ * Taylor sin to x^9, cos to x^10 (~2e-9). */
+void or_mot_sincos_turn(uint32_t b, double *so, double *co) {
+    uint32_t o = b >> 29;
+    uint32_t fr = b & 0x1FFFFFFFu;
+    uint32_t odd = o & 1u;
+    uint32_t fx = odd ? 0x20000000u - fr : fr;
+    double x = (double)fx * 1.4629180792671596e-09; /* pi / 2^31 */
+    double x2 = x * x;
+    double ps = fma(x2, 1.0 / 362880.0, -1.0 / 5040.0);
+    ps = fma(ps, x2, 1.0 / 120.0);
+    ps = fma(ps, x2, -1.0 / 6.0);
+    double sx = fma(x * x2, ps, x);
+    double pc = fma(x2, -1.0 / 3628800.0, 1.0 / 40320.0);
+    pc = fma(pc, x2, -1.0 / 720.0);
+    pc = fma(pc, x2, 1.0 / 24.0);
+    pc = fma(pc, x2, -0.5);
+    double cx = fma(x2, pc, 1.0);
+    uint32_t q = o >> 1;
+    uint32_t swap = odd ^ (q & 1u);
+    double sv = swap ? cx : sx;
+    double cv = swap ? sx : cx;
+    *so = (q >> 1) ? -sv : sv;
+    *co = ((q ^ (q >> 1)) & 1u) ? -cv : cv;
+}
+
+/* The 6 standard normals of particle i at generation seed gseed (motion noise): the particle
+ * key (cpu.rs:151-161) and the next two outputs of its splitmix64 stream (cu:323-328) give six
+ * 32-bit words; Box-Muller pair p takes its radius from word 2p (hi half of a 64-bit value) and
+ * its angle from word 2p + 1 (lo half). */
 void or_particle_normals(uint64_t gseed, uint64_t i, double n6[6]) {
     uint64_t x = particle_key(gseed, i);
+    uint64_t z[3];
+    z[0] = x;
+    z[1] = sm64_next(&x);
+    z[2] = sm64_next(&x);
     for (int p = 0; p < 3; ++p) {
-        uint64_t za = sm64_next(&x);
-        uint64_t zb = sm64_next(&x);
-        double u1 = u01_open_closed(za);
-        double u2 = u01_closed_open(zb);
-        double r = or_det_sqrt(-2.0 * or_det_log(u1));
+        double r = or_mot_sqrt(or_mot_neg2ln((uint32_t)(z[p] >> 32)));
         double s, c;
-        or_det_sincos(u2 * 6.283185307179586, &s, &c);
+        or_mot_sincos_turn((uint32_t)z[p], &s, &c);
         n6[2 * p] = r * c;
         n6[2 * p + 1] = r * s;
     }
@@ -222,30 +243,30 @@ typedef struct { double w, x, y, z; } quat;
 
 static inline quat qmul(quat a, quat b) {
     quat r;
-    r.w = a.w * b.w - a.x * b.x - a.y * b.y - a.z * b.z;
-    r.x = a.w * b.x + a.x * b.w + a.y * b.z - a.z * b.y;
-    r.y = a.w * b.y - a.x * b.z + a.y * b.w + a.z * b.x;
-    r.z = a.w * b.z + a.x * b.y - a.y * b.x + a.z * b.w;
+    r.w = fma(a.w, b.w, -fma(a.x, b.x, fma(a.y, b.y, a.z * b.z)));
+    r.x = fma(a.w, b.x, fma(a.x, b.w, fma(a.y, b.z, -(a.z * b.y))));
+    r.y = fma(a.w, b.y, fma(-a.x, b.z, fma(a.y, b.w, a.z * b.x)));
+    r.z = fma(a.w, b.z, fma(a.x, b.y, fma(-a.y, b.x, a.z * b.w)));
     return r;
 }
 
-/* v' = v + w*t + qv x t, t = 2 qv x v (cu:119-131; nalgebra transform_vector). */
+/* v' = v + 2 (w c + u x c), c = u x v (cu:119-131; nalgebra transform_vector with t = 2 c). */
 static inline void qrot(quat q, const double v[3], double o[3]) {
-    double tx = 2.0 * (q.y * v[2] - q.z * v[1]);
-    double ty = 2.0 * (q.z * v[0] - q.x * v[2]);
-    double tz = 2.0 * (q.x * v[1] - q.y * v[0]);
-    double cx = q.y * tz - q.z * ty;
-    double cy = q.z * tx - q.x * tz;
-    double cz = q.x * ty - q.y * tx;
-    o[0] = v[0] + q.w * tx + cx;
-    o[1] = v[1] + q.w * ty + cy;
-    o[2] = v[2] + q.w * tz + cz;
+    double cx = fma(q.y, v[2], -(q.z * v[1]));
+    double cy = fma(q.z, v[0], -(q.x * v[2]));
+    double cz = fma(q.x, v[1], -(q.y * v[0]));
+    double dx = fma(q.y, cz, -(q.z * cy));
+    double dy = fma(q.z, cx, -(q.x * cz));
+    double dz = fma(q.x, cy, -(q.y * cx));
+    o[0] = fma(2.0, fma(q.w, cx, dx), v[0]);
+    o[1] = fma(2.0, fma(q.w, cy, dy), v[1]);
+    o[2] = fma(2.0, fma(q.w, cz, dz), v[2]);
 }
 
 /* cpu.rs:30-42 noise_dq in the deterministic formulation (cos h, a*sinc(h)/2). */
 static inline quat noise_dq(const double std3[3], const double n3[3]) {
     double ax = std3[0] * n3[0], ay = std3[1] * n3[1], az = std3[2] * n3[2];
-    double ang2 = ax * ax + ay * ay + az * az;
+    double ang2 = fma(ax, ax, fma(ay, ay, az * az));
     quat d = {1.0, 0.0, 0.0, 0.0};
     if (ang2 > 1e-24) {
         double c, s;
@@ -357,9 +378,9 @@ void or_reinit(OrFilter *f, const double pose7[7], const double pos_std[3], cons
     for (int i = 0; i < f->n; ++i) {
         double nn[6];
         or_particle_normals(gs, (uint64_t)i, nn);
-        double p0 = pose7[0] + pos_std[0] * nn[0];
-        double p1 = pose7[1] + pos_std[1] * nn[1];
-        double p2 = pose7[2] + pos_std[2] * nn[2];
+        double p0 = fma(pos_std[0], nn[0], pose7[0]);
+        double p1 = fma(pos_std[1], nn[1], pose7[1]);
+        double p2 = fma(pos_std[2], nn[2], pose7[2]);
         quat dq = noise_dq(rot_std, nn + 3);
         quat q = qmul(q0, dq);
         f->px[i] = st(f, p0);
@@ -383,7 +404,7 @@ void or_predict(OrFilter *f, const double t[3], const double qodo[4], const doub
     for (int i = 0; i < f->n; ++i) {
         double nn[6];
         or_particle_normals(gs, (uint64_t)i, nn);
-        double body[3] = {t[0] + tstd[0] * nn[0], t[1] + tstd[1] * nn[1], t[2] + tstd[2] * nn[2]};
+        double body[3] = {fma(tstd[0], nn[0], t[0]), fma(tstd[1], nn[1], t[1]), fma(tstd[2], nn[2], t[2])};
         quat q = {f->qw[i], f->qx[i], f->qy[i], f->qz[i]};
         double wd[3];
         qrot(q, body, wd);

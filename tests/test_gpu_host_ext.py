@@ -98,6 +98,45 @@ def test_checkpoint_resume_is_bit_identical(gpu, tmp_path):
         small.load(path)
 
 
+def test_pf_bench_circle_workload_matches_oracle(gpu):
+    """The pf-bench PF workload (bench:21-60: circle scenario, seed 123, Cartesian observations,
+    no gate, prior (1, 1, 1) m / (0.5, 0.5, 0.5) rad) through the host layer against the oracle
+    driver (particle_filter.rs:110-124) at 10k particles for 40 cycles: resample decisions equal,
+    means within 1e-5, covariances within 1e-4 relative."""
+    from oracle import OBS_DTYPE, OracleFilter
+    from cuda_pf_localization_amd.sim import circle_scenario
+    f32 = lambda v: np.asarray(v, np.float32).astype(np.float64)
+    n = 10_000
+    sc = circle_scenario(40, 777)
+    pf = ParticleFilter(PfConfig(particle_count=n, seed=123, range_noise=0.05))
+    pf.set_map(sc.landmarks)
+    prior = [*sc.prior_pos, *sc.prior_q]
+    pf.reinitialize(prior, [1.0] * 3, [0.5] * 3)
+    o = OracleFilter(n, 123, True)
+    o.reinit(f32(prior), f32([1.0] * 3), f32([0.5] * 3))
+    lm = np.array([sc.landmarks[i] for i in range(len(sc.landmarks))], np.float32)
+    n_rs = 0
+    for c, st in enumerate(sc.steps):
+        eg = pf.update(OdomDelta(st.odom_t, st.odom_q, st.trans_noise, st.rot_noise),
+                       [Observation(i, m, v, r) for (i, m, v, r) in st.obs])
+        o.predict(f32(st.odom_t), f32(st.odom_q), f32(st.trans_noise), f32(st.rot_noise))
+        o.weight(np.array([(i, m, *v, r) for (i, m, v, r) in st.obs], OBS_DTYPE), lm, float(np.float32(0.05)),
+                 float(np.float32(0.02)), 0.0)
+        eo = o.estimate()
+        rs = eo["ess"] < 0.5 * n
+        if rs:
+            o.resample(np.float32(0.0))
+            eo = o.estimate()
+        assert eg.resampled == rs, c
+        n_rs += rs
+        assert np.all(np.abs(eg.position - eo["position"]) < 1e-5), c
+        assert abs(eg.ess - eo["ess"]) <= 1e-4 * eo["ess"], c
+        assert min(np.abs(eg.attitude - eo["attitude"]).max(), np.abs(eg.attitude + eo["attitude"]).max()) < 1e-5
+        assert np.allclose(eg.pos_cov, eo["pos_cov"], rtol=1e-4, atol=1e-9), c
+        assert np.allclose(eg.att_cov, eo["att_cov"], rtol=1e-4, atol=1e-9), c
+    assert n_rs > 0
+
+
 def test_pf_bench_cli_tables(gpu, capsys):
     """The pf-bench restatement prints the reference's table layout; the HIP filter tracks the
     circle scenario (bench:21-60) and the GL table localises the room_grid truth."""

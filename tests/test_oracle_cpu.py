@@ -2,7 +2,8 @@
 
 The reference ships no golden vectors (its tests are statistical plus a few known answers),
 so each oracle function is pinned by what the mathematics or the reference's own tests fix:
-  * deterministic math      vs libm (log, sqrt, sin/cos) and the literal axis-angle formula
+  * deterministic math      vs libm (motion -2 ln, sqrt, sin/cos; CBGL sin/cos) and the
+                            literal axis-angle formula
   * normals                 moments of the Box-Muller stream
   * Markley mean            vs numpy.linalg.eigh (independent eigen-solver)
   * integer systematic      == literal f64 cpu.rs:264-289 where that is exact (dyadic weights,
@@ -25,14 +26,9 @@ TAU = 2 * math.pi
 
 # ------------------------------------------------------------------ deterministic math
 
-def test_det_log_sqrt_sincos_vs_libm():
+def test_det_sincos_vs_libm():
+    """The full-precision sin/cos of the CBGL ray directions and hypotheses."""
     rng = np.random.default_rng(0)
-    for x in np.concatenate([rng.random(20000) * (1 - 2 ** -53) + 2 ** -53, [2 ** -53, 0.5, 1.0, 0.999999]]):
-        lx = math.log(x)
-        assert abs(O.det_log(x) - lx) <= 5e-16 * max(1.0, abs(lx)) + 1e-300
-    for y in np.concatenate([rng.random(20000) * 80, [1e-300, 1.0, 2.0, 73.7]]):
-        assert abs(O.det_sqrt(y) - math.sqrt(y)) <= 2.3e-16 * math.sqrt(y) + 1e-300
-    assert O.det_sqrt(0.0) == 0.0
     for a in np.concatenate([rng.random(20000) * 40 - 20, [0.0, math.pi / 4, math.pi / 2, math.pi]]):
         s, c = O.det_sincos(a)
         assert abs(s - math.sin(a)) < 3e-16 * max(1, abs(a)) and abs(c - math.cos(a)) < 3e-16 * max(1, abs(a))
@@ -48,6 +44,53 @@ def test_det_cos_sinc_is_axis_angle():
         lit_v = a / ang * math.sin(0.5 * ang)
         assert abs(c - lit_w) < 1e-13 * max(1.0, ang)
         assert np.all(np.abs(a * 0.5 * s - lit_v) < 1e-13 * max(1.0, ang))
+
+
+def _sm_final(z):
+    m = (1 << 64) - 1
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & m
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & m
+    return z ^ (z >> 31)
+
+
+def test_normals_are_box_muller_of_the_stated_words():
+    """The motion normals are the Box-Muller transform (libm log, sqrt, cos, sin here) of the
+    six 32-bit words the draw specification names: the particle key (cpu.rs:153-159) and the next
+    two splitmix64 outputs (pf_kernels.cu:323-328), radius from each value's high half, angle
+    from its low half; the short fma sequences agree with libm to 1e-7 relative."""
+    m = (1 << 64) - 1
+    phi = 0x9E3779B97F4A7C15
+    for gseed, i in [(12345, 0), (12345, 7), (0x9E37797979797979, 999_999), (42, 123_456), (1, 2 ** 31 - 1)]:
+        x = _sm_final((gseed * phi + i * 0xBF58476D1CE4E5B9) & m)
+        z = [x, _sm_final((x + phi) & m), _sm_final((x + 2 * phi) & m)]
+        want = []
+        for v in z:
+            r = math.sqrt(-2.0 * math.log(((v >> 32) + 1) / 2 ** 32))
+            th = 2 * math.pi * (v & 0xFFFFFFFF) / 2 ** 32
+            want += [r * math.cos(th), r * math.sin(th)]
+        got = O.particle_normals(gseed, i)
+        assert np.allclose(got, want, rtol=1e-7, atol=1e-9), (gseed, i, got, want)
+
+
+def test_motion_math_vs_libm():
+    """The motion draws' -2 ln, sqrt and sin/cos against libm over 100k random words and the
+    end points (words 0, 2^32 - 1, octant boundaries)."""
+    import ctypes as C
+    L = O.lib()
+    rng = np.random.default_rng(4)
+    words = [int(a) for a in rng.integers(0, 2 ** 32, 100_000)] + [0, 1, 2 ** 31, 2 ** 32 - 1]
+    for a in words:
+        ref = -2.0 * math.log((a + 1) / 2 ** 32)
+        assert abs(L.or_mot_neg2ln(a) - ref) <= 6e-8 * ref + 1e-300
+    assert L.or_mot_neg2ln(2 ** 32 - 1) == 0.0
+    for v in list(rng.random(100_000) * 45.0) + [0.0, 1e-12, 44.4]:
+        assert abs(L.or_mot_sqrt(v) - math.sqrt(v)) <= 1e-10 * math.sqrt(v)
+    s, c = C.c_double(), C.c_double()
+    for b in words + [k * 2 ** 29 + d for k in range(8) for d in (0, 1, 2 ** 29 - 1)]:
+        b &= 0xFFFFFFFF
+        L.or_mot_sincos_turn(b, C.byref(s), C.byref(c))
+        th = 2 * math.pi * b / 2 ** 32
+        assert abs(s.value - math.sin(th)) < 2e-9 and abs(c.value - math.cos(th)) < 2e-9, b
 
 
 def test_normals_moments():
