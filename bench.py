@@ -406,7 +406,9 @@ def _oracle_gl(n_hyp, n_poses, seed=3):
         top = g.localize64(scan, n_hyp, 64, seed + 100 + i, span=2 * math.pi, rmax=8.0)
         t += time.perf_counter() - a
         ok += math.hypot(top[0, 0] - tp[0], top[0, 1] - tp[1]) < 0.5
-    return t * 1000.0 / n_poses, ok
+    # algorithmic work of one scoring pass: DDA cell steps of the twin over its own hypotheses
+    steps = g.dda_steps64(g.disperse64(seed + 100, n_hyp), 360, 2 * math.pi, 8.0)
+    return t * 1000.0 / n_poses, ok, steps / (n_hyp * 360.0)
 
 
 def _oracle_pf_bench_rows(sizes=(1_000, 10_000, 100_000, 500_000), steps_n=100):
@@ -462,7 +464,7 @@ def cpu_baseline(sc, n, budget_s=12.0):
     finally:
         set_threads(threads)
     ms0, c0 = _oracle_pf(sc0, 10_000, 42, 0.0, 60.0, 100)
-    ms_gl, ok_gl = _oracle_gl(200_000, 3)
+    ms_gl, ok_gl, steps_per_ray = _oracle_gl(200_000, 3)
     pf_bench_rows = _oracle_pf_bench_rows()
     return {"value": ms1, "unit": "ms/cycle", "cores": threads, "kind": "port",
             "sample": f"{c1} cycles of the 1M-particle, 256-observation workload (oracle: C restatement of "
@@ -475,7 +477,8 @@ def cpu_baseline(sc, n, budget_s=12.0):
                 "config0_10k_all_threads": {"ms_per_cycle": round(ms0, 3), "cycles": c0, "threads": threads},
                 "config0_10k_1_thread": {"ms_per_cycle": round(ms0_1t, 3), "cycles": c0_1t, "threads": 1},
                 "gl_200k_hypotheses_all_threads": {"ms_per_localize": round(ms_gl, 1), "poses": 3,
-                                                   "success": f"{ok_gl}/3", "threads": threads},
+                                                   "success": f"{ok_gl}/3", "threads": threads,
+                                                   "dda_cell_steps_per_ray": round(steps_per_ray, 3)},
                 "pf_bench_circle_cpu_rows": pf_bench_rows,
             }}
 
@@ -608,6 +611,10 @@ def main():
         line["learned"] = run_learned()
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(sc, n, args.cpu_budget)
+        if "gl" in line and line["gl"].get("rays_per_s"):
+            spr = line["cpu_baseline"]["legs"]["gl_200k_hypotheses_all_threads"]["dda_cell_steps_per_ray"]
+            line["gl"]["dda_cell_steps_per_ray_algorithmic"] = spr
+            line["gl"]["cell_steps_per_s"] = round(line["gl"]["rays_per_s"] * spr, 1)
     print(json.dumps(line), flush=True)
 
 

@@ -57,6 +57,8 @@ _SIGS = [
     ("or_disperse64", C.c_int, [C.c_int, C.c_int, f64, f64, f64, P(C.c_uint8), u64, C.c_int, P(f64)]),
     ("or_caer64_batch", None, [C.c_int, C.c_int, f64, f64, f64, P(C.c_uint8), P(f64), C.c_int, f64, f64, P(f64),
                                C.c_int, P(f64)]),
+    ("or_dda64_steps", C.c_ulonglong, [C.c_int, C.c_int, f64, f64, f64, P(C.c_uint8), P(f64), C.c_int, C.c_int, f64,
+                                       f64]),
     ("or_gen_hyp32", None, [P(i32), C.c_int, f32, f32, f32, u64, C.c_int, P(f32)]),
     ("or_cast_ray32", f32, [C.c_int, C.c_int, f32, f32, f32, P(C.c_uint8), f32, f32, f32, f32, f32]),
     ("or_caer32_batch", None, [C.c_int, C.c_int, f32, f32, f32, P(C.c_uint8), P(f32), C.c_int, f32, f32, P(f32),
@@ -275,6 +277,12 @@ class OracleGrid:
         lib().or_caer64_batch(self.W, self.H, self.res, self.ox, self.oy, _u8(self.cells), _d(s), s.size, span, rmax,
                               _d(h), len(h), _d(out))
         return out
+
+    def dda_steps64(self, hyp, nrays, span, rmax):
+        """Total DDA cell steps of the f64 twin over the hypotheses' rays (algorithmic work)."""
+        h = np.ascontiguousarray(hyp, np.float64).reshape(-1, 3)
+        return int(lib().or_dda64_steps(self.W, self.H, self.res, self.ox, self.oy, _u8(self.cells), _d(h), len(h),
+                                        nrays, span, rmax))
 
     def localize64(self, scan, hypotheses, topk, seed, span=2 * np.pi, rmax=10.0):
         """CpuGlobalLoc::localize (global_loc.rs:384-398): (x, y, th, caer) rows ascending."""

@@ -755,10 +755,36 @@ static int w2c64(const or_grid *g, double wx, double wy, long *cx, long *cy) {
 }
 
 /* global_loc.rs:152-196 (f64 DDA). */
+/* The f64 DDA; *steps (when non-NULL) receives the number of cell steps the walk took. */
+static double cast_ray64_steps(int W, int H, double res, double ox, double oy, const uint8_t *cells, double px,
+                               double py, double dirx, double diry, double rmax, long *steps);
+
 double or_cast_ray64(int W, int H, double res, double ox, double oy, const uint8_t *cells, double px, double py,
                      double dirx, double diry, double rmax) {
+    return cast_ray64_steps(W, H, res, ox, oy, cells, px, py, dirx, diry, rmax, NULL);
+}
+
+/* Algorithmic work of a CBGL scoring pass: the total number of DDA cell steps of the f64 twin
+ * (global_loc.rs:151-196) over n hypotheses x nrays rays. */
+unsigned long long or_dda64_steps(int W, int H, double res, double ox, double oy, const uint8_t *cells,
+                                  const double *hyp, int n, int nrays, double span, double rmax) {
+    unsigned long long total = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : total)
+    for (int h = 0; h < n; ++h)
+        for (int r = 0; r < nrays; ++r) {
+            double a = hyp[3 * h + 2] + (nrays > 1 ? span * (double)r / (double)(nrays - 1) : 0.0);
+            long st = 0;
+            cast_ray64_steps(W, H, res, ox, oy, cells, hyp[3 * h], hyp[3 * h + 1], cos(a), sin(a), rmax, &st);
+            total += (unsigned long long)st;
+        }
+    return total;
+}
+
+static double cast_ray64_steps(int W, int H, double res, double ox, double oy, const uint8_t *cells, double px,
+                               double py, double dirx, double diry, double rmax, long *steps) {
     or_grid g = {W, H, res, ox, oy, cells};
     long cx, cy;
+    if (steps) *steps = 0;
     if (!w2c64(&g, px, py, &cx, &cy)) return rmax;
     long sx = dirx > 0.0 ? 1 : -1, sy = diry > 0.0 ? 1 : -1;
     double adx = fabs(dirx), ady = fabs(diry);
@@ -770,6 +796,7 @@ double or_cast_ray64(int W, int H, double res, double ox, double oy, const uint8
     double tmy = ady > 1e-9 ? (diry > 0.0 ? 1.0 - fry : fry) * tdy : INFINITY;
     for (int s = 0; s < 8192; ++s) {
         double dist;
+        if (steps) *steps = s + 1;
         if (tmx <= tmy) {
             dist = tmx;
             tmx += tdx;

@@ -348,6 +348,22 @@ def test_raycast_known_answers():
             g.cast_ray64(x, y, math.cos(a), math.sin(a), 8.0)
 
 
+def test_dda_step_count_known_answers():
+    """The algorithmic work count of a scoring pass: a ray along +x from cell (5, 5) of a 40 x 40
+    bordered grid steps through cells 6..39 (34 steps, the last one entering the wall); along +y
+    from cell (5, 30) it takes 9; a diagonal one from cell (5, 5) alternates x and y steps
+    (33 of each to cell (38, 38)) and enters the wall with its next x step: 67."""
+    cells = np.zeros((40, 40), np.uint8)
+    cells[0, :] = cells[-1, :] = cells[:, 0] = cells[:, -1] = 1
+    g = O.OracleGrid(cells, 0.05, (0.0, 0.0))
+    c = lambda i: (i + 0.5) * 0.05
+    assert g.dda_steps64(np.array([[c(5), c(5), 0.0]]), 1, 0.0, 10.0) == 34
+    assert g.dda_steps64(np.array([[c(5), c(30), math.pi / 2]]), 1, 0.0, 10.0) == 9
+    assert g.dda_steps64(np.array([[c(5), c(5), math.pi / 4]]), 1, 0.0, 10.0) == 67
+    both = np.array([[c(5), c(5), 0.0], [c(5), c(30), math.pi / 2]])
+    assert g.dda_steps64(both, 1, 0.0, 10.0) == 43
+
+
 def test_raycast_brute_force_marching():
     cells, res, org = sim.obstacle_map(11)
     g = O.OracleGrid(cells, res, org)
