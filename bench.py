@@ -322,6 +322,9 @@ def run_pf_fused(n, sc, steps, warmup, steps_host):
         inputs.append(bf.make_inputs([(st.odom_t, st.odom_q, st.trans_noise, st.rot_noise, obs)]))
     assert len(inputs) >= warmup + steps_host + steps
     bf.use_graph(True)
+    # the likelihood launch shape from the library's autotuner (times the shapes on this filter
+    # and restores the draw state it staged)
+    ppt = bf.autotune(inputs[0][0])
     for i in range(warmup):
         bf.cycle(inputs[i][0])
     t0 = time.perf_counter()
@@ -350,7 +353,7 @@ def run_pf_fused(n, sc, steps, warmup, steps_host):
     bf.profile(False)
     bf.close()
     stages = {k: {"ms_total": v[0], "launches": v[1], "ms_avg": v[0] / v[1] if v[1] else 0.0} for k, v in prof.items()}
-    return ms, ms_dev, ms_host, err, n_rs, stages, ms_prof, n_prof
+    return ms, ms_dev, ms_host, err, n_rs, stages, ms_prof, n_prof, ppt
 
 
 def cpu_model() -> str:
@@ -522,7 +525,7 @@ def main():
     steps_prof = min(args.steps, 50)
     sc = config1_scenario(cycles=max(args.steps + args.warmup + steps_host + steps_prof, 50), seed=7)
     (ms_fused, ms_fused_dev, ms_host, err_fused, n_rs_fused, fstages, ms_fprof,
-     n_fprof) = run_pf_fused(n, sc, args.steps, args.warmup, steps_host)
+     n_fprof, weight_ppt) = run_pf_fused(n, sc, args.steps, args.warmup, steps_host)
     ms_fast, err, n_rs, stages = run_pf(n, sc, args.steps, args.warmup, fast=True, profile=False)
     ms_exact, _, _, _ = run_pf(n, sc, min(args.steps, 50), args.warmup, fast=False, profile=False)
     _, _, _, stages = run_pf(n, sc, min(args.steps, 50), args.warmup, fast=True, profile=True)
@@ -594,6 +597,7 @@ def main():
                      "traffic_algorithmic": 36 * n,
                      "ms_avg": round(w["ms_avg"], 5),
                      "flop_per_pair": WEIGHT_FLOP_PER_PAIR, "pairs_per_launch": pairs,
+                     "particles_per_lane": int(weight_ppt),
                      "issued": {"flop_per_pair": WEIGHT_FLOP_PER_PAIR_ISSUED,
                                 "tflops": round(issued_tflops, 2) if issued_tflops else None,
                                 "frac": round(issued_tflops / FP64_VALU_PEAK_TFLOPS, 3) if issued_tflops else None}},

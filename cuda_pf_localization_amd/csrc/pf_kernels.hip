@@ -333,19 +333,23 @@ __device__ inline void cart_from_b(double bx, double by, double bz, double vx, d
     accr += fmin(r2, p.cart_gate2);
 }
 
-// Gated log-likelihood (cpu.rs:44-97, 220-248), kPPT particles per lane, plus fused
-// online-softmax block partials and a deterministic last-block finalise per filter.
+// Gated log-likelihood (cpu.rs:44-97, 220-248), kPPT particles per lane, plus the block's
+// online-softmax partials.  Within each category of observations (range-bearing, then
+// Cartesian) the even-numbered and the odd-numbered records are summed as two independent
+// chains, each in order, and the two sums added.
 template <int kPPT>
-__global__ __launch_bounds__(kTB) void k_weight(Layout L, PoseBufs B, FilterDev* F, double* __restrict__ ll,
-                                                double* __restrict__ part, ParamRef<WeightParams> A) {
+__global__ __launch_bounds__(kTB) __attribute__((amdgpu_waves_per_eu(kPPT == 1 ? 5 : 1, 8)))
+void k_weight(Layout L, PoseBufs B, FilterDev* F, double* __restrict__ ll, double* __restrict__ part,
+              ParamRef<WeightParams> A) {
     __shared__ double sh[4];
     constexpr int kSpan = kTB * kPPT;
+    const int tl = (int)threadIdx.x;
     const int blk = blockIdx.x;
-    const int f = L.blk_filter[blk * kPPT];
-    const int off = L.f_off[f];
-    const int nf = L.f_n[f];
-    const WeightParams p = A.get(f);
-    const PoseSet S = B.s[F[f].parity];
+    const int f = ((ConstPtr<int>)L.blk_filter)[blk * kPPT];
+    const int off = ((ConstPtr<int>)L.f_off)[f];
+    const int nf = ((ConstPtr<int>)L.f_n)[f];
+    const WeightParams p = uniform_params(A, f);
+    const PoseSet S = B.s[((ConstPtr<FilterDev>)F)[f].parity];
     int li[kPPT];
     bool valid[kPPT];
     BodyMap bm[kPPT];
@@ -354,7 +358,7 @@ __global__ __launch_bounds__(kTB) void k_weight(Layout L, PoseBufs B, FilterDev*
     const double hs = fast ? 0.5 : 1.0;  // Cartesian records see b = hs * (mapped offset)
 #pragma unroll
     for (int e = 0; e < kPPT; ++e) {
-        li[e] = blk * kSpan + e * kTB + (int)threadIdx.x - off;
+        li[e] = blk * kSpan + e * kTB + tl - off;
         valid[e] = li[e] < nf;
         bm[e] = body_map(S, off + (valid[e] ? li[e] : 0));
         if (fast) {  // doubled body map: every offset it forms is exactly 2 b
@@ -364,42 +368,81 @@ __global__ __launch_bounds__(kTB) void k_weight(Layout L, PoseBufs B, FilterDev*
             m.m20 *= 2.0; m.m21 *= 2.0; m.m22 *= 2.0;
             m.mpx *= 2.0; m.mpy *= 2.0; m.mpz *= 2.0;
         }
-        accr[e] = 0.0;
-        accb[e] = 0.0;
-        accc[e] = 0.0;
     }
     ConstObsPtr obs = (ConstObsPtr)p.obs;
     const int nobs = p.nobs, nrb = p.nrb;  // range-bearing records first, then Cartesian
-    if (fast) {
-#pragma unroll 2
-        for (int k = 0; k < nrb; ++k) {
-            const ObsRec o = obs[k];
+    // Each category is summed as (terms of its even-numbered records, in order) + (terms of its
+    // odd-numbered records, in order): chain s of a lane sums the records k = c0 + s, c0 + s + 2,
+    // ... of a category that starts at record c0.
+    double ar[2][kPPT], ab[2][kPPT], ac[2][kPPT];
 #pragma unroll
-            for (int e = 0; e < kPPT; ++e) {
-                double bx, by, bz;
-                body_offset(bm[e], o, bx, by, bz);
-                rb_scaled(bx, by, bz, o.vx, o.vy, o.vz, o.range, p, accr[e], accb[e]);
+    for (int e = 0; e < kPPT; ++e)
+        ar[0][e] = ab[0][e] = ac[0][e] = ar[1][e] = ab[1][e] = ac[1][e] = 0.0;
+    {
+        if (fast) {
+            int k = 0;
+#pragma unroll 2
+            for (; k + 1 < nrb; k += 2) {
+                const ObsRec o0 = obs[k], o1 = obs[k + 1];
+#pragma unroll
+                for (int e = 0; e < kPPT; ++e) {
+                    double bx, by, bz;
+                    body_offset(bm[e], o0, bx, by, bz);
+                    rb_scaled(bx, by, bz, o0.vx, o0.vy, o0.vz, o0.range, p, ar[0][e], ab[0][e]);
+                    body_offset(bm[e], o1, bx, by, bz);
+                    rb_scaled(bx, by, bz, o1.vx, o1.vy, o1.vz, o1.range, p, ar[1][e], ab[1][e]);
+                }
+            }
+            if (k < nrb) {
+                const ObsRec o = obs[k];
+#pragma unroll
+                for (int e = 0; e < kPPT; ++e) {
+                    double bx, by, bz;
+                    body_offset(bm[e], o, bx, by, bz);
+                    rb_scaled(bx, by, bz, o.vx, o.vy, o.vz, o.range, p, ar[0][e], ab[0][e]);
+                }
+            }
+        } else {
+            for (int k = 0; k < nrb; ++k) {
+                const ObsRec o = obs[k];
+                const int sk = k & 1;
+#pragma unroll
+                for (int e = 0; e < kPPT; ++e) {
+                    double bx, by, bz;
+                    body_offset(bm[e], o, bx, by, bz);
+                    double r = sk ? ar[1][e] : ar[0][e], b2 = sk ? ab[1][e] : ab[0][e];
+                    rb_from_b(bx, by, bz, o.vx, o.vy, o.vz, o.range, p, r, b2);
+                    if (sk) {
+                        ar[1][e] = r;
+                        ab[1][e] = b2;
+                    } else {
+                        ar[0][e] = r;
+                        ab[0][e] = b2;
+                    }
+                }
             }
         }
-    } else {
-        for (int k = 0; k < nrb; ++k) {
+        for (int k = nrb; k < nobs; ++k) {
             const ObsRec o = obs[k];
+            const int sk = (k - nrb) & 1;
 #pragma unroll
             for (int e = 0; e < kPPT; ++e) {
                 double bx, by, bz;
                 body_offset(bm[e], o, bx, by, bz);
-                rb_from_b(bx, by, bz, o.vx, o.vy, o.vz, o.range, p, accr[e], accb[e]);
+                double c = sk ? ac[1][e] : ac[0][e];
+                cart_from_b(bx, by, bz, o.vx, o.vy, o.vz, hs, p, c);
+                if (sk)
+                    ac[1][e] = c;
+                else
+                    ac[0][e] = c;
             }
         }
     }
-    for (int k = nrb; k < nobs; ++k) {
-        const ObsRec o = obs[k];
 #pragma unroll
-        for (int e = 0; e < kPPT; ++e) {
-            double bx, by, bz;
-            body_offset(bm[e], o, bx, by, bz);
-            cart_from_b(bx, by, bz, o.vx, o.vy, o.vz, hs, p, accc[e]);
-        }
+    for (int e = 0; e < kPPT; ++e) {
+        accr[e] = ar[0][e] + ar[1][e];
+        accb[e] = ab[0][e] + ab[1][e];
+        accc[e] = ac[0][e] + ac[1][e];
     }
     const double sr = fast ? p.inv2rr * 0.0625 : p.inv2rr;  // the scaled range sum is 16x
 #pragma unroll

@@ -163,7 +163,7 @@ int pfx_particle_count(void *h);
  * Jacobi rotations.  Shared by the host layer and the batched path. */
 int pfx_quat_mean(const double *M16, double *q4);
 /* Likelihood kernel launch shape: 1, 2 or 4 particles per lane (default 1).  All shapes give
- * bit-identical results. */
+ * bit-identical log-likelihoods; the softmax partials are per 256, 512 or 1024 particles. */
 int pfx_set_weight_variant(void *h, int variant);
 /* Resampling scan: reads of a predecessor tile's status before the waiting tile sums that
  * predecessor's weights itself (default 64; 0 sums every predecessor not yet published at the

@@ -281,6 +281,24 @@ def test_weight_variants_match_oracle(gpu, variant, case):
     assert np.abs(w.astype(np.float64) - ow).max() <= 1e-6 * ow.max()
 
 
+@pytest.mark.parametrize("case", ["rb_gated", "rb_wide", "mixed", "config1"])
+def test_weight_shapes_bit_identical(gpu, case):
+    """The launch shapes (1, 2, 4 particles per lane) give the same log-likelihoods bit for bit
+    (each particle's terms are summed in the same two chains in every shape)."""
+    n = 30_001 if case == "config1" else 5003
+    lm, obs, truth, q_t, gate, spread = _weight_case(case, 13)
+    res = []
+    for variant in (1, 2, 4):
+        k, _ = pair(n, 21)
+        k.set_weight_variant(variant)
+        pose = f32([*truth, *q_t])
+        k.reinit(pose, f32([spread] * 3), f32([0.1, 0.1, 0.3]))
+        k.upload_landmarks(lm)
+        k.weight(obs, 0.05, 0.02, gate)
+        res.append(k.get_ll())
+    assert np.array_equal(res[0], res[1]) and np.array_equal(res[0], res[2])
+
+
 def weighted_state(n, seed, conc=1.0):
     pos, quat = random_state(n, seed)
     rng = np.random.default_rng(seed + 1)
