@@ -172,7 +172,17 @@ void state_free_particles(PfState* s);
 void prof_begin(PfState* s, int stage);
 void prof_end(PfState* s);
 int launch_motion(PfState* s, ParamRef<MotionParams> A, int reinit);
-int launch_weight(PfState* s, ParamRef<WeightParams> A);
+// A covariance finalize deferred into the next likelihood launch: the 12 covariance sums of
+// every filter from part (the covariance partials of an earlier pass, value-major with the given
+// stride) into FilterDev.covs and, with every other scalar of the filter, into out[f].
+struct CovFin {
+    const double* part = nullptr;  // null: nothing deferred
+    FilterDev* out = nullptr;
+    int stride = 0;
+};
+// Likelihood pass and its softmax finalize; a deferred covariance finalize (cf.part non-null)
+// runs as one extra workgroup per filter at the end of the likelihood grid.
+int launch_weight(PfState* s, ParamRef<WeightParams> A, CovFin cf = {});
 // Normalised weights, ESS and (with d_ratio) the resample predicate; with scan (fused cycle,
 // d_ratio required) the normalisation pass also leaves every tile's fixed-point aggregate and
 // draws the injection picks inj of a possible resample, and the resampling scan of every filter
@@ -192,7 +202,10 @@ int set_inject_blocks(PfState* s, const std::vector<int>& kmax);
 // pass = false: the gather of the same cycle left the partials of every filter.
 int launch_est_sums(PfState* s, CovParams* d_cov, bool flip, bool pass = true);
 // Covariances; with results non-null every filter's scalars are also copied to results[f].
-int launch_est_covs(PfState* s, ParamRef<CovParams> A, FilterDev* results = nullptr);
+// finalize = false leaves the partials for a covariance finalize deferred into the next
+// likelihood launch (CovFin) or run by launch_cov_fin.
+int launch_est_covs(PfState* s, ParamRef<CovParams> A, FilterDev* results = nullptr, bool finalize = true);
+int launch_cov_fin(PfState* s, FilterDev* results);
 int launch_predicate(PfState* s, const float* d_ratio, int force);
 WeightParams make_weight_params(double sigma_r, double sigma_b, double gate);
 // Joins observations with their landmarks (unknown ids dropped); range-bearing records

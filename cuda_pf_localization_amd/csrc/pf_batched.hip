@@ -240,17 +240,26 @@ void fill_estimate(const FilterDev& fd, pfb_estimate& o) {
     o.valid = fd.qvalid;
 }
 
-// Device stages of one staged cycle: parameters read in place from its staging block, every
-// filter's scalars copied to results by the covariance finalize (no copy node).
-int stages_staged(Batch* b, const ParamBlock& dp, FilterDev* results) {
+// Device stages of one staged cycle: parameters read in place from its staging block.  The
+// covariance finalize of the cycle (which also copies every filter's scalars to its results
+// slot) is left pending: nothing in the next cycle needs the covariances, so it runs as extra
+// workgroups of the next cycle's likelihood launch (prev_results: the previous cycle's slot, or
+// null), and the run's last cycle finalizes on its own afterwards.
+int stages_staged(Batch* b, const ParamBlock& dp, FilterDev* prev_results) {
     PfState* s = &b->s;
     if (launch_motion(s, ParamRef<MotionParams>::array(dp.motion), 0) != 0) return -1;
-    if (launch_weight(s, ParamRef<WeightParams>::array(dp.weight)) != 0) return -1;
+    CovFin cf;
+    if (prev_results) {
+        cf.part = s->part_cov;
+        cf.out = prev_results;
+        cf.stride = s->NP / kTile;
+    }
+    if (launch_weight(s, ParamRef<WeightParams>::array(dp.weight), cf) != 0) return -1;
     if (launch_normalize(s, 0, b->d_ratio, true, ParamRef<InjectParams>::array(dp.inject)) != 0) return -1;
     if (launch_resample(s, ParamRef<InjectParams>::array(dp.inject), s->d_inj_blk, s->inj_nblk, false, false) != 0)
         return -1;
     if (launch_est_sums(s, dp.cov, true, false) != 0) return -1;
-    return launch_est_covs(s, ParamRef<CovParams>::array(dp.cov), results);
+    return launch_est_covs(s, ParamRef<CovParams>::array(dp.cov), nullptr, false);
 }
 
 void free_staged(Batch* b) {
@@ -417,8 +426,9 @@ int pfb_run_staged(void* h, pfb_estimate* out) {
     PB_CHECK(hipEventRecord(b->st_ev[0], s->stream));
     for (int k = 0; k < b->st_cycles; ++k)
         if (stages_staged(b, carve(b->d_staged + (size_t)k * b->stage_bytes, b->nf),
-                          b->d_results + (size_t)k * b->nf) != 0)
+                          k > 0 ? b->d_results + (size_t)(k - 1) * b->nf : nullptr) != 0)
             return -1;
+    if (launch_cov_fin(s, b->d_results + (size_t)(b->st_cycles - 1) * b->nf) != 0) return -1;
     PB_CHECK(hipEventRecord(b->st_ev[1], s->stream));
     s->prof_on = prof_saved;
     const size_t nfd = (size_t)b->st_cycles * b->nf;

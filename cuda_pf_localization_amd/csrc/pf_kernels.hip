@@ -333,18 +333,29 @@ __device__ inline void cart_from_b(double bx, double by, double bz, double vx, d
     accr += fmin(r2, p.cart_gate2);
 }
 
+// Covariance finalize of filter f by a 256-thread block, in the order of k_est_fin<kCovN>
+// (defined with it below).
+__device__ void cov_fin_block(const Layout& L, FilterDev* F, const double* __restrict__ part, int stride,
+                              FilterDev* out, int f);
+
 // Gated log-likelihood (cpu.rs:44-97, 220-248), kPPT particles per lane, plus the block's
-// online-softmax partials.  Within each category of observations (range-bearing, then
+// online-softmax partials.  Workgroups past the likelihood grid (cf.part non-null) run the
+// covariance finalize deferred from the previous cycle, one per filter; they are dispatched
+// last and occupy CUs the likelihood's final waves leave idle.  Within each category of observations (range-bearing, then
 // Cartesian) the even-numbered and the odd-numbered records are summed as two independent
 // chains, each in order, and the two sums added.
 template <int kPPT>
 __global__ __launch_bounds__(kTB) __attribute__((amdgpu_waves_per_eu(kPPT == 1 ? 5 : 1, 8)))
 void k_weight(Layout L, PoseBufs B, FilterDev* F, double* __restrict__ ll, double* __restrict__ part,
-              ParamRef<WeightParams> A) {
+              ParamRef<WeightParams> A, CovFin cf, int nblk) {
     __shared__ double sh[4];
     constexpr int kSpan = kTB * kPPT;
     const int tl = (int)threadIdx.x;
     const int blk = blockIdx.x;
+    if (blk >= nblk) {
+        cov_fin_block(L, F, cf.part, cf.stride, cf.out, blk - nblk);
+        return;
+    }
     const int f = ((ConstPtr<int>)L.blk_filter)[blk * kPPT];
     const int off = ((ConstPtr<int>)L.f_off)[f];
     const int nf = ((ConstPtr<int>)L.f_n)[f];
@@ -381,16 +392,53 @@ void k_weight(Layout L, PoseBufs B, FilterDev* F, double* __restrict__ ll, doubl
     {
         if (fast) {
             int k = 0;
-#pragma unroll 2
-            for (; k + 1 < nrb; k += 2) {
-                const ObsRec o0 = obs[k], o1 = obs[k + 1];
+            if (nrb >= 2) {
+                // Software-pipelined record loads: the next pair of records is requested before
+                // the current pair is evaluated, so its scalar-load latency overlaps the
+                // arithmetic.  Two register sets (a, b) alternate so that no loaded record is
+                // copied.  Scalar loads may return out of order, so each wait for a pair (issued
+                // one half-iteration earlier) is explicit, and the scheduler is kept from hoisting
+                // the next prefetch above it.
+                ObsRec a0 = obs[0], a1 = obs[1];
+                for (; k + 3 < nrb; k += 4) {
+                    __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): pair a has arrived
+                    __builtin_amdgcn_sched_barrier(0);
+                    const ObsRec b0 = obs[k + 2], b1 = obs[k + 3];
+                    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-                for (int e = 0; e < kPPT; ++e) {
-                    double bx, by, bz;
-                    body_offset(bm[e], o0, bx, by, bz);
-                    rb_scaled(bx, by, bz, o0.vx, o0.vy, o0.vz, o0.range, p, ar[0][e], ab[0][e]);
-                    body_offset(bm[e], o1, bx, by, bz);
-                    rb_scaled(bx, by, bz, o1.vx, o1.vy, o1.vz, o1.range, p, ar[1][e], ab[1][e]);
+                    for (int e = 0; e < kPPT; ++e) {
+                        double bx, by, bz;
+                        body_offset(bm[e], a0, bx, by, bz);
+                        rb_scaled(bx, by, bz, a0.vx, a0.vy, a0.vz, a0.range, p, ar[0][e], ab[0][e]);
+                        body_offset(bm[e], a1, bx, by, bz);
+                        rb_scaled(bx, by, bz, a1.vx, a1.vy, a1.vz, a1.range, p, ar[1][e], ab[1][e]);
+                    }
+                    __builtin_amdgcn_s_waitcnt(0xC07F);  // pair b has arrived
+                    __builtin_amdgcn_sched_barrier(0);
+                    const int kn = k + 5 < nrb ? k + 4 : k + 2;  // in bounds; unused past the end
+                    a0 = obs[kn];
+                    a1 = obs[kn + 1];
+                    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                    for (int e = 0; e < kPPT; ++e) {
+                        double bx, by, bz;
+                        body_offset(bm[e], b0, bx, by, bz);
+                        rb_scaled(bx, by, bz, b0.vx, b0.vy, b0.vz, b0.range, p, ar[0][e], ab[0][e]);
+                        body_offset(bm[e], b1, bx, by, bz);
+                        rb_scaled(bx, by, bz, b1.vx, b1.vy, b1.vz, b1.range, p, ar[1][e], ab[1][e]);
+                    }
+                }
+                if (k + 1 < nrb) {  // pair a = records k, k + 1 is loaded
+                    __builtin_amdgcn_s_waitcnt(0xC07F);
+#pragma unroll
+                    for (int e = 0; e < kPPT; ++e) {
+                        double bx, by, bz;
+                        body_offset(bm[e], a0, bx, by, bz);
+                        rb_scaled(bx, by, bz, a0.vx, a0.vy, a0.vz, a0.range, p, ar[0][e], ab[0][e]);
+                        body_offset(bm[e], a1, bx, by, bz);
+                        rb_scaled(bx, by, bz, a1.vx, a1.vy, a1.vz, a1.range, p, ar[1][e], ab[1][e]);
+                    }
+                    k += 2;
                 }
             }
             if (k < nrb) {
@@ -1432,6 +1480,33 @@ __global__ void k_flip(FilterDev* F, int nfilt, unsigned long long* scan_ctl) {
 // staged cycle) the filter's scalars are copied to out[f]: the other members from F[f] (final
 // since earlier kernels), the new values by the lanes that hold them.
 constexpr int kEstFinTB = 512;
+
+// Lane vt of the estimate finalize: the sum, in order, of the partials of tiles c + vt + u
+// kEstFinTB (u = 0..3, c = 0, 4 kEstFinTB, ...) of one filter (nb tiles; a missing tile reads as
+// 0, which leaves the sums unchanged).  kU tiles' loads are in flight at a time; the arithmetic
+// is the same for any kU.
+template <int NV, int kU>
+__device__ inline void fin_lane_sums(const double* __restrict__ pf, int stride, int nb, int vt, double (&acc)[NV]) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) acc[k] = 0.0;
+    for (int c = 0; c < nb; c += 4 * kEstFinTB) {
+#pragma unroll 1
+        for (int u0 = 0; u0 < 4; u0 += kU) {
+            double x[kU][NV];
+#pragma unroll
+            for (int u = 0; u < kU; ++u) {
+                const int b = c + vt + (u0 + u) * kEstFinTB;
+#pragma unroll
+                for (int k = 0; k < NV; ++k) x[u][k] = b < nb ? pf[(size_t)k * stride + b] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < kU; ++u)
+#pragma unroll
+                for (int k = 0; k < NV; ++k) acc[k] += x[u][k];
+        }
+    }
+}
+
 template <int NV>
 __global__ __launch_bounds__(kEstFinTB) void k_est_fin(Layout L, FilterDev* F, const double* __restrict__ part,
                                                        int stride, int which, CovParams* cov, int flip,
@@ -1442,22 +1517,7 @@ __global__ __launch_bounds__(kEstFinTB) void k_est_fin(Layout L, FilterDev* F, c
     const int b0 = L.f_off[f] / kTile, nb = L.f_padn[f] / kTile;
     const int t = (int)threadIdx.x;
     double acc[NV];
-#pragma unroll
-    for (int k = 0; k < NV; ++k) acc[k] = 0.0;
-    const double* pf = part + b0;
-    for (int c = 0; c < nb; c += 4 * kEstFinTB) {
-        double x[4][NV];  // a missing block reads as 0, which leaves the sums unchanged
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int b = c + t + u * kEstFinTB;
-#pragma unroll
-            for (int k = 0; k < NV; ++k) x[u][k] = b < nb ? pf[(size_t)k * stride + b] : 0.0;
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-#pragma unroll
-            for (int k = 0; k < NV; ++k) acc[k] += x[u][k];
-    }
+    fin_lane_sums<NV, 4>(part + b0, stride, nb, t, acc);
     wave_fold_store<NV>(acc, shv[t >> 6]);
     __syncthreads();
     if (t < NV) {
@@ -1485,6 +1545,36 @@ __global__ __launch_bounds__(kEstFinTB) void k_est_fin(Layout L, FilterDev* F, c
         __syncthreads();
         if (threadIdx.x == 0) markley_mean(F[f], sv, cov[f]);
     }
+}
+
+// The covariance finalize of k_est_fin<kCovN> (flip = 0, results into out) by 256 threads:
+// thread t takes the finalize's lanes t and t + 256 in turn, each wave folding them into the
+// rows of virtual waves w and w + 4, which are then added in the finalize's order.  The results
+// are bit-identical to the 512-thread kernel's.
+__device__ void cov_fin_block(const Layout& L, FilterDev* F, const double* __restrict__ part, int stride,
+                              FilterDev* out, int f) {
+    __shared__ double shv[kEstFinTB / 64][kCovN];
+    const int b0 = L.f_off[f] / kTile, nb = L.f_padn[f] / kTile;
+    const int t = (int)threadIdx.x;
+    static_assert(kEstFinTB == 2 * kTB, "two finalize lanes per thread");
+#pragma unroll 1
+    for (int h = 0; h < 2; ++h) {
+        double acc[kCovN];
+        fin_lane_sums<kCovN, 1>(part + b0, stride, nb, t + h * kTB, acc);
+        wave_fold_store<kCovN>(acc, shv[(t >> 6) + h * (kTB / 64)]);
+    }
+    __syncthreads();
+    if (t < kCovN) {
+        double v = shv[0][t];
+#pragma unroll
+        for (int wv = 1; wv < kEstFinTB / 64; ++wv) v += shv[wv][t];
+        F[f].covs[t] = v;
+        out[f].covs[t] = v;
+    }
+    constexpr int kWords = (int)(sizeof(FilterDev) / sizeof(uint64_t));
+    constexpr int v0 = (int)(offsetof(FilterDev, covs) / sizeof(uint64_t));
+    if (t < kWords && (t < v0 || t >= v0 + kCovN))
+        reinterpret_cast<uint64_t*>(out + f)[t] = reinterpret_cast<const uint64_t*>(F + f)[t];
 }
 
 // est layout (cu:354-404): [0..2] sum w p, [3..12] sum w q_r q_c (r<=c), [13] sum w^2.
@@ -1758,18 +1848,21 @@ int launch_motion(PfState* s, ParamRef<MotionParams> A, int reinit) {
     return 0;
 }
 
-int launch_weight(PfState* s, ParamRef<WeightParams> A) {
+int launch_weight(PfState* s, ParamRef<WeightParams> A, CovFin cf) {
     prof_begin(s, PFX_STAGE_WEIGHT);
     const int ppt = s->weight_ppt;
+    const int nblk = s->NP / (kTB * ppt);
+    const int grid = nblk + (cf.part ? s->nf : 0);
+    if (cf.part && (!cf.out || cf.stride != s->NP / kTile)) return -1;
     if (ppt == 1)
-        hipLaunchKernelGGL(k_weight<1>, dim3(s->NP / kTB), dim3(kTB), 0, s->stream, s->L, s->pose, s->fdev, s->ll,
-                           s->part_w, A);
+        hipLaunchKernelGGL(k_weight<1>, dim3(grid), dim3(kTB), 0, s->stream, s->L, s->pose, s->fdev, s->ll,
+                           s->part_w, A, cf, nblk);
     else if (ppt == 4)
-        hipLaunchKernelGGL(k_weight<4>, dim3(s->NP / (kTB * 4)), dim3(kTB), 0, s->stream, s->L, s->pose, s->fdev,
-                           s->ll, s->part_w, A);
+        hipLaunchKernelGGL(k_weight<4>, dim3(grid), dim3(kTB), 0, s->stream, s->L, s->pose, s->fdev, s->ll,
+                           s->part_w, A, cf, nblk);
     else
-        hipLaunchKernelGGL(k_weight<2>, dim3(s->NP / (kTB * 2)), dim3(kTB), 0, s->stream, s->L, s->pose, s->fdev,
-                           s->ll, s->part_w, A);
+        hipLaunchKernelGGL(k_weight<2>, dim3(grid), dim3(kTB), 0, s->stream, s->L, s->pose, s->fdev, s->ll,
+                           s->part_w, A, cf, nblk);
     prof_end(s);
     PF_LAUNCH_CHECK();
     prof_begin(s, PFX_STAGE_FINALIZE);
@@ -1855,12 +1948,16 @@ int launch_est_sums(PfState* s, CovParams* d_cov, bool flip, bool pass) {
     return 0;
 }
 
-int launch_est_covs(PfState* s, ParamRef<CovParams> A, FilterDev* results) {
+int launch_est_covs(PfState* s, ParamRef<CovParams> A, FilterDev* results, bool finalize) {
     prof_begin(s, PFX_STAGE_EST_COVS);
     hipLaunchKernelGGL(k_est_covs, dim3(s->NP / kTile), dim3(kTB), 0, s->stream, s->L, s->fdev, s->pose, s->w,
                        s->part_cov, s->NP / kTile, A);
     prof_end(s);
     PF_LAUNCH_CHECK();
+    return finalize ? launch_cov_fin(s, results) : 0;
+}
+
+int launch_cov_fin(PfState* s, FilterDev* results) {
     prof_begin(s, PFX_STAGE_FINALIZE);
     hipLaunchKernelGGL(k_est_fin<kCovN>, dim3(s->nf), dim3(kEstFinTB), 0, s->stream, s->L, s->fdev, s->part_cov,
                        s->NP / kTile, 1, nullptr, 0, s->scan_ctl, results);
