@@ -171,18 +171,22 @@ int state_alloc(PfState* s, const std::vector<int>& ns);
 void state_free_particles(PfState* s);
 void prof_begin(PfState* s, int stage);
 void prof_end(PfState* s);
-int launch_motion(PfState* s, ParamRef<MotionParams> A, int reinit);
-// A covariance finalize deferred into the next likelihood launch: the 12 covariance sums of
-// every filter from part (the covariance partials of an earlier pass, value-major with the given
-// stride) into FilterDev.covs and, with every other scalar of the filter, into out[f].
-struct CovFin {
-    const double* part = nullptr;  // null: nothing deferred
+// dst_flip = 1 (predict only): the new poses go to the idle pose set, leaving the current set
+// untouched until the likelihood launch's finalize makes the new set current (WeightDefer).
+int launch_motion(PfState* s, ParamRef<MotionParams> A, int reinit, int dst_flip = 0);
+// Work of a staged cycle carried into the next cycle's likelihood launch, which nothing in that
+// cycle before it needs.  flip: predict wrote the idle pose set; the likelihood reads it and its
+// softmax finalize makes it current.  cov (non-null, requires flip): the previous cycle's
+// covariance pass, run by extra likelihood workgroups on the still-current pose set and weights
+// with that cycle's covariance parameter blocks, and its finalize, run by extra softmax-finalize
+// workgroups, into FilterDev.covs and the results slot out.
+struct WeightDefer {
+    int flip = 0;
+    const CovParams* cov = nullptr;
     FilterDev* out = nullptr;
-    int stride = 0;
 };
-// Likelihood pass and its softmax finalize; a deferred covariance finalize (cf.part non-null)
-// runs as one extra workgroup per filter at the end of the likelihood grid.
-int launch_weight(PfState* s, ParamRef<WeightParams> A, CovFin cf = {});
+// Likelihood pass and its softmax finalize.
+int launch_weight(PfState* s, ParamRef<WeightParams> A, const WeightDefer& d = {});
 // Normalised weights, ESS and (with d_ratio) the resample predicate; with scan (fused cycle,
 // d_ratio required) the normalisation pass also leaves every tile's fixed-point aggregate and
 // draws the injection picks inj of a possible resample, and the resampling scan of every filter
@@ -202,10 +206,7 @@ int set_inject_blocks(PfState* s, const std::vector<int>& kmax);
 // pass = false: the gather of the same cycle left the partials of every filter.
 int launch_est_sums(PfState* s, CovParams* d_cov, bool flip, bool pass = true);
 // Covariances; with results non-null every filter's scalars are also copied to results[f].
-// finalize = false leaves the partials for a covariance finalize deferred into the next
-// likelihood launch (CovFin) or run by launch_cov_fin.
-int launch_est_covs(PfState* s, ParamRef<CovParams> A, FilterDev* results = nullptr, bool finalize = true);
-int launch_cov_fin(PfState* s, FilterDev* results);
+int launch_est_covs(PfState* s, ParamRef<CovParams> A, FilterDev* results = nullptr);
 int launch_predicate(PfState* s, const float* d_ratio, int force);
 WeightParams make_weight_params(double sigma_r, double sigma_b, double gate);
 // Joins observations with their landmarks (unknown ids dropped); range-bearing records

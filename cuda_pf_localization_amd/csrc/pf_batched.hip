@@ -241,25 +241,26 @@ void fill_estimate(const FilterDev& fd, pfb_estimate& o) {
 }
 
 // Device stages of one staged cycle: parameters read in place from its staging block.  The
-// covariance finalize of the cycle (which also copies every filter's scalars to its results
-// slot) is left pending: nothing in the next cycle needs the covariances, so it runs as extra
-// workgroups of the next cycle's likelihood launch (prev_results: the previous cycle's slot, or
-// null), and the run's last cycle finalizes on its own afterwards.
-int stages_staged(Batch* b, const ParamBlock& dp, FilterDev* prev_results) {
+// cycle's covariances are left pending: nothing in the next cycle needs them, so the next
+// cycle's likelihood launch computes them (its predict stores the new poses in the idle pose
+// set, leaving this cycle's poses and weights in place until the likelihood has read them) and
+// its softmax finalize reduces them into the results slot.  prev: the previous cycle's
+// parameter blocks and results slot (null for the first cycle of a run, whose predecessor
+// finished on its own); the run's last cycle finishes with a covariance pass of its own.
+int stages_staged(Batch* b, const ParamBlock& dp, const ParamBlock* prev, FilterDev* prev_results) {
     PfState* s = &b->s;
-    if (launch_motion(s, ParamRef<MotionParams>::array(dp.motion), 0) != 0) return -1;
-    CovFin cf;
-    if (prev_results) {
-        cf.part = s->part_cov;
-        cf.out = prev_results;
-        cf.stride = s->NP / kTile;
+    if (launch_motion(s, ParamRef<MotionParams>::array(dp.motion), 0, 1) != 0) return -1;
+    WeightDefer d;
+    d.flip = 1;
+    if (prev) {
+        d.cov = prev->cov;
+        d.out = prev_results;
     }
-    if (launch_weight(s, ParamRef<WeightParams>::array(dp.weight), cf) != 0) return -1;
+    if (launch_weight(s, ParamRef<WeightParams>::array(dp.weight), d) != 0) return -1;
     if (launch_normalize(s, 0, b->d_ratio, true, ParamRef<InjectParams>::array(dp.inject)) != 0) return -1;
     if (launch_resample(s, ParamRef<InjectParams>::array(dp.inject), s->d_inj_blk, s->inj_nblk, false, false) != 0)
         return -1;
-    if (launch_est_sums(s, dp.cov, true, false) != 0) return -1;
-    return launch_est_covs(s, ParamRef<CovParams>::array(dp.cov), nullptr, false);
+    return launch_est_sums(s, dp.cov, true, false);
 }
 
 void free_staged(Batch* b) {
@@ -424,11 +425,16 @@ int pfb_run_staged(void* h, pfb_estimate* out) {
     for (hipEvent_t& e : b->st_ev)
         if (!e) PB_CHECK(hipEventCreate(&e));
     PB_CHECK(hipEventRecord(b->st_ev[0], s->stream));
-    for (int k = 0; k < b->st_cycles; ++k)
-        if (stages_staged(b, carve(b->d_staged + (size_t)k * b->stage_bytes, b->nf),
-                          k > 0 ? b->d_results + (size_t)(k - 1) * b->nf : nullptr) != 0)
-            return -1;
-    if (launch_cov_fin(s, b->d_results + (size_t)(b->st_cycles - 1) * b->nf) != 0) return -1;
+    ParamBlock prev{};
+    for (int k = 0; k < b->st_cycles; ++k) {
+        const ParamBlock dp = carve(b->d_staged + (size_t)k * b->stage_bytes, b->nf);
+        FilterDev* prev_out = k > 0 ? b->d_results + (size_t)(k - 1) * b->nf : nullptr;
+        if (stages_staged(b, dp, k > 0 ? &prev : nullptr, prev_out) != 0) return -1;
+        prev = dp;
+    }
+    if (launch_est_covs(s, ParamRef<CovParams>::array(prev.cov), b->d_results + (size_t)(b->st_cycles - 1) * b->nf) !=
+        0)
+        return -1;
     PB_CHECK(hipEventRecord(b->st_ev[1], s->stream));
     s->prof_on = prof_saved;
     const size_t nfd = (size_t)b->st_cycles * b->nf;

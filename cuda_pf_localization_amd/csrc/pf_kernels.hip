@@ -114,10 +114,11 @@ __device__ inline void motion_pose(const MotionParams& p, const PoseSet& S, int 
 }
 
 // Reinit (reinit = 1) and predict, one particle per lane; the filter, its layout entries and its
-// parameter block are wave-uniform scalar loads.
+// parameter block are wave-uniform scalar loads.  dst_flip = 1: the new poses are stored in the
+// idle pose set (the current set is read and left as it is).
 template <int reinit>
 __global__ __launch_bounds__(kTB) void k_motion(Layout L, PoseBufs B, FilterDev* F, float* __restrict__ w,
-                                                ParamRef<MotionParams> A) {
+                                                ParamRef<MotionParams> A, int dst_flip) {
     const int blk = blockIdx.x;
     const int f = ((ConstPtr<int>)L.blk_filter)[blk];
     const int off = ((ConstPtr<int>)L.f_off)[f];
@@ -125,12 +126,14 @@ __global__ __launch_bounds__(kTB) void k_motion(Layout L, PoseBufs B, FilterDev*
     if (li >= ((ConstPtr<int>)L.f_n)[f]) return;
     const int gi = off + li;
     const MotionParams p = uniform_params(A, f);
-    const PoseSet S = B.s[((ConstPtr<FilterDev>)F)[f].parity];
+    const int par = ((ConstPtr<FilterDev>)F)[f].parity;
+    const PoseSet S = B.s[par];
+    const PoseSet D = B.s[par ^ dst_flip];
     float v[7];
     motion_pose(p, S, gi, li, reinit, v);
     if (li == 0) F[f].est_src = 0;  // poses change: partials of an earlier gather are stale
 #pragma unroll
-    for (int c = 0; c < 7; ++c) S.c[c][gi] = v[c];
+    for (int c = 0; c < 7; ++c) D.c[c][gi] = v[c];
     if (reinit) w[gi] = p.winv;
 }
 
@@ -333,28 +336,37 @@ __device__ inline void cart_from_b(double bx, double by, double bz, double vx, d
     accr += fmin(r2, p.cart_gate2);
 }
 
-// Covariance finalize of filter f by a 256-thread block, in the order of k_est_fin<kCovN>
-// (defined with it below).
+// Covariance partials of one 1024-particle tile, and the covariance finalize of one filter by a
+// 1024-thread block (both defined with the estimate kernels below).
+__device__ void covs_tile(const Layout& L, FilterDev* F, const PoseBufs& B, int swapped, const float* __restrict__ w,
+                          double* __restrict__ part, int stride, const ParamRef<CovParams>& A, int blk);
 __device__ void cov_fin_block(const Layout& L, FilterDev* F, const double* __restrict__ part, int stride,
                               FilterDev* out, int f);
 
 // Gated log-likelihood (cpu.rs:44-97, 220-248), kPPT particles per lane, plus the block's
-// online-softmax partials.  Workgroups past the likelihood grid (cf.part non-null) run the
-// covariance finalize deferred from the previous cycle, one per filter; they are dispatched
-// last and occupy CUs the likelihood's final waves leave idle.  Within each category of observations (range-bearing, then
+// online-softmax partials.  Within each category of observations (range-bearing, then
 // Cartesian) the even-numbered and the odd-numbered records are summed as two independent
 // chains, each in order, and the two sums added.
-template <int kPPT>
+//   kDefer (staged cycles, where predict stored the new poses in the idle set and B comes with
+// its two sets swapped, so that B.s[parity] is that idle set): workgroups past the likelihood
+// grid (blk >= nblk) compute the covariance partials of the previous cycle, one 1024-particle
+// tile each, from the current set; they are dispatched last and run on CUs the likelihood's
+// final waves leave idle.  (A separate instantiation, so that the deferred work's registers do
+// not constrain the plain kernel.)
+template <int kPPT, bool kDefer>
 __global__ __launch_bounds__(kTB) __attribute__((amdgpu_waves_per_eu(kPPT == 1 ? 5 : 1, 8)))
 void k_weight(Layout L, PoseBufs B, FilterDev* F, double* __restrict__ ll, double* __restrict__ part,
-              ParamRef<WeightParams> A, CovFin cf, int nblk) {
+              ParamRef<WeightParams> A, const float* __restrict__ cw, double* __restrict__ cpart, int cstride,
+              ParamRef<CovParams> cA, int nblk) {
     __shared__ double sh[4];
     constexpr int kSpan = kTB * kPPT;
     const int tl = (int)threadIdx.x;
     const int blk = blockIdx.x;
-    if (blk >= nblk) {
-        cov_fin_block(L, F, cf.part, cf.stride, cf.out, blk - nblk);
-        return;
+    if constexpr (kDefer) {
+        if (blk >= nblk) {
+            covs_tile(L, F, B, 1, cw, cpart, cstride, cA, blk - nblk);
+            return;
+        }
     }
     const int f = ((ConstPtr<int>)L.blk_filter)[blk * kPPT];
     const int off = ((ConstPtr<int>)L.f_off)[f];
@@ -559,8 +571,13 @@ __device__ __forceinline__ void markley_mean(FilterDev& fd, const double* sums, 
 // most blocks.  S and S2 are summed per lane, then over the 16 waves in order (block16_sum's
 // tree, two values at once).
 __global__ __launch_bounds__(kFinTB) void k_weight_fin(Layout L, FilterDev* F, const double* __restrict__ part,
-                                                       int span) {
+                                                       int span, int nfilt, int flip, const double* __restrict__ cpart,
+                                                       int cstride, FilterDev* cout) {
     __shared__ double sh[kFinTB / 64], sh1[kFinTB / 64], sh2[kFinTB / 64];
+    if ((int)blockIdx.x >= nfilt) {  // the previous staged cycle's covariance finalize
+        cov_fin_block(L, F, cpart, cstride, cout, blockIdx.x - nfilt);
+        return;
+    }
     const int f = blockIdx.x;
     const int b0 = L.f_off[f] / span, nb = L.f_padn[f] / span;
     const int t = (int)threadIdx.x;
@@ -620,6 +637,7 @@ __global__ __launch_bounds__(kFinTB) void k_weight_fin(Layout L, FilterDev* F, c
         fd.sum_e = Ssum;
         fd.sum_e2 = S2sum;
         fd.uniform = ok ? 0 : 1;
+        if (flip) fd.parity ^= 1;  // predict's new pose set becomes current
     }
 }
 
@@ -1547,21 +1565,20 @@ __global__ __launch_bounds__(kEstFinTB) void k_est_fin(Layout L, FilterDev* F, c
     }
 }
 
-// The covariance finalize of k_est_fin<kCovN> (flip = 0, results into out) by 256 threads:
-// thread t takes the finalize's lanes t and t + 256 in turn, each wave folding them into the
-// rows of virtual waves w and w + 4, which are then added in the finalize's order.  The results
-// are bit-identical to the 512-thread kernel's.
+// The covariance finalize of k_est_fin<kCovN> (flip = 0, results into out) run by a
+// 1024-thread block: threads below kEstFinTB are the finalize's lanes (same loads, folds and
+// wave order, so the same bits), the others only join the barrier.  Every scalar of the filter
+// goes to out[f] except the likelihood scalars and the word holding the pose parity, which the
+// softmax finalize running beside this block rewrites for the next cycle.
 __device__ void cov_fin_block(const Layout& L, FilterDev* F, const double* __restrict__ part, int stride,
                               FilterDev* out, int f) {
     __shared__ double shv[kEstFinTB / 64][kCovN];
     const int b0 = L.f_off[f] / kTile, nb = L.f_padn[f] / kTile;
     const int t = (int)threadIdx.x;
-    static_assert(kEstFinTB == 2 * kTB, "two finalize lanes per thread");
-#pragma unroll 1
-    for (int h = 0; h < 2; ++h) {
+    if (t < kEstFinTB) {
         double acc[kCovN];
-        fin_lane_sums<kCovN, 1>(part + b0, stride, nb, t + h * kTB, acc);
-        wave_fold_store<kCovN>(acc, shv[(t >> 6) + h * (kTB / 64)]);
+        fin_lane_sums<kCovN, 4>(part + b0, stride, nb, t, acc);
+        wave_fold_store<kCovN>(acc, shv[t >> 6]);
     }
     __syncthreads();
     if (t < kCovN) {
@@ -1572,8 +1589,11 @@ __device__ void cov_fin_block(const Layout& L, FilterDev* F, const double* __res
         out[f].covs[t] = v;
     }
     constexpr int kWords = (int)(sizeof(FilterDev) / sizeof(uint64_t));
-    constexpr int v0 = (int)(offsetof(FilterDev, covs) / sizeof(uint64_t));
-    if (t < kWords && (t < v0 || t >= v0 + kCovN))
+    constexpr int c0 = (int)(offsetof(FilterDev, covs) / sizeof(uint64_t));
+    constexpr int l1 = (int)(offsetof(FilterDev, sum_w2) / sizeof(uint64_t));  // words [0, l1): likelihood
+    constexpr int pw = (int)(offsetof(FilterDev, parity) / sizeof(uint64_t));
+    static_assert(offsetof(FilterDev, max_ll) == 0 && offsetof(FilterDev, uniform) / 8 == pw, "FilterDev words");
+    if (t < kWords && t >= l1 && t != pw && (t < c0 || t >= c0 + kCovN))
         reinterpret_cast<uint64_t*>(out + f)[t] = reinterpret_cast<const uint64_t*>(F + f)[t];
 }
 
@@ -1608,14 +1628,13 @@ __global__ __launch_bounds__(kTB) void k_est_sums(Layout L, FilterDev* F, PoseBu
 
 // cov layout (cu:406-471): [0..5] position, [6..11] attitude tangent (axis-angle of qm^-1 q,
 // canonical sign, angle = 2 atan2(|v|, w)).
-__global__ __launch_bounds__(kTB) void k_est_covs(Layout L, FilterDev* F, PoseBufs B, const float* __restrict__ w,
-                                                  double* __restrict__ part, int stride, ParamRef<CovParams> A) {
-    const int blk = blockIdx.x;
+__device__ void covs_tile(const Layout& L, FilterDev* F, const PoseBufs& B, int swapped, const float* __restrict__ w,
+                          double* __restrict__ part, int stride, const ParamRef<CovParams>& A, int blk) {
     const int f = L.blk_filter[blk * (kTile / kTB)];
     const int off = L.f_off[f];
     const int nf = L.f_n[f];
     const CovParams& p = A.get(f);
-    const PoseSet S = B.s[F[f].parity];
+    const PoseSet S = B.s[F[f].parity ^ swapped];  // the current set
     double acc[kCovN];
 #pragma unroll
     for (int k = 0; k < kCovN; ++k) acc[k] = 0.0;
@@ -1693,6 +1712,11 @@ __global__ __launch_bounds__(kTB) void k_est_covs(Layout L, FilterDev* F, PoseBu
         }
     }
     block_partials<kCovN>(acc, part, blk, stride);
+}
+
+__global__ __launch_bounds__(kTB) void k_est_covs(Layout L, FilterDev* F, PoseBufs B, const float* __restrict__ w,
+                                                  double* __restrict__ part, int stride, ParamRef<CovParams> A) {
+    covs_tile(L, F, B, 0, w, part, stride, A, blockIdx.x);
 }
 
 // Sets the per-filter resample predicate: force (ABI call) or ESS < ratio * n (fused cycle).
@@ -1839,35 +1863,36 @@ void prof_end(PfState* s) {
     (void)hipEventRecord(s->prof_pending.back().b, s->stream);
 }
 
-int launch_motion(PfState* s, ParamRef<MotionParams> A, int reinit) {
+int launch_motion(PfState* s, ParamRef<MotionParams> A, int reinit, int dst_flip) {
     prof_begin(s, reinit ? PFX_STAGE_REINIT : PFX_STAGE_PREDICT);
     hipLaunchKernelGGL(reinit ? k_motion<1> : k_motion<0>, dim3(s->NP / kTB), dim3(kTB), 0, s->stream, s->L, s->pose,
-                       s->fdev, s->w, A);
+                       s->fdev, s->w, A, dst_flip);
     prof_end(s);
     PF_LAUNCH_CHECK();
     return 0;
 }
 
-int launch_weight(PfState* s, ParamRef<WeightParams> A, CovFin cf) {
+int launch_weight(PfState* s, ParamRef<WeightParams> A, const WeightDefer& d) {
+    const bool dc = d.cov != nullptr;
+    if (dc && (!d.out || !d.flip)) return -1;
     prof_begin(s, PFX_STAGE_WEIGHT);
     const int ppt = s->weight_ppt;
     const int nblk = s->NP / (kTB * ppt);
-    const int grid = nblk + (cf.part ? s->nf : 0);
-    if (cf.part && (!cf.out || cf.stride != s->NP / kTile)) return -1;
-    if (ppt == 1)
-        hipLaunchKernelGGL(k_weight<1>, dim3(grid), dim3(kTB), 0, s->stream, s->L, s->pose, s->fdev, s->ll,
-                           s->part_w, A, cf, nblk);
-    else if (ppt == 4)
-        hipLaunchKernelGGL(k_weight<4>, dim3(grid), dim3(kTB), 0, s->stream, s->L, s->pose, s->fdev, s->ll,
-                           s->part_w, A, cf, nblk);
-    else
-        hipLaunchKernelGGL(k_weight<2>, dim3(grid), dim3(kTB), 0, s->stream, s->L, s->pose, s->fdev, s->ll,
-                           s->part_w, A, cf, nblk);
+    const int ntile = s->NP / kTile;
+    const int grid = nblk + (dc ? ntile : 0);
+    const ParamRef<CovParams> cA = ParamRef<CovParams>::array(d.cov);
+    auto kw = ppt == 1 ? (dc ? k_weight<1, true> : k_weight<1, false>)
+              : ppt == 4 ? (dc ? k_weight<4, true> : k_weight<4, false>)
+                         : (dc ? k_weight<2, true> : k_weight<2, false>);
+    PoseBufs B = s->pose;
+    if (d.flip) std::swap(B.s[0], B.s[1]);  // B.s[parity]: the set predict wrote
+    hipLaunchKernelGGL(kw, dim3(grid), dim3(kTB), 0, s->stream, s->L, B, s->fdev, s->ll, s->part_w, A, s->w,
+                       s->part_cov, ntile, cA, nblk);
     prof_end(s);
     PF_LAUNCH_CHECK();
     prof_begin(s, PFX_STAGE_FINALIZE);
-    hipLaunchKernelGGL(k_weight_fin, dim3(s->nf), dim3(kFinTB), 0, s->stream, s->L, s->fdev, s->part_w,
-                       kTB * ppt);
+    hipLaunchKernelGGL(k_weight_fin, dim3(s->nf + (dc ? s->nf : 0)), dim3(kFinTB), 0, s->stream, s->L, s->fdev,
+                       s->part_w, kTB * ppt, s->nf, d.flip, dc ? s->part_cov : nullptr, ntile, d.out);
     prof_end(s);
     PF_LAUNCH_CHECK();
     return 0;
@@ -1948,16 +1973,12 @@ int launch_est_sums(PfState* s, CovParams* d_cov, bool flip, bool pass) {
     return 0;
 }
 
-int launch_est_covs(PfState* s, ParamRef<CovParams> A, FilterDev* results, bool finalize) {
+int launch_est_covs(PfState* s, ParamRef<CovParams> A, FilterDev* results) {
     prof_begin(s, PFX_STAGE_EST_COVS);
     hipLaunchKernelGGL(k_est_covs, dim3(s->NP / kTile), dim3(kTB), 0, s->stream, s->L, s->fdev, s->pose, s->w,
                        s->part_cov, s->NP / kTile, A);
     prof_end(s);
     PF_LAUNCH_CHECK();
-    return finalize ? launch_cov_fin(s, results) : 0;
-}
-
-int launch_cov_fin(PfState* s, FilterDev* results) {
     prof_begin(s, PFX_STAGE_FINALIZE);
     hipLaunchKernelGGL(k_est_fin<kCovN>, dim3(s->nf), dim3(kEstFinTB), 0, s->stream, s->L, s->fdev, s->part_cov,
                        s->NP / kTile, 1, nullptr, 0, s->scan_ctl, results);
