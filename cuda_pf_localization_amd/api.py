@@ -220,6 +220,9 @@ class PfKernels:
         _check(self.L.pfx_get_rng(self.h, C.byref(g), C.byref(n)), "pfx_get_rng")
         return g.value, n.value
 
+    def set_rng(self, gseed: int, gen: int):
+        _check(self.L.pfx_set_rng(self.h, gseed, gen), "pfx_set_rng")
+
     def weight_scalars(self):
         out = np.zeros(4)
         _check(self.L.pfx_weight_scalars(self.h, _dp(out)), "pfx_weight_scalars")

@@ -156,12 +156,19 @@ struct PfState {
     ObsRec* d_obs = nullptr;
     ObsRec* h_obs = nullptr;  // pinned staging
     hipEvent_t obs_ev = nullptr;
+    // pinned landing buffer and completion event of the ABI's small synchronous reads
+    double* h_small = nullptr;
+    hipEvent_t small_ev = nullptr;
     std::vector<float> lm;  // host mirror of the dense landmark table
     int lm_cap = 0;
     // deterministic-draw state (single-filter ABI)
     uint64_t gseed = 42, gen = 0;
     // launch shape of the likelihood kernel (particles per lane: 1, 2 or 4)
     int weight_ppt = 1;
+    // part_q holds the fixed-point tile aggregates of the current weights (left by the last
+    // normalisation; cleared by every launch or copy that writes the weights otherwise), so the
+    // ABI's resampling scan can start from them instead of looking back
+    bool agg_ok = false;
     // profiling
     bool prof_on = false;
     std::vector<ProfEvent> prof_pending;

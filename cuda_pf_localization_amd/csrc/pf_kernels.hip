@@ -671,17 +671,31 @@ __device__ inline float4 tile_norm(const double* __restrict__ ll, int tile, cons
 
 // Normalised weights of every 1024-particle tile plus its partial of sum w^2 (the same tile
 // function and block tree as the fused cycle's k_norm_agg, so both give the same bits).
+// (defined with the resampling scan below)
+__device__ inline void tile_q(const float4 wv, int tile, int off, int nf, uint64_t (&q)[4]);
+__device__ inline uint64_t block_sum_u64(uint64_t v, uint64_t* s_w);
+
+// The tile's fixed-point aggregate sum q as well (part_q), which a later resampling scan starts
+// from while the weights stay as they are (k_scan_agg).
 __global__ __launch_bounds__(kTB) void k_normalize(Layout L, FilterDev* F, const double* __restrict__ ll,
-                                                   float* __restrict__ w, double* __restrict__ part, int force_uniform) {
+                                                   float* __restrict__ w, double* __restrict__ part,
+                                                   uint64_t* __restrict__ part_q, int force_uniform) {
     __shared__ double sh[4];
+    __shared__ uint64_t s_w[4];
     const int tile = blockIdx.x;
     const int f = L.blk_filter[tile * (kTile / kTB)];
     const FilterDev& fd = F[f];
     double w2;
     const float4 wv = tile_norm(ll, tile, L, f, fd, force_uniform || fd.uniform, w2);
     *reinterpret_cast<float4*>(w + tile * kTile + (int)threadIdx.x * 4) = wv;
+    uint64_t q[4];
+    tile_q(wv, tile, L.f_off[f], L.f_n[f], q);
     const double bs = block_sum(w2, sh);
-    if (threadIdx.x == 0) part[tile] = bs;
+    const uint64_t agg = block_sum_u64(q[0] + q[1] + q[2] + q[3], s_w);
+    if (threadIdx.x == 0) {
+        part[tile] = bs;
+        part_q[tile] = agg;
+    }
 }
 
 // Per filter: ESS = 1 / sum w^2 of the stored weights -- the quantity the reference driver
@@ -1006,7 +1020,8 @@ __global__ __launch_bounds__(kTB) void k_norm_agg(Layout L, FilterDev* F, const 
 // advance (pf_kernels.cu:647-657) -- the sum in k_norm_fin's order (1024 lanes, lane v summing
 // tiles v, v + 1024, ..., its 16 waves folded by wave_sum and added in sequence), which this
 // block's four waves reproduce as virtual waves w, w + 4, w + 8, w + 12, so both paths give the
-// same bits.
+// same bits.  A null ratio forces the predicate (the reference ABI's pfc_resample, scanning from
+// the aggregates k_normalize left).
 __global__ __launch_bounds__(kTB) void k_scan_agg(Layout L, FilterDev* F, const float* __restrict__ w,
                                                   const double* __restrict__ part_n,
                                                   const uint64_t* __restrict__ part_q, uint64_t* __restrict__ cum,
@@ -1076,8 +1091,12 @@ __global__ __launch_bounds__(kTB) void k_scan_agg(Layout L, FilterDev* F, const 
         fd.est_src = 0;  // weights changed
         fd.sum_w2 = tot;
         fd.ess = tot > 0.0 ? 1.0 / tot : 0.0;
-        const double r = (double)ratio[f];
-        fd.do_resample = (r > 0.0 && fd.ess < r * (double)nf) ? 1 : 0;
+        if (ratio) {
+            const double r = (double)ratio[f];
+            fd.do_resample = (r > 0.0 && fd.ess < r * (double)nf) ? 1 : 0;
+        } else {
+            fd.do_resample = 1;  // reference ABI: pfc_resample always resamples (cu:638-676)
+        }
         if (fd.do_resample && T > 0) {
             const uint64_t lcg = fd.lcg * 6364136223846793005ull + 1442695040888963407ull;  // cu:653
             fd.lcg = lcg;
@@ -1740,6 +1759,7 @@ namespace pfk {
 
 int state_alloc(PfState* s, const std::vector<int>& ns) {
     state_free_particles(s);
+    s->agg_ok = false;
     const int nf = (int)ns.size();
     s->nf = nf;
     s->n.assign(ns.begin(), ns.end());
@@ -1864,6 +1884,7 @@ void prof_end(PfState* s) {
 }
 
 int launch_motion(PfState* s, ParamRef<MotionParams> A, int reinit, int dst_flip) {
+    if (reinit) s->agg_ok = false;  // reinit writes the weights
     prof_begin(s, reinit ? PFX_STAGE_REINIT : PFX_STAGE_PREDICT);
     hipLaunchKernelGGL(reinit ? k_motion<1> : k_motion<0>, dim3(s->NP / kTB), dim3(kTB), 0, s->stream, s->L, s->pose,
                        s->fdev, s->w, A, dst_flip);
@@ -1902,6 +1923,7 @@ int launch_normalize(PfState* s, int force_uniform, const float* d_ratio, bool s
     const int ntile = s->NP / kTile;
     if (scan) {
         if (!d_ratio) return -1;  // the fused scan forms the resample predicate
+        s->agg_ok = false;        // (the gather that follows rewrites the weights)
         prof_begin(s, PFX_STAGE_NORMALIZE);
         hipLaunchKernelGGL(k_norm_agg, dim3(ntile), dim3(kTB), 0, s->stream, s->L, s->fdev, s->ll, s->w, s->part_n,
                            s->part_q, inj, s->inj_jit, s->inj_head, s->inj_next);
@@ -1916,8 +1938,9 @@ int launch_normalize(PfState* s, int force_uniform, const float* d_ratio, bool s
     }
     prof_begin(s, PFX_STAGE_NORMALIZE);
     hipLaunchKernelGGL(k_normalize, dim3(ntile), dim3(kTB), 0, s->stream, s->L, s->fdev, s->ll, s->w, s->part_n,
-                       force_uniform);
+                       s->part_q, force_uniform);
     prof_end(s);
+    s->agg_ok = true;
     PF_LAUNCH_CHECK();
     prof_begin(s, PFX_STAGE_FINALIZE);
     hipLaunchKernelGGL(k_norm_fin, dim3(s->nf), dim3(kFinTB), 0, s->stream, s->L, s->fdev, s->part_n, d_ratio);
@@ -1929,13 +1952,23 @@ int launch_normalize(PfState* s, int force_uniform, const float* d_ratio, bool s
 int launch_resample(PfState* s, ParamRef<InjectParams> inj, const int32_t* inj_tab, int inj_nblk, bool flip,
                     bool scan) {
     const int ntile = s->NP / kTile;
-    if (scan) {
+    if (scan && s->agg_ok) {
+        // the weights are those the last normalisation left aggregates for: reduce-then-scan
+        // from them (the same integer prefixes, total and LCG advance as the look-back scan;
+        // the predicate forced, as k_resample_predicate does for this path)
+        prof_begin(s, PFX_STAGE_RESAMPLE_SCAN);
+        hipLaunchKernelGGL(k_scan_agg, dim3(ntile), dim3(kTB), 0, s->stream, s->L, s->fdev, s->w, s->part_n,
+                           s->part_q, s->cum, nullptr);
+        prof_end(s);
+        PF_LAUNCH_CHECK();
+    } else if (scan) {
         prof_begin(s, PFX_STAGE_RESAMPLE_SCAN);
         hipLaunchKernelGGL(k_scan, dim3(ntile), dim3(kTB), 0, s->stream, s->L, s->fdev, s->w, s->cum, s->scan_st,
                            s->scan_ctl, s->scan_spin_limit);
         prof_end(s);
         PF_LAUNCH_CHECK();
     }
+    s->agg_ok = false;  // the gather rewrites the weights
     if (scan && inj_nblk > 0) {  // (the fused normalise-and-scan drew them otherwise)
         prof_begin(s, PFX_STAGE_INJECT);
         hipLaunchKernelGGL(k_inj_pick, dim3(inj_nblk), dim3(kTB), 0, s->stream, s->L, s->fdev, inj, inj_tab,
@@ -2091,6 +2124,8 @@ int pfc_device_available(void) {
     return prop.major >= 9 ? 1 : 0;
 }
 
+constexpr int kSmallRead = 16;  // doubles: the largest synchronous result (pfc_est_sums' 14)
+
 void* pfc_create(int cap_obs) {
     PfState* s = new (std::nothrow) PfState();
     if (!s) return nullptr;
@@ -2099,7 +2134,9 @@ void* pfc_create(int cap_obs) {
     if (hipSetDevice(0) != hipSuccess || hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess ||
         hipMalloc((void**)&s->d_obs, sizeof(ObsRec) * (size_t)cap_obs) != hipSuccess ||
         hipHostMalloc((void**)&s->h_obs, sizeof(ObsRec) * (size_t)cap_obs, hipHostMallocDefault) != hipSuccess ||
-        hipEventCreateWithFlags(&s->obs_ev, hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&s->obs_ev, hipEventDisableTiming) != hipSuccess ||
+        hipHostMalloc((void**)&s->h_small, sizeof(double) * kSmallRead, hipHostMallocDefault) != hipSuccess ||
+        hipEventCreateWithFlags(&s->small_ev, hipEventDisableTiming) != hipSuccess) {
         fprintf(stderr, "pf_hip pfc_create: device setup failed\n");
         pfc_destroy(s);
         return nullptr;
@@ -2114,6 +2151,8 @@ int pfc_destroy(void* h) {
     state_free_particles(s);
     (void)hipFree(s->d_obs);
     (void)hipHostFree(s->h_obs);
+    (void)hipHostFree(s->h_small);
+    if (s->small_ev) (void)hipEventDestroy(s->small_ev);
     for (auto& p : s->prof_pending) {
         (void)hipEventDestroy(p.a);
         (void)hipEventDestroy(p.b);
@@ -2221,14 +2260,26 @@ int pfc_resample(void* h, float random_inject_ratio) {
     return launch_resample(s, ParamRef<InjectParams>::value(ip), nullptr, (k + kTB - 1) / kTB, true);
 }
 
+// A synchronous read of n doubles after the work queued on the stream: copied into the handle's
+// pinned buffer, completion polled on an event (a short cycle's result is ready within tens of
+// microseconds; a sleeping wait would add the wake-up latency to every call), then handed to the
+// caller's memory.
+static int read_small(PfState* s, double* out, const double* dsrc, int n) {
+    PF_CHECK(hipMemcpyAsync(s->h_small, dsrc, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost, s->stream));
+    PF_CHECK(hipEventRecord(s->small_ev, s->stream));
+    hipError_t q;
+    while ((q = hipEventQuery(s->small_ev)) == hipErrorNotReady) {
+    }
+    PF_CHECK(q);
+    memcpy(out, s->h_small, sizeof(double) * (size_t)n);
+    return 0;
+}
+
 int pfc_est_sums(void* h, double* out) {
     PfState* s = static_cast<PfState*>(h);
     if (!s || s->nf != 1 || !out) return -1;
     if (launch_est_sums(s, nullptr, false) != 0) return -1;
-    PF_CHECK(hipMemcpyAsync(out, fdev_field<double>(s, 0, offsetof(FilterDev, sums)), sizeof(double) * kEstN,
-                            hipMemcpyDeviceToHost, s->stream));
-    PF_CHECK(hipStreamSynchronize(s->stream));
-    return 0;
+    return read_small(s, out, fdev_field<double>(s, 0, offsetof(FilterDev, sums)), kEstN);
 }
 
 int pfc_covs(void* h, float mx, float my, float mz, float qw, float qx, float qy, float qz, double* out) {
@@ -2243,10 +2294,7 @@ int pfc_covs(void* h, float mx, float my, float mz, float qw, float qx, float qy
     p.mq[2] = qy;
     p.mq[3] = qz;
     if (launch_est_covs(s, ParamRef<CovParams>::value(p)) != 0) return -1;
-    PF_CHECK(hipMemcpyAsync(out, fdev_field<double>(s, 0, offsetof(FilterDev, covs)), sizeof(double) * kCovN,
-                            hipMemcpyDeviceToHost, s->stream));
-    PF_CHECK(hipStreamSynchronize(s->stream));
-    return 0;
+    return read_small(s, out, fdev_field<double>(s, 0, offsetof(FilterDev, covs)), kCovN);
 }
 
 static int read_parity(PfState* s, int f, int* par) {
@@ -2293,6 +2341,7 @@ int pfx_set_state(void* h, const float* pos, const float* quat, const float* w) 
         if (tot >= 16.0) return -1;  // fixed-point resampling headroom
     }
     PF_CHECK(hipStreamSynchronize(s->stream));
+    s->agg_ok = false;  // the weights are replaced
     int par = 0;
     if (read_parity(s, 0, &par) != 0) return -1;
     std::vector<float> tmp((size_t)n);

@@ -147,12 +147,14 @@ def test_cuda_matches_cpu_ranking(gpu):
 
 
 def test_config2_localization_success(gpu):
-    """Config 2: obstacle map, 360 rays, 8 m, 0.02 m noise, 200k hypotheses, bottom-64:
-    the best hypothesis is within 0.5 m of the truth for most poses."""
+    """Config 2: obstacle map, 360 rays, 8 m, 0.02 m noise, 200k hypotheses, bottom-64, its 20
+    random true poses (seed 3): the best hypothesis is within 0.5 m of the truth (the success
+    criterion of pf_bench main.rs:96-176) for at least 19 of them (the reference reports 20/20 at
+    200k hypotheses on its own map, README.md:80)."""
     from cuda_pf_localization_amd.sim import free_poses
     cells, res, org = obstacle_map(3)
     og = OracleGrid(cells, res, org)
-    truths = free_poses(cells, res, org, 10, seed=3, margin_cells=6)
+    truths = free_poses(cells, res, org, 20, seed=3, margin_cells=6)
     gl = GlobalLoc(cells, res, org, GlParams(hypotheses=200_000, rmax=8.0, topk=64, seed=3))
     rng = np.random.default_rng(3)
     ok = 0
@@ -160,7 +162,8 @@ def test_config2_localization_success(gpu):
         scan = og.map_scan64(*t, 360, TAU, 8.0) + rng.normal(size=360) * 0.02
         top = gl.localize(scan.astype(np.float32))
         ok += math.hypot(top[0, 0] - t[0], top[0, 1] - t[1]) < 0.5
-    assert ok >= 8, f"{ok}/10"
+    print(f"config 2: {ok}/20 poses localised within 0.5 m")
+    assert ok >= 19, f"{ok}/20"
 
 
 @pytest.mark.parametrize("map_seed", [3, 4])

@@ -349,6 +349,45 @@ def test_resample_scan_direct_route_bitexact(gpu, n):
         assert k.get_lcg() == o.rng()[2]
 
 
+@pytest.mark.parametrize("n", [5000, 300_007])
+@pytest.mark.parametrize("ratio", [0.0, 0.01])
+def test_resample_after_weight_scans_from_aggregates(gpu, n, ratio):
+    """pfc_resample right after pfc_weight starts from the fixed-point tile aggregates the
+    normalisation left (reduce, then scan); after pfx_set_state it takes the look-back scan.
+    On the same weights, poses and LCG both routes give the same ancestors, LCG advance and
+    resampled poses, and the aggregate route matches the oracle."""
+    lm = make_landmarks(256, 9, box=(50.0, 50.0, 10.0))
+    q_t = (math.cos(1.0), 0.0, 0.0, math.sin(1.0))
+    truth = np.array([25.0, 25.0, 5.0])
+    obs = rb_obs_for(truth, q_t, lm, 10, outliers=0.01)
+    a, o = pair(n, 31)
+    pose = f32([*truth, *q_t])
+    a.reinit(pose, f32([0.5] * 3), f32([0.1] * 3))
+    o.reinit(pose, f32([0.5] * 3), f32([0.1] * 3))
+    a.upload_landmarks(lm)
+    a.weight(obs, 0.05, 0.02, 3.0)
+    pos, quat, w = a.get_state()
+    lcg = a.get_lcg()
+    rng = a.rng()
+    a.resample(ratio)  # aggregate route
+    b = PfKernels(256)
+    b.init(n, 31)
+    b.set_state(pos, quat, w)
+    b.set_lcg(lcg)
+    b.set_rng(*rng)  # the injection draws' generation
+    b.resample(ratio)  # look-back route
+    assert np.array_equal(a.ancestors(), b.ancestors())
+    assert a.get_lcg() == b.get_lcg()
+    pa, qa, wa = a.get_state()
+    pb, qb, wb = b.get_state()
+    assert np.array_equal(pa, pb) and np.array_equal(qa, qb) and np.array_equal(wa, wb)
+    o.set_state(pos.astype(np.float64), quat.astype(np.float64), w.astype(np.float64))
+    o.set_rng(o.rng()[0], rng[1], lcg)
+    o.resample(ratio)
+    assert np.array_equal(a.ancestors(), o.ancestors())
+    b.close()
+
+
 def test_resample_bitexact_more_tiles_than_resident(gpu):
     """4.2M particles: more scan tiles than the device holds at once (later tiles start while
     earlier ones are still finishing); ancestors still bit-exact, over two launches."""
@@ -572,10 +611,10 @@ def oracle_update(o, st, lm, cfg, n):
 @pytest.mark.parametrize("fast", [False, True])
 def test_full_cycle_matches_oracle_driver(gpu, fast):
     """Config-0 tracking (10k particles, 12 landmarks, RB obs, gate 3): the C++ host layer over
-    the HIP ABI vs the oracle driver (particle_filter.rs:110-124) for 30 cycles."""
+    the HIP ABI vs the oracle driver (particle_filter.rs:110-124) for the config's 100 cycles."""
     from cuda_pf_localization_amd import Observation, OdomDelta, ParticleFilter, PfConfig
     from cuda_pf_localization_amd.sim import config0_scenario
-    sc = config0_scenario(cycles=30, seed=42)
+    sc = config0_scenario(cycles=100, seed=42)
     n = 10_000
     cfg = PfConfig(particle_count=n, seed=42, range_noise=0.05, bearing_noise=0.02, gate_sigma=3.0,
                    resample_ess_ratio=0.5, random_inject_ratio=0.0, fast=fast)
