@@ -44,12 +44,14 @@ WEIGHT_FLOP_PER_PAIR_ISSUED = 54.0
 # fixed-point cumulative weight 8 B).  ABI driver stages:
 BYTES = {"predict": 56, "weight": 36, "normalize": 12, "resample_scan": 12, "resample_gather": 72,
          "est_sums": 32, "est_covs": 32}
-# Fused cycle stages: normalize = normalisation + tile aggregates + injection picks (read ll,
-# write w), resample_scan = the scan from the tile aggregates (read w, write cum), resample_gather
-# = gather + injection + estimate-sum partials (read cum window, pose, list head; write pose, w,
-# ancestor), finalize = the one-workgroup-per-filter reductions (no particle traffic).
-FUSED_BYTES = {"predict": 56, "weight": 36, "normalize": 12, "resample_scan": 12, "resample_gather": 76,
-               "est_covs": 32, "finalize": 0}
+# Fused (staged) cycle stages: weight = the likelihood (read pose, write ll: 36) plus the previous
+# cycle's covariance pass that runs as extra workgroups of the same launch (read pose + w: 32),
+# normalize = normalisation + tile aggregates + injection picks (read ll, write w), resample_scan
+# = the scan from the tile aggregates (read w, write cum), resample_gather = gather + injection +
+# estimate-sum partials (read cum window, pose, list head; write pose, w, ancestor), finalize =
+# the one-workgroup-per-filter reductions (no particle traffic).
+FUSED_BYTES = {"predict": 56, "weight": 68, "normalize": 12, "resample_scan": 12, "resample_gather": 76,
+               "finalize": 0}
 # Cycle with resampling (predict 56, weight 32, softmax 8, estimate 64, resample 64) and the
 # likelihood's share of it.
 CYCLE_BYTES = 224
@@ -596,6 +598,11 @@ def main():
                                        "run)",
                      "traffic_algorithmic": 36 * n,
                      "ms_avg": round(w["ms_avg"], 5),
+                     "launch_includes": "the previous cycle's covariance pass (one extra workgroup per "
+                                        "1024-particle tile, dispatched after the likelihood's): the achieved "
+                                        "rate is counted on the likelihood's FLOP alone",
+                     "ms_avg_plain_kernel_abi_driver_1_per_lane": round(stages["weight"]["ms_avg"], 5)
+                     if stages.get("weight") else None,
                      "flop_per_pair": WEIGHT_FLOP_PER_PAIR, "pairs_per_launch": pairs,
                      "particles_per_lane": int(weight_ppt),
                      "issued": {"flop_per_pair": WEIGHT_FLOP_PER_PAIR_ISSUED,

@@ -269,6 +269,9 @@ class GlKernels:
         in a second pass (same CAER bits)."""
         _check(self.L.pfglx_set_score_mode(self.h, mode), "pfglx_set_score_mode")
 
+    def set_prune(self, enable: bool):
+        _check(self.L.pfglx_set_prune(self.h, 1 if enable else 0), "pfglx_set_prune")
+
     def free_cells(self) -> int:
         return self.L.pfgl_free_cells(self.h)
 

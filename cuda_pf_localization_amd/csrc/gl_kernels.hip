@@ -106,6 +106,22 @@ struct GlState {
     float* d_poses = nullptr;
     float* d_ranges = nullptr;
     size_t render_poses_cap = 0, render_ranges_cap = 0;
+    // Pruned scoring (large batches): pfgl_score leaves a lower bound of every CAER (the sum
+    // over every second ray) and the next bottom-k scores only what can still be among the k
+    // smallest.  caer_state: 0 d_caer holds every hypothesis' CAER; 1 only the bounds (d_lb) of
+    // the scoring of pend_nrays rays at pend_rmax; 2 exact for every hypothesis that can be among
+    // the bottom pruned_k, +inf for the others.
+    bool prune = true;
+    int caer_state = 0;
+    int pend_route = 1;  // of the pending bounds: 1 the wave kernel, 2 one lane per ray
+    int pend_nrays = 0, pruned_k = 0;
+    float pend_rmax = 0.f;
+    float* d_lb = nullptr;
+    int32_t* d_surv = nullptr;
+    int* d_nsurv = nullptr;  // [0] survivor count
+    float* d_tau = nullptr;
+    float* d_cand = nullptr;   // packed poses of the candidates
+    float* d_cdiff = nullptr;  // their per-ray errors (kSortMax x kLdsRaysMax)
     // profiling
     bool prof_on = false;
     struct Ev {
@@ -199,11 +215,17 @@ __global__ __launch_bounds__(kTB) void k_gl_gen(const int32_t* __restrict__ free
 
 // CAER scoring.  LDS layout: [grid][real scan f32][cos phi f64][sin phi f64]; the grid is
 // staged as bytes (kBytes, 1024-thread blocks, two per CU) or as bits (256-thread blocks).
-template <int kBT, bool kBytes, bool kLdsGrid, bool kLdsRays>
+// kSub = 2: every second round of rays only (lane l: rays l, l + 128, l + 256, ...: blocks of 64
+// consecutive rays, so a wave's lanes still walk neighbouring directions), a lower bound of the
+// CAER up to rounding, for pruned scoring.  list (non-null): score the hypotheses list[i] for i below
+// *nlist (nlist non-null) or nh; mark (non-null): set mark[h] = -1 for every hypothesis scored.
+template <int kBT, bool kBytes, bool kLdsGrid, bool kLdsRays, int kSub = 1>
 __global__ __launch_bounds__(kBT) void k_gl_score(Grid g, const float* __restrict__ hyp, int nh,
                                                   const float* __restrict__ scan, const double* __restrict__ cphi,
                                                   const double* __restrict__ sphi, int nrays, float rmax,
-                                                  float* __restrict__ caer) {
+                                                  float* __restrict__ caer, const int32_t* __restrict__ list = nullptr,
+                                                  const int* __restrict__ nlist = nullptr,
+                                                  float* __restrict__ mark = nullptr) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int gbytes = kLdsGrid ? (kBytes ? g.nbytes : 4 * ((g.nwords + 3) & ~3)) : 0;
     double* scphi = reinterpret_cast<double*>(smem + gbytes);
@@ -231,12 +253,14 @@ __global__ __launch_bounds__(kBT) void k_gl_score(Grid g, const float* __restric
     const float* sc = kLdsRays ? sscan : scan;
     const int lane = threadIdx.x & 63;
     const int waves_total = gridDim.x * (kBT / 64);
-    for (int h = blockIdx.x * (kBT / 64) + (threadIdx.x >> 6); h < nh; h += waves_total) {
+    const int cnt = nlist ? *nlist : nh;
+    for (int i = blockIdx.x * (kBT / 64) + (threadIdx.x >> 6); i < cnt; i += waves_total) {
+        const int h = list ? list[i] : i;
         const float x = hyp[3 * h], y = hyp[3 * h + 1];
         double st, ct;
         det_sincos((double)hyp[3 * h + 2], st, ct);
         float acc = 0.f;
-        for (int r = lane; r < nrays; r += 64) {
+        for (int r = lane; r < nrays; r += 64 * kSub) {
             const double cp = tc[r], sp = ts[r];
             const float dx = (float)(ct * cp - st * sp);
             const float dy = (float)(st * cp + ct * sp);
@@ -249,8 +273,39 @@ __global__ __launch_bounds__(kBT) void k_gl_score(Grid g, const float* __restric
             acc += fabsf(sc[r] - vm);
         }
         acc = wave_sum_down(acc);
-        if (lane == 0) caer[h] = acc;
+        if (lane == 0) {
+            caer[h] = acc;
+            if (mark) mark[h] = -1.f;
+        }
     }
+}
+
+
+// Pruned scoring, survivors: a hypothesis whose bound, less a margin that covers the rounding of
+// both sums, exceeds the threshold cannot be among the bottom k (k candidates have CAER <= tau);
+// its CAER is set to +inf.  Every other one, except the candidates (bound marked -1, CAER exact
+// already), is appended to the survivor list, one atomic per wave.
+__global__ __launch_bounds__(kTB) void k_gl_prune(const float* __restrict__ lb, int n, const float* __restrict__ tau,
+                                                  float* __restrict__ caer, int32_t* __restrict__ surv,
+                                                  int* __restrict__ nsurv) {
+    const int i = blockIdx.x * kTB + threadIdx.x;
+    bool keep = false;
+    if (i < n) {
+        const float b = lb[i];
+        if (!(b < 0.f)) {  // (a NaN bound is kept: the comparison below is false for it)
+            if ((double)b * (1.0 - 1e-4) > (double)*tau)
+                caer[i] = INFINITY;
+            else
+                keep = true;
+        }
+    }
+    const uint64_t bal = __ballot(keep);
+    if (bal == 0) return;
+    const int lane = threadIdx.x & 63;
+    int base = 0;
+    if (lane == __ffsll((long long)bal) - 1) base = atomicAdd(nsurv, __popcll(bal));
+    base = __shfl(base, __ffsll((long long)bal) - 1, 64);
+    if (keep) surv[base + __popcll(bal & ((1ull << lane) - 1ull))] = i;
 }
 
 // One lane per (pose, ray) over a whole batch, consecutive lanes on consecutive rays of one
@@ -298,8 +353,10 @@ __global__ __launch_bounds__(kBT) void k_gl_rays(Grid g, const float* __restrict
     }
 }
 
+// idx (non-null): the sum of row h is the CAER of hypothesis idx[h], and mark[idx[h]] = -1.
 __global__ __launch_bounds__(kTB) void k_gl_caer_sum(const float* __restrict__ diff, int nh, int nrays,
-                                                     float* __restrict__ caer) {
+                                                     float* __restrict__ caer, const int32_t* __restrict__ idx = nullptr,
+                                                     float* __restrict__ mark = nullptr) {
     const int h = blockIdx.x * (kTB / 64) + (threadIdx.x >> 6);
     if (h >= nh) return;
     const int lane = threadIdx.x & 63;
@@ -307,7 +364,95 @@ __global__ __launch_bounds__(kTB) void k_gl_caer_sum(const float* __restrict__ d
     float acc = 0.f;
     for (int r = lane; r < nrays; r += 64) acc += d[r];
     acc = wave_sum_down(acc);
-    if (lane == 0) caer[h] = acc;
+    if (lane == 0) {
+        const int o = idx ? idx[h] : h;
+        caer[o] = acc;
+        if (mark) mark[o] = -1.f;
+    }
+}
+
+// Pruned scoring on the one-lane-per-ray route.  Ray r belongs to block (r >> 6); kSel = 1 casts
+// the rays of the even blocks (0-63, 128-191, ...: the rays lane l of k_gl_score casts in its
+// rounds 0, 2, 4, ...), kSel = 2 those of the odd blocks, each |scan - cast| stored at diff[h
+// nrays + r] exactly as k_gl_rays stores it (same direction arithmetic), for the hypotheses
+// h = list[i] (i < *nlist, or i < n when nlist is null; h = i when list is null).  The byte grid
+// is staged in LDS (1024-thread workgroups).
+template <int kSel>
+__global__ __launch_bounds__(1024) void k_gl_rays_sel(Grid g, const float* __restrict__ hyp, int n,
+                                                      const int32_t* __restrict__ list, const int* __restrict__ nlist,
+                                                      const float* __restrict__ scan, const double* __restrict__ cphi,
+                                                      const double* __restrict__ sphi, int nrays, float rmax,
+                                                      float* __restrict__ diff) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    {
+        const uint4* src = reinterpret_cast<const uint4*>(g.bytes);
+        uint4* dst = reinterpret_cast<uint4*>(smem);
+        for (int i = threadIdx.x; i < g.nbytes / 16; i += 1024) dst[i] = src[i];
+        __syncthreads();
+    }
+    const int cnt = nlist ? *nlist : n;
+    const int per = ((nrays + 127) >> 7) << 6;  // selected-ray slots per hypothesis
+    const size_t total = (size_t)cnt * (size_t)per;
+    for (size_t t = (size_t)blockIdx.x * 1024 + threadIdx.x; t < total; t += (size_t)gridDim.x * 1024) {
+        const int i = (int)(t / (size_t)per), j = (int)(t - (size_t)i * per);
+        const int r = ((j >> 6) << 7) + (kSel == 2 ? 64 : 0) + (j & 63);
+        if (r >= nrays) continue;
+        const int h = list ? list[i] : i;
+        const float x = hyp[3 * h], y = hyp[3 * h + 1];
+        double st, ct;
+        det_sincos((double)hyp[3 * h + 2], st, ct);
+        const double cp = cphi[r], sp = sphi[r];
+        const float dx = (float)(ct * cp - st * sp);
+        const float dy = (float)(st * cp + ct * sp);
+        const float vm = cast_ray(OccBytes{smem}, g.W, g.H, g.res, g.ox, g.oy, x, y, dx, dy, rmax);
+        diff[(size_t)h * nrays + r] = fabsf(scan[r] - vm);
+    }
+}
+
+// Bound of every hypothesis from the even blocks' errors: their sum, in any order (the margin of
+// k_gl_prune covers its rounding).
+__global__ __launch_bounds__(kTB) void k_gl_lb(const float* __restrict__ diff, int nh, int nrays,
+                                               float* __restrict__ lb) {
+    const int h = blockIdx.x * (kTB / 64) + (threadIdx.x >> 6);
+    if (h >= nh) return;
+    const int lane = threadIdx.x & 63;
+    const float* d = diff + (size_t)h * nrays;
+    float acc = 0.f;
+    for (int r = lane; r < nrays; r += 128) acc += d[r];
+    acc = wave_sum_down(acc);
+    if (lane == 0) lb[h] = acc;
+}
+
+// CAER of the listed hypotheses from their complete rows of errors, in k_gl_score's order (lane
+// l: rays l, l + 64, ...; then the shuffle-down tree): caer[h] for h = list[i], i < *nlist (or
+// < n); mark (non-null): mark[h] = -1.
+__global__ __launch_bounds__(kTB) void k_gl_caer_rows(const float* __restrict__ diff, const int32_t* __restrict__ list,
+                                                      const int* __restrict__ nlist, int n, int nrays,
+                                                      float* __restrict__ caer, float* __restrict__ mark) {
+    const int cnt = nlist ? *nlist : n;
+    const int lane = threadIdx.x & 63;
+    for (int i = blockIdx.x * (kTB / 64) + (threadIdx.x >> 6); i < cnt; i += gridDim.x * (kTB / 64)) {
+        const int h = list[i];
+        const float* d = diff + (size_t)h * nrays;
+        float acc = 0.f;
+        for (int r = lane; r < nrays; r += 64) acc += d[r];
+        acc = wave_sum_down(acc);
+        if (lane == 0) {
+            caer[h] = acc;
+            if (mark) mark[h] = -1.f;
+        }
+    }
+}
+
+// Poses of the listed hypotheses, packed: out[3 i ..] = hyp[3 idx[i] ..] for i < n.
+__global__ __launch_bounds__(kTB) void k_gl_pick(const float* __restrict__ hyp, const int32_t* __restrict__ idx, int n,
+                                                 float* __restrict__ out) {
+    const int i = blockIdx.x * kTB + threadIdx.x;
+    if (i >= n) return;
+    const int h = idx[i];
+    out[3 * i] = hyp[3 * h];
+    out[3 * i + 1] = hyp[3 * h + 1];
+    out[3 * i + 2] = hyp[3 * h + 2];
 }
 
 // ---------------------------------------------------------------- bottom-k selection
@@ -316,6 +461,38 @@ __device__ inline uint64_t comp_key(float c, int i) {
     uint32_t b = __float_as_uint(c);
     b = (b & 0x80000000u) ? ~b : (b | 0x80000000u);
     return ((uint64_t)b << 32) | (uint32_t)i;
+}
+
+// Pruned scoring, threshold: the CAER of the k-th smallest of the m fully scored candidates
+// cand[0 .. m) (k <= m <= kSortMax), in the (CAER bits, index) order of the bottom-k (LDS bitonic
+// sort); also resets the survivor count for k_gl_prune.
+__global__ __launch_bounds__(1024) void k_gl_tau(const float* __restrict__ caer, const int32_t* __restrict__ cand,
+                                                 int m, int k, float* __restrict__ tau, int* __restrict__ nsurv) {
+    __shared__ uint64_t key[4096];
+    const int t = threadIdx.x;
+    int p2 = 1;
+    while (p2 < m) p2 <<= 1;
+    for (int j = t; j < p2; j += 1024) key[j] = j < m ? comp_key(caer[cand[j]], cand[j]) : ~0ull;
+    __syncthreads();
+    for (int size = 2; size <= p2; size <<= 1)
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int j = t; j < p2; j += 1024) {
+                const int q = j ^ stride;
+                if (q > j) {
+                    const uint64_t a = key[j], b = key[q];
+                    const bool up = (j & size) == 0;
+                    if ((a > b) == up) {
+                        key[j] = b;
+                        key[q] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    if (t == 0) {
+        *tau = caer[(uint32_t)key[k - 1]];
+        *nsurv = 0;
+    }
 }
 
 // tk[0] prefix, tk[1] threshold, tk[2] remaining, tk[3] done, tk[4] collect counter
@@ -574,6 +751,7 @@ int score_launch(GlState* s, int nrays, float rmax) {
     }
     prof_e(s);
     GL_LAUNCH_CHECK();
+    s->caer_state = 0;
     return 0;
 }
 
@@ -727,25 +905,25 @@ __global__ __launch_bounds__(1024) void k_topk_small(const float* __restrict__ c
     }
 }
 
-int topk_launch(GlState* s, int k) {
-    const int n = s->nh;
+// Bottom-k of keys[0 .. n) (CAER values, or pruning bounds) by (value bits, index): the winners'
+// poses, values and indices into d_out, d_outcaer, d_outidx.
+int topk_select(GlState* s, const float* keys, int n, int k) {
     int blocks = (n + kTB - 1) / kTB;
     if (blocks > 1024) blocks = 1024;
-    prof_b(s, PFGLX_STAGE_TOPK);
     if (n <= kTopkSmallN && k <= kTopkSmallK) {
-        hipLaunchKernelGGL(k_topk_small, dim3(1), dim3(1024), 0, s->stream, s->d_caer, n, k, s->d_hyp, s->d_out,
+        hipLaunchKernelGGL(k_topk_small, dim3(1), dim3(1024), 0, s->stream, keys, n, k, s->d_hyp, s->d_out,
                            s->d_outcaer, s->d_outidx);
     } else if (k <= kSortMax) {
         hipLaunchKernelGGL(k_tk_init, dim3(1), dim3(256), 0, s->stream, s->d_tk, s->d_hist, k);
         for (int pass = 0; pass < 8; ++pass) {
             const int shift = 56 - 8 * pass;
-            hipLaunchKernelGGL(k_tk_hist, dim3(blocks), dim3(kTB), 0, s->stream, s->d_caer, n, s->d_tk, s->d_hist,
+            hipLaunchKernelGGL(k_tk_hist, dim3(blocks), dim3(kTB), 0, s->stream, keys, n, s->d_tk, s->d_hist,
                                shift);
             hipLaunchKernelGGL(k_tk_pick, dim3(1), dim3(256), 0, s->stream, s->d_tk, s->d_hist, shift);
         }
-        hipLaunchKernelGGL(k_tk_collect, dim3(blocks), dim3(kTB), 0, s->stream, s->d_caer, n, s->d_tk, s->d_win, k);
+        hipLaunchKernelGGL(k_tk_collect, dim3(blocks), dim3(kTB), 0, s->stream, keys, n, s->d_tk, s->d_win, k);
         hipLaunchKernelGGL(k_tk_sort, dim3(1), dim3(1024), 0, s->stream, s->d_win, s->d_tk, k, n, s->d_hyp,
-                           s->d_caer, s->d_out, s->d_outcaer, s->d_outidx);
+                           keys, s->d_out, s->d_outcaer, s->d_outidx);
     } else {
         int m = 1;
         while (m < n) m <<= 1;
@@ -756,17 +934,173 @@ int topk_launch(GlState* s, int k) {
             s->sort_cap = m;
         }
         const int gb = (m + kTB - 1) / kTB;
-        hipLaunchKernelGGL(k_fill_keys, dim3(gb), dim3(kTB), 0, s->stream, s->d_caer, n, m, s->d_sortbuf);
+        hipLaunchKernelGGL(k_fill_keys, dim3(gb), dim3(kTB), 0, s->stream, keys, n, m, s->d_sortbuf);
         for (int size = 2; size <= m; size <<= 1)
             for (int stride = size >> 1; stride > 0; stride >>= 1)
                 hipLaunchKernelGGL(k_bitonic_step, dim3(gb), dim3(kTB), 0, s->stream, s->d_sortbuf, m, size, stride);
         hipLaunchKernelGGL(k_gather_sorted, dim3((k + kTB - 1) / kTB), dim3(kTB), 0, s->stream, s->d_sortbuf, k, n,
-                           s->d_hyp, s->d_caer, s->d_out, s->d_outcaer, s->d_outidx);
+                           s->d_hyp, keys, s->d_out, s->d_outcaer, s->d_outidx);
     }
-    prof_e(s);
     GL_LAUNCH_CHECK();
+    return 0;
+}
+
+int topk_launch(GlState* s, int k) {
+    prof_b(s, PFGLX_STAGE_TOPK);
+    if (topk_select(s, s->d_caer, s->nh, k) != 0) return -1;
+    prof_e(s);
     s->last_k = k;
     return 0;
+}
+
+// ---------------------------------------------------------------- pruned scoring
+
+// Pruning applies, in the automatic kernel choice, to the scoring the large-batch byte-grid
+// kernel would run (route 1: the scan and ray tables in LDS) and to the one-lane-per-ray route of
+// smaller batches with the byte grid in LDS (route 2, from kPruneRaysMinHyp hypotheses; its
+// per-ray errors are kept, so the survivors cast only their remaining rays; below that its
+// extra launches cost more than the rays it saves: 10k hypotheses took 0.43 against 0.35 ms).
+// 0: not eligible.
+constexpr int kPruneRaysMinHyp = 32768;
+int prune_route(GlState* s, int nrays) {
+    if (!s->prune || s->score_mode != 0 || nrays > kLdsRaysMax || nrays < 128) return 0;
+    const size_t ray_lds = (size_t)nrays * (8 + 8 + 4);
+    if (s->nh >= kScoreBlockMaxHyp) return (size_t)s->g.nbytes + ray_lds <= kLdsScoreBytesMax ? 1 : 0;
+    const size_t total = (size_t)s->nh * (size_t)nrays;
+    if (s->nh < kPruneRaysMinHyp || (size_t)s->g.nbytes > kLdsScoreBytesMax || total > kRayDiffsMax ||
+        ray_diffs_reserve(s, total) != 0)
+        return 0;
+    return 2;
+}
+
+template <int kSel>
+int rays_sel_launch(GlState* s, int nrays, float rmax, const int32_t* list, const int* nlist, int count) {
+    Grid g = s->g;
+    g.bits = s->d_bits;
+    g.bytes = s->d_bytes;
+    const size_t per = (size_t)(((nrays + 127) >> 7) << 6);
+    const int blocks = (int)std::max<size_t>(1, std::min<size_t>(((size_t)count * per + 1023) / 1024, 2048));
+    hipLaunchKernelGGL((k_gl_rays_sel<kSel>), dim3(blocks), dim3(1024), (size_t)g.nbytes, s->stream, g, s->d_hyp,
+                       count, list, nlist, s->d_scan, s->d_cphi, s->d_sphi, nrays, rmax, s->d_diff);
+    GL_LAUNCH_CHECK();
+    return 0;
+}
+
+int prune_buffers(GlState* s) {
+    if (s->d_lb) return 0;
+    GL_CHECK(hipMalloc((void**)&s->d_lb, sizeof(float) * (size_t)s->cap_hyp));
+    GL_CHECK(hipMalloc((void**)&s->d_surv, sizeof(int32_t) * (size_t)s->cap_hyp));
+    GL_CHECK(hipMalloc((void**)&s->d_nsurv, sizeof(int)));
+    GL_CHECK(hipMalloc((void**)&s->d_tau, sizeof(float) * 2));
+    GL_CHECK(hipMalloc((void**)&s->d_cand, sizeof(float) * 3 * (size_t)kSortMax));
+    GL_CHECK(hipMalloc((void**)&s->d_cdiff, sizeof(float) * (size_t)kSortMax * (size_t)kLdsRaysMax));
+    return 0;
+}
+
+// The byte-grid scoring kernel over all hypotheses (kSub = 2: bounds into out) or over a list.
+// count: the hypotheses (or list entries) to score when nlist is null; else the grid's upper
+// bound of the device-side count *nlist.
+template <int kSub>
+int score_bytes_launch(GlState* s, int nrays, float rmax, float* out, const int32_t* list, const int* nlist, int count,
+                       float* mark) {
+    Grid g = s->g;
+    g.bits = s->d_bits;
+    g.bytes = s->d_bytes;
+    const size_t shm = (size_t)g.nbytes + (size_t)nrays * (8 + 8 + 4);
+    int blocks = (count + 15) / 16;
+    blocks = std::max(1, std::min(blocks, 1024));
+    hipLaunchKernelGGL((k_gl_score<1024, true, true, true, kSub>), dim3(blocks), dim3(1024), shm, s->stream, g,
+                       s->d_hyp, count, s->d_scan, s->d_cphi, s->d_sphi, nrays, rmax, out, list, nlist, mark);
+    GL_LAUNCH_CHECK();
+    return 0;
+}
+
+// First half of a pruned scoring: the bound of every hypothesis.
+int bounds_launch(GlState* s, int nrays, float rmax, int route) {
+    if (prune_buffers(s) != 0) return -1;
+    prof_b(s, PFGLX_STAGE_SCORE);
+    if (route == 1) {
+        if (score_bytes_launch<2>(s, nrays, rmax, s->d_lb, nullptr, nullptr, s->nh, nullptr) != 0) return -1;
+    } else {
+        if (rays_sel_launch<1>(s, nrays, rmax, nullptr, nullptr, s->nh) != 0) return -1;
+        hipLaunchKernelGGL(k_gl_lb, dim3((s->nh + 3) / 4), dim3(kTB), 0, s->stream, s->d_diff, s->nh, nrays, s->d_lb);
+        GL_LAUNCH_CHECK();
+    }
+    prof_e(s);
+    s->caer_state = 1;
+    s->pend_route = route;
+    s->pend_nrays = nrays;
+    s->pend_rmax = rmax;
+    return 0;
+}
+
+// Second half, for a bottom-k: the m = 4k hypotheses with the smallest bounds are scored in
+// full, the k-th smallest of their CAERs is the threshold tau, and of the rest only those whose
+// bound does not exceed tau are scored; the others cannot be among the k smallest and get +inf.
+// Every hypothesis that can be in the bottom k then holds its exact CAER, so the bottom k (ties
+// by index included) is the one full scoring gives.
+int finish_pruned(GlState* s, int k) {
+    const int n = s->nh, nrays = s->pend_nrays;
+    const float rmax = s->pend_rmax;
+    const int m = std::min(std::max(4 * k, 64), kSortMax);
+    if (m * 2 > n || k > m) return score_launch(s, nrays, rmax);  // nothing to gain: score everything
+    prof_b(s, PFGLX_STAGE_SCORE);
+    if (topk_select(s, s->d_lb, n, m) != 0) return -1;  // candidates -> d_outidx[0 .. m)
+    if (s->pend_route == 2) {
+        // the per-ray errors of the even blocks are kept: the candidates, then the survivors,
+        // cast only the odd blocks' rays, and each complete row is summed in the standard order
+        if (rays_sel_launch<2>(s, nrays, rmax, s->d_outidx, nullptr, m) != 0) return -1;
+        hipLaunchKernelGGL(k_gl_caer_rows, dim3((m + 3) / 4), dim3(kTB), 0, s->stream, s->d_diff, s->d_outidx,
+                           nullptr, m, nrays, s->d_caer, s->d_lb);
+        hipLaunchKernelGGL(k_gl_tau, dim3(1), dim3(1024), 0, s->stream, s->d_caer, s->d_outidx, m, k, s->d_tau,
+                           s->d_nsurv);
+        hipLaunchKernelGGL(k_gl_prune, dim3((n + kTB - 1) / kTB), dim3(kTB), 0, s->stream, s->d_lb, n, s->d_tau,
+                           s->d_caer, s->d_surv, s->d_nsurv);
+        GL_LAUNCH_CHECK();
+        if (rays_sel_launch<2>(s, nrays, rmax, s->d_surv, s->d_nsurv, n) != 0) return -1;
+        hipLaunchKernelGGL(k_gl_caer_rows, dim3(std::max(1, std::min((n + 3) / 4, 4096))), dim3(kTB), 0, s->stream,
+                           s->d_diff, s->d_surv, s->d_nsurv, n, nrays, s->d_caer, nullptr);
+        GL_LAUNCH_CHECK();
+        prof_e(s);
+        s->caer_state = 2;
+        s->pruned_k = k;
+        return 0;
+    }
+    // the candidates one lane per ray (few hypotheses: one wave each would leave most of the
+    // chip idle), their errors summed in k_gl_score's order (the same CAER bits)
+    hipLaunchKernelGGL(k_gl_pick, dim3((m + kTB - 1) / kTB), dim3(kTB), 0, s->stream, s->d_hyp, s->d_outidx, m,
+                       s->d_cand);
+    GL_LAUNCH_CHECK();
+    Grid g = s->g;
+    g.bits = s->d_bits;
+    g.bytes = s->d_bytes;
+    if (rays_launch(s, g, s->d_cand, m, s->d_scan, nrays, rmax, s->d_cdiff, s->stream) != 0) return -1;
+    hipLaunchKernelGGL(k_gl_caer_sum, dim3((m + 3) / 4), dim3(kTB), 0, s->stream, s->d_cdiff, m, nrays, s->d_caer,
+                       s->d_outidx, s->d_lb);
+    hipLaunchKernelGGL(k_gl_tau, dim3(1), dim3(1024), 0, s->stream, s->d_caer, s->d_outidx, m, k, s->d_tau,
+                       s->d_nsurv);
+    hipLaunchKernelGGL(k_gl_prune, dim3((n + kTB - 1) / kTB), dim3(kTB), 0, s->stream, s->d_lb, n, s->d_tau,
+                       s->d_caer, s->d_surv, s->d_nsurv);
+    GL_LAUNCH_CHECK();
+    if (score_bytes_launch<1>(s, nrays, rmax, s->d_caer, s->d_surv, s->d_nsurv, n, nullptr) != 0) return -1;
+    prof_e(s);
+    s->caer_state = 2;
+    s->pruned_k = k;
+    return 0;
+}
+
+// Before a bottom-k of k: finish a pending pruned scoring, or score everything again when the
+// pruned one was for a smaller k.
+int caer_for_topk(GlState* s, int k) {
+    if (s->caer_state == 1) return finish_pruned(s, k);
+    if (s->caer_state == 2 && k > s->pruned_k) return score_launch(s, s->pend_nrays, s->pend_rmax);
+    return 0;
+}
+
+// Before anything reads every CAER: a pruned scoring is completed in full.
+int caer_all(GlState* s) {
+    if (s->caer_state == 0) return 0;
+    return score_launch(s, s->pend_nrays, s->pend_rmax);
 }
 
 // Local refinement: `per` children per parent (the last top-k rows); child 0 is the parent
@@ -929,7 +1263,8 @@ int pfgl_destroy(void* h) {
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     void* ptrs[] = {s->d_bits, s->d_bytes, s->d_free,    s->d_hyp,      s->d_caer,   s->d_scan, s->d_cphi,
                     s->d_sphi, s->d_tk,      s->d_hist,     s->d_win,    s->d_sortbuf, s->d_out,
-                    s->d_outcaer, s->d_outidx, s->d_poses, s->d_ranges, s->d_diff};
+                    s->d_outcaer, s->d_outidx, s->d_poses, s->d_ranges, s->d_diff, s->d_lb, s->d_surv,
+                    s->d_nsurv, s->d_tau, s->d_cand, s->d_cdiff};
     for (void* p : ptrs) (void)hipFree(p);
     if (s->h_scan) (void)hipHostFree(s->h_scan);
     if (s->scan_ev) (void)hipEventDestroy(s->scan_ev);
@@ -952,6 +1287,7 @@ int pfgl_generate(void* h, int n, unsigned long long seed) {
     if (!s || s->nfree <= 0) return -1;
     if (n <= 0 || n > s->cap_hyp) return -1;
     s->nh = n;
+    s->caer_state = 0;
     prof_b(s, PFGLX_STAGE_GENERATE);
     hipLaunchKernelGGL(k_gl_gen, dim3((n + kTB - 1) / kTB), dim3(kTB), 0, s->stream, s->d_free, s->nfree, s->g.res,
                        s->g.ox, s->g.oy, (uint64_t)seed, n, s->d_hyp);
@@ -964,7 +1300,11 @@ int pfgl_score(void* h, const float* real_scan, int nrays, float rmax, float ang
     GlState* s = static_cast<GlState*>(h);
     if (!s || s->nh <= 0 || !real_scan || nrays <= 0) return -1;
     if (upload_scan(s, real_scan, nrays, ang_span) != 0) return -1;
-    if (score_launch(s, nrays, rmax) != 0) return -1;
+    if (const int route = prune_route(s, nrays)) {  // bounds now, the rest with the bottom-k
+        if (bounds_launch(s, nrays, rmax, route) != 0) return -1;
+    } else if (score_launch(s, nrays, rmax) != 0) {
+        return -1;
+    }
     GL_CHECK(hipStreamSynchronize(s->stream));
     return 0;
 }
@@ -972,12 +1312,14 @@ int pfgl_score(void* h, const float* real_scan, int nrays, float rmax, float ang
 int pfglx_set_scan(void* h, const float* scan, int nrays, float ang_span) {
     GlState* s = static_cast<GlState*>(h);
     if (!s || !scan || nrays <= 0) return -1;
+    s->caer_state = 0;  // (a pending pruned scoring was for the previous scan)
     return upload_scan(s, scan, nrays, ang_span);
 }
 
 int pfglx_score_async(void* h, int nrays, float rmax) {
     GlState* s = static_cast<GlState*>(h);
     if (!s || s->nh <= 0 || nrays <= 0 || nrays != s->scan_nrays) return -1;
+    if (const int route = prune_route(s, nrays)) return bounds_launch(s, nrays, rmax, route);
     return score_launch(s, nrays, rmax);
 }
 
@@ -985,7 +1327,7 @@ int pfglx_topk_ex(void* h, int k, float* out3k, float* caer_k, int32_t* idx_k) {
     GlState* s = static_cast<GlState*>(h);
     if (!s || k <= 0 || s->nh <= 0) return -1;
     if (k > s->nh) k = s->nh;
-    if (topk_launch(s, k) != 0) return -1;
+    if (caer_for_topk(s, k) != 0 || topk_launch(s, k) != 0) return -1;
     if (out3k) GL_CHECK(hipMemcpyAsync(out3k, s->d_out, sizeof(float) * 3 * (size_t)k, hipMemcpyDeviceToHost, s->stream));
     if (caer_k)
         GL_CHECK(hipMemcpyAsync(caer_k, s->d_outcaer, sizeof(float) * (size_t)k, hipMemcpyDeviceToHost, s->stream));
@@ -1006,6 +1348,7 @@ int pfglx_set_hypotheses(void* h, const float* hyp3n, int n) {
     GL_CHECK(hipMemcpyAsync(s->d_hyp, hyp3n, sizeof(float) * 3 * (size_t)n, hipMemcpyHostToDevice, s->stream));
     GL_CHECK(hipStreamSynchronize(s->stream));
     s->nh = n;
+    s->caer_state = 0;
     return 0;
 }
 
@@ -1020,6 +1363,7 @@ int pfglx_get_hypotheses(void* h, float* hyp3n) {
 int pfglx_get_caer(void* h, float* caer) {
     GlState* s = static_cast<GlState*>(h);
     if (!s || !caer || s->nh <= 0) return -1;
+    if (caer_all(s) != 0) return -1;
     GL_CHECK(hipMemcpyAsync(caer, s->d_caer, sizeof(float) * (size_t)s->nh, hipMemcpyDeviceToHost, s->stream));
     GL_CHECK(hipStreamSynchronize(s->stream));
     return s->nh;
@@ -1092,6 +1436,7 @@ int pfglx_generate_range(void* h, int offset, int n, uint64_t seed) {
         GL_LAUNCH_CHECK();
     }
     s->nh = offset + n;
+    s->caer_state = 0;
     return 0;
 }
 
@@ -1099,7 +1444,7 @@ int pfglx_topk_device(void* h, int k) {
     GlState* s = static_cast<GlState*>(h);
     if (!s || k <= 0 || s->nh <= 0) return -1;
     if (k > s->nh) k = s->nh;
-    if (topk_launch(s, k) != 0) return -1;
+    if (caer_for_topk(s, k) != 0 || topk_launch(s, k) != 0) return -1;
     return k;
 }
 
@@ -1114,6 +1459,7 @@ int pfglx_refine(void* h, int k_parents, int per_parent, float sigma_xy, float s
                        per_parent, sigma_xy, sigma_yaw, seed, x0, y0, x1, y1, s->d_hyp);
     GL_LAUNCH_CHECK();
     s->nh = n;
+    s->caer_state = 0;
     return 0;
 }
 
@@ -1138,12 +1484,20 @@ int pfglx_set_count(void* h, int n) {
     GlState* s = static_cast<GlState*>(h);
     if (!s || n < 0 || n > s->cap_hyp) return -1;
     s->nh = n;
+    s->caer_state = 0;
     return 0;
 }
 
 void* pfglx_stream(void* h) {
     GlState* s = static_cast<GlState*>(h);
     return s ? (void*)s->stream : nullptr;
+}
+
+int pfglx_set_prune(void* h, int enable) {
+    GlState* s = static_cast<GlState*>(h);
+    if (!s) return -1;
+    s->prune = enable != 0;
+    return 0;
 }
 
 int pfglx_set_score_mode(void* h, int mode) {

@@ -41,6 +41,14 @@ int pfglx_render_device(void *h, const float *d_poses3n, int n, int nrays, float
  * stride the rays), 2 one lane per ray with the per-ray errors summed in a second pass.  All
  * give the same CAER bits. */
 int pfglx_set_score_mode(void *h, int mode);
+/* Pruned scoring (default on, automatic kernel choice, map in LDS, at least 32,768 hypotheses
+ * and 128 rays): pfgl_score and pfglx_score_async compute a lower bound of every CAER (the sum
+ * over the rays of every second block of 64) and the next bottom-k scores in full only the
+ * max(4k, 64) hypotheses with the smallest bounds and those whose bound does not exceed the
+ * k-th smallest CAER among them.  The bottom-k, ties included, is the one full scoring gives;
+ * pfglx_get_caer, or a bottom-k with a larger k, completes the scoring of every hypothesis
+ * first.  0 disables it. */
+int pfglx_set_prune(void *h, int enable);
 int pfglx_profile_enable(void *h, int enable);
 int pfglx_profile_read(void *h, double *ms, int *count);
 int pfglx_sync(void *h);

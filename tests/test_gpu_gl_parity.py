@@ -204,3 +204,47 @@ def test_caer_vs_reference_f64_arithmetic(gpu, map_seed):
         for j, h in enumerate(bad[:32]):
             r64 = og.map_scan64(*hyp[h].astype(np.float64), 360, TAU, 8.0)
             assert np.count_nonzero(np.abs(r32[j] - r64) > 1e-4) <= 2, f"hypothesis {h}"
+
+
+@pytest.mark.parametrize("map_seed,n", [(3, 200_000), (4, 200_000), (100, 200_000), (3, 50_000), (100, 3_000)])
+def test_pruned_bottom_k_equals_full_scoring(gpu, map_seed, n):
+    """Pruned scoring (pfgl_score leaves every-second-ray bounds, the bottom-k finishes only the
+    hypotheses that can still be among the k smallest) gives the bottom-k of full scoring: the
+    same indices in the same order and the same CAER bits, for several k, with exact CAER ties
+    at the boundary (duplicated hypotheses), on the scan of a true pose and on a scan no
+    hypothesis fits well.  Reading every CAER afterwards completes the scoring: bit-identical
+    to full scoring."""
+    cells, res, org = obstacle_map(map_seed)
+    og = OracleGrid(cells, res, org)
+    truth = free_poses(cells, res, org, 1, seed=map_seed, margin_cells=6)[0]
+    rng = np.random.default_rng(map_seed)
+    scans = [(og.map_scan64(*truth, 360, TAU, 8.0) + rng.normal(size=360) * 0.02).astype(np.float32),
+             rng.uniform(0.2, 8.0, size=360).astype(np.float32)]
+    full = GlKernels(cells, res, org[0], org[1], n)
+    full.set_prune(False)
+    pruned = GlKernels(cells, res, org[0], org[1], n)
+    full.generate(n, 77)
+    hyp = full.hypotheses()
+    nd = n // 60
+    hyp[n // 2: n // 2 + nd] = hyp[:nd]  # exact CAER ties, near the bottom or not
+    for scan in scans:
+        full.set_hypotheses(hyp)
+        full.score(scan, 8.0, TAU)
+        ref_caer = full.caer()
+        for k in (1, 10, 64, min(1000, n // 8)):
+            _, c_ref, i_ref = full.topk_ex(k)
+            pruned.set_hypotheses(hyp)
+            pruned.score(scan, 8.0, TAU)
+            _, c_p, i_p = pruned.topk_ex(k)
+            assert np.array_equal(i_p, i_ref), (map_seed, k)
+            assert np.array_equal(c_p.view(np.uint32), c_ref.view(np.uint32)), (map_seed, k)
+            assert np.array_equal(i_ref, bottom_k(ref_caer, k))
+        # a larger k after a pruned bottom-k, then every CAER
+        _, _, i_big = pruned.topk_ex(n // 4)
+        assert np.array_equal(i_big, bottom_k(ref_caer, n // 4))
+        assert np.array_equal(pruned.caer().view(np.uint32), ref_caer.view(np.uint32))
+        # the ties really are at stake: duplicates among the reference bottom n / 8
+        dup = set(range(nd)) | set(range(n // 2, n // 2 + nd))
+        assert len(dup & set(bottom_k(ref_caer, n // 8).tolist())) > 0
+    full.close()
+    pruned.close()
