@@ -162,9 +162,25 @@ def run_gl(n_hyp, n_poses, seed=3):
     msv = (C.c_double * 3)()
     cnt = (C.c_int * 3)()
     L.pfglx_profile_read(kh, msv, cnt)
+    st = (C.c_int * 3)()
+    L.pfglx_prune_stats(kh, st)
     gl.close()
+    # rays actually cast by the profiled localisation: bounds over the rays of every second
+    # 64-ray block, then the candidates and survivors in full (one wave per hypothesis) or only
+    # their remaining rays (one lane per ray)
+    nr = 360
+    sel = sum(1 for r in range(nr) if (r >> 6) % 2 == 0)
+    m, surv, route = st[0], st[1], st[2]
+    if route == 0:
+        cast = n_hyp * nr
+    else:
+        cast = n_hyp * sel + (m + surv) * (nr if route == 1 else nr - sel)
     return {"ms_per_localize": ms, "success": f"{ok}/{len(truths)}", "hypotheses": n_hyp,
-            "hyp_per_s": n_hyp / (ms / 1000.0), "rays_per_s": n_hyp * 360 / (msv[1] / 1000.0) if msv[1] else None,
+            "hyp_per_s": n_hyp / (ms / 1000.0),
+            "hypothesis_rays_per_s": n_hyp * nr / (msv[1] / 1000.0) if msv[1] else None,
+            "rays_cast_per_localize": cast, "rays_cast_per_s": cast / (msv[1] / 1000.0) if msv[1] else None,
+            "pruning": {"candidates": m, "survivors": surv, "survivor_fraction": round(surv / n_hyp, 4),
+                        "route": route},
             "stage_ms": {"generate": msv[0], "score": msv[1], "topk": msv[2]}}
 
 
@@ -622,10 +638,11 @@ def main():
         line["learned"] = run_learned()
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(sc, n, args.cpu_budget)
-        if "gl" in line and line["gl"].get("rays_per_s"):
+        if "gl" in line and line["gl"].get("rays_cast_per_s"):
             spr = line["cpu_baseline"]["legs"]["gl_200k_hypotheses_all_threads"]["dda_cell_steps_per_ray"]
             line["gl"]["dda_cell_steps_per_ray_algorithmic"] = spr
-            line["gl"]["cell_steps_per_s"] = round(line["gl"]["rays_per_s"] * spr, 1)
+            # (the mean steps per ray of all hypotheses applied to the rays actually cast)
+            line["gl"]["cell_steps_per_s_est"] = round(line["gl"]["rays_cast_per_s"] * spr, 1)
     print(json.dumps(line), flush=True)
 
 

@@ -126,6 +126,7 @@ KERNEL_SYMBOLS = [
     ("pfglx_render_device", C.c_int, [vp, vp, C.c_int, C.c_int, f32, f32, vp, vp]),
     ("pfglx_set_score_mode", C.c_int, [vp, C.c_int]),
     ("pfglx_set_prune", C.c_int, [vp, C.c_int]),
+    ("pfglx_prune_stats", C.c_int, [vp, P(C.c_int)]),
     ("pfglx_profile_enable", C.c_int, [vp, C.c_int]),
     ("pfglx_profile_read", C.c_int, [vp, P(f64), P(C.c_int)]),
     ("pfglx_sync", C.c_int, [vp]),

@@ -120,6 +120,7 @@ struct GlState {
     int32_t* d_surv = nullptr;
     int* d_nsurv = nullptr;  // [0] survivor count
     float* d_tau = nullptr;
+    int last_m = 0, last_route = 0;  // candidates and route of the last pruned bottom-k (0: none)
     float* d_cand = nullptr;   // packed poses of the candidates
     float* d_cdiff = nullptr;  // their per-ray errors (kSortMax x kLdsRaysMax)
     // profiling
@@ -752,6 +753,7 @@ int score_launch(GlState* s, int nrays, float rmax) {
     prof_e(s);
     GL_LAUNCH_CHECK();
     s->caer_state = 0;
+    s->last_m = 0;
     return 0;
 }
 
@@ -1064,6 +1066,8 @@ int finish_pruned(GlState* s, int k) {
         prof_e(s);
         s->caer_state = 2;
         s->pruned_k = k;
+        s->last_m = m;
+        s->last_route = 2;
         return 0;
     }
     // the candidates one lane per ray (few hypotheses: one wave each would leave most of the
@@ -1086,6 +1090,8 @@ int finish_pruned(GlState* s, int k) {
     prof_e(s);
     s->caer_state = 2;
     s->pruned_k = k;
+    s->last_m = m;
+    s->last_route = 1;
     return 0;
 }
 
@@ -1491,6 +1497,19 @@ int pfglx_set_count(void* h, int n) {
 void* pfglx_stream(void* h) {
     GlState* s = static_cast<GlState*>(h);
     return s ? (void*)s->stream : nullptr;
+}
+
+int pfglx_prune_stats(void* h, int* out3) {
+    GlState* s = static_cast<GlState*>(h);
+    if (!s || !out3) return -1;
+    out3[0] = s->last_m;
+    out3[1] = 0;
+    out3[2] = s->last_route;
+    if (s->last_m > 0) {
+        GL_CHECK(hipMemcpyAsync(out3 + 1, s->d_nsurv, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+        GL_CHECK(hipStreamSynchronize(s->stream));
+    }
+    return 0;
 }
 
 int pfglx_set_prune(void* h, int enable) {

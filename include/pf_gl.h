@@ -49,6 +49,11 @@ int pfglx_set_score_mode(void *h, int mode);
  * pfglx_get_caer, or a bottom-k with a larger k, completes the scoring of every hypothesis
  * first.  0 disables it. */
 int pfglx_set_prune(void *h, int enable);
+/* Work of the last pruned bottom-k: out3 = {candidates scored in full, survivors scored after
+ * the threshold, route (1 one wave per hypothesis: candidates and survivors cast every ray
+ * again; 2 one lane per ray: they cast the rays the bounds left out)}; {0, 0, 0} after a full
+ * scoring. */
+int pfglx_prune_stats(void *h, int *out3);
 int pfglx_profile_enable(void *h, int enable);
 int pfglx_profile_read(void *h, double *ms, int *count);
 int pfglx_sync(void *h);
