@@ -317,7 +317,7 @@ def run_learned(maps=tuple(range(100, 110)), n_poses=20):
             "loss_first_last": losses}
 
 
-def run_pf_fused(n, sc, steps, warmup, steps_host):
+def run_pf_fused(n, sc, steps, warmup, steps_host, weight_ppt=0):
     """The 1M-particle cycle through the fused entry (one filter): every device stage (predict,
     weight, normalise + ESS predicate, resample, estimate sums, Markley mean, covariances) on the
     device.  Two timings:
@@ -342,7 +342,11 @@ def run_pf_fused(n, sc, steps, warmup, steps_host):
     bf.use_graph(True)
     # the likelihood launch shape from the library's autotuner (times the shapes on this filter
     # and restores the draw state it staged)
-    ppt = bf.autotune(inputs[0][0])
+    if weight_ppt:
+        bf.set_weight_ppt(weight_ppt)
+        ppt = weight_ppt
+    else:
+        ppt = bf.autotune(inputs[0][0])
     for i in range(warmup):
         bf.cycle(inputs[i][0])
     t0 = time.perf_counter()
@@ -527,6 +531,8 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--weight-ppt", type=int, default=0, choices=(0, 1, 2, 4),
+                    help="likelihood particles per lane of the headline cycle (0: the autotuner's choice)")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="only run a plain device-to-device copy of this many bytes (HBM counter calibration)")
     args = ap.parse_args()
@@ -543,7 +549,7 @@ def main():
     steps_prof = min(args.steps, 50)
     sc = config1_scenario(cycles=max(args.steps + args.warmup + steps_host + steps_prof, 50), seed=7)
     (ms_fused, ms_fused_dev, ms_host, err_fused, n_rs_fused, fstages, ms_fprof,
-     n_fprof, weight_ppt) = run_pf_fused(n, sc, args.steps, args.warmup, steps_host)
+     n_fprof, weight_ppt) = run_pf_fused(n, sc, args.steps, args.warmup, steps_host, args.weight_ppt)
     ms_fast, err, n_rs, stages = run_pf(n, sc, args.steps, args.warmup, fast=True, profile=False)
     ms_exact, _, _, _ = run_pf(n, sc, min(args.steps, 50), args.warmup, fast=False, profile=False)
     _, _, _, stages = run_pf(n, sc, min(args.steps, 50), args.warmup, fast=True, profile=True)
@@ -612,7 +618,7 @@ def main():
                      "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE of the same workload, copy-calibrated, "
                                        "read from profiles/pf_cycle_1M_particles_256_obs.txt (not measured in this "
                                        "run)",
-                     "traffic_algorithmic": 36 * n,
+                     "traffic_algorithmic": 68 * n,  # likelihood 36 B + the deferred covariance pass 32 B
                      "ms_avg": round(w["ms_avg"], 5),
                      "launch_includes": "the previous cycle's covariance pass (one extra workgroup per "
                                         "1024-particle tile, dispatched after the likelihood's): the achieved "

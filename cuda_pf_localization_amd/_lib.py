@@ -142,6 +142,7 @@ KERNEL_SYMBOLS = [
     ("pfb_staged_device_ms", C.c_int, [vp, P(f32)]),
     ("pfb_use_graph", C.c_int, [vp, C.c_int]),
     ("pfb_autotune", C.c_int, [vp, P(PfbCycleIn)]),
+    ("pfb_set_weight_ppt", C.c_int, [vp, C.c_int]),
     ("pfb_get_state", C.c_int, [vp, C.c_int, P(f32), P(f32), P(f32)]),
     ("pfb_get_ancestors", C.c_int, [vp, C.c_int, P(i32)]),
     ("pfb_get_cum", C.c_int, [vp, C.c_int, P(u64)]),

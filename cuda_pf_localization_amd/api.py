@@ -749,6 +749,9 @@ class BatchedFilters:
     def use_graph(self, on: bool = True):
         _check(self.L.pfb_use_graph(self.h, 1 if on else 0), "pfb_use_graph")
 
+    def set_weight_ppt(self, ppt: int):
+        _check(self.L.pfb_set_weight_ppt(self.h, ppt), "pfb_set_weight_ppt")
+
     def autotune(self, inputs) -> int:
         r = self.L.pfb_autotune(self.h, inputs)
         if r < 0:

@@ -502,6 +502,15 @@ int pfb_autotune(void* h, const pfb_cycle_in* in) {
     return best_ppt;
 }
 
+int pfb_set_weight_ppt(void* h, int ppt) {
+    Batch* b = static_cast<Batch*>(h);
+    if (!b || (ppt != 1 && ppt != 2 && ppt != 4)) return -1;
+    PB_CHECK(hipStreamSynchronize(b->s.stream));
+    b->s.weight_ppt = ppt;
+    drop_graphs(b);  // (a captured cycle holds the launch shape)
+    return 0;
+}
+
 int pfb_get_state(void* h, int f, float* pos, float* quat, float* w) {
     Batch* b = static_cast<Batch*>(h);
     if (!b || f < 0 || f >= b->nf) return -1;

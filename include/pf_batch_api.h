@@ -69,6 +69,9 @@ int pfb_use_graph(void *h, int enable);
 /* Times the tunable launch shapes on the current state and keeps the fastest;
  * returns the chosen particles-per-lane of the likelihood kernel (1, 2 or 4). */
 int pfb_autotune(void *h, const pfb_cycle_in *in);
+/* Sets the likelihood kernel's particles per lane (1, 2 or 4) instead of tuning it: a fixed
+ * shape for profiling runs.  Every shape gives bit-identical results. */
+int pfb_set_weight_ppt(void *h, int ppt);
 /* Particles of filter f (pos 3n, quat 4n, weights n; any may be NULL). */
 int pfb_get_state(void *h, int f, float *pos3n, float *quat4n, float *w);
 int pfb_get_ancestors(void *h, int f, int32_t *anc);
