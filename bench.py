@@ -268,11 +268,15 @@ def run_batched(nf=64, cycles=100, warmup=2, seed=11):
 def run_learned(maps=tuple(range(100, 110)), n_poses=20):
     """Learned proposal on a 10-map family: per map, train the MLP on 1M GPU-rendered scans
     (Adam 1e-3, batch 4096), then localise n_poses scans with the learned proposal (8k
-    mixture + 2k uniform, refinement, uniform fallback) and with 200k uniform hypotheses."""
+    mixture + 2k uniform, refinement, uniform fallback) and with 200k uniform hypotheses, both
+    with the library's pruned scoring (the default) and with every CAER scored, as the
+    reference's GL path does."""
     from cuda_pf_localization_amd import GlobalLoc, GlParams, LearnedParams, LearnedProposal
     from cuda_pf_localization_amd.sim import f64_scan, free_poses, obstacle_map
-    ok_l = ok_u = fb = 0
-    t_l = t_u = t_train = 0.0
+    from cuda_pf_localization_amd import _lib
+    L = _lib.kernels()
+    ok_l = ok_u = ok_f = fb = 0
+    t_l = t_u = t_f = t_train = 0.0
     losses = []
     params = None
     for ms in maps:
@@ -304,6 +308,13 @@ def run_learned(maps=tuple(range(100, 110)), n_poses=20):
             top = gl.localize(s)
             t_u += time.perf_counter() - a
             ok_u += math.hypot(top[0, 0] - t[0], top[0, 1] - t[1]) < 0.5
+            L.pfglx_set_prune(gl.kernels_handle(), 0)
+            gl.set_seed(ms * 1000 + i)
+            a = time.perf_counter()
+            top = gl.localize(s)
+            t_f += time.perf_counter() - a
+            ok_f += math.hypot(top[0, 0] - t[0], top[0, 1] - t[1]) < 0.5
+            L.pfglx_set_prune(gl.kernels_handle(), 1)
         net.close()
         gl.close()
     n = len(maps) * n_poses
@@ -312,6 +323,8 @@ def run_learned(maps=tuple(range(100, 110)), n_poses=20):
                         "rmax 8 m, noise 0.02 m",
             "learned_ms_per_fix": round(t_l * 1000.0 / n, 4), "uniform_ms_per_fix": round(t_u * 1000.0 / n, 4),
             "speedup": round(t_u / t_l, 2), "learned_success": f"{ok_l}/{n}", "uniform_success": f"{ok_u}/{n}",
+            "uniform_full_scoring_ms_per_fix": round(t_f * 1000.0 / n, 4),
+            "speedup_vs_full_scoring": round(t_f / t_l, 2), "uniform_full_scoring_success": f"{ok_f}/{n}",
             "fallbacks": int(fb), "train_s_per_map": round(t_train / len(maps), 3),
             "train_tflops_end_to_end": round(train_flop / t_train / 1e12, 2), "params": int(params),
             "loss_first_last": losses}

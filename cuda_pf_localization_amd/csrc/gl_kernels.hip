@@ -216,12 +216,69 @@ __global__ __launch_bounds__(kTB) void k_gl_gen(const int32_t* __restrict__ free
 
 // CAER scoring.  LDS layout: [grid][real scan f32][cos phi f64][sin phi f64]; the grid is
 // staged as bytes (kBytes, 1024-thread blocks, two per CU) or as bits (256-thread blocks).
+template <int kBT, bool kBytes, bool kLdsGrid, bool kLdsRays>
+__global__ __launch_bounds__(kBT) void k_gl_score(Grid g, const float* __restrict__ hyp, int nh,
+                                                  const float* __restrict__ scan, const double* __restrict__ cphi,
+                                                  const double* __restrict__ sphi, int nrays, float rmax,
+                                                  float* __restrict__ caer) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int gbytes = kLdsGrid ? (kBytes ? g.nbytes : 4 * ((g.nwords + 3) & ~3)) : 0;
+    double* scphi = reinterpret_cast<double*>(smem + gbytes);
+    double* ssphi = scphi + (kLdsRays ? nrays : 0);
+    float* sscan = reinterpret_cast<float*>(ssphi + (kLdsRays ? nrays : 0));
+    if (kLdsGrid) {
+        if (kBytes) {
+            const uint4* src = reinterpret_cast<const uint4*>(g.bytes);
+            uint4* dst = reinterpret_cast<uint4*>(smem);
+            for (int i = threadIdx.x; i < g.nbytes / 16; i += kBT) dst[i] = src[i];
+        } else {
+            uint32_t* sbits = reinterpret_cast<uint32_t*>(smem);
+            for (int i = threadIdx.x; i < g.nwords; i += kBT) sbits[i] = g.bits[i];
+        }
+    }
+    if (kLdsRays)
+        for (int i = threadIdx.x; i < nrays; i += kBT) {
+            scphi[i] = cphi[i];
+            ssphi[i] = sphi[i];
+            sscan[i] = scan[i];
+        }
+    __syncthreads();
+    const double* tc = kLdsRays ? scphi : cphi;
+    const double* ts = kLdsRays ? ssphi : sphi;
+    const float* sc = kLdsRays ? sscan : scan;
+    const int lane = threadIdx.x & 63;
+    const int waves_total = gridDim.x * (kBT / 64);
+    for (int h = blockIdx.x * (kBT / 64) + (threadIdx.x >> 6); h < nh; h += waves_total) {
+        const float x = hyp[3 * h], y = hyp[3 * h + 1];
+        double st, ct;
+        det_sincos((double)hyp[3 * h + 2], st, ct);
+        float acc = 0.f;
+        for (int r = lane; r < nrays; r += 64) {
+            const double cp = tc[r], sp = ts[r];
+            const float dx = (float)(ct * cp - st * sp);
+            const float dy = (float)(st * cp + ct * sp);
+            float vm;
+            if (kBytes)
+                vm = cast_ray(OccBytes{kLdsGrid ? smem : g.bytes}, g.W, g.H, g.res, g.ox, g.oy, x, y, dx, dy, rmax);
+            else
+                vm = cast_ray(OccBits{kLdsGrid ? reinterpret_cast<const uint32_t*>(smem) : g.bits}, g.W, g.H, g.res,
+                              g.ox, g.oy, x, y, dx, dy, rmax);
+            acc += fabsf(sc[r] - vm);
+        }
+        acc = wave_sum_down(acc);
+        if (lane == 0) caer[h] = acc;
+    }
+}
+
+// Scoring for pruned scoring (the large-batch byte-grid kernel above, with a hypothesis list and
+// a ray subset; a kernel of its own so that these options leave the plain scoring kernel's code
+// as it is: folded into it, they made that kernel 17 % slower with the same inner loop).
 // kSub = 2: every second round of rays only (lane l: rays l, l + 128, l + 256, ...: blocks of 64
 // consecutive rays, so a wave's lanes still walk neighbouring directions), a lower bound of the
 // CAER up to rounding, for pruned scoring.  list (non-null): score the hypotheses list[i] for i below
 // *nlist (nlist non-null) or nh; mark (non-null): set mark[h] = -1 for every hypothesis scored.
 template <int kBT, bool kBytes, bool kLdsGrid, bool kLdsRays, int kSub = 1>
-__global__ __launch_bounds__(kBT) void k_gl_score(Grid g, const float* __restrict__ hyp, int nh,
+__global__ __launch_bounds__(kBT) void k_gl_score_sel(Grid g, const float* __restrict__ hyp, int nh,
                                                   const float* __restrict__ scan, const double* __restrict__ cphi,
                                                   const double* __restrict__ sphi, int nrays, float rmax,
                                                   float* __restrict__ caer, const int32_t* __restrict__ list = nullptr,
@@ -1011,7 +1068,7 @@ int score_bytes_launch(GlState* s, int nrays, float rmax, float* out, const int3
     const size_t shm = (size_t)g.nbytes + (size_t)nrays * (8 + 8 + 4);
     int blocks = (count + 15) / 16;
     blocks = std::max(1, std::min(blocks, 1024));
-    hipLaunchKernelGGL((k_gl_score<1024, true, true, true, kSub>), dim3(blocks), dim3(1024), shm, s->stream, g,
+    hipLaunchKernelGGL((k_gl_score_sel<1024, true, true, true, kSub>), dim3(blocks), dim3(1024), shm, s->stream, g,
                        s->d_hyp, count, s->d_scan, s->d_cphi, s->d_sphi, nrays, rmax, out, list, nlist, mark);
     GL_LAUNCH_CHECK();
     return 0;
