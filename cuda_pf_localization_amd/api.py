@@ -699,7 +699,10 @@ class BatchedFilters:
         _check(self.L.pfb_reinit(self.h, _fp(p), _fp(ps), _fp(rs)), "pfb_reinit")
 
     def make_inputs(self, per_filter):
-        """per_filter: list of (t3, q4, tstd3, rstd3, obs ndarray OBS_DTYPE or None)."""
+        """per_filter: list of (t3, q4, tstd3, rstd3, obs ndarray OBS_DTYPE or None).
+
+        Returns (array, keep): keep holds the contiguous observation copies the array points
+        at; the array also holds them itself, so either one keeps the pointers valid."""
         arr = (_lib.PfbCycleIn * self.nf)()
         keep = []
         for f, (t, q, ts, rs, obs) in enumerate(per_filter):
@@ -715,6 +718,7 @@ class BatchedFilters:
             else:
                 arr[f].obs = None
                 arr[f].nobs = 0
+        arr._keep = keep  # the array's obs pointers stay valid as long as the array lives
         return arr, keep
 
     def cycle(self, inputs) -> C.Array:

@@ -388,6 +388,27 @@ def test_resample_after_weight_scans_from_aggregates(gpu, n, ratio):
     b.close()
 
 
+def test_resample_after_empty_weight_scans_from_aggregates(gpu):
+    """pfc_weight with no observations resets the weights to uniform through the normalisation
+    (which leaves the aggregates); the resample that follows starts from them and gives the
+    look-back route's ancestors and LCG advance."""
+    n = 70_001
+    a, _ = pair(n, 8)
+    a.reinit(f32([1.0, 2.0, 3.0, 1.0, 0.0, 0.0, 0.0]), f32([0.5] * 3), f32([0.1] * 3))
+    a.weight(np.zeros(0, OBS_DTYPE), 0.05, 0.02, 3.0)
+    pos, quat, w = a.get_state()
+    lcg = a.get_lcg()
+    a.resample(0.0)
+    b = PfKernels(256)
+    b.init(n, 8)
+    b.set_state(pos, quat, w)
+    b.set_lcg(lcg)
+    b.resample(0.0)
+    assert np.array_equal(a.ancestors(), b.ancestors())
+    assert a.get_lcg() == b.get_lcg()
+    b.close()
+
+
 def test_resample_bitexact_more_tiles_than_resident(gpu):
     """4.2M particles: more scan tiles than the device holds at once (later tiles start while
     earlier ones are still finishing); ancestors still bit-exact, over two launches."""
