@@ -148,6 +148,7 @@ struct PfState {
     // reduction partials / scan state
     double *part_w = nullptr, *part_n = nullptr, *part_est = nullptr, *part_cov = nullptr;
     uint64_t* part_q = nullptr;                // fixed-point weight aggregate of every 1024-particle tile
+    uint64_t* tcum = nullptr;                  // cumulative fixed-point weight at every tile's last particle
     uint64_t* scan_st = nullptr;               // look-back status words, two per 1024-particle tile
     unsigned long long* scan_ctl = nullptr;     // status epoch of the look-back words
     int scan_spin_limit = 64;                   // reads of a predecessor's status before summing it here

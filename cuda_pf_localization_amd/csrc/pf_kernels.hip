@@ -835,7 +835,8 @@ __device__ inline uint64_t block_sum_u64(uint64_t v, uint64_t* s_w) {
 }
 
 __global__ __launch_bounds__(kTB) void k_scan(Layout L, FilterDev* F, const float* __restrict__ w,
-                                              uint64_t* __restrict__ cum, uint64_t* st, const unsigned long long* ctl,
+                                              uint64_t* __restrict__ cum, uint64_t* __restrict__ tcum, uint64_t* st,
+                                              const unsigned long long* ctl,
                                               int spin_limit) {
     __shared__ uint64_t s_w[4];
     __shared__ int s_dmin, s_nun;
@@ -961,6 +962,7 @@ __global__ __launch_bounds__(kTB) void k_scan(Layout L, FilterDev* F, const floa
     cp[1] = pre + q[1];
     cp[2] = pre + q[2];
     cp[3] = pre + q[3];
+    if (threadIdx.x == kTB - 1) tcum[tile] = pre + q[3];
 }
 
 // Fused cycle, first pass of the resampling scan: the normalised weights of every 1024-particle
@@ -1025,7 +1027,7 @@ __global__ __launch_bounds__(kTB) void k_norm_agg(Layout L, FilterDev* F, const 
 __global__ __launch_bounds__(kTB) void k_scan_agg(Layout L, FilterDev* F, const float* __restrict__ w,
                                                   const double* __restrict__ part_n,
                                                   const uint64_t* __restrict__ part_q, uint64_t* __restrict__ cum,
-                                                  const float* __restrict__ ratio) {
+                                                  uint64_t* __restrict__ tcum, const float* __restrict__ ratio) {
     __shared__ uint64_t s_w[4];
     __shared__ double s_vw[kFinTB / 64];
     const int tile = blockIdx.x;
@@ -1068,6 +1070,7 @@ __global__ __launch_bounds__(kTB) void k_scan_agg(Layout L, FilterDev* F, const 
     cp[1] = pre + q[1];
     cp[2] = pre + q[2];
     cp[3] = pre + q[3];
+    if (threadIdx.x == kTB - 1) tcum[tile] = pre + q[3];
     if (!last) return;
 #pragma unroll
     for (int j = 0; j < kFinTB / kTB; ++j) {
@@ -1132,29 +1135,6 @@ __device__ inline U128 pointer_key(uint64_t T, uint64_t Bi) {
         if (k.lo == 0) k.hi += 1;
     }
     return k;
-}
-
-// Merge-path split of the merged sequence of outputs (keys K_i, i < n) and particles (keys
-// cum_j n, j < n), particle j ahead of output i when cum_j n < K_i: the number of outputs among
-// its first d items.  By one wave (result wave-uniform), 64-ary: lane l tests output
-// i = lo + l * step against particle d - 1 - i; the outputs that still precede it form a prefix.
-__device__ inline int wave_merge_split(const uint64_t* c, uint32_t n, uint64_t T, uint64_t U, int d) {
-    const int lane = threadIdx.x & 63;
-    int lo = max(0, d - (int)n), hi = min(d, (int)n);  // answer in [lo, hi]
-    // ahead(i): output i comes before particle d - 1 - i, i.e. K_i <= cum_{d-1-i} n (the split
-    // lies above i); false from some i on
-    while (hi - lo > 64) {
-        const int step = (hi - lo + 63) >> 6;
-        const int i = lo + lane * step;
-        const bool ahead = i < hi && !lt128(mul_n(c[d - 1 - i], n), pointer_key(T, U + ((uint64_t)i << 32)));
-        const int k = __popcll(__ballot(ahead));
-        const int nlo = k > 0 ? lo + (k - 1) * step + 1 : lo;
-        hi = min(lo + k * step, hi);
-        lo = nlo;
-    }
-    const int i = lo + lane;
-    const bool ahead = i < hi && !lt128(mul_n(c[d - 1 - i], n), pointer_key(T, U + ((uint64_t)i << 32)));
-    return lo + __popcll(__ballot(ahead));
 }
 
 // Cross-lane folds of 64-bit values on CDNA4 (one 32-bit permutation per half).  Rows are 16
@@ -1290,27 +1270,52 @@ __device__ inline void apply_picks(int off, int hd, const int32_t* __restrict__ 
     v[2] = (float)az;
 }
 
-// Merge-path gather.  The outputs (pointer keys K_i) and the particles (cum_j n) form one merged
-// sequence of 2n items, particle j ahead of output i when cum_j n < K_i, so that ancestor(i) is
-// the number of particles ahead of output i; block b of a filter takes items [2048 b, 2048 b +
-// 2048) of it (two one-wave searches find where its outputs and particles start and end), stages
-// its particles' cum_j n in LDS and finds each of its outputs' ancestors there.  Every block has
-// 2048 items whatever the weights (with a few hundred heavy particles the ancestors of 1024
-// consecutive outputs can span thousands of particles), and the grid is one block per tile.
-// Poses move from the current set to the other set (no copies); weights -> 1/n.  An output that
-// injection picks target (linked into a list at head[output]) adds their jitters (apply_picks).
-// Every block then leaves the estimate-sum partial of its outputs' final poses and weights
-// (est_src = 1); for a filter that does not resample (its current poses and weights) or whose
-// total is zero (weights set to 1/n) the block takes the tile's particles instead, so the
-// estimate needs no separate pass.  The other set becomes current afterwards (k_flip, or the
+// Merge-path gather.  The outputs (pointer keys K_i = K_0 + i T) and the particles (cum_j n)
+// form one merged sequence of 2n items, output i ahead of particle j when K_i <= cum_j n, so that
+// ancestor(i) is the first particle behind output i; block b of a filter takes items [2048 b,
+// 2048 b + 2048) of it (every block has 2048 items whatever the weights: with a few hundred
+// heavy particles the ancestors of 1024 consecutive outputs can span thousands of particles), and
+// the grid is one block per tile.  J(d), the number of particles among the first d items, is
+// found for the block's two ends in two memory round trips: every tile's last cumulative weight
+// (tcum, left by the scan) shows which tile holds J(d) (a particle is among the first d items
+// when the output d - 1 - e is not ahead of it); those tiles -- at most three, J(d1) - J(d0) <=
+// 2048 -- are staged in LDS, where J(d0), J(d1) and then every output's ancestor are found, the
+// latter by a merge of eight items per lane.  Outputs are then moved one per lane (consecutive
+// lanes, consecutive outputs: the pose loads and stores coalesce), a lane's loads in flight
+// together.  Poses move from the current set to the other set (no copies); weights -> 1/n.  An
+// output that injection picks target (linked into a list at head[output]) adds their jitters
+// (apply_picks).  Every block then leaves the estimate-sum partial of its outputs' final poses
+// and weights (est_src = 1); for a filter that does not resample (its current poses and weights)
+// or whose total is zero (weights set to 1/n) the block takes the tile's particles instead, so
+// the estimate needs no separate pass.  The other set becomes current afterwards (k_flip, or the
 // sums finalize of the fused cycle).
+constexpr int kGatherWin = 3 * kTile;  // staged cumulative weights: the tiles from J(d0)'s on
+constexpr int kGatherBatch = 4;       // outputs per lane whose loads are in flight together
+
+__device__ inline U128 add128(const U128& a, const U128& b) {
+    U128 r;
+    r.lo = a.lo + b.lo;
+    r.hi = a.hi + b.hi + (r.lo < a.lo ? 1u : 0u);
+    return r;
+}
+// Particle e (filter-local, key cne = cum_e n) lies among the first d merged items: fewer than
+// d - e outputs are ahead of it, i.e. output d - 1 - e (if there is one) is not.
+__device__ inline bool among_first(int e, int d, const U128& cne, int n, uint64_t T, const U128& K0) {
+    const int i = d - 1 - e;
+    if (i < 0) return false;
+    if (i >= n) return true;
+    return lt128(cne, add128(K0, mul_n(T, (uint32_t)i)));
+}
+
 __global__ __launch_bounds__(kTB) void k_gather(Layout L, FilterDev* F, PoseBufs B, const uint64_t* __restrict__ cum,
-                                                float* __restrict__ w, int32_t* __restrict__ anc,
-                                                int32_t* __restrict__ head, const int32_t* __restrict__ next,
-                                                const double* __restrict__ jit, double* __restrict__ part_est,
-                                                int stride) {
-    __shared__ int s_lo, s_hi;
-    __shared__ U128 s_cn[kMergeItems];
+                                                const uint64_t* __restrict__ tcum, float* __restrict__ w,
+                                                int32_t* __restrict__ anc, int32_t* __restrict__ head,
+                                                const int32_t* __restrict__ next, const double* __restrict__ jit,
+                                                double* __restrict__ part_est, int stride) {
+    __shared__ uint64_t s_c[kGatherWin];      // cum of the staged tiles
+    __shared__ int32_t s_anc[kMergeItems];     // ancestor of each of the block's outputs
+    __shared__ int s_first[3][2];              // tile search: first probe past J(d), per round and end
+    __shared__ int s_j[2];                     // J(d0), J(d1)
     const int blk = blockIdx.x;
     const int f = L.blk_filter[blk * (kGatherBlk / kTB)];
     FilterDev& fd = F[f];
@@ -1352,28 +1357,10 @@ __global__ __launch_bounds__(kTB) void k_gather(Layout L, FilterDev* F, PoseBufs
     const uint32_t n32 = (uint32_t)nf;
     const uint64_t U = (uint64_t)fd.u32;
     const uint64_t* c = cum + off;
-    const int d0 = min((blk - off / kTile) * kMergeItems, 2 * nf), d1 = min(d0 + kMergeItems, 2 * nf);
+    const int t0 = off / kTile, nt = L.f_padn[f] / kTile;
+    const int d0 = min((blk - t0) * kMergeItems, 2 * nf), d1 = min(d0 + kMergeItems, 2 * nf);
     if (d0 < d1) {
-        if (t < 64) {
-            const int r = wave_merge_split(c, n32, T, U, d0);
-            if (t == 0) s_lo = r;
-        } else if (t < 128) {
-            const int r = wave_merge_split(c, n32, T, U, d1);
-            if (t == 64) s_hi = r;
-        }
-        __syncthreads();
-        const int ia = s_lo, ib = s_hi;          // outputs [ia, ib)
-        const int ja = d0 - ia, jb = d1 - ib;    // particles [ja, jb), at most kMergeItems of them
-        for (int u = t; u < jb - ja; u += kTB) s_cn[u] = mul_n(c[ja + u], n32);
-        __syncthreads();
-        const PoseSet D = B.s[par ^ 1];
-        const int np = jb - ja;
-        // Lane t takes groups of four consecutive outputs a0 + 4 g (g = t, t + 256, ...), a0 the
-        // multiple of 4 at or below ia, so that a full group stores 16 bytes per stream.  The
-        // group's first output is found by a binary search over the staged particles; every
-        // further output starts from the previous ancestor (ancestors do not decrease) and is
-        // usually the same particle, whose pose is then reused.  K_i = K_0 + i T (K_0 =
-        // ceil(T U / 2^32)), so no 64 x 64-bit product per output.
+        // K_i = K_0 + i T with K_0 = ceil(T U / 2^32): no 64 x 64-bit product per output
         const U128 TU = mul_n(T, (uint32_t)U);
         U128 K0;
         K0.lo = (TU.hi << 32) | (TU.lo >> 32);
@@ -1382,94 +1369,149 @@ __global__ __launch_bounds__(kTB) void k_gather(Layout L, FilterDev* F, PoseBufs
             K0.lo += 1;
             if (K0.lo == 0) K0.hi += 1;
         }
-        const int a0 = ia & ~3;
-        const int ngroups = (ib - a0 + 3) >> 2;
-        for (int g = t; g < ngroups; g += kTB) {
-            const int o0 = a0 + 4 * g;
-            int jr[4];
-            bool ok[4];
-            float v[4][7];
-            int lo = 0;
+        // 1. The tile holding J(d) for d = d0 (threads 0-127) and d = d1 (128-255): the first tile
+        //    whose last particle is not among the first d items (nt: every particle is).  Each
+        //    round probes 1024 evenly spaced tiles, eight loads per lane in flight; one round
+        //    settles up to 1024 tiles (1M particles), two up to 2^20 tiles.
+        const int half = t >> 7, ht = t & 127;
+        if (t < 6) s_first[t >> 1][t & 1] = INT_MAX;
+        __syncthreads();
+        const int rounds = nt <= 1024 ? 1 : nt <= (1 << 20) ? 2 : 3;
+        {
+            const int d = half ? d1 : d0;
+            int lo = 0, hi = nt;  // the answer lies in [lo, hi]; tiles below lo are among the first d
+            for (int r = 0; r < rounds; ++r) {
+                const int span = hi - lo;
+                const int step = (span + 1023) >> 10;
+                uint64_t tv[8];
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int o = o0 + r;
-                ok[r] = o >= ia && o < ib;
-                jr[r] = -1;
-                if (!ok[r]) continue;
-                const U128 io = mul_n(T, (uint32_t)o);
-                U128 K;
-                K.lo = K0.lo + io.lo;
-                K.hi = K0.hi + io.hi + (K.lo < K0.lo ? 1u : 0u);
-                // first staged cn >= K in [lo, np] (np: the ancestor is particle jb)
-                if (lo < np && lt128(s_cn[lo], K)) {
-                    int hi = np;
-                    ++lo;
-                    while (lo < hi) {
-                        const int mid = (lo + hi) >> 1;
-                        if (lt128(s_cn[mid], K))
-                            lo = mid + 1;
-                        else
-                            hi = mid;
+                for (int p = 0; p < 8; ++p) {
+                    const int k = ht + 128 * p;
+                    tv[p] = span > 0 && k * step < span ? tcum[t0 + lo + k * step] : 0;
+                }
+                int kmin = INT_MAX;
+#pragma unroll
+                for (int p = 0; p < 8; ++p) {
+                    const int k = ht + 128 * p;
+                    if (span > 0 && k * step < span && k < kmin) {
+                        const int e = min((lo + k * step) * kTile + kTile - 1, nf - 1);
+                        if (!among_first(e, d, mul_n(tv[p], n32), nf, T, K0)) kmin = k;
                     }
                 }
-                jr[r] = ja + lo;
-            }
-            // poses of the distinct ancestors, list heads
-            int hd[4];
-            const bool full = ok[0] && ok[3];
-            if (full) {
-                const int4 h4 = *reinterpret_cast<const int4*>(head + off + o0);
-                hd[0] = h4.x;
-                hd[1] = h4.y;
-                hd[2] = h4.z;
-                hd[3] = h4.w;
-            } else {
-#pragma unroll
-                for (int r = 0; r < 4; ++r) hd[r] = ok[r] ? head[off + o0 + r] : -1;
-            }
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                if (!ok[r]) continue;
-                if (r > 0 && jr[r] == jr[r - 1]) {
-#pragma unroll
-                    for (int k = 0; k < 7; ++k) v[r][k] = v[r - 1][k];
-                } else {
-#pragma unroll
-                    for (int k = 0; k < 7; ++k) v[r][k] = S.c[k][off + jr[r]];
+                if (kmin != INT_MAX) atomicMin(&s_first[r][half], kmin);
+                __syncthreads();
+                const int kf = s_first[r][half];
+                if (span > 0) {
+                    if (kf == INT_MAX) {
+                        lo += ((span - 1) / step) * step + 1;
+                    } else {
+                        hi = lo + kf * step;
+                        lo = kf == 0 ? lo : lo + (kf - 1) * step + 1;
+                    }
                 }
             }
-            // injection, stores, estimate terms (in output order)
+            if (ht == 0) s_j[half] = lo;  // the tile (lo == hi after the last round)
+        }
+        __syncthreads();
+        const int ta = s_j[0], tb = s_j[1];
+        // 2. Stage the tiles from J(d0)'s on (J(d1) <= J(d0) + 2048 lies at most two tiles further).
+        const int wb = ta * kTile;  // filter-local index of s_c[0]
+        const int wn = min(kGatherWin, nt * kTile - wb);
+        {
+            uint64_t cv[kGatherWin / kTB];
 #pragma unroll
-            for (int r = 0; r < 4; ++r)
-                if (ok[r] && hd[r] >= 0) {
-                    float pv[7];
+            for (int u = 0; u < kGatherWin / kTB; ++u) {
+                const int x = t + kTB * u;
+                cv[u] = x < wn ? c[wb + x] : 0;
+            }
 #pragma unroll
-                    for (int k = 0; k < 7; ++k) pv[k] = v[r][k];
-                    apply_picks(off, hd[r], next, jit, pv);
-                    v[r][0] = pv[0];
-                    v[r][1] = pv[1];
-                    v[r][2] = pv[2];
-                    head[off + o0 + r] = -1;
+            for (int u = 0; u < kGatherWin / kTB; ++u) s_c[t + kTB * u] = cv[u];
+        }
+        __syncthreads();
+        // J(d0) and J(d1) inside their tiles (wave 0 and wave 1): a 64-ary search, then the exact
+        // particle among the 16 the first step leaves.
+        const int wv = t >> 6, lane = t & 63;
+        if (wv < 2) {
+            const int d = wv ? d1 : d0;
+            const int tile = wv ? tb : ta;
+            int J = nf;
+            if (tile < nt) {
+                const int A = tile * kTile, E = min(A + kTile - 1, nf - 1);
+                auto beyond = [&](int e) {  // e is not among the first d items
+                    return !among_first(e, d, mul_n(s_c[min(e - wb, kGatherWin - 1)], n32), nf, T, K0);
+                };
+                const int e1 = min(A + 16 * lane + 15, E);
+                const uint64_t m1 = __ballot(A + 16 * lane <= E && beyond(e1));
+                const int l1 = m1 ? __ffsll((long long)m1) - 1 : 0;
+                const int e2 = A + 16 * l1 + lane;
+                const uint64_t m2 = __ballot(lane < 16 && e2 <= E && beyond(e2));
+                J = m2 ? A + 16 * l1 + __ffsll((long long)m2) - 1 : E;
+            }
+            if (lane == 0) s_j[wv] = J;
+        }
+        __syncthreads();
+        const int ja = s_j[0], jb = s_j[1];
+        const int ia = d0 - ja, ib = d1 - jb;  // outputs [ia, ib), ancestors in [ja, jb]
+        // 3. Every output's ancestor: lane t merges items [8 t, 8 t + 8) of the block's 2048.
+        {
+            const int m = d1 - d0;
+            int d = d0 + 8 * t;
+            if (d < d1) {
+                int lo = ja, hi = jb;  // first particle not among the first d items
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (among_first(mid, d, mul_n(s_c[mid - wb], n32), nf, T, K0))
+                        lo = mid + 1;
+                    else
+                        hi = mid;
                 }
-            if (full) {
+                int e = lo, i = d - lo;
+                U128 K = add128(K0, mul_n(T, (uint32_t)min(i, nf - 1)));
+                const int dend = min(d0 + 8 * t + 8, d0 + m);
+                for (; d < dend; ++d) {
+                    const bool out = i < nf && (e >= nf || !lt128(mul_n(s_c[min(e - wb, kGatherWin - 1)], n32), K));
+                    if (out) {
+                        s_anc[i - ia] = min(e, nf - 1);
+                        ++i;
+                        K = add128(K, U128{0, T});
+                    } else {
+                        ++e;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        // 4. Move the outputs, one per lane, kGatherBatch per lane with their loads in flight.
+        const PoseSet D = B.s[par ^ 1];
+        const int no = ib - ia;
+        for (int u0 = 0; u0 < no; u0 += kTB * kGatherBatch) {
+            int jj[kGatherBatch], hd[kGatherBatch];
+            float v[kGatherBatch][7];
 #pragma unroll
-                for (int k = 0; k < 7; ++k)
-                    *reinterpret_cast<float4*>(D.c[k] + off + o0) = make_float4(v[0][k], v[1][k], v[2][k], v[3][k]);
-                *reinterpret_cast<int4*>(anc + off + o0) = make_int4(jr[0], jr[1], jr[2], jr[3]);
-                *reinterpret_cast<float4*>(w + off + o0) = make_float4(winv, winv, winv, winv);
-            } else {
+            for (int r = 0; r < kGatherBatch; ++r) {
+                const int u = u0 + t + kTB * r;
+                jj[r] = -1;
+                hd[r] = -1;
+                if (u < no) {
+                    const int j = min(max(s_anc[u], 0), nf - 1);
+                    jj[r] = j;
+                    hd[r] = head[off + ia + u];
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    if (!ok[r]) continue;
-#pragma unroll
-                    for (int k = 0; k < 7; ++k) D.c[k][off + o0 + r] = v[r][k];
-                    anc[off + o0 + r] = jr[r];
-                    w[off + o0 + r] = winv;
+                    for (int k = 0; k < 7; ++k) v[r][k] = S.c[k][off + j];
                 }
             }
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                if (!ok[r]) continue;
+            for (int r = 0; r < kGatherBatch; ++r) {
+                if (jj[r] < 0) continue;
+                const int o = off + ia + u0 + t + kTB * r;
+                if (hd[r] >= 0) {
+                    apply_picks(off, hd[r], next, jit, v[r]);
+                    head[o] = -1;
+                }
+#pragma unroll
+                for (int k = 0; k < 7; ++k) D.c[k][o] = v[r][k];
+                anc[o] = jj[r];
+                w[o] = winv;
                 const double q[4] = {(double)v[r][3], (double)v[r][4], (double)v[r][5], (double)v[r][6]};
                 est_terms((double)winv, (double)v[r][0], (double)v[r][1], (double)v[r][2], q, acc);
             }
@@ -1757,7 +1799,18 @@ __global__ void k_resample_predicate(FilterDev* F, int nfilt, const float* ratio
 
 namespace pfk {
 
+constexpr int kMaxFilterN = 1 << 30;
+
 int state_alloc(PfState* s, const std::vector<int>& ns) {
+    // Particles are indexed with int: every filter's 2n merge-path items, and the padded total,
+    // must fit (at most 2^30 particles per filter and in all); a rejected size leaves the state
+    // as it was.
+    long long np_total = 0;
+    for (int n : ns) {
+        if (n <= 0 || n > kMaxFilterN) return -1;
+        np_total += (n + kTile - 1) / kTile * kTile;
+    }
+    if (ns.empty() || np_total > kMaxFilterN) return -1;
     state_free_particles(s);
     s->agg_ok = false;
     const int nf = (int)ns.size();
@@ -1793,6 +1846,7 @@ int state_alloc(PfState* s, const std::vector<int>& ns) {
     PF_CHECK(hipMalloc((void**)&s->part_w, sizeof(double) * 3 * (size_t)nb256));
     PF_CHECK(hipMalloc((void**)&s->part_n, sizeof(double) * (size_t)nb256));
     PF_CHECK(hipMalloc((void**)&s->part_q, sizeof(uint64_t) * (size_t)ntile));
+    PF_CHECK(hipMalloc((void**)&s->tcum, sizeof(uint64_t) * (size_t)ntile));
     PF_CHECK(hipMalloc((void**)&s->part_est, sizeof(double) * kEstN * (size_t)ntile));
     PF_CHECK(hipMalloc((void**)&s->part_cov, sizeof(double) * kCovN * (size_t)ntile));
     PF_CHECK(hipMalloc((void**)&s->scan_st, sizeof(uint64_t) * 2 * (size_t)ntile));
@@ -1841,7 +1895,7 @@ void state_free_particles(PfState* s) {
         }
     void** ptrs[] = {(void**)&s->w,        (void**)&s->ll,          (void**)&s->cum,       (void**)&s->anc,
                      (void**)&s->inj_next,    (void**)&s->inj_head,  (void**)&s->inj_jit,
-                     (void**)&s->part_w,   (void**)&s->part_n,  (void**)&s->part_q,  (void**)&s->part_est,    (void**)&s->part_cov,  (void**)&s->scan_st,
+                     (void**)&s->part_w,   (void**)&s->part_n,  (void**)&s->part_q,  (void**)&s->tcum,  (void**)&s->part_est,    (void**)&s->part_cov,  (void**)&s->scan_st,
                      (void**)&s->scan_ctl, (void**)&s->d_blk_filter, (void**)&s->d_fn,     (void**)&s->d_foff,
                      (void**)&s->d_fpadn,  (void**)&s->d_fwinv,     (void**)&s->fdev,      (void**)&s->d_inj_blk};
     for (void** p : ptrs) {
@@ -1931,7 +1985,7 @@ int launch_normalize(PfState* s, int force_uniform, const float* d_ratio, bool s
         PF_LAUNCH_CHECK();
         prof_begin(s, PFX_STAGE_RESAMPLE_SCAN);
         hipLaunchKernelGGL(k_scan_agg, dim3(ntile), dim3(kTB), 0, s->stream, s->L, s->fdev, s->w, s->part_n,
-                           s->part_q, s->cum, d_ratio);
+                           s->part_q, s->cum, s->tcum, d_ratio);
         prof_end(s);
         PF_LAUNCH_CHECK();
         return 0;
@@ -1958,12 +2012,12 @@ int launch_resample(PfState* s, ParamRef<InjectParams> inj, const int32_t* inj_t
         // the predicate forced, as k_resample_predicate does for this path)
         prof_begin(s, PFX_STAGE_RESAMPLE_SCAN);
         hipLaunchKernelGGL(k_scan_agg, dim3(ntile), dim3(kTB), 0, s->stream, s->L, s->fdev, s->w, s->part_n,
-                           s->part_q, s->cum, nullptr);
+                           s->part_q, s->cum, s->tcum, nullptr);
         prof_end(s);
         PF_LAUNCH_CHECK();
     } else if (scan) {
         prof_begin(s, PFX_STAGE_RESAMPLE_SCAN);
-        hipLaunchKernelGGL(k_scan, dim3(ntile), dim3(kTB), 0, s->stream, s->L, s->fdev, s->w, s->cum, s->scan_st,
+        hipLaunchKernelGGL(k_scan, dim3(ntile), dim3(kTB), 0, s->stream, s->L, s->fdev, s->w, s->cum, s->tcum, s->scan_st,
                            s->scan_ctl, s->scan_spin_limit);
         prof_end(s);
         PF_LAUNCH_CHECK();
@@ -1978,7 +2032,7 @@ int launch_resample(PfState* s, ParamRef<InjectParams> inj, const int32_t* inj_t
     }
     prof_begin(s, PFX_STAGE_RESAMPLE_GATHER);
     hipLaunchKernelGGL(k_gather, dim3(s->NP / kGatherBlk), dim3(kTB), 0, s->stream, s->L, s->fdev, s->pose, s->cum,
-                       s->w, s->anc, s->inj_head, s->inj_next, s->inj_jit, s->part_est, s->NP / kTile);
+                       s->tcum, s->w, s->anc, s->inj_head, s->inj_next, s->inj_jit, s->part_est, s->NP / kTile);
     prof_end(s);
     PF_LAUNCH_CHECK();
     if (flip) {
