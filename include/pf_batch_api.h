@@ -43,7 +43,8 @@ typedef struct pfb_estimate {
     int32_t resampled, valid;
 } pfb_estimate;
 
-/* nf filters; obs_cap observations per filter per cycle (clamped like pfc_create). */
+/* nf filters; obs_cap observations per filter per cycle (clamped like pfc_create).  Every
+ * filter's n lies in [1, 2^30], and so does the batch's total rounded up to 1024 per filter. */
 void *pfb_create(int nf, const pfb_filter_cfg *cfg, int obs_cap);
 int pfb_destroy(void *h);
 int pfb_filter_count(void *h);

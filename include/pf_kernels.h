@@ -54,7 +54,8 @@ void *pfc_create(int cap_obs);
 int pfc_destroy(void *h);
 
 /* (Re)allocates for n particles; seed 0 becomes 42; resets the generation counter.
- * Replaces pf_kernels.cu:519-551. */
+ * n must lie in [1, 2^30] (the resampling merge path indexes 2n items with int); a
+ * rejected n returns -1 and leaves the handle as it was.  Replaces pf_kernels.cu:519-551. */
 int pfc_init(void *h, int n, unsigned int seed);
 
 /* Dense landmark table, 3 floats per id; count = max id + 1 (0 = no landmarks).
