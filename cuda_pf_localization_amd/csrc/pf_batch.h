@@ -51,6 +51,7 @@ struct Layout {
     const int* f_off;
     const int* f_padn;      // n rounded up to kTile
     const float* f_winv;    // (float)(1/n)
+    int nf;                 // filters (1: every block belongs to filter 0, no lookup)
 };
 
 // Observation joined with its landmark, read as wave-uniform scalar loads (64 B).  For a

@@ -67,6 +67,13 @@ template <class T>
 using ConstPtr = const T*;  // host pass of the single-source compile: never executed
 #endif
 
+// Filter of layout block b (256 particles).  A single-filter layout needs no lookup, so a
+// kernel's first loads (its filter's record, sizes and parameters) do not wait for this one.
+__device__ inline int layout_filter(const Layout& L, int b) {
+    if (L.nf == 1) return 0;
+    return ((ConstPtr<int>)L.blk_filter)[b];
+}
+
 // Parameter block of filter f (wave-uniform): from the per-filter device array or the launch
 // argument.
 template <class T>
@@ -120,7 +127,7 @@ template <int reinit>
 __global__ __launch_bounds__(kTB) void k_motion(Layout L, PoseBufs B, FilterDev* F, float* __restrict__ w,
                                                 ParamRef<MotionParams> A, int dst_flip) {
     const int blk = blockIdx.x;
-    const int f = ((ConstPtr<int>)L.blk_filter)[blk];
+    const int f = layout_filter(L, blk);
     const int off = ((ConstPtr<int>)L.f_off)[f];
     const int li = blk * kTB + (int)threadIdx.x - off;
     if (li >= ((ConstPtr<int>)L.f_n)[f]) return;
@@ -368,7 +375,7 @@ void k_weight(Layout L, PoseBufs B, FilterDev* F, double* __restrict__ ll, doubl
             return;
         }
     }
-    const int f = ((ConstPtr<int>)L.blk_filter)[blk * kPPT];
+    const int f = layout_filter(L, blk * kPPT);
     const int off = ((ConstPtr<int>)L.f_off)[f];
     const int nf = ((ConstPtr<int>)L.f_n)[f];
     const WeightParams p = uniform_params(A, f);
@@ -683,7 +690,7 @@ __global__ __launch_bounds__(kTB) void k_normalize(Layout L, FilterDev* F, const
     __shared__ double sh[4];
     __shared__ uint64_t s_w[4];
     const int tile = blockIdx.x;
-    const int f = L.blk_filter[tile * (kTile / kTB)];
+    const int f = layout_filter(L, tile * (kTile / kTB));
     const FilterDev& fd = F[f];
     double w2;
     const float4 wv = tile_norm(ll, tile, L, f, fd, force_uniform || fd.uniform, w2);
@@ -845,7 +852,7 @@ __global__ __launch_bounds__(kTB) void k_scan(Layout L, FilterDev* F, const floa
     __shared__ uint64_t s_un_val[kLbWin];  // their aggregates, summed here
     const int tile = blockIdx.x;
     const uint32_t epoch = (uint32_t)ctl[0] & 0x3fffffffu;
-    const int f = L.blk_filter[tile * (kTile / kTB)];
+    const int f = layout_filter(L, tile * (kTile / kTB));
     FilterDev& fd = F[f];
     const float4 wv = tile_w(w, tile);
     if (!fd.do_resample) {
@@ -978,7 +985,7 @@ __global__ __launch_bounds__(kTB) void k_norm_agg(Layout L, FilterDev* F, const 
     __shared__ double sh[4];
     __shared__ uint64_t s_w[4];
     const int tile = blockIdx.x;
-    const int f = L.blk_filter[tile * (kTile / kTB)];
+    const int f = layout_filter(L, tile * (kTile / kTB));
     const FilterDev& fd = F[f];
     const int off = L.f_off[f], nf = L.f_n[f];
     // the first pick of each lane is pushed before the tile's own work, its link stored at the
@@ -1031,7 +1038,7 @@ __global__ __launch_bounds__(kTB) void k_scan_agg(Layout L, FilterDev* F, const 
     __shared__ uint64_t s_w[4];
     __shared__ double s_vw[kFinTB / 64];
     const int tile = blockIdx.x;
-    const int f = L.blk_filter[tile * (kTile / kTB)];
+    const int f = layout_filter(L, tile * (kTile / kTB));
     const int off = L.f_off[f], nf = L.f_n[f];
     const int t0 = off / kTile, nt = L.f_padn[f] / kTile;
     const bool last = tile == t0 + nt - 1;
@@ -1317,7 +1324,7 @@ __global__ __launch_bounds__(kTB) void k_gather(Layout L, FilterDev* F, PoseBufs
     __shared__ int s_first[3][2];              // tile search: first probe past J(d), per round and end
     __shared__ int s_j[2];                     // J(d0), J(d1)
     const int blk = blockIdx.x;
-    const int f = L.blk_filter[blk * (kGatherBlk / kTB)];
+    const int f = layout_filter(L, blk * (kGatherBlk / kTB));
     FilterDev& fd = F[f];
     const int off = L.f_off[f];
     const int nf = L.f_n[f];
@@ -1529,7 +1536,7 @@ __global__ __launch_bounds__(kTB) void k_inj_pick(Layout L, FilterDev* F, ParamR
                                                   double* __restrict__ jit, int32_t* __restrict__ head,
                                                   int32_t* __restrict__ next) {
     const int blk = tab ? tab[blockIdx.x] : (int)blockIdx.x;
-    const int f = L.blk_filter[blk];
+    const int f = layout_filter(L, blk);
     const FilterDev& fd = F[f];
     const InjectParams& p = A.get(f);
     const int off = L.f_off[f];
@@ -1666,7 +1673,7 @@ __device__ void cov_fin_block(const Layout& L, FilterDev* F, const double* __res
 __global__ __launch_bounds__(kTB) void k_est_sums(Layout L, FilterDev* F, PoseBufs B, const float* __restrict__ w,
                                                   double* __restrict__ part, int stride) {
     const int blk = blockIdx.x;
-    const int f = L.blk_filter[blk * (kTile / kTB)];
+    const int f = layout_filter(L, blk * (kTile / kTB));
     if (F[f].est_src) return;
     const int off = L.f_off[f];
     const int nf = L.f_n[f];
@@ -1691,7 +1698,7 @@ __global__ __launch_bounds__(kTB) void k_est_sums(Layout L, FilterDev* F, PoseBu
 // canonical sign, angle = 2 atan2(|v|, w)).
 __device__ void covs_tile(const Layout& L, FilterDev* F, const PoseBufs& B, int swapped, const float* __restrict__ w,
                           double* __restrict__ part, int stride, const ParamRef<CovParams>& A, int blk) {
-    const int f = L.blk_filter[blk * (kTile / kTB)];
+    const int f = layout_filter(L, blk * (kTile / kTB));
     const int off = L.f_off[f];
     const int nf = L.f_n[f];
     const CovParams& p = A.get(f);
@@ -1884,6 +1891,7 @@ int state_alloc(PfState* s, const std::vector<int>& ns) {
     s->L.f_off = s->d_foff;
     s->L.f_padn = s->d_fpadn;
     s->L.f_winv = s->d_fwinv;
+    s->L.nf = nf;
     return 0;
 }
 
@@ -2634,6 +2642,7 @@ extern "C" int pfx_est_sums_noalloc(const float* px, const float* py, const floa
     L.f_off = sc + 1;
     L.f_padn = sc + 2;
     L.f_winv = reinterpret_cast<const float*>(sc + 3);
+    L.nf = 1;
     FilterDev* F = reinterpret_cast<FilterDev*>(base + kNaFilterOff);
     double* part = reinterpret_cast<double*>(base + kNaPartOff);
     PoseBufs B{};

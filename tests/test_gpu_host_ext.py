@@ -62,6 +62,22 @@ def test_particle_count_beyond_int_is_rejected(gpu):
         ParticleFilter(PfConfig(particle_count=2 ** 32 + 5, seed=1))
 
 
+def test_particle_count_beyond_merge_path_limit_is_rejected(gpu):
+    """The resampling merge path indexes 2n items with int: pfc_init refuses n > 2^30 before
+    allocating, and the handle keeps its particles."""
+    from cuda_pf_localization_amd import PfKernels
+    k = PfKernels(16)
+    k.init(1000, 3)
+    with pytest.raises(RuntimeError):
+        k.init(2 ** 30 + 1, 3)
+    with pytest.raises(RuntimeError):
+        k.init(0, 3)
+    assert k.L.pfx_particle_count(k.h) == 1000
+    pos, quat, w = k.get_state()
+    assert len(pos) == 1000
+    k.close()
+
+
 def _steps(sc, k0, k1):
     out = []
     for st in sc.steps[k0:k1]:
