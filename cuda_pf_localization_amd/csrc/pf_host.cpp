@@ -342,8 +342,9 @@ using namespace pfh;
 extern "C" {
 
 void* pfh_filter_new(const pfh_config* cfg) {
-    // The device filter indexes particles with int (pfc_init takes int n, cu:534).
-    if (!cfg || cfg->particle_count <= 0 || cfg->particle_count > INT_MAX) return nullptr;
+    // The device filter indexes particles with int (pfc_init takes int n, cu:534) and its
+    // resampling merge path 2n items: at most 2^30 particles.
+    if (!cfg || cfg->particle_count <= 0 || cfg->particle_count > (int64_t)1 << 30) return nullptr;
     ParticleFilter* f = new (std::nothrow) ParticleFilter(*cfg);
     if (!f) return nullptr;
     if (!f->create()) {
