@@ -61,7 +61,8 @@ struct Grid {
     // (W + 2) x (H + 2): bit (cy + 1) * (W + 2) + cx + 1 is map cell (cx, cy), 1 = occupied.
     // The frame cells are set, so a ray stops on the frame where it would leave the map.
     const uint32_t* bits;
-    const uint8_t* bytes;  // the same framed grid, one byte per cell (nbytes, a multiple of 16)
+    const uint8_t* bytes;  // the same framed grid, one byte per cell: 1 obstacle, 2 frame, 0 free
+                           // (nbytes, a multiple of 16)
     int W, H, nwords, nbytes;
     float res, ox, oy;
 };
@@ -132,23 +133,28 @@ struct GlState {
     std::vector<Ev> prof;
 };
 
-__device__ inline bool occ_bit(const uint32_t* bits, int idx) { return (bits[idx >> 5] >> (idx & 31)) & 1u; }
-
 // Cell lookups for the raycaster: packed bits (small footprint) or one byte per cell (one LDS
-// read and one compare per step).
+// read and one compare per step).  get() is non-zero for a set cell; frame() tells, for the
+// set cell a walk ended on, whether it is a frame cell (the ray left the map): the byte grid
+// stores frame cells as 2 (obstacles 1), the bit grid is asked by position.
 struct OccBits {
     const uint32_t* bits;
-    __device__ bool operator()(int idx) const { return occ_bit(bits, idx); }
+    __device__ uint32_t get(int idx) const { return (bits[idx >> 5] >> (idx & 31)) & 1u; }
+    __device__ bool frame(int idx, uint32_t, int Wp, int H) const {
+        const int qy = idx / Wp, qx = idx - qy * Wp;
+        return qx == 0 || qy == 0 || qx == Wp - 1 || qy == H + 1;
+    }
 };
 struct OccBytes {
     const uint8_t* bytes;
-    __device__ bool operator()(int idx) const { return bytes[idx] != 0; }
+    __device__ uint32_t get(int idx) const { return bytes[idx]; }
+    __device__ bool frame(int, uint32_t v, int, int) const { return v == 2u; }
 };
 
 // f32 DDA with the operation sequence of pf_kernels.cu:784-808 (distance to the first
 // occupied cell along a unit direction; rmax on exit, miss or beyond rmax).  The walk tracks
 // only the framed linear index: leaving the map means entering the set frame, so the loop
-// tests one bit per step and the frame is told apart from an obstacle once, at the end.
+// tests one cell per step and the frame is told apart from an obstacle once, at the end.
 template <class Occ>
 __device__ inline float cast_ray(Occ occ_at, int W, int H, float res, float ox, float oy, float px, float py,
                                  float dirx, float diry, float rmax) {
@@ -166,29 +172,26 @@ __device__ inline float cast_ray(Occ occ_at, int W, int H, float res, float ox, 
     int idx = (cy + 1) * Wp + cx + 1;
     const int dxi = sx, dyi = sy * Wp;
     // Branch-free step: both candidate boundary times are formed and one is selected, which
-    // rounds exactly as the reference's if/else.
-    float hit = rmax;
-    int hidx = -1;
+    // rounds exactly as the reference's if/else.  The loop carries only the walk and the last
+    // step's distance and cell; which test ended it is decided once afterwards (beyond rmax
+    // first, as the reference tests it first).  The frame ends every walk within W + H steps
+    // whatever the direction or rmax; the step bound is a guard only.
+    float dist = 0.f;
+    uint32_t v = 0;
     for (int step = 0; step < 8192; ++step) {
         const bool xs = tmx <= tmy;
-        const float dist = xs ? tmx : tmy;
+        dist = xs ? tmx : tmy;
         const float tmx1 = tmx + tdx, tmy1 = tmy + tdy;
         tmx = xs ? tmx1 : tmx;
         tmy = xs ? tmy : tmy1;
         idx += xs ? dxi : dyi;
-        const bool occ = occ_at(idx);
-        if (dist > rmax) break;
-        if (occ) {
-            hit = dist;
-            hidx = idx;
-            break;
-        }
+        v = occ_at.get(idx);
+        if (v != 0u || dist > rmax) break;
     }
-    if (hidx >= 0) {  // a frame cell: the ray left the map
-        const int qy = hidx / Wp, qx = hidx - qy * Wp;
-        if (qx == 0 || qy == 0 || qx == Wp - 1 || qy == H + 1) hit = rmax;
-    }
-    return hit;
+    // A set cell found past rmax (on the step that crossed it) yields rmax: the distance never
+    // decreases along the walk, so clamping equals stopping at the first step beyond rmax.
+    const float r = (v == 0u || occ_at.frame(idx, v, Wp, H)) ? rmax : dist;
+    return fminf(r, rmax);
 }
 
 // Hypothesis h (deterministic-draw mode): cell = floor(u53 * nfree); x/y offsets and yaw
@@ -1277,7 +1280,7 @@ void* pfgl_create(int W, int H, float res, float ox, float oy, const unsigned ch
             if (frame || occ[(y - 1) * W + (x - 1)]) {
                 const long long i = y * Wp + x;
                 bits[(size_t)(i >> 5)] |= 1u << (i & 31);
-                bytes[(size_t)i] = 1;
+                bytes[(size_t)i] = frame ? 2 : 1;
             }
         }
     for (long long i = 0; i < ncell; ++i)
