@@ -452,6 +452,45 @@ def test_resample_single_heavy_particle(gpu):
     assert np.all(p == pos[1234])
 
 
+def skewed_weights(n, kind, seed):
+    rng = np.random.default_rng(seed)
+    w = np.zeros(n, np.float64)
+    if kind == "first":
+        w[0] = 1.0
+    elif kind == "last":
+        w[-1] = 1.0
+    elif kind == "ends":
+        w[0], w[-1] = 0.3, 0.7
+    elif kind == "sparse":  # a few hundred heavy particles among zeros (the tracking regime)
+        idx = rng.choice(n, size=300, replace=False)
+        w[idx] = rng.gamma(0.5, 1.0, size=300)
+    else:  # "tail": weight rising steeply towards the end (long runs of single copies)
+        w = np.exp(np.linspace(-60.0, 0.0, n))
+    return (w / w.sum()).astype(np.float32)
+
+
+@pytest.mark.parametrize("n", [70_001, 1_048_576, 1_048_577])
+@pytest.mark.parametrize("kind", ["first", "last", "ends", "sparse", "tail"])
+def test_resample_skewed_weights_bitexact(gpu, n, kind):
+    """Weights that put whole blocks of outputs on one particle or leave long runs of particles
+    without copies: each gather block's merge-path window and its tile search at their edges
+    (a filter of exactly 1024 tiles, one more, a ragged last tile); ancestors bit-exact."""
+    pos, quat = random_state(n, 31)
+    w = skewed_weights(n, kind, n)
+    k, o = pair(n, 5)
+    k.set_state(pos, quat, w)
+    o.set_state(pos.astype(np.float64), quat.astype(np.float64), w.astype(np.float64))
+    lcg = 0x0F1E_2D3C_4B5A_6978 ^ n
+    k.set_lcg(lcg)
+    o.set_rng(o.rng()[0], 0, lcg)
+    k.resample(0.0)
+    o.resample(0.0)
+    anc = k.ancestors()
+    assert np.array_equal(anc, o.ancestors()), f"{kind}: {np.sum(anc != o.ancestors())} ancestor mismatches"
+    p, q, _ = k.get_state()
+    assert np.array_equal(p, pos[anc]) and np.array_equal(q, quat[anc])
+
+
 def test_inject_duplicates_deterministic(gpu):
     """ratio 1.0 -> n picks, many duplicate targets: pick-ordered f64 sums, bit-exact."""
     n = 3000
