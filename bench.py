@@ -36,10 +36,10 @@ FP64_VALU_PEAK_TFLOPS = 78.6
 # residual + square + scale 3, direction dot product 5 + normalise 3, bearing square + scale 2,
 # two gated accumulates 4 -> 40 FLOP, plus sqrt, reciprocal and acos).  This is the figure the
 # roofline is computed on.  The kernel's own formulation issues more arithmetic per pair
-# (Newton step after the reciprocal square root, the acos^2 series, fma = 2): 54 FLOP, reported
-# beside it as issued work, not as the algorithm.
+# (Newton step after the reciprocal square root, the 3-term acos^2 series, fma = 2): 52 FLOP,
+# reported beside it as issued work, not as the algorithm.
 WEIGHT_FLOP_PER_PAIR = 40.0
-WEIGHT_FLOP_PER_PAIR_ISSUED = 54.0
+WEIGHT_FLOP_PER_PAIR_ISSUED = 52.0
 # Algorithmic HBM bytes per particle (SoA f32 pose 28 B, weight 4 B, log-likelihood 8 B,
 # fixed-point cumulative weight 8 B).  ABI driver stages:
 BYTES = {"predict": 56, "weight": 36, "normalize": 12, "resample_scan": 12, "resample_gather": 72,

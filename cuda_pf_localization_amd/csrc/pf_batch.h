@@ -80,11 +80,11 @@ struct WeightParams {
     double rng_gate;               // |range residual| gate g sr (+inf: no gate)
     double u_gate;                            // 1 - cos of the bearing gate (+inf: none)
     // Scaled range-bearing path (u_gate <= 0.005): residuals formed from 2 b, the bearing as
-    // us = u / u_gate so that the gate is the [0, 1] clamp, eb^2 = us (c0 + c1 us + c2 us^2 +
-    // c3 us^3), the range residual at 4x scale against rng_gate4 = 4 g sr.
+    // us = u / u_gate so that the gate is the [0, 1] clamp, eb^2 = us (c0 + c1 us + c2 us^2),
+    // the range residual at 4x scale against rng_gate4 = 4 g sr.
     int fast_rb, pad_fast;
     double cg;                 // 1 / u_gate
-    double c0, c1, c2, c3;     // series coefficients in us
+    double c0, c1, c2;         // series coefficients in us
     double rng_gate4;
     const ObsRec* obs;
     int nobs;  // joined records

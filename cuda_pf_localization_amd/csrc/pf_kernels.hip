@@ -232,15 +232,15 @@ __device__ inline BodyMap body_map(const PoseSet& S, int gi) {
 // acos(1 - u)^2 as a power series in u = 1 - dot: acos(1-u)^2 = sum_n 2^(n+1) u^n / (n^2 C(2n,n))
 // (from asin(x)^2 = 1/2 sum (2x)^(2n) / (n^2 C(2n,n)) with x^2 = u/2).  Six terms hold to
 // ~1e-18 relative for u <= 0.005, twelve for u <= 0.125.
-// Four terms for the gated fast path (u <= u_gate <= 0.005): the first omitted term,
-// (64/6300) u^5, is below 3.2e-14 there, i.e. < 1e-11 relative.
-__host__ __device__ inline double acos_sq_p4(double u) {  // the series divided by u
-    double p = 32.0 / 1120.0;
-    p = fma(p, u, 16.0 / 180.0);
-    p = fma(p, u, 8.0 / 24.0);
+// Three terms for the gated fast path (u <= u_gate <= 0.005): the first omitted term,
+// (32/1120) u^4, is below 1.8e-11 there (< 2e-9 relative), i.e. at most 0.0286 u_gate^4 / (2 sb^2)
+// in a log-likelihood term -- 3.8e-10 for sb = 0.02 and a 3-sigma gate -- against the
+// 1e-5 relative the weights are held to.
+__host__ __device__ inline double acos_sq_p3(double u) {  // the series divided by u
+    const double p = fma(16.0 / 180.0, u, 8.0 / 24.0);
     return fma(p, u, 2.0);
 }
-__host__ __device__ inline double acos_sq_u4(double u) { return u * acos_sq_p4(u); }
+__host__ __device__ inline double acos_sq_u3(double u) { return u * acos_sq_p3(u); }
 __host__ __device__ inline double acos_sq_u6(double u) {
     double p = 128.0 / 33264.0;
     p = fma(p, u, 64.0 / 6300.0);
@@ -286,14 +286,14 @@ __device__ inline void body_offset(const BodyMap& b, const ObsRec& o, double& bx
     bz = __fma_rn(b.m20, o.lx, __fma_rn(b.m21, o.ly, __fma_rn(b.m22, o.lz, -b.mpz)));
 }
 
-// Gated range-bearing pair on the scaled path (27 FP64 instructions with the body offset).
+// Gated range-bearing pair on the scaled path (26 FP64 instructions with the body offset).
 // In: b2 = 2 b (the particle's body map doubled, exact), the record's vs = v / (2 u_gate) and
 // rng4 = 4 range.  X = |b2|^2 = 4 |b|^2; y0h = rsq(X) ~ 1 / (2 |b|); one Newton step
 // written as q = 3 - X y0h^2 gives 1/|b| = y0h q and |b| = X y0h q / 4, so
 //   ec4 = X y0h q - rng4 = 4 (|b| - range)            (gated as min(|ec4|, 4 g sr))
 //   us  = clamp((1 + (b.v) / |b|) / u_gate, 0, 1)      (the clamp bit of the FMA is the gate)
-// and eb^2 = u acos-series(u) = us (c0 + c1 us + c2 us^2 + c3 us^3) with u = u_gate us.  The range
-// sum comes out 16x, which the caller's final scale removes.
+// and eb^2 = u acos-series(u) = us (c0 + c1 us + c2 us^2) with u = u_gate us.  The range sum
+// comes out 16x, which the caller's final scale removes.
 __device__ inline void rb_scaled(double bx, double by, double bz, double vx, double vy, double vz, double rng4,
                                  const WeightParams& p, double& accr, double& accb) {
     const double X = __fma_rn(bx, bx, __fma_rn(by, by, __fma_rn(bz, bz, 4e-18)));
@@ -305,8 +305,7 @@ __device__ inline void rb_scaled(double bx, double by, double bz, double vx, dou
     accr = __fma_rn(ec, ec, accr);
     const double y = y0h * q;
     const double us = fmin(fmax(__fma_rn(pvs, y, p.cg), 0.0), 1.0);
-    double s = __fma_rn(p.c3, us, p.c2);
-    s = __fma_rn(s, us, p.c1);
+    double s = __fma_rn(p.c2, us, p.c1);
     s = __fma_rn(s, us, p.c0);
     accb = __fma_rn(us, s, accb);
 }
@@ -2102,7 +2101,7 @@ WeightParams make_weight_params(double sigma_r, double sigma_b, double gate) {
     p.cart_gate2 = gated ? g2 * 3.0 * sr * sr : INFINITY;
     p.rng_gate = gated ? fabs(gate) * sr : INFINITY;
     p.brg_gate2 = gated ? g2 * sb * sb : INFINITY;
-    // Bearing gate in u = 1 - cos(eb): the largest u whose 4-term series value is <= g^2 sb^2
+    // Bearing gate in u = 1 - cos(eb): the largest u whose 3-term series value is <= g^2 sb^2
     // (bisection over doubles with the kernel's own series), so that series(min(u, u_gate))
     // is the gated term; +inf when ungated.
     const double gb = gate * sb;
@@ -2110,20 +2109,20 @@ WeightParams make_weight_params(double sigma_r, double sigma_b, double gate) {
     if (gated && gb < 0.5) {
         const double target = p.brg_gate2;
         double lo = 0.0, hi = 2.0 * (1.0 - cos(gb)) + 1e-300;
-        while (acos_sq_u4(hi) <= target) hi *= 2.0;
+        while (acos_sq_u3(hi) <= target) hi *= 2.0;
         for (int it = 0; it < 2000; ++it) {
             const double mid = 0.5 * (lo + hi);
             if (!(mid > lo && mid < hi)) break;
-            if (acos_sq_u4(mid) <= target)
+            if (acos_sq_u3(mid) <= target)
                 lo = mid;
             else
                 hi = mid;
         }
         p.u_gate = lo;
     }
-    // Scaled range-bearing path when the bearing gate bounds u below the 4-term series' range:
-    // the series coefficients in us = u / u_gate (u acos-series(u) = us (c0 + c1 us + c2 us^2 +
-    // c3 us^3)), so that us = 1 yields the gated value.
+    // Scaled range-bearing path when the bearing gate bounds u below the 3-term series' range:
+    // the series coefficients in us = u / u_gate (u acos-series(u) = us (c0 + c1 us +
+    // c2 us^2)), so that us = 1 yields the gated value.
     p.fast_rb = (p.u_gate > 0.0 && p.u_gate <= 0.005) ? 1 : 0;
     p.pad_fast = 0;
     if (p.fast_rb) {
@@ -2132,7 +2131,6 @@ WeightParams make_weight_params(double sigma_r, double sigma_b, double gate) {
         p.c0 = 2.0 * ug;
         p.c1 = (8.0 / 24.0) * ug * ug;
         p.c2 = (16.0 / 180.0) * ug * ug * ug;
-        p.c3 = (32.0 / 1120.0) * ug * ug * ug * ug;
         p.rng_gate4 = 4.0 * p.rng_gate;
     }
     return p;

@@ -3,7 +3,9 @@
 Every test calls the HIP library through its C ABI and compares with oracle/pf_oracle.c
 (the C restatement of crates/pf_core/src/cpu.rs) on identical inputs:
   * reinit / predict poses          bit-exact (deterministic-draw mode, f32 storage)
-  * log-likelihoods                 |dll| <= 1e-9 + 1e-12 |ll|
+  * log-likelihoods                 |dll| <= 1e-9 + 1e-12 |ll|, plus on the gated range-bearing
+                                    path the truncation bound of its 3-term bearing series
+                                    (series_bound: below 1e-7 for 256 observations)
   * weights                         |dw| <= 1e-6 * w_max  (1e-5)
   * resampling ancestors, poses     bit-exact for identical weights and LCG state
   * injection picks                 bit-exact
@@ -141,13 +143,28 @@ def weight_pair(n, obs, lm, sr, sb, gate, seed=5, spread=1.0):
     return k, o
 
 
-def check_weights(k, o, ll_check=True):
+def series_bound(obs, sb, gate):
+    """Largest log-likelihood change the gated range-bearing path's 3-term acos^2 series can
+    make: its first omitted term, (32/1120) u^4, at u <= u_gate ~ 1 - cos(gate sb), times
+    1 / (2 sb^2), per range-bearing observation (10 % margin for the kernel's bisected u_gate);
+    zero where that path is off (no gate, or u_gate above 0.005)."""
+    sb = float(np.float32(sb))
+    if not gate > 0.0:
+        return 0.0
+    ug = 1.0 - math.cos(float(gate) * sb)
+    if ug > 0.005:
+        return 0.0
+    nrb = int(np.sum(np.asarray(obs)["mode"] == 1))
+    return 1.1 * nrb * (32.0 / 1120.0) * ug ** 4 / (2.0 * sb * sb)
+
+
+def check_weights(k, o, ll_check=True, bound=0.0):
     _, _, w = k.get_state()
     _, _, ow, oll = o.state()
     if ll_check:
         ll = k.get_ll()
         dll = np.abs(ll - oll)
-        tol = 1e-9 + 1e-12 * np.abs(oll)
+        tol = 1e-9 + 1e-12 * np.abs(oll) + bound
         assert np.all(dll <= tol), f"max |dll| {dll.max():.3e} at {np.argmax(dll - tol)}"
     dw = np.abs(w.astype(np.float64) - ow)
     assert dw.max() <= 1e-6 * ow.max(), f"max |dw| {dw.max():.3e} vs w_max {ow.max():.3e}"
@@ -163,7 +180,7 @@ def test_weight_range_bearing_parity(gpu, gate, spread):
     q_t = (math.cos(0.4), 0.0, 0.0, math.sin(0.4))
     obs = rb_obs_for(np.array([10.0, 10.0, 2.5]), q_t, lm, 2)
     k, o = weight_pair(n, obs, lm, 0.05, 0.02, gate, spread=spread)
-    check_weights(k, o)
+    check_weights(k, o, bound=series_bound(obs, 0.02, gate))
 
 
 @pytest.mark.parametrize("gate", [0.0, 3.0])
@@ -190,7 +207,7 @@ def test_weight_256_obs_outliers_parity(gpu):
     k.upload_landmarks(lm)
     k.weight(obs, 0.05, 0.02, 3.0)
     o.weight(obs, lm, float(np.float32(0.05)), float(np.float32(0.02)), 3.0)
-    check_weights(k, o)
+    check_weights(k, o, bound=series_bound(obs, 0.02, 3.0))
 
 
 def test_weight_mixed_modes_and_unknown_ids(gpu):
@@ -276,7 +293,7 @@ def test_weight_variants_match_oracle(gpu, variant, case):
     _, _, ow, oll = o.state()
     dll = np.abs(ll - oll)
     print(f"variant {variant} {case}: max |dll| {dll.max():.3e} (|ll| max {np.abs(oll).max():.3e})")
-    assert np.all(dll <= 1e-9 + 1e-12 * np.abs(oll)), f"max |dll| {dll.max():.3e}"
+    assert np.all(dll <= 1e-9 + 1e-12 * np.abs(oll) + series_bound(obs, 0.02, gate)), f"max |dll| {dll.max():.3e}"
     _, _, w = k.get_state()
     assert np.abs(w.astype(np.float64) - ow).max() <= 1e-6 * ow.max()
 
