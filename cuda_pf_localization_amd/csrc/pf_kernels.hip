@@ -343,7 +343,11 @@ __device__ inline void cart_from_b(double bx, double by, double bz, double vx, d
 }
 
 // Covariance partials of one 1024-particle tile, and the covariance finalize of one filter by a
-// 1024-thread block (both defined with the estimate kernels below).
+// 1024-thread block (both defined with the estimate kernels below).  kWave: each wave stores its
+// own partial and the finalize adds a tile's four in the block tree's order (no barrier in the
+// tile's code: a barrier there makes the compiler keep a prefetched observation record of the
+// likelihood loop, which shares the kernel, in vector registers).
+template <bool kWave>
 __device__ void covs_tile(const Layout& L, FilterDev* F, const PoseBufs& B, int swapped, const float* __restrict__ w,
                           double* __restrict__ part, int stride, const ParamRef<CovParams>& A, int blk);
 __device__ void cov_fin_block(const Layout& L, FilterDev* F, const double* __restrict__ part, int stride,
@@ -370,7 +374,7 @@ void k_weight(Layout L, PoseBufs B, FilterDev* F, double* __restrict__ ll, doubl
     const int blk = blockIdx.x;
     if constexpr (kDefer) {
         if (blk >= nblk) {
-            covs_tile(L, F, B, 1, cw, cpart, cstride, cA, blk - nblk);
+            covs_tile<true>(L, F, B, 1, cw, cpart, cstride, cA, blk - nblk);
             return;
         }
     }
@@ -1181,7 +1185,7 @@ struct Fold {
     static constexpr int H1 = (NV + 1) / 2, H2 = (H1 + 1) / 2;
 };
 template <int NV>
-__device__ inline void wave_fold_store(const double (&acc)[NV], double* __restrict__ row) {
+__device__ inline void wave_fold_store(const double (&acc)[NV], double* __restrict__ row, int rs = 1) {
     constexpr int H1 = Fold<NV>::H1, H2 = Fold<NV>::H2;
     const int lane = threadIdx.x & 63;
     double s1[H1], s2[H2];
@@ -1202,7 +1206,7 @@ __device__ inline void wave_fold_store(const double (&acc)[NV], double* __restri
         const int end = r == 0 ? H2 : r == 1 ? H1 : r == 2 ? (H1 + H2 < NV ? H1 + H2 : NV) : NV;
 #pragma unroll
         for (int i = 0; i < H2; ++i)
-            if (base + i < end) row[base + i] = s2[i];
+            if (base + i < end) row[(size_t)(base + i) * rs] = s2[i];
     }
 }
 
@@ -1592,6 +1596,40 @@ __device__ inline void fin_lane_sums(const double* __restrict__ pf, int stride, 
     }
 }
 
+// The same sums over per-wave partials (4 per tile, value k of tile b's wave v at
+// k * 4 stride + 4 b + v): each tile's value is formed as (v0 + v1) + (v2 + v3), block_partials'
+// order, before it joins the lane's sum, so both layouts give the same bits.
+template <int NV, int kU>
+__device__ inline void fin_lane_sums_w4(const double* __restrict__ pf, int stride, int nb, int vt,
+                                        double (&acc)[NV]) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) acc[k] = 0.0;
+    for (int c = 0; c < nb; c += 4 * kEstFinTB) {
+#pragma unroll 1
+        for (int u0 = 0; u0 < 4; u0 += kU) {
+            double x[kU][NV];
+#pragma unroll
+            for (int u = 0; u < kU; ++u) {
+                const int b = c + vt + (u0 + u) * kEstFinTB;
+#pragma unroll
+                for (int k = 0; k < NV; ++k) {
+                    if (b < nb) {
+                        const double2 lo = *reinterpret_cast<const double2*>(pf + (size_t)k * 4 * stride + 4 * b);
+                        const double2 hi = *reinterpret_cast<const double2*>(pf + (size_t)k * 4 * stride + 4 * b + 2);
+                        x[u][k] = (lo.x + lo.y) + (hi.x + hi.y);
+                    } else {
+                        x[u][k] = 0.0;
+                    }
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kU; ++u)
+#pragma unroll
+                for (int k = 0; k < NV; ++k) acc[k] += x[u][k];
+        }
+    }
+}
+
 template <int NV>
 __global__ __launch_bounds__(kEstFinTB) void k_est_fin(Layout L, FilterDev* F, const double* __restrict__ part,
                                                        int stride, int which, CovParams* cov, int flip,
@@ -1644,7 +1682,7 @@ __device__ void cov_fin_block(const Layout& L, FilterDev* F, const double* __res
     const int t = (int)threadIdx.x;
     if (t < kEstFinTB) {
         double acc[kCovN];
-        fin_lane_sums<kCovN, 4>(part + b0, stride, nb, t, acc);
+        fin_lane_sums_w4<kCovN, 2>(part + 4 * b0, stride, nb, t, acc);
         wave_fold_store<kCovN>(acc, shv[t >> 6]);
     }
     __syncthreads();
@@ -1695,6 +1733,7 @@ __global__ __launch_bounds__(kTB) void k_est_sums(Layout L, FilterDev* F, PoseBu
 
 // cov layout (cu:406-471): [0..5] position, [6..11] attitude tangent (axis-angle of qm^-1 q,
 // canonical sign, angle = 2 atan2(|v|, w)).
+template <bool kWave>
 __device__ void covs_tile(const Layout& L, FilterDev* F, const PoseBufs& B, int swapped, const float* __restrict__ w,
                           double* __restrict__ part, int stride, const ParamRef<CovParams>& A, int blk) {
     const int f = layout_filter(L, blk * (kTile / kTB));
@@ -1778,12 +1817,15 @@ __device__ void covs_tile(const Layout& L, FilterDev* F, const PoseBufs& B, int 
             }
         }
     }
-    block_partials<kCovN>(acc, part, blk, stride);
+    if constexpr (kWave)
+        wave_fold_store<kCovN>(acc, part + 4 * blk + (threadIdx.x >> 6), 4 * stride);
+    else
+        block_partials<kCovN>(acc, part, blk, stride);
 }
 
 __global__ __launch_bounds__(kTB) void k_est_covs(Layout L, FilterDev* F, PoseBufs B, const float* __restrict__ w,
                                                   double* __restrict__ part, int stride, ParamRef<CovParams> A) {
-    covs_tile(L, F, B, 0, w, part, stride, A, blockIdx.x);
+    covs_tile<false>(L, F, B, 0, w, part, stride, A, blockIdx.x);
 }
 
 // Sets the per-filter resample predicate: force (ABI call) or ESS < ratio * n (fused cycle).
@@ -1854,7 +1896,7 @@ int state_alloc(PfState* s, const std::vector<int>& ns) {
     PF_CHECK(hipMalloc((void**)&s->part_q, sizeof(uint64_t) * (size_t)ntile));
     PF_CHECK(hipMalloc((void**)&s->tcum, sizeof(uint64_t) * (size_t)ntile));
     PF_CHECK(hipMalloc((void**)&s->part_est, sizeof(double) * kEstN * (size_t)ntile));
-    PF_CHECK(hipMalloc((void**)&s->part_cov, sizeof(double) * kCovN * (size_t)ntile));
+    PF_CHECK(hipMalloc((void**)&s->part_cov, sizeof(double) * kCovN * 4 * (size_t)ntile));  // per-wave layout
     PF_CHECK(hipMalloc((void**)&s->scan_st, sizeof(uint64_t) * 2 * (size_t)ntile));
     PF_CHECK(hipMemsetAsync(s->scan_st, 0, sizeof(uint64_t) * 2 * (size_t)ntile, s->stream));
     PF_CHECK(hipMalloc((void**)&s->scan_ctl, sizeof(unsigned long long)));
