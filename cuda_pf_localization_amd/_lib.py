@@ -104,6 +104,7 @@ KERNEL_SYMBOLS = [
     ("pfx_get_state", C.c_int, [vp, P(f32), P(f32), P(f32)]),
     ("pfx_get_ll", C.c_int, [vp, P(f64)]),
     ("pfx_get_ancestors", C.c_int, [vp, P(i32)]),
+    ("pfx_get_cum", C.c_int, [vp, P(u64)]),
     ("pfx_set_lcg", C.c_int, [vp, u64]),
     ("pfx_get_lcg", C.c_int, [vp, P(u64)]),
     ("pfx_get_rng", C.c_int, [vp, P(u64), P(u64)]),

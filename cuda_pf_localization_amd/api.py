@@ -207,6 +207,12 @@ class PfKernels:
         _check(self.L.pfx_get_ancestors(self.h, _ip(a)), "pfx_get_ancestors")
         return a
 
+    def cum(self):
+        """The last resampling scan's inclusive fixed-point prefix (u64 per particle)."""
+        c = np.zeros(self.n, np.uint64)
+        _check(self.L.pfx_get_cum(self.h, c.ctypes.data_as(_lib.P(_lib.u64))), "pfx_get_cum")
+        return c
+
     def set_lcg(self, v: int):
         _check(self.L.pfx_set_lcg(self.h, v), "pfx_set_lcg")
 

@@ -85,8 +85,23 @@ __device__ inline void det_sincos(double x, double& so, double& co) {
     }
 }
 
-// cos(h) and sin(h)/h from h2 = h^2: Taylor in h2 after halving h2 <= 1, then doubling.
+// cos(h) and sin(h)/h from h2 = h^2.  Small angles (h2 <= 2^-10, every predict draw at
+// rotation noise up to ~0.05 rad): five Taylor terms each, whose first omitted terms are below
+// 1e-21 relative.  Otherwise Taylor to h^20 after halving h2 <= 1, then doubling.
 __device__ inline void det_cos_sinc(double h2, double& co, double& sinc) {
+    if (h2 <= 0x1.0p-10) {
+        double c = __fma_rn(h2, -1.0 / 3628800.0, 1.0 / 40320.0);
+        c = __fma_rn(c, -h2, 1.0 / 720.0);
+        c = __fma_rn(c, -h2, 1.0 / 24.0);
+        c = __fma_rn(c, -h2, 0.5);
+        co = __fma_rn(c, -h2, 1.0);
+        double s = __fma_rn(h2, -1.0 / 39916800.0, 1.0 / 362880.0);
+        s = __fma_rn(s, -h2, 1.0 / 5040.0);
+        s = __fma_rn(s, -h2, 1.0 / 120.0);
+        s = __fma_rn(s, -h2, 1.0 / 6.0);
+        sinc = __fma_rn(s, -h2, 1.0);
+        return;
+    }
     int m = 0;
     while (h2 > 1.0 && m < 64) {
         h2 *= 0.25;
@@ -122,80 +137,82 @@ __device__ inline void det_cos_sinc(double h2, double& co, double& sinc) {
     sinc = s;
 }
 
-// Motion-noise draws.  Normals only need f32-level accuracy (the poses they perturb are stored
-// as f32), so the log, square root and sine/cosine of the Box-Muller transform are short
-// fma/mul/add sequences (~1e-9 relative) evaluated from the integer words directly.
+// Motion-noise draws.  The normals perturb poses stored as f32 (the reference's GPU path draws
+// f32 normals), so the Box-Muller transform runs in f32 arithmetic, which gfx950 issues at twice
+// the f64 rate: -2 ln, the square root and sine/cosine are short fmaf/mul/add sequences
+// (~1e-7 relative) evaluated from the integer words.  Every operation is an IEEE f32 operation
+// (correctly rounded fmaf, *, +, conversions), so a CPU evaluation of the same sequence is
+// bit-identical.
 
-// -2 ln((a + 1) 2^-32): k = a + 1 = 2^e m, m in [sqrt(1/2), sqrt(2)), ln m = 2 atanh(s) with
-// s = (m - 1)/(m + 1); 1/(m + 1) from a linear minimax seed (relative error 0.015) and two
-// Newton steps; the atanh series to s^9.
-__device__ inline double mot_neg2ln(uint32_t a) {
-    const double k = (double)a + 1.0;
-    const uint64_t b = __double_as_longlong(k);
-    int e = (int)(b >> 52) - 1023;
-    double m = __longlong_as_double((long long)((b & 0x000FFFFFFFFFFFFFull) | 0x3FF0000000000000ull));
-    if (m > 1.4142135623730951) {
-        m = m * 0.5;
+// -2 ln((a + 1) 2^-32) in f32: k = (float)a + 1 = 2^e m, m in [sqrt(1/2), sqrt(2)),
+// ln m = 2 atanh(s) with s = (m - 1)/(m + 1); 1/(m + 1) from a linear minimax seed (relative
+// error 0.015) and two Newton steps; the atanh series to s^9.
+__device__ inline float mot_neg2ln(uint32_t a) {
+    const float k = __fadd_rn(__uint2float_rn(a), 1.0f);
+    const uint32_t b = __float_as_uint(k);
+    int e = (int)(b >> 23) - 127;
+    float m = __uint_as_float((b & 0x007FFFFFu) | 0x3F800000u);
+    if (m > 0x1.6a09e6p+0f) {  // f32 sqrt(2)
+        m = __fmul_rn(m, 0.5f);
         e += 1;
     }
-    const double f = m - 1.0;
-    const double d = m + 1.0;
-    double y = __fma_rn(d, -0.23901599922648442, 0.9850615000483455);
-    y = __fma_rn(y, __fma_rn(-d, y, 1.0), y);
-    y = __fma_rn(y, __fma_rn(-d, y, 1.0), y);
-    const double s = f * y;
-    const double s2 = s * s;
-    double p = __fma_rn(s2, 1.0 / 9.0, 1.0 / 7.0);
-    p = __fma_rn(p, s2, 1.0 / 5.0);
-    p = __fma_rn(p, s2, 1.0 / 3.0);
-    const double h = __fma_rn(s * s2, p, s);  // ln(m) / 2
-    return __fma_rn((double)(32 - e), 1.3862943611198906, -4.0 * h);
+    const float f = __fsub_rn(m, 1.0f);
+    const float d = __fadd_rn(m, 1.0f);
+    float y = __fmaf_rn(d, -0.23901600f, 0.98506150f);
+    y = __fmaf_rn(y, __fmaf_rn(-d, y, 1.0f), y);
+    y = __fmaf_rn(y, __fmaf_rn(-d, y, 1.0f), y);
+    const float s = __fmul_rn(f, y);
+    const float s2 = __fmul_rn(s, s);
+    float p = __fmaf_rn(s2, 1.0f / 9.0f, 1.0f / 7.0f);
+    p = __fmaf_rn(p, s2, 1.0f / 5.0f);
+    p = __fmaf_rn(p, s2, 1.0f / 3.0f);
+    const float h = __fmaf_rn(__fmul_rn(s, s2), p, s);  // ln(m) / 2
+    return __fmaf_rn(__int2float_rn(32 - e), 0x1.62e430p+0f, __fmul_rn(-4.0f, h));  // f32 2 ln 2
 }
 
-// sqrt(x), x in [0, 45]: three Newton steps on the reciprocal square root from the integer seed
-// (~1e-11 relative); x = 0 gives 0 (the seed stays finite).
-__device__ inline double mot_sqrt(double x) {
-    double y = __longlong_as_double((long long)(0x5FE6EB50C7B537A9ull - (__double_as_longlong(x) >> 1)));
-    const double h = 0.5 * x;
+// sqrt(x), x in [0, 45], in f32: three Newton steps on the reciprocal square root from the
+// integer seed (~1e-7 relative after f32 rounding); x = 0 gives 0 (the seed stays finite).
+__device__ inline float mot_sqrt(float x) {
+    float y = __uint_as_float(0x5F3759DFu - (__float_as_uint(x) >> 1));
+    const float h = __fmul_rn(0.5f, x);
 #pragma unroll
     for (int it = 0; it < 3; ++it) {
-        const double t = h * y;
-        const double r = __fma_rn(-t, y, 0.5);
-        y = __fma_rn(y, r, y);
+        const float t = __fmul_rn(h, y);
+        const float r = __fmaf_rn(-t, y, 0.5f);
+        y = __fmaf_rn(y, r, y);
     }
-    return x * y;
+    return __fmul_rn(x, y);
 }
 
-// sin and cos of the angle 2 pi b / 2^32: octant o = b >> 29, the 29-bit remainder reflected in
-// odd octants, x = remainder * pi / 2^31 in [0, pi/4]; Taylor sin to x^9, cos to x^10 (~2e-9);
-// the octant picks, swaps and negates.
-__device__ inline void mot_sincos_turn(uint32_t b, double& so, double& co) {
+// sin and cos of the angle 2 pi b / 2^32 in f32: octant o = b >> 29, the 29-bit remainder
+// reflected in odd octants, x = remainder * pi / 2^31 in [0, pi/4]; Taylor sin to x^9, cos to
+// x^8 (first omitted terms below 3e-8); the octant picks, swaps and negates.
+__device__ inline void mot_sincos_turn(uint32_t b, float& so, float& co) {
     const uint32_t o = b >> 29;
     const uint32_t fr = b & 0x1FFFFFFFu;
     const uint32_t odd = o & 1u;
     const uint32_t fx = odd ? 0x20000000u - fr : fr;
-    const double x = (double)fx * 1.4629180792671596e-09;
-    const double x2 = x * x;
-    double ps = __fma_rn(x2, 1.0 / 362880.0, -1.0 / 5040.0);
-    ps = __fma_rn(ps, x2, 1.0 / 120.0);
-    ps = __fma_rn(ps, x2, -1.0 / 6.0);
-    const double sx = __fma_rn(x * x2, ps, x);
-    double pc = __fma_rn(x2, -1.0 / 3628800.0, 1.0 / 40320.0);
-    pc = __fma_rn(pc, x2, -1.0 / 720.0);
-    pc = __fma_rn(pc, x2, 1.0 / 24.0);
-    pc = __fma_rn(pc, x2, -0.5);
-    const double cx = __fma_rn(x2, pc, 1.0);
+    const float x = __fmul_rn(__uint2float_rn(fx), 0x1.921fb6p-30f);  // f32 pi / 2^31
+    const float x2 = __fmul_rn(x, x);
+    float ps = __fmaf_rn(x2, 1.0f / 362880.0f, -1.0f / 5040.0f);
+    ps = __fmaf_rn(ps, x2, 1.0f / 120.0f);
+    ps = __fmaf_rn(ps, x2, -1.0f / 6.0f);
+    const float sx = __fmaf_rn(__fmul_rn(x, x2), ps, x);
+    float pc = __fmaf_rn(x2, 1.0f / 40320.0f, -1.0f / 720.0f);
+    pc = __fmaf_rn(pc, x2, 1.0f / 24.0f);
+    pc = __fmaf_rn(pc, x2, -0.5f);
+    const float cx = __fmaf_rn(x2, pc, 1.0f);
     const uint32_t q = o >> 1;
     const uint32_t swap = odd ^ (q & 1u);
-    const double sv = swap ? cx : sx;
-    const double cv = swap ? sx : cx;
+    const float sv = swap ? cx : sx;
+    const float cv = swap ? sx : cx;
     so = (q >> 1) ? -sv : sv;
     co = ((q ^ (q >> 1)) & 1u) ? -cv : cv;
 }
 
 // Six standard normals of particle i: the particle key and the next two outputs of its
 // splitmix64 stream give six 32-bit words; Box-Muller pair p takes its radius from the high
-// half of value p, its angle from the low half.
+// half of value p, its angle from the low half.  The f32 products are widened to f64 exactly.
 __device__ inline void particle_normals(uint64_t gseed, uint64_t i, double n6[6]) {
     uint64_t x = particle_key(gseed, i);
     uint64_t z[3];
@@ -204,11 +221,16 @@ __device__ inline void particle_normals(uint64_t gseed, uint64_t i, double n6[6]
     z[2] = sm_next(x);
 #pragma unroll
     for (int p = 0; p < 3; ++p) {
-        const double r = mot_sqrt(mot_neg2ln((uint32_t)(z[p] >> 32)));
-        double sn, cs;
+        // The radius word as an opaque 32-bit register: the empty asm emits nothing, but without
+        // it the compiler sees (float)(uint32_t)(z >> 32) as a 64-bit conversion and expands it to
+        // five instructions instead of one v_cvt_f32_u32 (same value either way).
+        uint32_t hi = (uint32_t)(z[p] >> 32);
+        asm("" : "+v"(hi));
+        const float r = mot_sqrt(mot_neg2ln(hi));
+        float sn, cs;
         mot_sincos_turn((uint32_t)z[p], sn, cs);
-        n6[2 * p] = r * cs;
-        n6[2 * p + 1] = r * sn;
+        n6[2 * p] = (double)__fmul_rn(r, cs);
+        n6[2 * p + 1] = (double)__fmul_rn(r, sn);
     }
 }
 

@@ -2482,6 +2482,14 @@ int pfx_get_ancestors(void* h, int32_t* anc) {
     return 0;
 }
 
+int pfx_get_cum(void* h, uint64_t* cum) {
+    PfState* s = static_cast<PfState*>(h);
+    if (!s || s->nf != 1 || !cum) return -1;
+    PF_CHECK(hipStreamSynchronize(s->stream));
+    PF_CHECK(hipMemcpy(cum, s->cum, sizeof(uint64_t) * (size_t)s->n[0], hipMemcpyDeviceToHost));
+    return 0;
+}
+
 int pfx_set_lcg(void* h, uint64_t lcg) {
     PfState* s = static_cast<PfState*>(h);
     if (!s || s->nf < 1) return -1;

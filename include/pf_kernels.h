@@ -143,6 +143,9 @@ int pfx_get_state(void *h, float *pos3n, float *quat4n, float *w);
 int pfx_get_ll(void *h, double *ll);
 /* Ancestor index of every output particle of the last resample (n ints). */
 int pfx_get_ancestors(void *h, int32_t *anc);
+/* Inclusive fixed-point prefix of the last resampling scan (n u64: cum_i = sum_{j<=i}
+ * floor(w_j 2^56)), from which the weights the scan used are recovered exactly. */
+int pfx_get_cum(void *h, uint64_t *cum);
 /* Sets / reads the resampling LCG state (the u0 and injection stream). */
 int pfx_set_lcg(void *h, uint64_t lcg);
 int pfx_get_lcg(void *h, uint64_t *lcg);

@@ -29,9 +29,9 @@ _SIGS = [
     ("or_det_sincos", None, [f64, P(f64), P(f64)]),
     ("or_det_cos_sinc", None, [f64, P(f64), P(f64)]),
     ("or_particle_normals", None, [u64, u64, P(f64)]),
-    ("or_mot_neg2ln", f64, [C.c_uint32]),
-    ("or_mot_sqrt", f64, [f64]),
-    ("or_mot_sincos_turn", None, [C.c_uint32, P(f64), P(f64)]),
+    ("or_mot_neg2ln", C.c_float, [C.c_uint32]),
+    ("or_mot_sqrt", C.c_float, [C.c_float]),
+    ("or_mot_sincos_turn", None, [C.c_uint32, P(C.c_float), P(C.c_float)]),
     ("or_create", vp, [C.c_int, u64, C.c_int]),
     ("or_destroy", None, [vp]),
     ("or_set_rng", None, [vp, u64, u64, u64]),

@@ -21,8 +21,9 @@
  *   words   = key, then the splitmix64 outputs of key + phi, key + 2 phi (cu:323-328 mixer),
  *             each split into its high and low 32 bits;
  *   normals = Box-Muller on the three (hi, lo) pairs: radius sqrt(-2 ln((hi + 1) 2^-32)),
- *             angle 2 pi lo 2^-32, with the short fma-only log, sqrt and sincos below
- *             (~1e-9 relative), so oracle and GPU produce bit-identical draws.
+ *             angle 2 pi lo 2^-32, in f32 arithmetic with the short fmaf-only log, sqrt and
+ *             sincos below (~1e-7 relative), widened to f64, so oracle and GPU produce
+ *             bit-identical draws.
  * With f32 storage on (the GPU's storage precision), poses and weights are rounded to
  * f32 at the end of every stage, so per-stage parity can be checked against the GPU on
  * identical inputs.
@@ -107,8 +108,22 @@ void or_det_sincos(double x, double *so, double *co) {
     }
 }
 
-/* cos(h) and sin(h)/h from h2 = h*h (Taylor in h2 after halving to h2 <= 1). */
+/* cos(h) and sin(h)/h from h2 = h*h.  h2 <= 2^-10: five Taylor terms each (first omitted terms
+ * below 1e-21 relative); otherwise Taylor to h^20 after halving to h2 <= 1, then doubling. */
 void or_det_cos_sinc(double h2, double *co, double *sinc) {
+    if (h2 <= 0x1.0p-10) {
+        double c = fma(h2, -1.0 / 3628800.0, 1.0 / 40320.0);
+        c = fma(c, -h2, 1.0 / 720.0);
+        c = fma(c, -h2, 1.0 / 24.0);
+        c = fma(c, -h2, 0.5);
+        *co = fma(c, -h2, 1.0);
+        double s = fma(h2, -1.0 / 39916800.0, 1.0 / 362880.0);
+        s = fma(s, -h2, 1.0 / 5040.0);
+        s = fma(s, -h2, 1.0 / 120.0);
+        s = fma(s, -h2, 1.0 / 6.0);
+        *sinc = fma(s, -h2, 1.0);
+        return;
+    }
     int m = 0;
     while (h2 > 1.0 && m < 64) {
         h2 *= 0.25;
@@ -144,76 +159,77 @@ void or_det_cos_sinc(double h2, double *co, double *sinc) {
     *sinc = s;
 }
 
-/* -2 ln((a + 1) 2^-32) for a 32-bit word a, ~1e-9 relative: k = a + 1 = 2^e m with m in
+/* The motion draws' Box-Muller transform in f32 arithmetic (each step one IEEE f32 operation:
+ * fmaf, *, +, -, conversions; C evaluates float expressions in float on x86-64).
+ * -2 ln((a + 1) 2^-32) for a 32-bit word a, ~1e-7 relative: k = (float)a + 1 = 2^e m with m in
  * [sqrt(1/2), sqrt(2)), ln m = 2 atanh(s), s = (m - 1)/(m + 1); the reciprocal from a linear
  * minimax seed on [1 + sqrt(1/2), 1 + sqrt(2)] (relative error 0.015) and two Newton steps, the
  * atanh series to s^9. */
-double or_mot_neg2ln(uint32_t a) {
-    double k = (double)a + 1.0;
-    uint64_t b;
-    memcpy(&b, &k, 8);
-    int e = (int)(b >> 52) - 1023;
-    b = (b & 0x000FFFFFFFFFFFFFull) | 0x3FF0000000000000ull;
-    double m;
-    memcpy(&m, &b, 8);
-    if (m > 1.4142135623730951) {
-        m = m * 0.5;
+float or_mot_neg2ln(uint32_t a) {
+    float k = (float)a + 1.0f;
+    uint32_t b;
+    memcpy(&b, &k, 4);
+    int e = (int)(b >> 23) - 127;
+    b = (b & 0x007FFFFFu) | 0x3F800000u;
+    float m;
+    memcpy(&m, &b, 4);
+    if (m > 0x1.6a09e6p+0f) { /* f32 sqrt(2) */
+        m = m * 0.5f;
         e += 1;
     }
-    double f = m - 1.0;
-    double d = m + 1.0;
-    double y = fma(d, -0.23901599922648442, 0.9850615000483455);
-    y = fma(y, fma(-d, y, 1.0), y);
-    y = fma(y, fma(-d, y, 1.0), y);
-    double s = f * y;
-    double s2 = s * s;
-    double p = fma(s2, 1.0 / 9.0, 1.0 / 7.0);
-    p = fma(p, s2, 1.0 / 5.0);
-    p = fma(p, s2, 1.0 / 3.0);
-    double h = fma(s * s2, p, s); /* ln(m) / 2 */
-    return fma((double)(32 - e), 1.3862943611198906, -4.0 * h);
+    float f = m - 1.0f;
+    float d = m + 1.0f;
+    float y = fmaf(d, -0.23901600f, 0.98506150f);
+    y = fmaf(y, fmaf(-d, y, 1.0f), y);
+    y = fmaf(y, fmaf(-d, y, 1.0f), y);
+    float s = f * y;
+    float s2 = s * s;
+    float p = fmaf(s2, 1.0f / 9.0f, 1.0f / 7.0f);
+    p = fmaf(p, s2, 1.0f / 5.0f);
+    p = fmaf(p, s2, 1.0f / 3.0f);
+    float h = fmaf(s * s2, p, s); /* ln(m) / 2 */
+    return fmaf((float)(32 - e), 0x1.62e430p+0f, -4.0f * h); /* f32 2 ln 2 */
 }
 
-/* sqrt(x) for x in [0, 45] by three Newton steps on the reciprocal square root from the integer
- * seed (~1e-11 relative); x = 0 gives 0 (the seed stays finite). */
-double or_mot_sqrt(double x) {
-    uint64_t b;
-    memcpy(&b, &x, 8);
-    b = 0x5FE6EB50C7B537A9ull - (b >> 1);
-    double y;
-    memcpy(&y, &b, 8);
-    double h = 0.5 * x;
+/* sqrt(x) for x in [0, 45] in f32 by three Newton steps on the reciprocal square root from the
+ * integer seed (~1e-7 relative); x = 0 gives 0 (the seed stays finite). */
+float or_mot_sqrt(float x) {
+    uint32_t b;
+    memcpy(&b, &x, 4);
+    b = 0x5F3759DFu - (b >> 1);
+    float y;
+    memcpy(&y, &b, 4);
+    float h = 0.5f * x;
     for (int it = 0; it < 3; ++it) {
-        double t = h * y;
-        double r = fma(-t, y, 0.5);
-        y = fma(y, r, y);
+        float t = h * y;
+        float r = fmaf(-t, y, 0.5f);
+        y = fmaf(y, r, y);
     }
     return x * y;
 }
 
-/* sin and cos of the turn fraction b 2^-32 (angle 2 pi b / 2^32) from the word's bits: octant
- * o = b >> 29, the 29-bit remainder reflected in odd octants, x in [0, pi/4] by one product;
- * Taylor sin to x^9, cos to x^10 (~2e-9). */
-void or_mot_sincos_turn(uint32_t b, double *so, double *co) {
+/* sin and cos of the turn fraction b 2^-32 (angle 2 pi b / 2^32) in f32 from the word's bits:
+ * octant o = b >> 29, the 29-bit remainder reflected in odd octants, x in [0, pi/4] by one
+ * product; Taylor sin to x^9, cos to x^8 (first omitted terms below 3e-8). */
+void or_mot_sincos_turn(uint32_t b, float *so, float *co) {
     uint32_t o = b >> 29;
     uint32_t fr = b & 0x1FFFFFFFu;
     uint32_t odd = o & 1u;
     uint32_t fx = odd ? 0x20000000u - fr : fr;
-    double x = (double)fx * 1.4629180792671596e-09; /* pi / 2^31 */
-    double x2 = x * x;
-    double ps = fma(x2, 1.0 / 362880.0, -1.0 / 5040.0);
-    ps = fma(ps, x2, 1.0 / 120.0);
-    ps = fma(ps, x2, -1.0 / 6.0);
-    double sx = fma(x * x2, ps, x);
-    double pc = fma(x2, -1.0 / 3628800.0, 1.0 / 40320.0);
-    pc = fma(pc, x2, -1.0 / 720.0);
-    pc = fma(pc, x2, 1.0 / 24.0);
-    pc = fma(pc, x2, -0.5);
-    double cx = fma(x2, pc, 1.0);
+    float x = (float)fx * 0x1.921fb6p-30f; /* f32 pi / 2^31 */
+    float x2 = x * x;
+    float ps = fmaf(x2, 1.0f / 362880.0f, -1.0f / 5040.0f);
+    ps = fmaf(ps, x2, 1.0f / 120.0f);
+    ps = fmaf(ps, x2, -1.0f / 6.0f);
+    float sx = fmaf(x * x2, ps, x);
+    float pc = fmaf(x2, 1.0f / 40320.0f, -1.0f / 720.0f);
+    pc = fmaf(pc, x2, 1.0f / 24.0f);
+    pc = fmaf(pc, x2, -0.5f);
+    float cx = fmaf(x2, pc, 1.0f);
     uint32_t q = o >> 1;
     uint32_t swap = odd ^ (q & 1u);
-    double sv = swap ? cx : sx;
-    double cv = swap ? sx : cx;
+    float sv = swap ? cx : sx;
+    float cv = swap ? sx : cx;
     *so = (q >> 1) ? -sv : sv;
     *co = ((q ^ (q >> 1)) & 1u) ? -cv : cv;
 }
@@ -229,11 +245,11 @@ void or_particle_normals(uint64_t gseed, uint64_t i, double n6[6]) {
     z[1] = sm64_next(&x);
     z[2] = sm64_next(&x);
     for (int p = 0; p < 3; ++p) {
-        double r = or_mot_sqrt(or_mot_neg2ln((uint32_t)(z[p] >> 32)));
-        double s, c;
+        float r = or_mot_sqrt(or_mot_neg2ln((uint32_t)(z[p] >> 32)));
+        float s, c;
         or_mot_sincos_turn((uint32_t)z[p], &s, &c);
-        n6[2 * p] = r * c;
-        n6[2 * p + 1] = r * s;
+        n6[2 * p] = (double)(r * c);
+        n6[2 * p + 1] = (double)(r * s);
     }
 }
 

@@ -10,7 +10,9 @@ when ESS < N/2 (particle_filter.rs:111-124, cpu.rs:220-305).  Three drivers are 
     range-bearing path's 3-term bearing series (series_bound), weights within 1e-5 relative, estimate
     sums and covariances within 1e-9 of the oracle's f64 sums;
   * the C++ host layer (particle_filter.rs:110-124) in the reference's exact call order and in
-    the fast order, against the oracle driver with no synchronisation at all;
+    the fast order, against an oracle driver that runs its own predict, likelihood and estimate
+    and only takes, for its resampling, the weights the GPU's scan used (recovered exactly from
+    the scan's fixed-point prefix, pfx_get_cum, after a 1e-5 check against its own);
   * the fused cycle (pfb_cycle, hipGraph), against the same oracle driver.
 Config 3: the 64-filter batch (N log-uniform in [1k, 500k], L uniform in [4, 256]) in one
 launch per stage; the smallest, the largest, the fewest-landmark filter and a filter given 4
@@ -62,6 +64,36 @@ def assert_weights_close(w, ow, what):
     dw = np.abs(w - ow)
     bad = dw > 1e-5 * ow + 1e-38
     assert not bad.any(), f"{what}: {bad.sum()} weights off by more than 1e-5 relative, worst {dw.max():.3e}"
+
+
+def scan_weights(cum):
+    """The weights a resampling scan used, recovered exactly from its inclusive fixed-point
+    prefix: q = floor(w 2^56) of an f32 weight has at most 24 significant bits."""
+    return np.diff(cum, prepend=np.uint64(0)).astype(np.float64) * 2.0 ** -56
+
+
+def assert_scan_weights_close(w_used, ow, what):
+    big = ow >= 2.0 ** -32
+    assert np.all(np.abs(w_used[big] - ow[big]) <= 1e-5 * ow[big]), f"{what}: scan weights"
+    assert np.all(np.abs(w_used[~big] - ow[~big]) <= 2.0 ** -56 + 1e-5 * ow[~big]), f"{what}: scan weights"
+
+
+def oracle_cycle_on_gpu_weights(o, st, lm, sr, sb, gate, ratio, inject, n, gpu_cum, what):
+    """oracle_cycle with the resampling run on the weights the GPU's scan used (gpu_cum() reads
+    its prefix), after checking them against the oracle's own within 1e-5: two independent f64
+    likelihoods agree to ~1e-8, and weights that close can still move a systematic pointer
+    across a particle boundary, so ancestors are compared on identical weights and offset."""
+    o.predict(f32(st.odom_t), f32(st.odom_q), f32(st.trans_noise), f32(st.rot_noise))
+    o.weight(obs_array(st), lm, sr, sb, gate)
+    e = o.estimate()
+    if not (ratio > 0 and e["ess"] < ratio * n):
+        return e, False
+    opos, oquat, ow, _ = o.state()
+    w_used = scan_weights(gpu_cum())
+    assert_scan_weights_close(w_used, ow, what)
+    o.set_state(opos, oquat, w_used)
+    o.resample(np.float32(inject))
+    return o.estimate(), True
 
 
 def oracle_cycle(o, st, lm, sr, sb, gate, ratio, inject, n):
@@ -178,7 +210,11 @@ def test_config1_host_layer_vs_oracle_driver(gpu, config1, fast):
         print(f"config 1 host layer (fast={fast}) cycle {c}", flush=True)
         eg = pf.update(OdomDelta(st.odom_t, st.odom_q, st.trans_noise, st.rot_noise),
                        [Observation(i, m, v, r) for (i, m, v, r) in st.obs])
-        eo, rs = oracle_cycle(o, st, lm, sr, sb, gate, 0.5, 0.01, N1)
+        def gpu_cum():
+            c = np.zeros(N1, np.uint64)
+            assert L.pfx_get_cum(kh, c.ctypes.data_as(_lib.P(_lib.u64))) == 0
+            return c
+        eo, rs = oracle_cycle_on_gpu_weights(o, st, lm, sr, sb, gate, 0.5, 0.01, N1, gpu_cum, f"cycle {c}")
         assert eg.resampled == rs
         if rs:
             anc = np.zeros(N1, np.int32)
@@ -214,7 +250,8 @@ def test_config1_fused_cycle_vs_oracle_driver(gpu, config1):
         print(f"config 1 fused cycle {c}", flush=True)
         inp, _keep = bf.make_inputs([(st.odom_t, st.odom_q, st.trans_noise, st.rot_noise, obs_array(st))])
         e = bf.cycle(inp)[0]
-        eo, rs = oracle_cycle(o, st, lm, sr, sb, gate, 0.5, 0.01, N1)
+        eo, rs = oracle_cycle_on_gpu_weights(o, st, lm, sr, sb, gate, 0.5, 0.01, N1, lambda: bf.cum(0),
+                                             f"fused cycle {c}")
         assert bool(e.resampled) == rs
         if rs:
             assert np.array_equal(bf.ancestors(0), o.ancestors()), f"cycle {c}: ancestors differ"
@@ -287,11 +324,8 @@ def test_config3_64_filters_vs_oracle(gpu):
             assert bool(e.resampled) == rs, what
             # the weights the GPU's resampling scan used, recovered exactly from its fixed-point
             # prefix (q = floor(w 2^56) of an f32 weight has at most 24 significant bits)
-            q = np.diff(bf.cum(f), prepend=np.uint64(0))
-            w_used = q.astype(np.float64) * 2.0 ** -56
-            big = ow >= 2.0 ** -32
-            assert np.all(np.abs(w_used[big] - ow[big]) <= 1e-5 * ow[big]), f"{what}: scan weights"
-            assert np.all(np.abs(w_used[~big] - ow[~big]) <= 2.0 ** -56 + 1e-5 * ow[~big]), f"{what}: scan weights"
+            w_used = scan_weights(bf.cum(f))
+            assert_scan_weights_close(w_used, ow, what)
             pos, quat, w = bf.get_state(f)
             if rs:
                 # identical weights and offset: the oracle resamples the GPU's weights

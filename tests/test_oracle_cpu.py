@@ -44,6 +44,10 @@ def test_det_cos_sinc_is_axis_angle():
         lit_v = a / ang * math.sin(0.5 * ang)
         assert abs(c - lit_w) < 1e-13 * max(1.0, ang)
         assert np.all(np.abs(a * 0.5 * s - lit_v) < 1e-13 * max(1.0, ang))
+    # either side of the short-series threshold h^2 = 2^-10 (|a| = 1/16)
+    for ang in (0.0625 * (1 - 1e-12), 0.0625, 0.0625 * (1 + 1e-12), 0.06, 0.07):
+        c, s = O.det_cos_sinc(0.25 * ang * ang)
+        assert abs(c - math.cos(0.5 * ang)) < 1e-16 and abs(0.5 * ang * s - math.sin(0.5 * ang)) < 1e-16, ang
 
 
 def _sm_final(z):
@@ -57,7 +61,7 @@ def test_normals_are_box_muller_of_the_stated_words():
     """The motion normals are the Box-Muller transform (libm log, sqrt, cos, sin here) of the
     six 32-bit words the draw specification names: the particle key (cpu.rs:153-159) and the next
     two splitmix64 outputs (pf_kernels.cu:323-328), radius from each value's high half, angle
-    from its low half; the short fma sequences agree with libm to 1e-7 relative."""
+    from its low half; the short f32 fmaf sequences agree with libm to 1e-6."""
     m = (1 << 64) - 1
     phi = 0x9E3779B97F4A7C15
     for gseed, i in [(12345, 0), (12345, 7), (0x9E37797979797979, 999_999), (42, 123_456), (1, 2 ** 31 - 1)]:
@@ -69,28 +73,33 @@ def test_normals_are_box_muller_of_the_stated_words():
             th = 2 * math.pi * (v & 0xFFFFFFFF) / 2 ** 32
             want += [r * math.cos(th), r * math.sin(th)]
         got = O.particle_normals(gseed, i)
-        assert np.allclose(got, want, rtol=1e-7, atol=1e-9), (gseed, i, got, want)
+        assert np.allclose(got, want, rtol=1e-6, atol=1e-6), (gseed, i, got, want)
 
 
 def test_motion_math_vs_libm():
-    """The motion draws' -2 ln, sqrt and sin/cos against libm over 100k random words and the
-    end points (words 0, 2^32 - 1, octant boundaries)."""
+    """The motion draws' f32 -2 ln, sqrt and sin/cos against libm over 100k random words and the
+    end points (words 0, 2^32 - 1, the f32 rounding edges near 2^24 and 2^32, octant
+    boundaries).  -2 ln is held to 2e-7 relative plus 2e-7 absolute: the word is rounded to f32
+    (24 bits) before the logarithm, which moves ln u by up to 6e-8 near u = 1."""
     import ctypes as C
     L = O.lib()
     rng = np.random.default_rng(4)
-    words = [int(a) for a in rng.integers(0, 2 ** 32, 100_000)] + [0, 1, 2 ** 31, 2 ** 32 - 1]
+    words = [int(a) for a in rng.integers(0, 2 ** 32, 100_000)] + [0, 1, 2 ** 31, 2 ** 32 - 1, 2 ** 24,
+                                                                    2 ** 24 + 1, 2 ** 32 - 256, 2 ** 32 - 129]
     for a in words:
         ref = -2.0 * math.log((a + 1) / 2 ** 32)
-        assert abs(L.or_mot_neg2ln(a) - ref) <= 6e-8 * ref + 1e-300
+        assert abs(L.or_mot_neg2ln(a) - ref) <= 2e-7 * ref + 2e-7, a
     assert L.or_mot_neg2ln(2 ** 32 - 1) == 0.0
-    for v in list(rng.random(100_000) * 45.0) + [0.0, 1e-12, 44.4]:
-        assert abs(L.or_mot_sqrt(v) - math.sqrt(v)) <= 1e-10 * math.sqrt(v)
-    s, c = C.c_double(), C.c_double()
+    assert L.or_mot_sqrt(0.0) == 0.0
+    for v in list(rng.random(100_000) * 45.0) + [1e-12, 44.4]:
+        v = float(np.float32(v))
+        assert abs(L.or_mot_sqrt(v) - math.sqrt(v)) <= 2e-7 * math.sqrt(v), v
+    s, c = C.c_float(), C.c_float()
     for b in words + [k * 2 ** 29 + d for k in range(8) for d in (0, 1, 2 ** 29 - 1)]:
         b &= 0xFFFFFFFF
         L.or_mot_sincos_turn(b, C.byref(s), C.byref(c))
         th = 2 * math.pi * b / 2 ** 32
-        assert abs(s.value - math.sin(th)) < 2e-9 and abs(c.value - math.cos(th)) < 2e-9, b
+        assert abs(s.value - math.sin(th)) < 1.5e-7 and abs(c.value - math.cos(th)) < 1.5e-7, b
 
 
 def test_normals_moments():
