@@ -543,6 +543,7 @@ def main():
     ap.add_argument("--no-gl", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--no-sweep", action="store_true", help="skip the 4M / 16M-particle staged cycles")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--weight-ppt", type=int, default=0, choices=(0, 1, 2, 4),
                     help="likelihood particles per lane of the headline cycle (0: the autotuner's choice)")
@@ -583,6 +584,25 @@ def main():
         return out
     stage_bw = stage_table(fstages, FUSED_BYTES)
     stage_bw_abi = stage_table(stages, BYTES)
+    # the same staged cycle at 4M and 16M particles: the per-stage HBM fractions once the small
+    # passes are long enough to amortise their launch-to-retire latency
+    size_sweep = {}
+    if not args.no_sweep:
+        for n_sw in (4 * 1024 * 1024, 16 * 1024 * 1024):
+            ms_sw, ms_sw_dev, _, err_sw, _, st_sw, _, _, ppt_sw = run_pf_fused(n_sw, sc, 20, 5, 5, args.weight_ppt)
+            tab = {}
+            for s_, b_ in FUSED_BYTES.items():
+                st = st_sw.get(s_)
+                if st and st["launches"]:
+                    gbs = b_ * n_sw / (st["ms_avg"] * 1e-3) / 1e9 if b_ else None
+                    tab[s_] = {"ms_avg": round(st["ms_avg"], 4), "GB/s": round(gbs, 1) if gbs else None,
+                               "frac_hbm": round(gbs / HBM_PEAK_GBS, 3) if gbs else None}
+            wl = st_sw["weight"]["ms_avg"]
+            size_sweep[str(n_sw)] = {
+                "ms_per_cycle": round(ms_sw, 4), "particles_per_s": round(n_sw / (ms_sw * 1e-3), 1),
+                "mean_track_err_m": round(err_sw, 4), "particles_per_lane": ppt_sw,
+                "likelihood_tflops": round(WEIGHT_FLOP_PER_PAIR * n_sw * 256 / (wl * 1e-3) / 1e12, 2) if wl else None,
+                "stages": tab}
     # cycle roofline: every stage at the HBM peak on its algorithmic bytes, the likelihood at the
     # FP64 peak on its FLOP (the larger of its two bounds)
     t_roof_us = ((CYCLE_BYTES - BYTES_WEIGHT_ALG) * n / (HBM_PEAK_GBS * 1e9)
@@ -618,6 +638,7 @@ def main():
         "stages_profiled_ms_per_cycle": round(ms_fprof, 4),
         "stages_profiled_cycles": n_fprof,
         "stages_abi_driver": stage_bw_abi,
+        "size_sweep": size_sweep,
         "cycle_roofline": {"t_roof_us": round(t_roof_us, 1), "t_measured_us": round(ms_fused * 1e3, 1),
                            "frac": round(t_roof_us / (ms_fused * 1e3), 3),
                            "basis": f"{CYCLE_BYTES} B/particle at 8 TB/s, likelihood at the FP64 peak on "

@@ -148,8 +148,10 @@ struct PfState {
     int inj_nblk = 0;
     // reduction partials / scan state
     double *part_w = nullptr, *part_n = nullptr, *part_est = nullptr, *part_cov = nullptr;
+    double* cov_fold = nullptr;  // staged covariance finalize: one fold per lane group and filter
     uint64_t* part_q = nullptr;                // fixed-point weight aggregate of every 1024-particle tile
     uint64_t* tcum = nullptr;                  // cumulative fixed-point weight at every tile's last particle
+    uint64_t* tpre = nullptr;                  // exclusive prefix of every tile's aggregate (large filters)
     uint64_t* scan_st = nullptr;               // look-back status words, two per 1024-particle tile
     unsigned long long* scan_ctl = nullptr;     // status epoch of the look-back words
     int scan_spin_limit = 64;                   // reads of a predecessor's status before summing it here
@@ -188,7 +190,8 @@ int launch_motion(PfState* s, ParamRef<MotionParams> A, int reinit, int dst_flip
 // softmax finalize makes it current.  cov (non-null, requires flip): the previous cycle's
 // covariance pass, run by extra likelihood workgroups on the still-current pose set and weights
 // with that cycle's covariance parameter blocks, and its finalize, run by extra softmax-finalize
-// workgroups, into FilterDev.covs and the results slot out.
+// workgroups and completed by the normalisation launch (launch_normalize's cov_out), into
+// FilterDev.covs and the results slot out.
 struct WeightDefer {
     int flip = 0;
     const CovParams* cov = nullptr;
@@ -200,8 +203,10 @@ int launch_weight(PfState* s, ParamRef<WeightParams> A, const WeightDefer& d = {
 // d_ratio required) the normalisation pass also leaves every tile's fixed-point aggregate and
 // draws the injection picks inj of a possible resample, and the resampling scan of every filter
 // follows, its last tile forming ESS, the predicate and the LCG advance.
+// cov_out (staged cycles, with scan): extra workgroups finish the previous cycle's deferred
+// covariance finalize into FilterDev.covs and cov_out.
 int launch_normalize(PfState* s, int force_uniform, const float* d_ratio, bool scan = false,
-                     ParamRef<InjectParams> inj = ParamRef<InjectParams>::value({}));
+                     ParamRef<InjectParams> inj = ParamRef<InjectParams>::value({}), FilterDev* cov_out = nullptr);
 // Scan (unless launch_normalize already scanned), injection picks, gather (with the picks' jitter
 // and the estimate-sum partials of every filter); with flip the resampled set then becomes
 // current (else the fused cycle's sums finalize does it).

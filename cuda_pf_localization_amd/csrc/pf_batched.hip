@@ -244,7 +244,7 @@ void fill_estimate(const FilterDev& fd, pfb_estimate& o) {
 // cycle's covariances are left pending: nothing in the next cycle needs them, so the next
 // cycle's likelihood launch computes them (its predict stores the new poses in the idle pose
 // set, leaving this cycle's poses and weights in place until the likelihood has read them) and
-// its softmax finalize reduces them into the results slot.  prev: the previous cycle's
+// its softmax finalize and normalisation launches reduce them into the results slot.  prev: the previous cycle's
 // parameter blocks and results slot (null for the first cycle of a run, whose predecessor
 // finished on its own); the run's last cycle finishes with a covariance pass of its own.
 int stages_staged(Batch* b, const ParamBlock& dp, const ParamBlock* prev, FilterDev* prev_results) {
@@ -257,7 +257,9 @@ int stages_staged(Batch* b, const ParamBlock& dp, const ParamBlock* prev, Filter
         d.out = prev_results;
     }
     if (launch_weight(s, ParamRef<WeightParams>::array(dp.weight), d) != 0) return -1;
-    if (launch_normalize(s, 0, b->d_ratio, true, ParamRef<InjectParams>::array(dp.inject)) != 0) return -1;
+    if (launch_normalize(s, 0, b->d_ratio, true, ParamRef<InjectParams>::array(dp.inject), prev ? prev_results : nullptr) !=
+        0)
+        return -1;
     if (launch_resample(s, ParamRef<InjectParams>::array(dp.inject), s->d_inj_blk, s->inj_nblk, false, false) != 0)
         return -1;
     return launch_est_sums(s, dp.cov, true, false);

@@ -350,8 +350,9 @@ __device__ inline void cart_from_b(double bx, double by, double bz, double vx, d
 template <bool kWave>
 __device__ void covs_tile(const Layout& L, FilterDev* F, const PoseBufs& B, int swapped, const float* __restrict__ w,
                           double* __restrict__ part, int stride, const ParamRef<CovParams>& A, int blk);
-__device__ void cov_fin_block(const Layout& L, FilterDev* F, const double* __restrict__ part, int stride,
-                              FilterDev* out, int f);
+__device__ void cov_fin_wave(const Layout& L, FilterDev* F, const double* __restrict__ part, int stride,
+                             FilterDev* out, double* __restrict__ fold, int r);
+__device__ void cov_fold_sum(FilterDev* F, const double* __restrict__ fold, FilterDev* out, int f);
 
 // Gated log-likelihood (cpu.rs:44-97, 220-248), kPPT particles per lane, plus the block's
 // online-softmax partials.  Within each category of observations (range-bearing, then
@@ -582,10 +583,10 @@ __device__ __forceinline__ void markley_mean(FilterDev& fd, const double* sums, 
 // tree, two values at once).
 __global__ __launch_bounds__(kFinTB) void k_weight_fin(Layout L, FilterDev* F, const double* __restrict__ part,
                                                        int span, int nfilt, int flip, const double* __restrict__ cpart,
-                                                       int cstride, FilterDev* cout) {
+                                                       int cstride, FilterDev* cout, double* __restrict__ cfold) {
     __shared__ double sh[kFinTB / 64], sh1[kFinTB / 64], sh2[kFinTB / 64];
-    if ((int)blockIdx.x >= nfilt) {  // the previous staged cycle's covariance finalize
-        cov_fin_block(L, F, cpart, cstride, cout, blockIdx.x - nfilt);
+    if ((int)blockIdx.x >= nfilt) {  // the previous staged cycle's covariance finalize (its lane groups)
+        cov_fin_wave(L, F, cpart, cstride, cout, cfold, blockIdx.x - nfilt);
         return;
     }
     const int f = blockIdx.x;
@@ -980,13 +981,20 @@ __global__ __launch_bounds__(kTB) void k_scan(Layout L, FilterDev* F, const floa
 // bits), the tile's fixed-point aggregate sum q (the value a look-back tile publishes), and the
 // tile's share of the filter's injection picks: drawn before the resample predicate is known,
 // with the LCG value its advance will give (k_scan_agg advances fd.lcg only afterwards).
+// cout (staged cycles): workgroups past the tiles (blockIdx.x >= ntile) add the previous
+// cycle's covariance folds (cov_fold_sum) of filter blockIdx.x - ntile.
 __global__ __launch_bounds__(kTB) void k_norm_agg(Layout L, FilterDev* F, const double* __restrict__ ll,
                                                   float* __restrict__ w, double* __restrict__ part_n,
                                                   uint64_t* __restrict__ part_q, ParamRef<InjectParams> inj,
                                                   double* __restrict__ jit, int32_t* __restrict__ head,
-                                                  int32_t* __restrict__ next) {
+                                                  int32_t* __restrict__ next, const double* __restrict__ cfold,
+                                                  FilterDev* cout, int ntile) {
     __shared__ double sh[4];
     __shared__ uint64_t s_w[4];
+    if (cout && (int)blockIdx.x >= ntile) {
+        cov_fold_sum(F, cfold, cout, blockIdx.x - ntile);
+        return;
+    }
     const int tile = blockIdx.x;
     const int f = layout_filter(L, tile * (kTile / kTB));
     const FilterDev& fd = F[f];
@@ -1034,10 +1042,63 @@ __global__ __launch_bounds__(kTB) void k_norm_agg(Layout L, FilterDev* F, const 
 // block's four waves reproduce as virtual waves w, w + 4, w + 8, w + 12, so both paths give the
 // same bits.  A null ratio forces the predicate (the reference ABI's pfc_resample, scanning from
 // the aggregates k_normalize left).
+// Exclusive prefixes of the tile aggregates of every filter (one 1024-thread workgroup per
+// filter), for filters too large for k_scan_agg's per-tile sum of the earlier aggregates, which
+// grows with the square of the tile count (at 16M particles, 16,384 tiles: 99 us).  Chunks of
+// kPrefixRun * 1024 tiles: thread t takes the run of kPrefixRun consecutive tiles from
+// c + kPrefixRun t (one 128-byte span, loaded at once), the run totals are scanned across the
+// block once, and each thread writes its run's prefixes.  Integer sums: the same prefixes in any
+// order.
+constexpr int kPrefixTiles = 2048;  // filters above this many tiles take the prefix pass
+constexpr int kPrefixRun = 16;
+__global__ __launch_bounds__(kFinTB) void k_tile_prefix(Layout L, const uint64_t* __restrict__ part_q,
+                                                        uint64_t* __restrict__ tpre) {
+    __shared__ uint64_t s_w[kFinTB / 64];
+    const int f = blockIdx.x;
+    const int t0 = L.f_off[f] / kTile, nt = L.f_padn[f] / kTile;
+    if (nt <= kPrefixTiles) return;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    uint64_t carry = 0;
+    for (int c = 0; c < nt; c += kPrefixRun * kFinTB) {
+        const int b0 = c + kPrefixRun * (int)threadIdx.x;
+        uint64_t a[kPrefixRun];
+#pragma unroll
+        for (int k = 0; k < kPrefixRun; ++k) a[k] = b0 + k < nt ? part_q[t0 + b0 + k] : 0;
+        uint64_t run = 0;
+#pragma unroll
+        for (int k = 0; k < kPrefixRun; ++k) run += a[k];
+        uint64_t inc = run;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint64_t y = __shfl_up(inc, d, 64);
+            if (lane >= d) inc += y;
+        }
+        __syncthreads();  // (s_w of the previous chunk has been read)
+        if (lane == 63) s_w[wv] = inc;
+        __syncthreads();
+        uint64_t pre = carry, tot = 0;
+#pragma unroll
+        for (int w2 = 0; w2 < kFinTB / 64; ++w2) {
+            if (w2 < wv) pre += s_w[w2];
+            tot += s_w[w2];
+        }
+        pre += inc - run;
+#pragma unroll
+        for (int k = 0; k < kPrefixRun; ++k) {
+            if (b0 + k < nt) tpre[t0 + b0 + k] = pre;
+            pre += a[k];
+        }
+        carry += tot;
+    }
+}
+
+// tpre (non-null): the exclusive prefix of each tile of a filter above kPrefixTiles tiles, from
+// k_tile_prefix; smaller filters sum their earlier aggregates here.
 __global__ __launch_bounds__(kTB) void k_scan_agg(Layout L, FilterDev* F, const float* __restrict__ w,
                                                   const double* __restrict__ part_n,
                                                   const uint64_t* __restrict__ part_q, uint64_t* __restrict__ cum,
-                                                  uint64_t* __restrict__ tcum, const float* __restrict__ ratio) {
+                                                  uint64_t* __restrict__ tcum, const float* __restrict__ ratio,
+                                                  const uint64_t* __restrict__ tpre) {
     __shared__ uint64_t s_w[4];
     __shared__ double s_vw[kFinTB / 64];
     const int tile = blockIdx.x;
@@ -1055,18 +1116,23 @@ __global__ __launch_bounds__(kTB) void k_scan_agg(Layout L, FilterDev* F, const 
         const int vl = (int)threadIdx.x + kTB * j;
         pn[j] = last && nt <= kFinTB && vl < nt ? part_n[t0 + vl] : 0.0;
     }
-    uint64_t ex = 0;
-    for (int c = t0; c < tile; c += 4 * kTB) {  // (four aggregates per lane in flight)
-        uint64_t a[4];
+    uint64_t excl;
+    if (tpre && nt > kPrefixTiles) {
+        excl = tpre[tile];
+    } else {
+        uint64_t ex = 0;
+        for (int c = t0; c < tile; c += 4 * kTB) {  // (four aggregates per lane in flight)
+            uint64_t a[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int b = c + (int)threadIdx.x + u * kTB;
-            a[u] = b < tile ? part_q[b] : 0;
+            for (int u = 0; u < 4; ++u) {
+                const int b = c + (int)threadIdx.x + u * kTB;
+                a[u] = b < tile ? part_q[b] : 0;
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) ex += a[u];
         }
-#pragma unroll
-        for (int u = 0; u < 4; ++u) ex += a[u];
+        excl = block_sum_u64(ex, s_w);
     }
-    const uint64_t excl = block_sum_u64(ex, s_w);
     uint64_t q[4];
     tile_q(wv, tile, off, nf, q);
     q[1] += q[0];
@@ -1670,36 +1736,44 @@ __global__ __launch_bounds__(kEstFinTB) void k_est_fin(Layout L, FilterDev* F, c
     }
 }
 
-// The covariance finalize of k_est_fin<kCovN> (flip = 0, results into out) run by a
-// 1024-thread block: threads below kEstFinTB are the finalize's lanes (same loads, folds and
-// wave order, so the same bits), the others only join the barrier.  Every scalar of the filter
-// goes to out[f] except the likelihood scalars and the word holding the pose parity, which the
-// softmax finalize running beside this block rewrites for the next cycle.
-__device__ void cov_fin_block(const Layout& L, FilterDev* F, const double* __restrict__ part, int stride,
-                              FilterDev* out, int f) {
-    __shared__ double shv[kEstFinTB / 64][kCovN];
+// The covariance finalize of a staged cycle, split over kCovSplit workgroups per filter so that
+// no single CU loads all the per-wave partials its deferred tiles left (four per tile: at 16M
+// particles 12.6 MB, which one CU took 100 us to load): workgroup r of filter f runs lane group r
+// of k_est_fin<kCovN>'s lanes (lanes 64 r .. 64 r + 63, one wave; the same loads, lane sums and
+// fold) and stores its fold in fold[f][r]; the next kernel adds the kCovSplit folds in
+// k_est_fin's wave order (cov_fold_sum).  Group 0 also copies every scalar of the filter to
+// out[f] except the likelihood scalars, the word holding the pose parity (which the softmax
+// finalize running beside it rewrites for the next cycle) and the covariances.
+constexpr int kCovSplit = kEstFinTB / 64;
+__device__ void cov_fin_wave(const Layout& L, FilterDev* F, const double* __restrict__ part, int stride,
+                             FilterDev* out, double* __restrict__ fold, int r) {
+    if (threadIdx.x >= 64) return;  // one wave per workgroup
+    const int f = r / kCovSplit, j = r % kCovSplit;
     const int b0 = L.f_off[f] / kTile, nb = L.f_padn[f] / kTile;
-    const int t = (int)threadIdx.x;
-    if (t < kEstFinTB) {
-        double acc[kCovN];
-        fin_lane_sums_w4<kCovN, 2>(part + 4 * b0, stride, nb, t, acc);
-        wave_fold_store<kCovN>(acc, shv[t >> 6]);
-    }
-    __syncthreads();
-    if (t < kCovN) {
-        double v = shv[0][t];
-#pragma unroll
-        for (int wv = 1; wv < kEstFinTB / 64; ++wv) v += shv[wv][t];
-        F[f].covs[t] = v;
-        out[f].covs[t] = v;
-    }
+    const int lane = (int)threadIdx.x;
+    double acc[kCovN];
+    fin_lane_sums_w4<kCovN, 2>(part + 4 * b0, stride, nb, 64 * j + lane, acc);
+    wave_fold_store<kCovN>(acc, fold + ((size_t)f * kCovSplit + j) * kCovN);
+    if (j != 0) return;
     constexpr int kWords = (int)(sizeof(FilterDev) / sizeof(uint64_t));
     constexpr int c0 = (int)(offsetof(FilterDev, covs) / sizeof(uint64_t));
     constexpr int l1 = (int)(offsetof(FilterDev, sum_w2) / sizeof(uint64_t));  // words [0, l1): likelihood
     constexpr int pw = (int)(offsetof(FilterDev, parity) / sizeof(uint64_t));
     static_assert(offsetof(FilterDev, max_ll) == 0 && offsetof(FilterDev, uniform) / 8 == pw, "FilterDev words");
-    if (t < kWords && t >= l1 && t != pw && (t < c0 || t >= c0 + kCovN))
-        reinterpret_cast<uint64_t*>(out + f)[t] = reinterpret_cast<const uint64_t*>(F + f)[t];
+    for (int t = lane; t < kWords; t += 64)
+        if (t >= l1 && t != pw && (t < c0 || t >= c0 + kCovN))
+            reinterpret_cast<uint64_t*>(out + f)[t] = reinterpret_cast<const uint64_t*>(F + f)[t];
+}
+// The covariances of filter f from the kCovSplit folds cov_fin_wave stored (lanes t < kCovN).
+__device__ void cov_fold_sum(FilterDev* F, const double* __restrict__ fold, FilterDev* out, int f) {
+    const int t = (int)threadIdx.x;
+    if (t >= kCovN) return;
+    const double* fr = fold + (size_t)f * kCovSplit * kCovN;
+    double v = fr[t];
+#pragma unroll
+    for (int wv = 1; wv < kCovSplit; ++wv) v += fr[wv * kCovN + t];
+    F[f].covs[t] = v;
+    out[f].covs[t] = v;
 }
 
 // est layout (cu:354-404): [0..2] sum w p, [3..12] sum w q_r q_c (r<=c), [13] sum w^2.
@@ -1895,8 +1969,10 @@ int state_alloc(PfState* s, const std::vector<int>& ns) {
     PF_CHECK(hipMalloc((void**)&s->part_n, sizeof(double) * (size_t)nb256));
     PF_CHECK(hipMalloc((void**)&s->part_q, sizeof(uint64_t) * (size_t)ntile));
     PF_CHECK(hipMalloc((void**)&s->tcum, sizeof(uint64_t) * (size_t)ntile));
+    PF_CHECK(hipMalloc((void**)&s->tpre, sizeof(uint64_t) * (size_t)ntile));
     PF_CHECK(hipMalloc((void**)&s->part_est, sizeof(double) * kEstN * (size_t)ntile));
     PF_CHECK(hipMalloc((void**)&s->part_cov, sizeof(double) * kCovN * 4 * (size_t)ntile));  // per-wave layout
+    PF_CHECK(hipMalloc((void**)&s->cov_fold, sizeof(double) * kCovN * kCovSplit * (size_t)nf));
     PF_CHECK(hipMalloc((void**)&s->scan_st, sizeof(uint64_t) * 2 * (size_t)ntile));
     PF_CHECK(hipMemsetAsync(s->scan_st, 0, sizeof(uint64_t) * 2 * (size_t)ntile, s->stream));
     PF_CHECK(hipMalloc((void**)&s->scan_ctl, sizeof(unsigned long long)));
@@ -1944,7 +2020,7 @@ void state_free_particles(PfState* s) {
         }
     void** ptrs[] = {(void**)&s->w,        (void**)&s->ll,          (void**)&s->cum,       (void**)&s->anc,
                      (void**)&s->inj_next,    (void**)&s->inj_head,  (void**)&s->inj_jit,
-                     (void**)&s->part_w,   (void**)&s->part_n,  (void**)&s->part_q,  (void**)&s->tcum,  (void**)&s->part_est,    (void**)&s->part_cov,  (void**)&s->scan_st,
+                     (void**)&s->part_w,   (void**)&s->part_n,  (void**)&s->part_q,  (void**)&s->tcum,  (void**)&s->tpre,  (void**)&s->part_est,    (void**)&s->part_cov,  (void**)&s->cov_fold, (void**)&s->scan_st,
                      (void**)&s->scan_ctl, (void**)&s->d_blk_filter, (void**)&s->d_fn,     (void**)&s->d_foff,
                      (void**)&s->d_fpadn,  (void**)&s->d_fwinv,     (void**)&s->fdev,      (void**)&s->d_inj_blk};
     for (void** p : ptrs) {
@@ -2015,29 +2091,46 @@ int launch_weight(PfState* s, ParamRef<WeightParams> A, const WeightDefer& d) {
     prof_end(s);
     PF_LAUNCH_CHECK();
     prof_begin(s, PFX_STAGE_FINALIZE);
-    hipLaunchKernelGGL(k_weight_fin, dim3(s->nf + (dc ? s->nf : 0)), dim3(kFinTB), 0, s->stream, s->L, s->fdev,
-                       s->part_w, kTB * ppt, s->nf, d.flip, dc ? s->part_cov : nullptr, ntile, d.out);
+    hipLaunchKernelGGL(k_weight_fin, dim3(s->nf + (dc ? s->nf * kCovSplit : 0)), dim3(kFinTB), 0, s->stream, s->L,
+                       s->fdev, s->part_w, kTB * ppt, s->nf, d.flip, dc ? s->part_cov : nullptr, ntile, d.out,
+                       s->cov_fold);
     prof_end(s);
     PF_LAUNCH_CHECK();
     return 0;
 }
 
-int launch_normalize(PfState* s, int force_uniform, const float* d_ratio, bool scan, ParamRef<InjectParams> inj) {
+// The scan from the tile aggregates (k_scan_agg), after the prefix pass when a filter has more
+// than kPrefixTiles tiles.
+static int launch_scan_agg(PfState* s, const float* d_ratio) {
+    const int ntile = s->NP / kTile;
+    int max_nt = 0;
+    for (int p : s->padn) max_nt = std::max(max_nt, p / kTile);
+    prof_begin(s, PFX_STAGE_RESAMPLE_SCAN);
+    if (max_nt > kPrefixTiles) {
+        hipLaunchKernelGGL(k_tile_prefix, dim3(s->nf), dim3(kFinTB), 0, s->stream, s->L, s->part_q, s->tpre);
+        PF_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(k_scan_agg, dim3(ntile), dim3(kTB), 0, s->stream, s->L, s->fdev, s->w, s->part_n, s->part_q,
+                       s->cum, s->tcum, d_ratio, max_nt > kPrefixTiles ? s->tpre : nullptr);
+    prof_end(s);
+    PF_LAUNCH_CHECK();
+    return 0;
+}
+
+int launch_normalize(PfState* s, int force_uniform, const float* d_ratio, bool scan, ParamRef<InjectParams> inj,
+                     FilterDev* cov_out) {
+    if (cov_out && !scan) return -1;
     const int ntile = s->NP / kTile;
     if (scan) {
         if (!d_ratio) return -1;  // the fused scan forms the resample predicate
         s->agg_ok = false;        // (the gather that follows rewrites the weights)
         prof_begin(s, PFX_STAGE_NORMALIZE);
-        hipLaunchKernelGGL(k_norm_agg, dim3(ntile), dim3(kTB), 0, s->stream, s->L, s->fdev, s->ll, s->w, s->part_n,
-                           s->part_q, inj, s->inj_jit, s->inj_head, s->inj_next);
+        hipLaunchKernelGGL(k_norm_agg, dim3(ntile + (cov_out ? s->nf : 0)), dim3(kTB), 0, s->stream, s->L, s->fdev,
+                           s->ll, s->w, s->part_n, s->part_q, inj, s->inj_jit, s->inj_head, s->inj_next, s->cov_fold,
+                           cov_out, ntile);
         prof_end(s);
         PF_LAUNCH_CHECK();
-        prof_begin(s, PFX_STAGE_RESAMPLE_SCAN);
-        hipLaunchKernelGGL(k_scan_agg, dim3(ntile), dim3(kTB), 0, s->stream, s->L, s->fdev, s->w, s->part_n,
-                           s->part_q, s->cum, s->tcum, d_ratio);
-        prof_end(s);
-        PF_LAUNCH_CHECK();
-        return 0;
+        return launch_scan_agg(s, d_ratio);
     }
     prof_begin(s, PFX_STAGE_NORMALIZE);
     hipLaunchKernelGGL(k_normalize, dim3(ntile), dim3(kTB), 0, s->stream, s->L, s->fdev, s->ll, s->w, s->part_n,
@@ -2059,11 +2152,7 @@ int launch_resample(PfState* s, ParamRef<InjectParams> inj, const int32_t* inj_t
         // the weights are those the last normalisation left aggregates for: reduce-then-scan
         // from them (the same integer prefixes, total and LCG advance as the look-back scan;
         // the predicate forced, as k_resample_predicate does for this path)
-        prof_begin(s, PFX_STAGE_RESAMPLE_SCAN);
-        hipLaunchKernelGGL(k_scan_agg, dim3(ntile), dim3(kTB), 0, s->stream, s->L, s->fdev, s->w, s->part_n,
-                           s->part_q, s->cum, s->tcum, nullptr);
-        prof_end(s);
-        PF_LAUNCH_CHECK();
+        if (launch_scan_agg(s, nullptr) != 0) return -1;
     } else if (scan) {
         prof_begin(s, PFX_STAGE_RESAMPLE_SCAN);
         hipLaunchKernelGGL(k_scan, dim3(ntile), dim3(kTB), 0, s->stream, s->L, s->fdev, s->w, s->cum, s->tcum, s->scan_st,
