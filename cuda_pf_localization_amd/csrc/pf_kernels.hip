@@ -1736,23 +1736,58 @@ __global__ __launch_bounds__(kEstFinTB) void k_est_fin(Layout L, FilterDev* F, c
     }
 }
 
-// The covariance finalize of a staged cycle, split over kCovSplit workgroups per filter so that
-// no single CU loads all the per-wave partials its deferred tiles left (four per tile: at 16M
-// particles 12.6 MB, which one CU took 100 us to load): workgroup r of filter f runs lane group r
-// of k_est_fin<kCovN>'s lanes (lanes 64 r .. 64 r + 63, one wave; the same loads, lane sums and
-// fold) and stores its fold in fold[f][r]; the next kernel adds the kCovSplit folds in
-// k_est_fin's wave order (cov_fold_sum).  Group 0 also copies every scalar of the filter to
-// out[f] except the likelihood scalars, the word holding the pose parity (which the softmax
-// finalize running beside it rewrites for the next cycle) and the covariances.
+// The covariance finalize of a staged cycle, split over kCovSplit workgroups per filter: workgroup
+// r of filter f computes lane group r of k_est_fin<kCovN>'s lanes (lanes vt = 64 r + l), whose
+// lane sums run over the tiles vt + 512 u, u = 0, 1, ... (4 per 2048 tiles, tiles past the filter
+// counted as 0.0: fin_lane_sums' additions exactly) of the per-wave partials the deferred tiles
+// left.  All 1024 threads load and pair up the four per-wave partials of kCovStage rounds of
+// those tiles into LDS, then the group's wave adds them in order; the wave's fold goes to
+// fold[f][r], and the next kernel adds the kCovSplit folds in k_est_fin's wave order
+// (cov_fold_sum).  (One wave loading its own tiles took 116 us at 16M particles.)  Group 0 also
+// copies every scalar of the filter to out[f] except the likelihood scalars, the word holding the
+// pose parity (which the softmax finalize running beside it rewrites for the next cycle) and the
+// covariances.
 constexpr int kCovSplit = kEstFinTB / 64;
+constexpr int kCovStage = 8;  // rounds of tiles staged at a time: 8 x 12 x 64 doubles (48 KB)
 __device__ void cov_fin_wave(const Layout& L, FilterDev* F, const double* __restrict__ part, int stride,
                              FilterDev* out, double* __restrict__ fold, int r) {
-    if (threadIdx.x >= 64) return;  // one wave per workgroup
+    __shared__ double s_t[kCovStage][kCovN][64];
     const int f = r / kCovSplit, j = r % kCovSplit;
     const int b0 = L.f_off[f] / kTile, nb = L.f_padn[f] / kTile;
-    const int lane = (int)threadIdx.x;
+    const int t = (int)threadIdx.x, lane = t & 63;
+    const int nround = 4 * ((nb + 4 * kEstFinTB - 1) / (4 * kEstFinTB));
+    const double* pf = part + 4 * (size_t)b0;
     double acc[kCovN];
-    fin_lane_sums_w4<kCovN, 2>(part + 4 * b0, stride, nb, 64 * j + lane, acc);
+#pragma unroll
+    for (int k = 0; k < kCovN; ++k) acc[k] = 0.0;
+    for (int u0 = 0; u0 < nround; u0 += kCovStage) {
+        // item i = t + kFinTB m: lane i % 64, value (i / 64) % kCovN, round u0 + i / (64 kCovN)
+        constexpr int kItems = kCovStage * kCovN * 64 / kFinTB;
+        double2 lo[kItems], hi[kItems];
+#pragma unroll
+        for (int m = 0; m < kItems; ++m) {
+            const int i = t + kFinTB * m, l = i & 63, k = (i >> 6) % kCovN, u = u0 + (i >> 6) / kCovN;
+            const int b = 64 * j + l + kEstFinTB * u;
+            if (u < nround && b < nb) {
+                lo[m] = *reinterpret_cast<const double2*>(pf + (size_t)k * 4 * stride + 4 * b);
+                hi[m] = *reinterpret_cast<const double2*>(pf + (size_t)k * 4 * stride + 4 * b + 2);
+            } else {
+                lo[m] = hi[m] = make_double2(0.0, 0.0);
+            }
+        }
+        __syncthreads();  // (the previous stage has been added)
+#pragma unroll
+        for (int m = 0; m < kItems; ++m) {
+            const int i = t + kFinTB * m, l = i & 63, k = (i >> 6) % kCovN, uu = (i >> 6) / kCovN;
+            s_t[uu][k][l] = (lo[m].x + lo[m].y) + (hi[m].x + hi[m].y);
+        }
+        __syncthreads();
+        if (t < 64)
+            for (int uu = 0; uu < kCovStage && u0 + uu < nround; ++uu)
+#pragma unroll
+                for (int k = 0; k < kCovN; ++k) acc[k] += s_t[uu][k][lane];
+    }
+    if (t >= 64) return;
     wave_fold_store<kCovN>(acc, fold + ((size_t)f * kCovSplit + j) * kCovN);
     if (j != 0) return;
     constexpr int kWords = (int)(sizeof(FilterDev) / sizeof(uint64_t));
@@ -1760,9 +1795,9 @@ __device__ void cov_fin_wave(const Layout& L, FilterDev* F, const double* __rest
     constexpr int l1 = (int)(offsetof(FilterDev, sum_w2) / sizeof(uint64_t));  // words [0, l1): likelihood
     constexpr int pw = (int)(offsetof(FilterDev, parity) / sizeof(uint64_t));
     static_assert(offsetof(FilterDev, max_ll) == 0 && offsetof(FilterDev, uniform) / 8 == pw, "FilterDev words");
-    for (int t = lane; t < kWords; t += 64)
-        if (t >= l1 && t != pw && (t < c0 || t >= c0 + kCovN))
-            reinterpret_cast<uint64_t*>(out + f)[t] = reinterpret_cast<const uint64_t*>(F + f)[t];
+    for (int t2 = lane; t2 < kWords; t2 += 64)
+        if (t2 >= l1 && t2 != pw && (t2 < c0 || t2 >= c0 + kCovN))
+            reinterpret_cast<uint64_t*>(out + f)[t2] = reinterpret_cast<const uint64_t*>(F + f)[t2];
 }
 // The covariances of filter f from the kCovSplit folds cov_fin_wave stored (lanes t < kCovN).
 __device__ void cov_fold_sum(FilterDev* F, const double* __restrict__ fold, FilterDev* out, int f) {
