@@ -1389,7 +1389,9 @@ __global__ __launch_bounds__(kTB) void k_gather(Layout L, FilterDev* F, PoseBufs
                                                 const int32_t* __restrict__ next, const double* __restrict__ jit,
                                                 double* __restrict__ part_est, int stride) {
     __shared__ uint64_t s_c[kGatherWin];      // cum of the staged tiles
-    __shared__ int32_t s_anc[kMergeItems];     // ancestor of each of the block's outputs
+    // ancestor of each of the block's outputs, less J(d0) (ancestors span at most 2049 particles):
+    // 16-bit entries keep the block's LDS at 29 KB, five blocks per CU
+    __shared__ uint16_t s_anc[kMergeItems];
     __shared__ int s_first[3][2];              // tile search: first probe past J(d), per round and end
     __shared__ int s_j[2];                     // J(d0), J(d1)
     const int blk = blockIdx.x;
@@ -1547,7 +1549,7 @@ __global__ __launch_bounds__(kTB) void k_gather(Layout L, FilterDev* F, PoseBufs
                 for (; d < dend; ++d) {
                     const bool out = i < nf && (e >= nf || !lt128(mul_n(s_c[min(e - wb, kGatherWin - 1)], n32), K));
                     if (out) {
-                        s_anc[i - ia] = min(e, nf - 1);
+                        s_anc[i - ia] = (uint16_t)(min(e, nf - 1) - ja);
                         ++i;
                         K = add128(K, U128{0, T});
                     } else {
@@ -1569,7 +1571,7 @@ __global__ __launch_bounds__(kTB) void k_gather(Layout L, FilterDev* F, PoseBufs
                 jj[r] = -1;
                 hd[r] = -1;
                 if (u < no) {
-                    const int j = min(max(s_anc[u], 0), nf - 1);
+                    const int j = min(ja + (int)s_anc[u], nf - 1);
                     jj[r] = j;
                     hd[r] = head[off + ia + u];
 #pragma unroll

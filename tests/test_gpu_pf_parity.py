@@ -405,6 +405,47 @@ def test_resample_after_weight_scans_from_aggregates(gpu, n, ratio):
     b.close()
 
 
+@pytest.mark.parametrize("n", [2_200_001, 16_900_003])
+def test_aggregate_route_with_tile_prefix_pass(gpu, n):
+    """Above 2048 tiles of 1024 particles the aggregate route first scans the tile sums in a
+    prefix pass (runs of 16 tiles, 16384 tiles a chunk: 16.9M particles takes two chunks);
+    its ancestors, LCG advance and resampled state equal the look-back route's, and at 2.2M
+    the oracle's."""
+    lm = make_landmarks(64, 4, box=(50.0, 50.0, 10.0))
+    q_t = (math.cos(0.4), 0.0, 0.0, math.sin(0.4))
+    truth = np.array([20.0, 30.0, 4.0])
+    obs = rb_obs_for(truth, q_t, lm, 6, outliers=0.0)
+    a = PfKernels(64)
+    a.init(n, 77)
+    a.reinit(f32([*truth, *q_t]), f32([0.8] * 3), f32([0.2] * 3))
+    a.upload_landmarks(lm)
+    a.weight(obs, 0.05, 0.02, 3.0)
+    pos, quat, w = a.get_state()
+    lcg = a.get_lcg()
+    rng = a.rng()
+    a.resample(0.01)  # aggregate route, prefix pass
+    b = PfKernels(64)
+    b.init(n, 77)
+    b.set_state(pos, quat, w)
+    b.set_lcg(lcg)
+    b.set_rng(*rng)
+    b.resample(0.01)  # look-back route
+    anc = a.ancestors()
+    assert np.array_equal(anc, b.ancestors())
+    assert a.get_lcg() == b.get_lcg()
+    pa, qa, wa = a.get_state()
+    pb, qb, wb = b.get_state()
+    assert np.array_equal(pa, pb) and np.array_equal(qa, qb) and np.array_equal(wa, wb)
+    b.close()
+    if n < 4_000_000:
+        o = OracleFilter(n, 77, True)
+        o.set_state(pos.astype(np.float64), quat.astype(np.float64), w.astype(np.float64))
+        o.set_rng(o.rng()[0], rng[1], lcg)
+        o.resample(0.01)
+        assert np.array_equal(anc, o.ancestors())
+    a.close()
+
+
 def test_resample_after_empty_weight_scans_from_aggregates(gpu):
     """pfc_weight with no observations resets the weights to uniform through the normalisation
     (which leaves the aggregates); the resample that follows starts from them and gives the
