@@ -185,6 +185,37 @@ def test_staged_run_equals_cycles(gpu, graph):
         assert np.array_equal(a.get_state(f)[0], b.get_state(f)[0])
 
 
+def test_staged_run_equals_cycles_many_tiles(gpu):
+    """Staged cycles at 1M and 2.2M particles equal the same cycles through pfb_cycle.  Their
+    covariance finalize takes two and five rounds of 512 tiles per lane group, and the 2.2M
+    filter's scan (2149 tiles) starts with the tile-prefix pass."""
+    sizes = [1_000_000, 3000, 2_200_001]
+    lms = [64, 8, 32]
+    cycles = 3
+    scs = scenarios(sizes, lms, cycles + 2, 700)
+    a = make_batch(sizes, scs, 0.01)
+    b = make_batch(sizes, scs, 0.01)
+    for bf in (a, b):
+        bf.cycle(batch_inputs(bf, scs, 0)[0])
+    ref = [a.cycle(batch_inputs(a, scs, k)[0]) for k in range(1, cycles + 1)]
+    staged = [batch_inputs(b, scs, k) for k in range(1, cycles + 1)]
+    b.stage_cycles([inp for inp, keep in staged])
+    got = b.run_staged()
+    n_rs = 0
+    for k in range(cycles):
+        for f in range(len(sizes)):
+            r, g = ref[k][f], got[k][f]
+            for name in ("pos", "quat", "pos_cov", "att_cov"):
+                assert np.array_equal(np.array(getattr(r, name)[:]), np.array(getattr(g, name)[:])), (k, f, name)
+            assert r.ess == g.ess and r.resampled == g.resampled and r.valid == g.valid
+            n_rs += r.resampled
+    assert n_rs > 0
+    for f in range(len(sizes)):
+        pa, qa, wa = a.get_state(f)
+        pb, qb, wb = b.get_state(f)
+        assert np.array_equal(pa, pb) and np.array_equal(qa, qb) and np.array_equal(wa, wb)
+
+
 @pytest.mark.parametrize("cycles", [1, 2, 5])
 def test_staged_runs_mixed_resampling(gpu, cycles):
     """Staged runs of 1, 2 and 5 cycles where, in the same cycles, one filter resamples and
